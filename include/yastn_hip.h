@@ -1,0 +1,211 @@
+/*
+ * yastn_hip -- C ABI of the MI355X (gfx950) block-sparse backend hot path.
+ *
+ * Every entry point takes raw device pointers, element counts, a device-resident
+ * work list ("plan") built by the host from the reference planner's meta tuples,
+ * and a hipStream_t.  No torch types cross this boundary.  Entry points return
+ * YH_OK (0) or a negative status; yastn_hip_last_error() returns the message.
+ * They never allocate or free device memory: every output and scratch buffer is
+ * passed in by the caller.  All launches are asynchronous on `stream`.
+ *
+ * Data model (reference yastn/tensor/_auxiliary.py:26-38): a tensor's data is one
+ * flat 1-D array of float64 or complex128 holding C-ordered blocks; meta slices
+ * are element offsets into it.
+ *
+ * The reference's "FFI" for this path is the duck-typed backend module
+ * (yastn/backend/backend_torch.py:32-49); each entry point below names the
+ * backend function it replaces.  yastn_amd/backend_hip.py is the binding.
+ */
+#ifndef YASTN_HIP_H
+#define YASTN_HIP_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status / dtypes ---------------------------------------------------- */
+#define YH_OK 0
+#define YH_ERR_ARG -1      /* invalid argument or plan out of bounds          */
+#define YH_ERR_HIP -2      /* HIP runtime error                               */
+#define YH_ERR_NUMERIC -3  /* numerical failure flagged by a kernel           */
+
+#define YH_F64 0
+#define YH_C128 1
+
+#define YH_ABI_VERSION 3
+
+/* ---- box-gather plan (one job per copied or zero-filled box) -------------
+ * int64 words per job (YH_JOB_WORDS):
+ *   0 src_off  element offset in the source, -1 = zero fill
+ *   1 dst_off  element offset in the destination
+ *   2 rank     1..YH_MAXR (after merging dimensions contiguous in both)
+ *   3 kind     0 = element copy along the destination-contiguous dim,
+ *              1 = 32x32 LDS-staged transpose of (tdim, rank-1)
+ *   4 tdim     kind 1: dimension whose source stride is 1
+ *   5 tile_begin  index of the job's first tile
+ *   6..11  shape[YH_MAXR]    (destination C order; the last dim has dst stride 1)
+ *   12..17 src_stride[YH_MAXR]
+ *   18..23 dst_stride[YH_MAXR]
+ * tile_job[t] (int32) is the job owning tile t.
+ */
+#define YH_MAXR 6
+#define YH_JOB_WORDS 24
+#define YH_GATHER_TILE 2048 /* elements per kind-0 tile */
+#define YH_TRANSPOSE_TILE 32 /* kind-1 tile edge */
+
+/* Host-side bounds check of a gather plan against buffer sizes (elements). */
+int yastn_hip_check_gather_plan(const int64_t* h_jobs, int64_t njobs, const int32_t* h_tile_job,
+                                int64_t ntiles, int64_t src_numel, int64_t dst_numel);
+
+/* Generic engine behind the four entry points that follow. */
+int yastn_hip_block_gather(void* dst, int64_t dst_numel, const void* src, int64_t src_numel, int dtype,
+                           const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job, int64_t ntiles,
+                           hipStream_t stream);
+
+/* A1: replaces backend.transpose_and_merge(data, order, meta_new, meta_mrg, Dsize)
+ *     (reference yastn/backend/backend_np.py:653-661, backend_torch.py:588-589,
+ *      _backend_torch_backwards.py:338-364).  Uncovered destination boxes are
+ *     zero-fill jobs of the same plan, so every destination element is written once. */
+int yastn_hip_transpose_and_merge(void* newdata, int64_t Dsize, const void* data, int64_t data_numel, int dtype,
+                                  const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job,
+                                  int64_t ntiles, hipStream_t stream);
+
+/* A6: replaces kernel_transpose_and_merge.backward (_backend_torch_backwards.py:376-392):
+ *     gather under the inverse permutation. */
+int yastn_hip_transpose_and_merge_adjoint(void* grad_src, int64_t src_numel, const void* grad_new,
+                                          int64_t Dsize, int dtype, const int64_t* d_jobs, int64_t njobs,
+                                          const int32_t* d_tile_job, int64_t ntiles, hipStream_t stream);
+
+/* A2: replaces backend.unmerge(data, meta) (backend_np.py:664-669,
+ *     _backend_torch_backwards.py:395-408). */
+int yastn_hip_unmerge(void* newdata, const void* data, int64_t numel, int dtype, const int64_t* d_jobs,
+                      int64_t njobs, const int32_t* d_tile_job, int64_t ntiles, hipStream_t stream);
+
+/* A8: replaces kernel_unmerge.backward (_backend_torch_backwards.py:417-426). */
+int yastn_hip_unmerge_adjoint(void* grad_src, const void* grad_new, int64_t numel, int dtype,
+                              const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job, int64_t ntiles,
+                              hipStream_t stream);
+
+/* ---- grouped GEMM plan ---------------------------------------------------
+ * C_g (=|+=) sum_t op(A_t) op(B_t), FP64 MFMA (v_mfma_f64_16x16x4f64); complex128 as
+ * four real products.  Operands are read through (row, col) strides so that
+ * transposes and conjugate transposes need no copies.
+ * gemm (int64 x YH_GEMM_WORDS): 0 M, 1 N, 2 c_off, 3 c_rs, 4 c_cs, 5 term_begin,
+ *                               6 term_count, 7 flags (bit0 accumulate, bits8.. out buffer id)
+ * term (int64 x YH_TERM_WORDS): 0 a_off, 1 a_rs, 2 a_cs, 3 b_off, 4 b_rs, 5 b_cs, 6 K,
+ *                               7 flags (bit0 conj A, bit1 conj B, bits8.. A buffer id, bits16.. B buffer id)
+ * tile (int32 x 3): gemm index, first row, first column.
+ */
+#define YH_GEMM_WORDS 8
+#define YH_TERM_WORDS 8
+#define YH_MAX_BUFS 4
+
+/* tile configurations selectable by the autotuner */
+#define YH_TILE_32x32 0
+#define YH_TILE_64x64 1
+#define YH_TILE_128x64 2
+
+int yastn_hip_check_gemm_plan(const int64_t* h_gemms, int64_t ngemms, const int64_t* h_terms, int64_t nterms,
+                              const int32_t* h_tiles, int64_t ntiles, int tile_config,
+                              const int64_t* in_numel, int nin, const int64_t* out_numel, int nout);
+
+int yastn_hip_grouped_gemm(void* const* outs, const void* const* ins, int dtype, const int64_t* d_gemms,
+                           int64_t ngemms, const int64_t* d_terms, int64_t nterms, const int32_t* d_tiles,
+                           int64_t ntiles, int tile_config, hipStream_t stream);
+
+/* A3: replaces backend.dot(Adata, Bdata, meta_dot, Dsize) (backend_np.py:580-587,
+ *     _backend_torch_backwards.py:98-109): all sector GEMMs of one tensordot in one launch. */
+int yastn_hip_dot(void* C, const void* A, const void* B, int dtype, const int64_t* d_gemms, int64_t ngemms,
+                  const int64_t* d_terms, int64_t nterms, const int32_t* d_tiles, int64_t ntiles, int tile_config,
+                  hipStream_t stream);
+
+/* A7: replaces kernel_dot.backward (_backend_torch_backwards.py:117-138): gA = sum gC B^H and
+ *     gB = sum A^H gC, accumulated per operand block without atomics, one launch. */
+int yastn_hip_dot_adjoint(void* gA, void* gB, const void* gC, const void* A, const void* B, int dtype,
+                          const int64_t* d_gemms, int64_t ngemms, const int64_t* d_terms, int64_t nterms,
+                          const int32_t* d_tiles, int64_t ntiles, int tile_config, hipStream_t stream);
+
+/* ---- batched one-sided Jacobi SVD / Hermitian Jacobi eigh -----------------
+ * block (int64 x YH_SVD_WORDS): 0 a_off, 1 m, 2 n, 3 u_off, 4 s_off, 5 v_off, 6 work_off, 7 unused
+ * (eigh: m == n, v_off unused).  order[] lists block indices: the first n_lds blocks run with their
+ * working matrices in LDS (lds_bytes per workgroup, <= YH_SVD_LDS_BYTES), the rest in `work`
+ * (global memory), each needing (p*q + q*q) elements at work_off, p = max(m,n), q = min(m,n).
+ * info[b] = sweeps used, or -1 if not converged within max_sweeps.  q <= YH_SVD_MAX_Q.
+ */
+#define YH_SVD_WORDS 8
+#define YH_SVD_MAX_Q 2048
+#define YH_SVD_LDS_BYTES 49152
+
+int yastn_hip_check_svd_plan(const int64_t* h_blocks, int64_t nblocks, const int32_t* h_order, int64_t n_lds,
+                             int64_t lds_bytes, int mode, int dtype, int64_t a_numel, int64_t u_numel,
+                             int64_t s_numel, int64_t v_numel, int64_t work_numel);
+
+/* A4: replaces backend.svd(data, meta, sizes) (backend_np.py:330-340,
+ *     _backend_torch_backwards.py:26-39): thin SVD of every block, S descending. */
+int yastn_hip_svd(void* U, void* S, void* Vh, const void* A, int dtype, const int64_t* d_blocks, int64_t nblocks,
+                  const int32_t* d_order, int64_t n_lds, int64_t lds_bytes, void* work, int32_t* d_info,
+                  int max_sweeps, hipStream_t stream);
+
+/* A5: replaces backend.eigh(data, meta, sizes) (backend_np.py:461-474, backend_torch.py:369-384):
+ *     Hermitian eigendecomposition of every block, eigenvalues ascending. */
+int yastn_hip_eigh(void* S, void* U, const void* A, int dtype, const int64_t* d_blocks, int64_t nblocks,
+                   const int32_t* d_order, int64_t n_lds, int64_t lds_bytes, void* work, int32_t* d_info,
+                   int max_sweeps, hipStream_t stream);
+
+/* A11: replaces backend.fix_svd_signs(Udata, Vdata, meta) (backend_np.py:447-458,
+ *      backend_torch.py:353-366): in place on U and Vh; blocks are SVD plan words. */
+int yastn_hip_fix_svd_signs(void* U, void* Vh, int dtype, const int64_t* d_blocks, int64_t nblocks,
+                            hipStream_t stream);
+
+/* A11 (differentiable form): only the pivot of each U column, piv[s_off + j] = flat index in U of
+ *      the first entry of largest 2^40-quantised modulus (backend_torch.py:353-366 takes the phase of
+ *      that entry with differentiable tensor ops; the binding does the same with the pivots). */
+int yastn_hip_svd_sign_pivots(const void* U, int64_t* piv, int dtype, const int64_t* d_blocks, int64_t nblocks,
+                              hipStream_t stream);
+
+/* A9 core: the k x k middle factor of kernel_svd.backward / SVDGESDD.backward
+ *     (_backend_torch_backwards.py:53-66, linalg/torch_svd_gesdd.py:296-428) for every block:
+ *       J = 1/2 (F+G) o (X - X^H) + 1/2 (F-G) o (Y - Y^H) + diag(gS) + i diag(Im X_jj / s_j)
+ *           - [m>k] X diag(1/s) - [n>k] diag(1/s) Y^H
+ *     with X = U^H gU, Y = Vh gVh^H, F_ij = l(s_j - s_i), G_ij = l(s_j + s_i), l(x) = x/(x^2+eps),
+ *     1/s -> l(s); and T = [m>k] gU diag(1/s) or [n>k] diag(1/s) gVh (else 0).  The caller then
+ *     forms gA = (T + U J) Vh (m >= n) or U (T + J Vh) (m < n) with grouped GEMMs.
+ *     desc words (x8): 0 m, 1 n, 2 k, 3 u_off, 4 s_off, 5 v_off, 6 offset in X/Y/J, 7 offset in T. */
+int yastn_hip_svd_adjoint_core(void* J, void* T, const void* X, const void* Y, const void* S, const void* gS,
+                               const void* gU, const void* gVh, int dtype, const int64_t* d_descs,
+                               int64_t nblocks, double eps, int has_gS, int has_gU, int has_gV,
+                               hipStream_t stream);
+
+/* A10 core: the n x n middle factor of the Hermitian eigen adjoint (torch.linalg.eigh backward,
+ *     used by backend_torch.py:378): Xo = skew(X) / E, E_ij = w_j - w_i (E_ii = 1), diag(Xo) = gS
+ *     when given; X = U^H gU.  gA = U Xo U^H follows with grouped GEMMs.
+ *     desc words (x8): 0 n, 1 s_off, 2 offset in X/Xo. */
+int yastn_hip_eigh_adjoint_core(void* Xo, const void* X, const void* S, const void* gS, int dtype,
+                                const int64_t* d_descs, int64_t nblocks, int has_gS, int has_gU,
+                                hipStream_t stream);
+
+/* A12: replaces backend.apply_mask(Adata, mask, meta, Dsize, axis, ndim) and embed_mask
+ *      (backend_np.py:626-643, _backend_torch_backwards.py:251-310): the kept index list of each
+ *      block is split into runs of consecutive indices, one box-gather job per run; embed_mask adds
+ *      zero-fill jobs for the complement.  Also the adjoints (apply <-> embed). */
+int yastn_hip_apply_mask(void* newdata, int64_t Dsize, const void* data, int64_t data_numel, int dtype,
+                         const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job, int64_t ntiles,
+                         hipStream_t stream);
+int yastn_hip_embed_mask(void* newdata, int64_t Dsize, const void* data, int64_t data_numel, int dtype,
+                         const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job, int64_t ntiles,
+                         hipStream_t stream);
+
+/* Backend transpose(data, axes, meta_transpose) (backend_np.py:646-650): per-block permutation. */
+int yastn_hip_transpose(void* newdata, const void* data, int64_t numel, int dtype, const int64_t* d_jobs,
+                        int64_t njobs, const int32_t* d_tile_job, int64_t ntiles, hipStream_t stream);
+
+const char* yastn_hip_last_error(void);
+int yastn_hip_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YASTN_HIP_H */

@@ -1,0 +1,90 @@
+"""numpy interpreters of the device work lists, used by CPU tests to check plans without a GPU.
+
+They execute the documented semantics of include/yastn_hip.h (jobs, gemm/term words) with numpy
+indexing; they are independent of the HIP kernels, which the GPU tests check.
+"""
+import numpy as np
+
+YH_MAXR = 6
+
+
+def gather(h_jobs, src, dst):
+    for w in h_jobs:
+        src_off, dst_off, r = int(w[0]), int(w[1]), int(w[2])
+        shape = tuple(int(x) for x in w[6:6 + r])
+        ss = np.array(w[12:12 + r], dtype=np.int64)
+        ds = np.array(w[18:18 + r], dtype=np.int64)
+        idx = np.indices(shape).reshape(r, -1)
+        d = dst_off + ds @ idx
+        if src_off < 0:
+            dst[d] = 0
+        else:
+            dst[d] = src[src_off + ss @ idx]
+    return dst
+
+
+def gemm(h_gemms, h_terms, outs, ins):
+    for g in h_gemms:
+        M, N, c_off, c_rs, c_cs, tb, tc, fl = (int(x) for x in g)
+        C = outs[(fl >> 8) & 0xff]
+        acc = np.zeros((M, N), dtype=C.dtype)
+        for t in h_terms[tb:tb + tc]:
+            a_off, a_rs, a_cs, b_off, b_rs, b_cs, K, tf = (int(x) for x in t)
+            A = ins[(tf >> 8) & 0xff]
+            B = ins[(tf >> 16) & 0xff]
+            ia = a_off + np.arange(M)[:, None] * a_rs + np.arange(K)[None, :] * a_cs
+            ib = b_off + np.arange(K)[:, None] * b_rs + np.arange(N)[None, :] * b_cs
+            a = A[ia] if K and M else np.zeros((M, K), C.dtype)
+            b = B[ib] if K and N else np.zeros((K, N), C.dtype)
+            if tf & 1:
+                a = np.conj(a)
+            if tf & 2:
+                b = np.conj(b)
+            acc = acc + a @ b
+        ic = c_off + np.arange(M)[:, None] * c_rs + np.arange(N)[None, :] * c_cs
+        if fl & 1:
+            C[ic] += acc
+        else:
+            C[ic] = acc
+    return outs
+
+
+def _lor(x, eps):
+    return x / (x * x + eps)
+
+
+def svd_adjoint_core(descs, J, T, X, Y, S, gS, gU, gVh, eps, has_gS, has_gU, has_gV):
+    """numpy transliteration of the svd adjoint core kernel (yastn_amd/csrc/misc.hip)."""
+    for m, n, k, u_off, s_off, v_off, kk, tt in (tuple(int(x) for x in d) for d in descs):
+        s = S[s_off:s_off + k]
+        F = _lor(s[None, :] - s[:, None], eps)
+        G = _lor(s[None, :] + s[:, None], eps)
+        np.fill_diagonal(F, 0)
+        np.fill_diagonal(G, 0)
+        sinv = _lor(s, eps)
+        Jb = np.zeros((k, k), dtype=J.dtype)
+        if has_gU:
+            x = X[kk:kk + k * k].reshape(k, k)
+            Jb += 0.5 * (F + G) * (x - x.conj().T)
+            if np.iscomplexobj(J):
+                Jb[np.arange(k), np.arange(k)] += 1j * np.diag(x).imag * sinv
+            if m > k:
+                Jb -= x * sinv[None, :]
+        if has_gV:
+            y = Y[kk:kk + k * k].reshape(k, k)
+            Jb += 0.5 * (F - G) * (y - y.conj().T)
+            if n > k:
+                Jb -= sinv[:, None] * y.conj().T
+        if has_gS:
+            Jb[np.arange(k), np.arange(k)] += gS[s_off:s_off + k]
+        J[kk:kk + k * k] = Jb.ravel()
+        if m >= n:
+            t = np.zeros((m, k), dtype=J.dtype)
+            if has_gU and m > k:
+                t = gU[u_off:u_off + m * k].reshape(m, k) * sinv[None, :]
+            T[tt:tt + m * k] = t.ravel()
+        else:
+            t = np.zeros((k, n), dtype=J.dtype)
+            if has_gV:
+                t = sinv[:, None] * gVh[v_off:v_off + k * n].reshape(k, n)
+            T[tt:tt + k * n] = t.ravel()

@@ -1,0 +1,279 @@
+"""Pin the CPU oracle (oracle/np_backend.py) before trusting it.
+
+Checks: the reference numpy backend's golden vectors, the reference's own C merge worker
+(oracle/_ref, when built), dense closed forms, round trips, adjoint identities, finite
+differences, the reference torch backend's recorded gradients and the known-answer tests of
+the reference test-suite.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import golden_io as G
+from oracle import np_backend as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TM_WORKER = os.path.join(ROOT, 'oracle', '_ref', 'libtm_worker.so')
+
+
+def _cmp_exact(a, b):
+    assert a.dtype == b.dtype and a.shape == b.shape
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize('scenario', G.SCENARIOS)
+def test_oracle_copies_match_golden(scenario):
+    n = 0
+    for fn, args, kw, outs in G.calls(scenario):
+        if fn == 'transpose_and_merge':
+            _cmp_exact(O.transpose_and_merge(*args), outs[0]); n += 1
+        elif fn == 'unmerge':
+            _cmp_exact(O.unmerge(*args), outs[0]); n += 1
+        elif fn == 'transpose':
+            _cmp_exact(O.transpose(*args), outs[0]); n += 1
+        elif fn == 'apply_mask':
+            _cmp_exact(O.apply_mask(*args), outs[0]); n += 1
+    assert n > 0 or scenario == 'fix_signs_kat'
+
+
+@pytest.mark.parametrize('scenario', G.SCENARIOS)
+def test_oracle_dot_matches_golden(scenario):
+    for fn, args, kw, outs in G.calls(scenario, 'dot'):
+        got = O.dot(*args)
+        ref = outs[0]
+        assert got.dtype == ref.dtype
+        assert np.linalg.norm(got - ref) <= 1e-13 * max(1.0, np.linalg.norm(ref))
+
+
+@pytest.mark.parametrize('scenario', G.SCENARIOS)
+def test_oracle_svd_matches_golden(scenario):
+    for fn, args, kw, outs in G.calls(scenario, 'svd'):
+        data, meta, sizes = args[:3]
+        U, S, V = O.svd(data, meta, sizes)
+        np.testing.assert_allclose(S, outs[1], rtol=1e-12, atol=1e-13 * max(1.0, outs[1].max(initial=0)))
+        for sl, D, slU, DU, slS, slV, DV in meta:
+            a = data[sl[0]:sl[1]].reshape(D)
+            u = U[slU[0]:slU[1]].reshape(DU)
+            v = V[slV[0]:slV[1]].reshape(DV)
+            s = S[slS[0]:slS[1]]
+            assert np.linalg.norm(u * s @ v - a) <= 1e-12 * max(1.0, np.linalg.norm(a))
+            # gauge-invariant: the recorded factors reconstruct the same block
+            ur = outs[0][slU[0]:slU[1]].reshape(DU)
+            vr = outs[2][slV[0]:slV[1]].reshape(DV)
+            assert np.linalg.norm(ur * outs[1][slS[0]:slS[1]] @ vr - a) <= 1e-12 * max(1.0, np.linalg.norm(a))
+
+
+def test_oracle_eigh_matches_golden():
+    for fn, args, kw, outs in G.calls('eigh_u1_c128', 'eigh'):
+        data, meta, sizes = args
+        S, U = O.eigh(data, meta, sizes)
+        np.testing.assert_allclose(S, outs[0], rtol=1e-12, atol=1e-12)
+        for sl, D, slU, DU, slS in meta:
+            a = data[sl[0]:sl[1]].reshape(D)
+            u = U[slU[0]:slU[1]].reshape(DU)
+            w = S[slS[0]:slS[1]]
+            assert np.linalg.norm(u * w @ u.conj().T - a) < 1e-12 * np.linalg.norm(a)
+            assert np.all(np.diff(w) >= 0)
+
+
+def test_oracle_fix_signs_kat():
+    """Reference tests/tensor/test_svd.py:159-186: the fixed U has real positive largest entries."""
+    calls = G.calls('fix_signs_kat')
+    fn, args, kw, outs = [c for c in calls if c[0] == 'fix_svd_signs'][0]
+    U, V = O.fix_svd_signs(*args)
+    np.testing.assert_allclose(U, outs[0], atol=1e-14)
+    np.testing.assert_allclose(V, outs[1], atol=1e-14)
+    is2 = 0.5 ** 0.5
+    expect = np.array([[is2, 1j / 2], [-is2, 1j / 2], [0, is2]])
+    t = G.tensors('fix_signs_kat')['U']
+    blk0 = t['data'][t['slices'][0][0]:t['slices'][0][1]].reshape(t['D'][0])
+    np.testing.assert_allclose(blk0, expect, atol=1e-12)
+
+
+def test_oracle_fix_signs_golden():
+    for fn, args, kw, outs in G.calls('u1xu1_c128', 'fix_svd_signs'):
+        U, V = O.fix_svd_signs(*args)
+        np.testing.assert_allclose(U, outs[0], atol=1e-14)
+        np.testing.assert_allclose(V, outs[1], atol=1e-14)
+
+
+def _tm_worker():
+    if not os.path.exists(TM_WORKER):
+        pytest.skip('reference C worker not built (oracle/Makefile needs the reference tree)')
+    lib = ctypes.CDLL(TM_WORKER)
+    f = lib.tm_worker
+    ip = ctypes.POINTER(ctypes.c_int)
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ip, ip, ctypes.c_void_p, ctypes.c_int, ip, ip, ip,
+                  ctypes.c_int, ctypes.c_int]
+    f.restype = ctypes.c_int
+    return f
+
+
+def _ints(x):
+    arr = (ctypes.c_int * max(1, len(x)))(*[int(v) for v in x])
+    return arr
+
+
+@pytest.mark.parametrize('scenario', G.SCENARIOS)
+def test_oracle_merge_vs_reference_c_worker(scenario):
+    """The numpy restatement of transpose_and_merge agrees bit for bit with the reference's own C
+    merge worker (experimental/tm_worker.c, built from the reference tree into oracle/_ref)."""
+    f = _tm_worker()
+    for fn, args, kw, outs in G.calls(scenario, 'transpose_and_merge'):
+        data, order, meta_new, meta_mrg, Dsize = args
+        data = np.ascontiguousarray(data)
+        out = np.zeros(Dsize, dtype=data.dtype)
+        for (tn, Dn, sln), rows in O._groups(meta_new, meta_mrg):
+            for _, slo, Do, Dslc, Drsh in rows:
+                st = f(out.ctypes.data, sln[0], len(Dn), _ints(Dn), _ints([s[0] for s in Dslc]),
+                       data.ctypes.data, slo[0], _ints(Do), _ints(order), _ints(Drsh), len(Do), data.itemsize)
+                assert st == 0
+        _cmp_exact(out, O.transpose_and_merge(*args))
+
+
+def _densify(t):
+    """Dense array of a golden tensor from its block charges (legs sorted by charge)."""
+    s = t['s']
+    nsym = len(t['n'])
+    legs = []
+    for ax in range(len(s)):
+        td = {}
+        for tt, DD in zip(t['t'], t['D']):
+            td[tuple(tt[ax * nsym:(ax + 1) * nsym])] = DD[ax]
+        keys = sorted(td)
+        offs, acc = {}, 0
+        for k in keys:
+            offs[k] = acc
+            acc += td[k]
+        legs.append((offs, acc))
+    out = np.zeros(tuple(l[1] for l in legs), dtype=t['data'].dtype)
+    for tt, DD, sl in zip(t['t'], t['D'], t['slices']):
+        idx = tuple(slice(legs[ax][0][tuple(tt[ax * nsym:(ax + 1) * nsym])],
+                          legs[ax][0][tuple(tt[ax * nsym:(ax + 1) * nsym])] + DD[ax]) for ax in range(len(s)))
+        out[idx] = t['data'][sl[0]:sl[1]].reshape(DD)
+    return out, legs
+
+
+@pytest.mark.parametrize('scenario', ['mps_u1_fuse_contracted', 'mps_u1_fuse_to_matrix', 'u1xu1_c128'])
+def test_golden_transfer_matrix_closed_form(scenario):
+    """tensordot(A, A*, ((0,1),(0,1))) equals np.tensordot of the densified tensors (both policies)."""
+    ts = G.tensors(scenario)
+    A, _ = _densify(ts['A'])
+    T, _ = _densify(ts['T'])
+    ref = np.tensordot(A, A.conj(), axes=((0, 1), (0, 1)))
+    assert T.shape == ref.shape
+    assert np.linalg.norm(T - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_merge_unmerge_round_trip_and_adjoint():
+    rng = np.random.default_rng(0)
+    for scenario in ('fuse_u1_fuse_contracted', 'mps_u1_fuse_to_matrix', 'u1xu1_c128'):
+        for fn, args, kw, outs in G.calls(scenario, 'transpose_and_merge'):
+            data, order, meta_new, meta_mrg, Dsize = args
+            x = rng.standard_normal(data.size)
+            y = rng.standard_normal(Dsize)
+            mx = O.transpose_and_merge(x, order, meta_new, meta_mrg, Dsize)
+            aty = O.transpose_and_merge_adjoint(y, order, meta_new, meta_mrg, data.size)
+            assert abs(mx @ y - x @ aty) <= 1e-12 * np.linalg.norm(x) * np.linalg.norm(y)
+        for fn, args, kw, outs in G.calls(scenario, 'unmerge'):
+            data, meta = args
+            x = rng.standard_normal(data.size)
+            y = rng.standard_normal(data.size)
+            ux = O.unmerge(x, meta)
+            aty = O.unmerge_adjoint(y, meta, data.size)
+            assert abs(ux @ y - x @ aty) <= 1e-12 * np.linalg.norm(x) * np.linalg.norm(y)
+            # unmerge is a permutation of the elements: its adjoint is its inverse
+            np.testing.assert_array_equal(O.unmerge_adjoint(ux, meta, data.size), x)
+
+
+def _num_grad(f, x, h=1e-6):
+    g = np.zeros_like(x)
+    for i in range(x.size):
+        e = np.zeros_like(x)
+        e.flat[i] = h
+        g.flat[i] = (f(x + e) - f(x - e)) / (2 * h)
+    return g
+
+
+def test_oracle_dot_adjoint_fd():
+    fn, args, kw, outs = G.calls('mps_u1_fuse_contracted', 'dot')[0]
+    A, B, meta, Dsize = args
+    rng = np.random.default_rng(1)
+    W = rng.standard_normal(Dsize)
+    gA, gB = O.dot_adjoint(W, A, B, meta)
+    fA = _num_grad(lambda a: O.dot(a, B, meta, Dsize) @ W, A)
+    fB = _num_grad(lambda b: O.dot(A, b, meta, Dsize) @ W, B)
+    np.testing.assert_allclose(gA, fA, rtol=1e-7, atol=1e-7)
+    np.testing.assert_allclose(gB, fB, rtol=1e-7, atol=1e-7)
+
+
+def test_oracle_svd_adjoint_vs_reference_torch_gradients():
+    js, arrays = G.load('torch_grads')
+    for bi, blk in enumerate(js['svd_blocks']):
+        a = arrays[f'svdb{bi}_a']
+        g_ref = arrays[f'svdb{bi}_g']
+        u, s, vh = np.linalg.svd(a, full_matrices=False)
+        if blk['loss'] == 'linear':
+            # reference factors, so that the gauge of gU/gV matches
+            u, s, vh = arrays[f'svdb{bi}_u'], arrays[f'svdb{bi}_s'], arrays[f'svdb{bi}_vh']
+            g = O.svd_block_adjoint(u, s, vh, arrays[f'svdb{bi}_gu'], arrays[f'svdb{bi}_gs'], arrays[f'svdb{bi}_gv'])
+        else:
+            g = _complex_svd_grad_fd_free(a, arrays[f'svdb{bi}_w'])
+        np.testing.assert_allclose(g, g_ref, rtol=1e-8, atol=1e-9)
+
+
+def _complex_svd_grad_fd_free(a, w):
+    """Adjoint of loss(a) = Re tr(U S Vh W) + Im tr(U Vh W) + sum s^3 via the oracle's svd adjoint.
+
+    For a real loss L(z), torch's gradient is conj(dL/dz) (= 2 dL/d conj z / 2 per its convention,
+    reported as dL/dRe + i dL/dIm).  With M = W:  Re tr(U S Vh W) gives gU = conj(S Vh W)^T, gVh =
+    conj(W U S)^T, gS = Re diag(Vh W U); Im tr(U Vh W) = Re tr(-i U Vh W) gives gU = conj(-i Vh W)^T, gVh =
+    conj(-i W U)^T.
+    """
+    u, s, vh = np.linalg.svd(a, full_matrices=False)
+    gu = np.conj((s[:, None] * vh) @ w).T + np.conj(-1j * (vh @ w)).T
+    gvh = np.conj(w @ (u * s[None, :])).T + np.conj(-1j * (w @ u)).T
+    gs = np.real(np.diag(vh @ w @ u)) + 3 * s ** 2
+    return O.svd_block_adjoint(u, s, vh, gu, gs, gvh)
+
+
+def test_oracle_svd_adjoint_fd_real():
+    rng = np.random.default_rng(3)
+    for m, n in ((5, 5), (6, 3), (3, 7)):
+        a = rng.standard_normal((m, n))
+        k = min(m, n)
+        cu, cs, cv = rng.standard_normal((m, k)), rng.standard_normal(k), rng.standard_normal((k, n))
+
+        def loss(x):
+            u, s, vh = np.linalg.svd(x, full_matrices=False)
+            # fix the sign gauge so that the loss is a smooth function of x
+            sg = np.sign(u[np.argmax(np.abs(u), axis=0), np.arange(k)])
+            u, vh = u * sg, vh * sg[:, None]
+            return np.sum(u * cu) + s @ cs + np.sum(vh * cv)
+        u, s, vh = np.linalg.svd(a, full_matrices=False)
+        sg = np.sign(u[np.argmax(np.abs(u), axis=0), np.arange(k)])
+        u, vh = u * sg, vh * sg[:, None]
+        g = O.svd_block_adjoint(u, s, vh, cu, cs, cv)
+        np.testing.assert_allclose(g, _num_grad(loss, a), rtol=1e-6, atol=1e-6)
+
+
+def test_oracle_eigh_adjoint_fd():
+    rng = np.random.default_rng(4)
+    n = 6
+    x = rng.standard_normal((n, n))
+    a = x + x.T
+    cw, cu = rng.standard_normal(n), rng.standard_normal((n, n))
+
+    def loss(y):
+        y = (y + y.T) / 2
+        w, u = np.linalg.eigh(y)
+        sg = np.sign(u[np.argmax(np.abs(u), axis=0), np.arange(n)])
+        return w @ cw + np.sum(u * sg * cu)
+    w, u = np.linalg.eigh(a)
+    sg = np.sign(u[np.argmax(np.abs(u), axis=0), np.arange(n)])
+    g = O.eigh_block_adjoint(w, u * sg, cw, cu)
+    # the loss symmetrises its argument, so compare the symmetric part
+    fd = _num_grad(loss, a)
+    np.testing.assert_allclose((g + g.T) / 2, (fd + fd.T) / 2, rtol=1e-6, atol=1e-6)

@@ -1,0 +1,97 @@
+"""ctypes binding of the C-ABI library ``libyastn_hip.so`` (declared in include/yastn_hip.h).
+
+The library must exist (built by ``__graft_entry__.build()`` / ``python -m yastn_amd.build``);
+there is no fallback: a missing library or a non-zero status raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get('YASTN_HIP_LIB', os.path.join(HERE, 'libyastn_hip.so'))
+
+# constants mirrored from include/yastn_hip.h
+YH_OK, YH_ERR_ARG, YH_ERR_HIP, YH_ERR_NUMERIC = 0, -1, -2, -3
+YH_F64, YH_C128 = 0, 1
+YH_ABI_VERSION = 3
+YH_MAXR = 6
+YH_JOB_WORDS = 24
+YH_GATHER_TILE = 2048
+YH_TRANSPOSE_TILE = 32
+YH_GEMM_WORDS = 8
+YH_TERM_WORDS = 8
+YH_MAX_BUFS = 4
+YH_TILE_32x32, YH_TILE_64x64, YH_TILE_128x64 = 0, 1, 2
+TILE_DIMS = {YH_TILE_32x32: (32, 32), YH_TILE_64x64: (64, 64), YH_TILE_128x64: (128, 64)}
+YH_SVD_WORDS = 8
+YH_SVD_MAX_Q = 2048
+YH_SVD_LDS_BYTES = 49152
+
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+_dbl = ctypes.c_double
+
+# name -> argtypes (all return int)
+_SIGS = {
+    'yastn_hip_abi_version': [],
+    'yastn_hip_check_gather_plan': [_p, _i64, _p, _i64, _i64, _i64],
+    'yastn_hip_block_gather': [_p, _i64, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
+    'yastn_hip_transpose_and_merge': [_p, _i64, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
+    'yastn_hip_transpose_and_merge_adjoint': [_p, _i64, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
+    'yastn_hip_unmerge': [_p, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
+    'yastn_hip_unmerge_adjoint': [_p, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
+    'yastn_hip_transpose': [_p, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
+    'yastn_hip_apply_mask': [_p, _i64, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
+    'yastn_hip_embed_mask': [_p, _i64, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
+    'yastn_hip_check_gemm_plan': [_p, _i64, _p, _i64, _p, _i64, _i32, _p, _i32, _p, _i32],
+    'yastn_hip_grouped_gemm': [_p, _p, _i32, _p, _i64, _p, _i64, _p, _i64, _i32, _p],
+    'yastn_hip_dot': [_p, _p, _p, _i32, _p, _i64, _p, _i64, _p, _i64, _i32, _p],
+    'yastn_hip_dot_adjoint': [_p, _p, _p, _p, _p, _i32, _p, _i64, _p, _i64, _p, _i64, _i32, _p],
+    'yastn_hip_check_svd_plan': [_p, _i64, _p, _i64, _i64, _i32, _i32, _i64, _i64, _i64, _i64, _i64],
+    'yastn_hip_svd': [_p, _p, _p, _p, _i32, _p, _i64, _p, _i64, _i64, _p, _p, _i32, _p],
+    'yastn_hip_eigh': [_p, _p, _p, _i32, _p, _i64, _p, _i64, _i64, _p, _p, _i32, _p],
+    'yastn_hip_fix_svd_signs': [_p, _p, _i32, _p, _i64, _p],
+    'yastn_hip_svd_sign_pivots': [_p, _p, _i32, _p, _i64, _p],
+    'yastn_hip_svd_adjoint_core': [_p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _i64, _dbl, _i32, _i32, _i32, _p],
+    'yastn_hip_eigh_adjoint_core': [_p, _p, _p, _p, _i32, _p, _i64, _i32, _i32, _p],
+}
+
+_lib = None
+
+
+class HipBackendError(RuntimeError):
+    """Raised when the C-ABI library reports a non-zero status."""
+
+
+def lib():
+    """Load the library once; raise loudly if it is absent or of a different ABI version."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"yastn_amd: HIP library not built ({LIB_PATH}); run `python -m yastn_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, argt in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = argt
+            f.restype = ctypes.c_int
+        L.yastn_hip_last_error.argtypes = []
+        L.yastn_hip_last_error.restype = ctypes.c_char_p
+        ver = L.yastn_hip_abi_version()
+        if ver != YH_ABI_VERSION:
+            raise ImportError(f"yastn_amd: ABI version {ver} != {YH_ABI_VERSION}; rebuild the library")
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return tuple(_SIGS) + ('yastn_hip_last_error',)
+
+
+def call(name, *args):
+    """Call an entry point and raise HipBackendError on a non-zero status."""
+    L = lib()
+    st = getattr(L, name)(*args)
+    if st != YH_OK:
+        msg = L.yastn_hip_last_error().decode(errors='replace')
+        raise HipBackendError(f"{name} failed (status {st}): {msg}")
+    return st

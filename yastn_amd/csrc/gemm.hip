@@ -1,0 +1,302 @@
+// Grouped FP64 / complex128 GEMM on CDNA4 FP64 matrix cores.
+//
+// One launch computes every output block of a plan:
+//   C_g (=|+=) sum_t op(A_t) op(B_t)
+// where op() reads an operand through (row, col) strides with optional
+// conjugation, so A^H / B^H of the dot adjoint need no copies.  Output tiles are
+// owned by exactly one workgroup and terms are summed in a fixed order, so results
+// are bitwise reproducible (no atomics).
+//
+// Tile: BM x BN per workgroup of 4 waves (WM x WN), BK = 16, staged through LDS
+// k-major; each wave issues v_mfma_f64_16x16x4f64 on FM x FN 16x16 fragments.
+// Complex data is held as separate real / imaginary LDS planes and multiplied
+// with four real MFMAs per fragment.
+#include "common.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+constexpr int BK = 16;
+constexpr int PAD = 2;
+
+struct Bufs {
+  const void* in[YH_MAX_BUFS];
+  void* out[YH_MAX_BUFS];
+};
+
+template <bool CPLX>
+struct Elt;
+template <>
+struct Elt<false> {
+  typedef double T;
+};
+template <>
+struct Elt<true> {
+  typedef double2 T;
+};
+
+__device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+template <bool CPLX, int BM, int BN, int WM, int WN>
+__global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t* __restrict__ gemms,
+                                                        const int64_t* __restrict__ terms,
+                                                        const int32_t* __restrict__ tiles) {
+  typedef typename Elt<CPLX>::T T;
+  constexpr int FM = BM / WM / 16;
+  constexpr int FN = BN / WN / 16;
+  constexpr int NPL = CPLX ? 2 : 1;  // LDS planes
+  __shared__ double As[NPL][BK][BM + PAD];
+  __shared__ double Bs[NPL][BK][BN + PAD];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+
+  const int32_t* tl = tiles + 3 * (int64_t)blockIdx.x;
+  const int64_t g = tl[0];
+  const int64_t m0 = tl[1], n0 = tl[2];
+  const int64_t* gw = gemms + g * YH_GEMM_WORDS;
+  const int64_t M = gw[0], N = gw[1], c_off = gw[2], c_rs = gw[3], c_cs = gw[4];
+  const int64_t tb = gw[5], tc = gw[6], gflags = gw[7];
+
+  d4 accR[FM][FN], accI[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      accR[i][j] = d4{0, 0, 0, 0};
+      accI[i][j] = d4{0, 0, 0, 0};
+    }
+
+  for (int64_t q = tb; q < tb + tc; ++q) {
+    const int64_t* tw = terms + q * YH_TERM_WORDS;
+    const int64_t a_off = tw[0], a_rs = tw[1], a_cs = tw[2];
+    const int64_t b_off = tw[3], b_rs = tw[4], b_cs = tw[5], K = tw[6], tf = tw[7];
+    const bool conjA = tf & 1, conjB = (tf >> 1) & 1;
+    const T* __restrict__ A = (const T*)bufs.in[(tf >> 8) & 0xff];
+    const T* __restrict__ B = (const T*)bufs.in[(tf >> 16) & 0xff];
+    const bool a_kfast = (a_cs == 1);
+    const bool b_nfast = (b_cs == 1);
+    for (int64_t k0 = 0; k0 < K; k0 += BK) {
+      // ---- stage op(A)[m0:m0+BM, k0:k0+BK] into As[k][m]
+#pragma unroll
+      for (int p = 0; p < (BM * BK) / kThreads; ++p) {
+        const int e = tid + p * kThreads;
+        int ii, kk;
+        if (a_kfast) {
+          kk = e % BK;
+          ii = e / BK;
+        } else {
+          ii = e % BM;
+          kk = e / BM;
+        }
+        const int64_t gi = m0 + ii, gk = k0 + kk;
+        double re = 0.0, im = 0.0;
+        if (gi < M && gk < K) {
+          const T v = A[a_off + gi * a_rs + gk * a_cs];
+          if constexpr (CPLX) {
+            re = v.x;
+            im = conjA ? -v.y : v.y;
+          } else {
+            re = v;
+          }
+        }
+        As[0][kk][ii] = re;
+        if constexpr (CPLX) As[1][kk][ii] = im;
+      }
+      // ---- stage op(B)[k0:k0+BK, n0:n0+BN] into Bs[k][n]
+#pragma unroll
+      for (int p = 0; p < (BN * BK) / kThreads; ++p) {
+        const int e = tid + p * kThreads;
+        int jj, kk;
+        if (b_nfast) {
+          jj = e % BN;
+          kk = e / BN;
+        } else {
+          kk = e % BK;
+          jj = e / BK;
+        }
+        const int64_t gj = n0 + jj, gk = k0 + kk;
+        double re = 0.0, im = 0.0;
+        if (gj < N && gk < K) {
+          const T v = B[b_off + gk * b_rs + gj * b_cs];
+          if constexpr (CPLX) {
+            re = v.x;
+            im = conjB ? -v.y : v.y;
+          } else {
+            re = v;
+          }
+        }
+        Bs[0][kk][jj] = re;
+        if constexpr (CPLX) Bs[1][kk][jj] = im;
+      }
+      __syncthreads();
+      // ---- MFMA over the BK slice, 4 k per instruction
+#pragma unroll
+      for (int ks = 0; ks < BK; ks += 4) {
+        const int kr = ks + (lane >> 4);
+        double ar[FM], ai[FM], br[FN], bi[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          const int col = wm * (BM / WM) + i * 16 + (lane & 15);
+          ar[i] = As[0][kr][col];
+          if constexpr (CPLX) ai[i] = As[1][kr][col];
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const int col = wn * (BN / WN) + j * 16 + (lane & 15);
+          br[j] = Bs[0][kr][col];
+          if constexpr (CPLX) bi[j] = Bs[1][kr][col];
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            accR[i][j] = mfma(ar[i], br[j], accR[i][j]);
+            if constexpr (CPLX) {
+              accR[i][j] = mfma(-ai[i], bi[j], accR[i][j]);
+              accI[i][j] = mfma(ar[i], bi[j], accI[i][j]);
+              accI[i][j] = mfma(ai[i], br[j], accI[i][j]);
+            }
+          }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- epilogue: lane holds rows 4*(lane/16)+v, column lane%16 of each fragment
+  T* __restrict__ C = (T*)bufs.out[(gflags >> 8) & 0xff];
+  const bool accumulate = gflags & 1;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int64_t col = n0 + wn * (BN / WN) + j * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = m0 + wm * (BM / WM) + i * 16 + 4 * (lane >> 4) + v;
+        if (row < M && col < N) {
+          T* p = C + c_off + row * c_rs + col * c_cs;
+          if constexpr (CPLX) {
+            double2 r = make_double2(accR[i][j][v], accI[i][j][v]);
+            if (accumulate) {
+              const double2 o = *p;
+              r.x += o.x;
+              r.y += o.y;
+            }
+            *p = r;
+          } else {
+            double r = accR[i][j][v];
+            if (accumulate) r += *p;
+            *p = r;
+          }
+        }
+      }
+    }
+}
+
+template <bool CPLX>
+int launch(const Bufs& b, const int64_t* gemms, const int64_t* terms, const int32_t* tiles, int64_t ntiles,
+           int cfg, hipStream_t s) {
+  const dim3 grid((unsigned)ntiles), block(kThreads);
+  switch (cfg) {
+    case YH_TILE_32x32:
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 32, 32, 2, 2>), grid, block, 0, s, b, gemms, terms, tiles);
+      break;
+    case YH_TILE_64x64:
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 64, 64, 2, 2>), grid, block, 0, s, b, gemms, terms, tiles);
+      break;
+    case YH_TILE_128x64:
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 128, 64, 2, 2>), grid, block, 0, s, b, gemms, terms, tiles);
+      break;
+    default:
+      yh::set_error("unknown tile config %d", cfg);
+      return YH_ERR_ARG;
+  }
+  return yh::hip_status(hipGetLastError(), "grouped_gemm launch");
+}
+
+void tile_dims(int cfg, int* bm, int* bn) {
+  *bm = cfg == YH_TILE_32x32 ? 32 : (cfg == YH_TILE_64x64 ? 64 : 128);
+  *bn = cfg == YH_TILE_32x32 ? 32 : 64;
+}
+
+}  // namespace
+
+extern "C" int yastn_hip_check_gemm_plan(const int64_t* h_gemms, int64_t ngemms, const int64_t* h_terms,
+                                         int64_t nterms, const int32_t* h_tiles, int64_t ntiles, int cfg,
+                                         const int64_t* in_numel, int nin, const int64_t* out_numel, int nout) {
+  YH_CHECK_ARG(cfg >= YH_TILE_32x32 && cfg <= YH_TILE_128x64, "tile config %d", cfg);
+  YH_CHECK_ARG(nin <= YH_MAX_BUFS && nout <= YH_MAX_BUFS, "too many buffers");
+  int bm, bn;
+  tile_dims(cfg, &bm, &bn);
+  for (int64_t g = 0; g < ngemms; ++g) {
+    const int64_t* w = h_gemms + g * YH_GEMM_WORDS;
+    const int64_t M = w[0], N = w[1], c_off = w[2], c_rs = w[3], c_cs = w[4], tb = w[5], tc = w[6];
+    const int out = (int)((w[7] >> 8) & 0xff);
+    YH_CHECK_ARG(M >= 0 && N >= 0 && c_rs >= 0 && c_cs >= 0, "gemm %lld: negative dims", (long long)g);
+    YH_CHECK_ARG(out < nout, "gemm %lld: output buffer %d", (long long)g, out);
+    if (M > 0 && N > 0)
+      YH_CHECK_ARG(c_off >= 0 && c_off + (M - 1) * c_rs + (N - 1) * c_cs < out_numel[out],
+                   "gemm %lld: output out of bounds", (long long)g);
+    YH_CHECK_ARG(tb >= 0 && tc >= 0 && tb + tc <= nterms, "gemm %lld: term range", (long long)g);
+    for (int64_t q = tb; q < tb + tc; ++q) {
+      const int64_t* t = h_terms + q * YH_TERM_WORDS;
+      const int64_t K = t[6];
+      const int ia = (int)((t[7] >> 8) & 0xff), ib = (int)((t[7] >> 16) & 0xff);
+      YH_CHECK_ARG(ia < nin && ib < nin, "term %lld: input buffer", (long long)q);
+      YH_CHECK_ARG(K >= 0 && t[1] >= 0 && t[2] >= 0 && t[4] >= 0 && t[5] >= 0, "term %lld: negative", (long long)q);
+      if (M > 0 && K > 0)
+        YH_CHECK_ARG(t[0] >= 0 && t[0] + (M - 1) * t[1] + (K - 1) * t[2] < in_numel[ia],
+                     "term %lld: A out of bounds", (long long)q);
+      if (N > 0 && K > 0)
+        YH_CHECK_ARG(t[3] >= 0 && t[3] + (K - 1) * t[4] + (N - 1) * t[5] < in_numel[ib],
+                     "term %lld: B out of bounds", (long long)q);
+    }
+  }
+  for (int64_t t = 0; t < ntiles; ++t) {
+    const int32_t* x = h_tiles + 3 * t;
+    YH_CHECK_ARG(x[0] >= 0 && x[0] < ngemms, "tile %lld: gemm index", (long long)t);
+    const int64_t* w = h_gemms + (int64_t)x[0] * YH_GEMM_WORDS;
+    YH_CHECK_ARG(x[1] >= 0 && x[1] < w[0] && x[1] % bm == 0 && x[2] >= 0 && x[2] < w[1] && x[2] % bn == 0,
+                 "tile %lld: origin", (long long)t);
+  }
+  return YH_OK;
+}
+
+extern "C" int yastn_hip_grouped_gemm(void* const* outs, const void* const* ins, int dtype, const int64_t* d_gemms,
+                                      int64_t ngemms, const int64_t* d_terms, int64_t nterms,
+                                      const int32_t* d_tiles, int64_t ntiles, int cfg, hipStream_t stream) {
+  (void)ngemms;
+  (void)nterms;
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(ntiles >= 0 && ntiles < (int64_t)INT32_MAX, "tile count");
+  if (ntiles == 0) return YH_OK;
+  Bufs b;
+  for (int i = 0; i < YH_MAX_BUFS; ++i) {
+    b.in[i] = ins[i];
+    b.out[i] = outs[i];
+  }
+  return dtype == YH_F64 ? launch<false>(b, d_gemms, d_terms, d_tiles, ntiles, cfg, stream)
+                         : launch<true>(b, d_gemms, d_terms, d_tiles, ntiles, cfg, stream);
+}
+
+extern "C" int yastn_hip_dot(void* C, const void* A, const void* B, int dtype, const int64_t* d_gemms,
+                             int64_t ngemms, const int64_t* d_terms, int64_t nterms, const int32_t* d_tiles,
+                             int64_t ntiles, int cfg, hipStream_t stream) {
+  void* outs[YH_MAX_BUFS] = {C, nullptr, nullptr, nullptr};
+  const void* ins[YH_MAX_BUFS] = {A, B, nullptr, nullptr};
+  return yastn_hip_grouped_gemm(outs, ins, dtype, d_gemms, ngemms, d_terms, nterms, d_tiles, ntiles, cfg, stream);
+}
+
+extern "C" int yastn_hip_dot_adjoint(void* gA, void* gB, const void* gC, const void* A, const void* B, int dtype,
+                                     const int64_t* d_gemms, int64_t ngemms, const int64_t* d_terms,
+                                     int64_t nterms, const int32_t* d_tiles, int64_t ntiles, int cfg,
+                                     hipStream_t stream) {
+  void* outs[YH_MAX_BUFS] = {gA, gB, nullptr, nullptr};
+  const void* ins[YH_MAX_BUFS] = {A, B, gC, nullptr};
+  return yastn_hip_grouped_gemm(outs, ins, dtype, d_gemms, ngemms, d_terms, nterms, d_tiles, ntiles, cfg, stream);
+}

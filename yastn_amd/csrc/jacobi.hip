@@ -1,0 +1,368 @@
+// Batched one-sided Jacobi SVD (Hestenes) and Hermitian eigendecomposition, FP64 / complex128.
+//
+// One workgroup per block.  The block is copied into a column-major working
+// matrix W (p x q, p = max(m,n), q = min(m,n); W = A, or A^H when m < n) and a
+// q x q rotation accumulator V = I.  Each sweep visits all column pairs in the
+// round-robin (circle) order, so the q/2 pairs of a round are disjoint and the
+// 4 waves work on different pairs concurrently; a pair is rotated when
+// |w_i^H w_j| > tol * ||w_i|| ||w_j||.  Converged when a sweep rotates nothing.
+// Then sigma_j = ||w_j||, U = W Sigma^-1 (or V for m < n), sorted descending.
+//
+// Hermitian eigh reuses the same iteration on B = A + mu I, mu = max row
+// abs-sum (Gershgorin, so B is positive semidefinite): the right rotations V are
+// the eigenvectors and lambda_j = sigma_j - mu, written ascending.
+//
+// Blocks whose W and V fit in 64 KiB go through an LDS-resident instance of the
+// kernel; larger blocks use the caller's workspace in global memory.
+#include <float.h>
+
+#include "common.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+template <bool CPLX>
+struct Num;
+template <>
+struct Num<false> {
+  typedef double T;
+  static __device__ inline double re(double x) { return x; }
+  static __device__ inline double im(double) { return 0.0; }
+  static __device__ inline double make(double r, double) { return r; }
+  static __device__ inline double conj(double x) { return x; }
+  static __device__ inline double abs2(double x) { return x * x; }
+};
+template <>
+struct Num<true> {
+  typedef double2 T;
+  static __device__ inline double re(double2 x) { return x.x; }
+  static __device__ inline double im(double2 x) { return x.y; }
+  static __device__ inline double2 make(double r, double i) { return make_double2(r, i); }
+  static __device__ inline double2 conj(double2 x) { return make_double2(x.x, -x.y); }
+  static __device__ inline double abs2(double2 x) { return x.x * x.x + x.y * x.y; }
+};
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// circle-method pairing: qe (even) players, round r in [0, qe-1), pair k in [0, qe/2)
+__device__ inline void pair_of(int qe, int r, int k, int& i, int& j) {
+  const int n1 = qe - 1;
+  if (k == 0) {
+    i = r;
+    j = n1;
+  } else {
+    i = (r + k) % n1;
+    j = (r - k + n1) % n1;
+  }
+  if (i > j) {
+    const int t = i;
+    i = j;
+    j = t;
+  }
+}
+
+struct BlockDesc {
+  int64_t a_off, m, n, u_off, s_off, v_off, work_off;
+};
+
+__device__ inline BlockDesc load_desc(const int64_t* w) {
+  BlockDesc d;
+  d.a_off = w[0];
+  d.m = w[1];
+  d.n = w[2];
+  d.u_off = w[3];
+  d.s_off = w[4];
+  d.v_off = w[5];
+  d.work_off = w[6];
+  return d;
+}
+
+// MODE 0: SVD (U, S, Vh).  MODE 1: Hermitian eigh (S ascending, U).
+template <bool CPLX, bool IN_LDS, int MODE>
+__global__ void __launch_bounds__(kThreads)
+    jacobi_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ order,
+                  const typename Num<CPLX>::T* __restrict__ A, typename Num<CPLX>::T* __restrict__ U,
+                  double* __restrict__ S, typename Num<CPLX>::T* __restrict__ Vh,
+                  typename Num<CPLX>::T* __restrict__ work, int32_t* __restrict__ info, int max_sweeps) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  extern __shared__ double smem_raw[];
+  __shared__ int s_rotated;
+  __shared__ double s_red[kWaves];
+
+  const int bidx = order[blockIdx.x];
+  const BlockDesc d = load_desc(blocks + (int64_t)bidx * YH_SVD_WORDS);
+  const bool tr = (MODE == 0) && (d.m < d.n);
+  const int64_t p = tr ? d.n : d.m;
+  const int64_t q = tr ? d.m : d.n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  T* W = IN_LDS ? reinterpret_cast<T*>(smem_raw) : work + d.work_off;
+  T* V = W + p * q;
+
+  // ---- load W (column-major p x q) and V = I
+  for (int64_t e = tid; e < p * q; e += kThreads) {
+    const int64_t c = e / p, r = e - c * p;
+    T v = tr ? N::conj(A[d.a_off + c * d.n + r]) : A[d.a_off + r * d.n + c];
+    W[e] = v;
+  }
+  for (int64_t e = tid; e < q * q; e += kThreads) {
+    const int64_t c = e / q, r = e - c * q;
+    V[e] = N::make(r == c ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+
+  double mu = 0.0;
+  if (MODE == 1) {
+    // Gershgorin bound of the Hermitian block: max_r sum_c |a_rc|
+    double local = 0.0;
+    for (int64_t r = tid; r < p; r += kThreads) {
+      double acc = 0.0;
+      for (int64_t c = 0; c < q; ++c) acc += sqrt(N::abs2(W[r + c * p]));
+      local = fmax(local, acc);
+    }
+    local = wave_max(local);
+    if (lane == 0) s_red[wave] = local;
+    __syncthreads();
+    mu = s_red[0];
+    for (int w = 1; w < kWaves; ++w) mu = fmax(mu, s_red[w]);
+    for (int64_t r = tid; r < p; r += kThreads) {
+      T x = W[r + r * p];
+      W[r + r * p] = N::make(N::re(x) + mu, N::im(x));
+    }
+    __syncthreads();
+  }
+
+  const double tol = DBL_EPSILON * sqrt((double)(p > 1 ? p : 1));
+  const int qe = (int)(q + (q & 1));
+  int sweeps = 0;
+  bool converged = (q <= 1);
+  while (!converged && sweeps < max_sweeps) {
+    if (tid == 0) s_rotated = 0;
+    __syncthreads();
+    for (int r = 0; r < qe - 1; ++r) {
+      for (int k = wave; k < qe / 2; k += kWaves) {
+        int i, j;
+        pair_of(qe, r, k, i, j);
+        if (j >= q) continue;
+        T* wi = W + (int64_t)i * p;
+        T* wj = W + (int64_t)j * p;
+        double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;
+        for (int64_t x = lane; x < p; x += 64) {
+          const T u = wi[x], v = wj[x];
+          a += N::abs2(u);
+          b += N::abs2(v);
+          // conj(u) * v
+          gr += N::re(u) * N::re(v) + N::im(u) * N::im(v);
+          gi += N::re(u) * N::im(v) - N::im(u) * N::re(v);
+        }
+        a = wave_sum(a);
+        b = wave_sum(b);
+        gr = wave_sum(gr);
+        gi = wave_sum(gi);
+        const double g = sqrt(gr * gr + gi * gi);
+        if (!(g > tol * sqrt(a * b))) continue;  // also skips zero columns
+        // rotation J = [[c, s e^{i phi}], [-s e^{-i phi}, c]] with e^{i phi} = gamma / |gamma|
+        const double zeta = (b - a) / (2.0 * g);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / sqrt(1.0 + t * t);
+        const double s = c * t;
+        const double er = gr / g, ei = gi / g;  // e^{i phi}
+        // w_i' = c w_i - s e^{-i phi} w_j ; w_j' = s e^{i phi} w_i + c w_j
+        for (int64_t x = lane; x < p; x += 64) {
+          const T u = wi[x], v = wj[x];
+          const double vr = er * N::re(v) + ei * N::im(v), vi = er * N::im(v) - ei * N::re(v);  // e^{-i phi} v
+          const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);  // e^{i phi} u
+          wi[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vi);
+          wj[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
+        }
+        T* vi_ = V + (int64_t)i * q;
+        T* vj_ = V + (int64_t)j * q;
+        for (int64_t x = lane; x < q; x += 64) {
+          const T u = vi_[x], v = vj_[x];
+          const double vr = er * N::re(v) + ei * N::im(v), vv = er * N::im(v) - ei * N::re(v);
+          const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);
+          vi_[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vv);
+          vj_[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
+        }
+        if (lane == 0) s_rotated = 1;
+      }
+      __syncthreads();
+    }
+    ++sweeps;
+    converged = (s_rotated == 0);
+    __syncthreads();
+  }
+
+  // ---- finalize: sigma_j = ||w_j|| staged in the block's S slice, ranks by counting (descending,
+  // ties by index), then vectors written by whole waves, then S in sorted order.
+  __shared__ int s_rank_of[YH_SVD_MAX_Q];
+  for (int64_t j = wave; j < q; j += kWaves) {
+    const T* wj = W + j * p;
+    double a = 0.0;
+    for (int64_t x = lane; x < p; x += 64) a += N::abs2(wj[x]);
+    a = wave_sum(a);
+    if (lane == 0) S[d.s_off + j] = sqrt(a);
+  }
+  __syncthreads();
+  constexpr int kMaxCols = YH_SVD_MAX_Q / kThreads;
+  int rk[kMaxCols];
+  double sg[kMaxCols];
+#pragma unroll
+  for (int z = 0; z < kMaxCols; ++z) {
+    const int64_t j = tid + (int64_t)z * kThreads;
+    rk[z] = -1;
+    sg[z] = 0.0;
+    if (j < q) {
+      const double sj = S[d.s_off + j];
+      int cnt = 0;
+      for (int64_t i = 0; i < q; ++i) {
+        const double si = S[d.s_off + i];
+        cnt += (si > sj) || (si == sj && i < j);
+      }
+      rk[z] = cnt;
+      sg[z] = sj;
+      s_rank_of[j] = cnt;
+    }
+  }
+  __syncthreads();
+  for (int64_t j = wave; j < q; j += kWaves) {
+    const int64_t rkj = s_rank_of[j];
+    const double sj = S[d.s_off + j];
+    const T* wj = W + j * p;
+    const T* vj = V + j * q;
+    if (MODE == 1) {
+      const int64_t pos = q - 1 - rkj;
+      for (int64_t x = lane; x < q; x += 64) U[d.u_off + x * q + pos] = vj[x];
+    } else if (!tr) {
+      // A V = W: U (m x k) column rkj = w_j / sigma_j, Vh (k x n) row rkj = v_j^H
+      const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+      for (int64_t x = lane; x < p; x += 64) {
+        const T w = wj[x];
+        U[d.u_off + x * q + rkj] = N::make(N::re(w) * inv, N::im(w) * inv);
+      }
+      for (int64_t x = lane; x < q; x += 64) Vh[d.v_off + rkj * q + x] = N::conj(vj[x]);
+    } else {
+      // A^H V = W, so A = V Sigma (W Sigma^-1)^H: U (m x m) column rkj = v_j, Vh (m x n) row rkj = w_j^H / sigma_j
+      const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+      for (int64_t x = lane; x < q; x += 64) U[d.u_off + x * q + rkj] = vj[x];
+      for (int64_t x = lane; x < p; x += 64) {
+        const T w = wj[x];
+        Vh[d.v_off + rkj * p + x] = N::make(N::re(w) * inv, -N::im(w) * inv);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int z = 0; z < kMaxCols; ++z) {
+    if (rk[z] < 0) continue;
+    if (MODE == 0)
+      S[d.s_off + rk[z]] = sg[z];
+    else
+      S[d.s_off + (q - 1 - rk[z])] = sg[z] - mu;
+  }
+  if (tid == 0) info[bidx] = converged ? sweeps : -1;
+}
+
+template <bool CPLX, int MODE>
+int launch_jacobi(const int64_t* d_blocks, const int32_t* d_order, int64_t n_lds, int64_t lds_bytes,
+                  int64_t nblocks, const void* A, void* U, void* S, void* Vh, void* work, int32_t* info,
+                  int max_sweeps, hipStream_t stream) {
+  typedef typename Num<CPLX>::T T;
+  if (n_lds > 0) {
+    hipLaunchKernelGGL((jacobi_kernel<CPLX, true, MODE>), dim3((unsigned)n_lds), dim3(kThreads), (size_t)lds_bytes,
+                       stream, d_blocks, d_order, (const T*)A, (T*)U, (double*)S, (T*)Vh, (T*)work, info,
+                       max_sweeps);
+    const int st = yh::hip_status(hipGetLastError(), "jacobi (LDS) launch");
+    if (st) return st;
+  }
+  if (nblocks > n_lds) {
+    hipLaunchKernelGGL((jacobi_kernel<CPLX, false, MODE>), dim3((unsigned)(nblocks - n_lds)), dim3(kThreads), 0,
+                       stream, d_blocks, d_order + n_lds, (const T*)A, (T*)U, (double*)S, (T*)Vh, (T*)work, info,
+                       max_sweeps);
+    const int st = yh::hip_status(hipGetLastError(), "jacobi (global) launch");
+    if (st) return st;
+  }
+  return YH_OK;
+}
+
+}  // namespace
+
+extern "C" int yastn_hip_check_svd_plan(const int64_t* h_blocks, int64_t nblocks, const int32_t* h_order,
+                                        int64_t n_lds, int64_t lds_bytes, int mode, int dtype, int64_t a_numel,
+                                        int64_t u_numel, int64_t s_numel, int64_t v_numel, int64_t work_numel) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(mode == 0 || mode == 1, "mode %d", mode);
+  YH_CHECK_ARG(nblocks >= 0 && n_lds >= 0 && n_lds <= nblocks, "block counts");
+  YH_CHECK_ARG(lds_bytes >= 0 && lds_bytes <= YH_SVD_LDS_BYTES, "lds bytes %lld", (long long)lds_bytes);
+  const int64_t isz = yh::itemsize_of(dtype);
+  char* seen = (char*)calloc(nblocks > 0 ? nblocks : 1, 1);
+  YH_CHECK_ARG(seen != nullptr, "host allocation");
+  int st = YH_OK;
+  for (int64_t o = 0; o < nblocks && st == YH_OK; ++o) {
+    const int32_t b = h_order[o];
+    if (b < 0 || b >= nblocks || seen[b]) {
+      yh::set_error("order[%lld] = %d invalid or repeated", (long long)o, b);
+      st = YH_ERR_ARG;
+      break;
+    }
+    seen[b] = 1;
+    const int64_t* w = h_blocks + (int64_t)b * YH_SVD_WORDS;
+    const int64_t m = w[1], n = w[2];
+    const bool tr = (mode == 0) && (m < n);
+    const int64_t p = tr ? n : m, q = tr ? m : n;
+    if (m < 1 || n < 1 || q > YH_SVD_MAX_Q || (mode == 1 && m != n)) {
+      yh::set_error("block %d: shape %lld x %lld", b, (long long)m, (long long)n);
+      st = YH_ERR_ARG;
+      break;
+    }
+    const bool ok_a = w[0] >= 0 && w[0] + m * n <= a_numel;
+    const bool ok_u = w[3] >= 0 && w[3] + (mode == 0 ? m * q : n * n) <= u_numel;
+    const bool ok_s = w[4] >= 0 && w[4] + q <= s_numel;
+    const bool ok_v = mode == 1 || (w[5] >= 0 && w[5] + q * n <= v_numel);
+    const int64_t need = (p * q + q * q);
+    const bool ok_w = o < n_lds ? need * isz <= lds_bytes : (w[6] >= 0 && w[6] + need <= work_numel);
+    if (!(ok_a && ok_u && ok_s && ok_v && ok_w)) {
+      yh::set_error("block %d: offsets out of bounds (a%d u%d s%d v%d work%d)", b, ok_a, ok_u, ok_s, ok_v, ok_w);
+      st = YH_ERR_ARG;
+    }
+  }
+  free(seen);
+  return st;
+}
+
+extern "C" int yastn_hip_svd(void* U, void* S, void* Vh, const void* A, int dtype, const int64_t* d_blocks,
+                             int64_t nblocks, const int32_t* d_order, int64_t n_lds, int64_t lds_bytes, void* work,
+                             int32_t* d_info, int max_sweeps, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(lds_bytes >= 0 && lds_bytes <= YH_SVD_LDS_BYTES, "lds bytes");
+  if (nblocks == 0) return YH_OK;
+  return dtype == YH_F64 ? launch_jacobi<false, 0>(d_blocks, d_order, n_lds, lds_bytes, nblocks, A, U, S, Vh, work,
+                                                   d_info, max_sweeps, stream)
+                         : launch_jacobi<true, 0>(d_blocks, d_order, n_lds, lds_bytes, nblocks, A, U, S, Vh, work,
+                                                  d_info, max_sweeps, stream);
+}
+
+extern "C" int yastn_hip_eigh(void* S, void* U, const void* A, int dtype, const int64_t* d_blocks, int64_t nblocks,
+                              const int32_t* d_order, int64_t n_lds, int64_t lds_bytes, void* work, int32_t* d_info,
+                              int max_sweeps, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(lds_bytes >= 0 && lds_bytes <= YH_SVD_LDS_BYTES, "lds bytes");
+  if (nblocks == 0) return YH_OK;
+  return dtype == YH_F64 ? launch_jacobi<false, 1>(d_blocks, d_order, n_lds, lds_bytes, nblocks, A, U, S, nullptr,
+                                                   work, d_info, max_sweeps, stream)
+                         : launch_jacobi<true, 1>(d_blocks, d_order, n_lds, lds_bytes, nblocks, A, U, S, nullptr,
+                                                  work, d_info, max_sweeps, stream);
+}

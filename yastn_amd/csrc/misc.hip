@@ -1,0 +1,251 @@
+// Small per-block kernels around the decompositions:
+//  - fix_svd_signs: phase of the largest (2^40-quantised) entry of each U column
+//  - the k x k core of the SVD adjoint (Lorentzian-broadened F, G terms)
+//  - the n x n core of the Hermitian eigen adjoint
+// The surrounding products (U^H gU, Vh gVh^H, U J Vh, ...) run through the grouped GEMM.
+#include "common.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+template <bool CPLX>
+struct Z;
+template <>
+struct Z<false> {
+  typedef double T;
+  static __device__ inline double re(double x) { return x; }
+  static __device__ inline double im(double) { return 0.0; }
+  static __device__ inline double mk(double r, double) { return r; }
+  static __device__ inline double absv(double x) { return fabs(x); }
+};
+template <>
+struct Z<true> {
+  typedef double2 T;
+  static __device__ inline double re(double2 x) { return x.x; }
+  static __device__ inline double im(double2 x) { return x.y; }
+  static __device__ inline double2 mk(double r, double i) { return make_double2(r, i); }
+  static __device__ inline double absv(double2 x) { return hypot(x.x, x.y); }
+};
+
+// fix_svd_signs: one wave per (block, column); blocks given as SVD plan words
+// (0 a_off, 1 m, 2 n, 3 u_off, 4 s_off, 5 v_off).  U block is m x k, Vh block k x n.
+// With piv != nullptr only the pivot positions are written: piv[s_off + j] = flat index of the
+// pivot of column j in U (the caller applies the phases with differentiable tensor ops).
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads) fix_signs_kernel(typename Z<CPLX>::T* __restrict__ U,
+                                                             typename Z<CPLX>::T* __restrict__ Vh,
+                                                             const int64_t* __restrict__ blocks,
+                                                             int64_t* __restrict__ piv) {
+  typedef Z<CPLX> N;
+  typedef typename N::T T;
+  const int64_t* w = blocks + (int64_t)blockIdx.x * YH_SVD_WORDS;
+  const int64_t m = w[1], n = w[2], u_off = w[3], s_off = w[4], v_off = w[5];
+  const int64_t k = m < n ? m : n;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t j = wave; j < k; j += kThreads / 64) {
+    // argmax of int64(|U[r, j]| * 2^40), ties to the smallest row
+    int64_t best = -1, brow = 0;
+    for (int64_t r = lane; r < m; r += 64) {
+      const int64_t qv = (int64_t)(N::absv(U[u_off + r * k + j]) * 1099511627776.0);
+      if (qv > best) {
+        best = qv;
+        brow = r;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const int64_t ob = __shfl_xor(best, o, 64);
+      const int64_t orow = __shfl_xor(brow, o, 64);
+      if (ob > best || (ob == best && orow < brow)) {
+        best = ob;
+        brow = orow;
+      }
+    }
+    if (piv) {
+      if (lane == 0) piv[s_off + j] = u_off + brow * k + j;
+      continue;
+    }
+    const T pv = U[u_off + brow * k + j];
+    const double a = N::absv(pv);
+    const double pr = N::re(pv) / a, pi = N::im(pv) / a;  // phase
+    for (int64_t r = lane; r < m; r += 64) {
+      const T u = U[u_off + r * k + j];
+      // u * conj(phase)
+      U[u_off + r * k + j] = N::mk(N::re(u) * pr + N::im(u) * pi, N::im(u) * pr - N::re(u) * pi);
+    }
+    for (int64_t c = lane; c < n; c += 64) {
+      const T v = Vh[v_off + j * n + c];
+      // v * phase
+      Vh[v_off + j * n + c] = N::mk(N::re(v) * pr - N::im(v) * pi, N::im(v) * pr + N::re(v) * pi);
+    }
+  }
+}
+
+__device__ inline double lor(double x, double eps) { return x / (x * x + eps); }
+
+// SVD adjoint core.  desc words: 0 m, 1 n, 2 k, 3 u_off, 4 s_off, 5 v_off, 6 kk_off, 7 t_off.
+// J = 1/2 (F+G) o (X - X^H) + 1/2 (F-G) o (Y - Y^H) + diag(gS) + i diag(Im X_jj sinv_j)   [complex]
+//     - [m>k] X diag(sinv) - [n>k] diag(sinv) Y^H
+// T = [m>k] gU diag(sinv) | [n>k] diag(sinv) gVh | 0
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    svd_adj_core_kernel(typename Z<CPLX>::T* __restrict__ J, typename Z<CPLX>::T* __restrict__ T_out,
+                        const typename Z<CPLX>::T* __restrict__ X, const typename Z<CPLX>::T* __restrict__ Y,
+                        const double* __restrict__ S, const double* __restrict__ gS,
+                        const typename Z<CPLX>::T* __restrict__ gU, const typename Z<CPLX>::T* __restrict__ gVh,
+                        const int64_t* __restrict__ descs, double eps, int has_gS, int has_gU, int has_gV) {
+  typedef Z<CPLX> N;
+  typedef typename N::T T;
+  const int64_t* w = descs + (int64_t)blockIdx.x * 8;
+  const int64_t m = w[0], n = w[1], k = w[2], u_off = w[3], s_off = w[4], v_off = w[5], kk = w[6], t_off = w[7];
+  const double* s = S + s_off;
+  for (int64_t e = threadIdx.x; e < k * k; e += kThreads) {
+    const int64_t i = e / k, j = e - i * k;
+    double jr = 0.0, ji = 0.0;
+    const double sinv_j = lor(s[j], eps), sinv_i = lor(s[i], eps);
+    if (i != j) {
+      const double f = lor(s[j] - s[i], eps), g = lor(s[j] + s[i], eps);
+      if (has_gU) {
+        const T xij = X[kk + i * k + j], xji = X[kk + j * k + i];
+        // X - X^H at (i, j) = x_ij - conj(x_ji)
+        const double dr = N::re(xij) - N::re(xji), di = N::im(xij) + N::im(xji);
+        jr += 0.5 * (f + g) * dr;
+        ji += 0.5 * (f + g) * di;
+      }
+      if (has_gV) {
+        const T yij = Y[kk + i * k + j], yji = Y[kk + j * k + i];
+        const double dr = N::re(yij) - N::re(yji), di = N::im(yij) + N::im(yji);
+        jr += 0.5 * (f - g) * dr;
+        ji += 0.5 * (f - g) * di;
+      }
+    } else {
+      if (has_gS) jr += gS[s_off + i];
+      if (CPLX && has_gU) ji += N::im(X[kk + i * k + i]) * sinv_i;
+    }
+    if (has_gU && m > k) {  // - X diag(sinv): (i, j) -> x_ij sinv_j
+      const T xij = X[kk + i * k + j];
+      jr -= N::re(xij) * sinv_j;
+      ji -= N::im(xij) * sinv_j;
+    }
+    if (has_gV && n > k) {  // - diag(sinv) Y^H: (i, j) -> sinv_i conj(y_ji)
+      const T yji = Y[kk + j * k + i];
+      jr -= sinv_i * N::re(yji);
+      ji += sinv_i * N::im(yji);
+    }
+    J[kk + i * k + j] = N::mk(jr, ji);
+  }
+  if (m >= n) {  // T is m x k
+    for (int64_t e = threadIdx.x; e < m * k; e += kThreads) {
+      const int64_t jcol = e % k;
+      T v = N::mk(0.0, 0.0);
+      if (has_gU && m > k) {
+        const T g = gU[u_off + e];
+        const double sv = lor(s[jcol], eps);
+        v = N::mk(N::re(g) * sv, N::im(g) * sv);
+      }
+      T_out[t_off + e] = v;
+    }
+  } else {  // T is k x n
+    for (int64_t e = threadIdx.x; e < k * n; e += kThreads) {
+      const int64_t irow = e / n;
+      T v = N::mk(0.0, 0.0);
+      if (has_gV) {
+        const T g = gVh[v_off + e];
+        const double sv = lor(s[irow], eps);
+        v = N::mk(N::re(g) * sv, N::im(g) * sv);
+      }
+      T_out[t_off + e] = v;
+    }
+  }
+}
+
+// eigh adjoint core.  desc words: 0 n, 1 s_off, 2 kk_off.
+// Xo = (X - X^H)/2 / E, E_ij = w_j - w_i (E_ii = 1); diag(Xo) = gS when given.
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    eigh_adj_core_kernel(typename Z<CPLX>::T* __restrict__ Xo, const typename Z<CPLX>::T* __restrict__ X,
+                         const double* __restrict__ S, const double* __restrict__ gS,
+                         const int64_t* __restrict__ descs, int has_gS, int has_gU) {
+  typedef Z<CPLX> N;
+  const int64_t* w = descs + (int64_t)blockIdx.x * 8;
+  const int64_t n = w[0], s_off = w[1], kk = w[2];
+  for (int64_t e = threadIdx.x; e < n * n; e += kThreads) {
+    const int64_t i = e / n, j = e - i * n;
+    double r = 0.0, im = 0.0;
+    if (has_gU) {
+      const typename N::T xij = X[kk + i * n + j], xji = X[kk + j * n + i];
+      r = 0.5 * (N::re(xij) - N::re(xji));
+      im = 0.5 * (N::im(xij) + N::im(xji));
+      const double E = (i == j) ? 1.0 : (S[s_off + j] - S[s_off + i]);
+      r /= E;
+      im /= E;
+    }
+    if (i == j && has_gS) {
+      r = gS[s_off + i];
+      im = 0.0;
+    }
+    Xo[kk + e] = N::mk(r, im);
+  }
+}
+
+}  // namespace
+
+extern "C" int yastn_hip_fix_svd_signs(void* U, void* Vh, int dtype, const int64_t* d_blocks, int64_t nblocks,
+                                       hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  if (nblocks == 0) return YH_OK;
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(fix_signs_kernel<false>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, (double*)U,
+                       (double*)Vh, d_blocks, (int64_t*)nullptr);
+  else
+    hipLaunchKernelGGL(fix_signs_kernel<true>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, (double2*)U,
+                       (double2*)Vh, d_blocks, (int64_t*)nullptr);
+  return yh::hip_status(hipGetLastError(), "fix_svd_signs launch");
+}
+
+extern "C" int yastn_hip_svd_sign_pivots(const void* U, int64_t* piv, int dtype, const int64_t* d_blocks,
+                                         int64_t nblocks, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(piv != nullptr, "null pivot buffer");
+  if (nblocks == 0) return YH_OK;
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(fix_signs_kernel<false>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream,
+                       (double*)const_cast<void*>(U), (double*)nullptr, d_blocks, piv);
+  else
+    hipLaunchKernelGGL(fix_signs_kernel<true>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream,
+                       (double2*)const_cast<void*>(U), (double2*)nullptr, d_blocks, piv);
+  return yh::hip_status(hipGetLastError(), "svd_sign_pivots launch");
+}
+
+extern "C" int yastn_hip_svd_adjoint_core(void* J, void* T, const void* X, const void* Y, const void* S,
+                                          const void* gS, const void* gU, const void* gVh, int dtype,
+                                          const int64_t* d_descs, int64_t nblocks, double eps, int has_gS,
+                                          int has_gU, int has_gV, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  if (nblocks == 0) return YH_OK;
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(svd_adj_core_kernel<false>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, (double*)J,
+                       (double*)T, (const double*)X, (const double*)Y, (const double*)S, (const double*)gS,
+                       (const double*)gU, (const double*)gVh, d_descs, eps, has_gS, has_gU, has_gV);
+  else
+    hipLaunchKernelGGL(svd_adj_core_kernel<true>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, (double2*)J,
+                       (double2*)T, (const double2*)X, (const double2*)Y, (const double*)S, (const double*)gS,
+                       (const double2*)gU, (const double2*)gVh, d_descs, eps, has_gS, has_gU, has_gV);
+  return yh::hip_status(hipGetLastError(), "svd_adjoint_core launch");
+}
+
+extern "C" int yastn_hip_eigh_adjoint_core(void* Xo, const void* X, const void* S, const void* gS, int dtype,
+                                           const int64_t* d_descs, int64_t nblocks, int has_gS, int has_gU,
+                                           hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  if (nblocks == 0) return YH_OK;
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(eigh_adj_core_kernel<false>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream,
+                       (double*)Xo, (const double*)X, (const double*)S, (const double*)gS, d_descs, has_gS, has_gU);
+  else
+    hipLaunchKernelGGL(eigh_adj_core_kernel<true>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream,
+                       (double2*)Xo, (const double2*)X, (const double*)S, (const double*)gS, d_descs, has_gS,
+                       has_gU);
+  return yh::hip_status(hipGetLastError(), "eigh_adjoint_core launch");
+}

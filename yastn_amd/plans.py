@@ -1,0 +1,550 @@
+"""Compile the reference planner's meta tuples into device work lists for the C-ABI kernels.
+
+The reference planner (yastn/tensor/_merging.py, _contractions.py, linalg.py) emits
+nested tuples of ints that are lru-cached, so the same object comes back every
+iteration of an algorithm.  Plans are therefore cached by the identity of the meta
+object (a strong reference is kept so that the id cannot be reused), plus the
+scalar arguments.  Each plan is validated once on the host by the library's
+``yastn_hip_check_*_plan`` before it is uploaded, so no kernel is ever launched
+with an out-of-bounds work list.
+"""
+import ctypes
+import math
+from itertools import product
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+# --------------------------------------------------------------------------- cache
+
+class _Cache:
+    def __init__(self, maxsize=4096):
+        self.d = {}
+        self.maxsize = maxsize
+
+    def get(self, key, refs, build):
+        hit = self.d.get(key)
+        if hit is not None and all(a is b for a, b in zip(hit[0], refs)):
+            return hit[1]
+        val = build()
+        if len(self.d) >= self.maxsize:
+            self.d.clear()
+        self.d[key] = (refs, val)
+        return val
+
+
+_CACHE = _Cache()
+
+
+def _device_key(device):
+    return (device.type, device.index)
+
+
+# --------------------------------------------------------------------------- box gather
+
+def _c_strides(shape):
+    st, acc = [], 1
+    for d in reversed(shape):
+        st.append(acc)
+        acc *= d
+    return tuple(reversed(st))
+
+
+def _coalesce(shape, ss, ds):
+    """Drop unit dims and merge neighbours that are contiguous in both source and destination."""
+    dims = [(n, s, d) for n, s, d in zip(shape, ss, ds) if n != 1]
+    if not dims:
+        return (1,), (0,), (1,)
+    out = [list(dims[-1])]
+    for n, s, d in reversed(dims[:-1]):
+        n0, s0, d0 = out[-1]
+        if s == s0 * n0 and d == d0 * n0:
+            out[-1] = [n0 * n, s0, d0]
+        else:
+            out.append([n, s, d])
+    out.reverse()
+    return tuple(x[0] for x in out), tuple(x[1] for x in out), tuple(x[2] for x in out)
+
+
+class GatherPlan:
+    """Device work list of the box-gather engine (see include/yastn_hip.h)."""
+
+    def __init__(self, jobs, src_numel, dst_numel, device):
+        # jobs: list of (src_off, dst_off, shape, src_strides, dst_strides) in destination C order
+        rows, tile_job = [], []
+        for (src_off, dst_off, shape, ss, ds) in jobs:
+            if any(n == 0 for n in shape):
+                continue
+            shape, ss, ds = _coalesce(shape, ss, ds)
+            if len(shape) > L.YH_MAXR:
+                raise ValueError(f"box of rank {len(shape)} after coalescing exceeds {L.YH_MAXR}")
+            if ds[-1] != 1:  # destination inner dim must be unit stride: append a unit dim
+                shape, ss, ds = shape + (1,), ss + (0,), ds + (1,)
+            r = len(shape)
+            kind, tdim = 0, 0
+            if src_off >= 0 and ss[-1] != 1:
+                cand = [k for k in range(r - 1) if ss[k] == 1]
+                if cand and shape[cand[0]] >= 8 and shape[-1] >= 8:
+                    kind, tdim = 1, cand[0]
+            numel = math.prod(shape)
+            if kind == 0:
+                ntile = -(-numel // L.YH_GATHER_TILE)
+            else:
+                te = L.YH_TRANSPOSE_TILE
+                ntile = (-(-shape[tdim] // te)) * (-(-shape[-1] // te)) * (numel // (shape[tdim] * shape[-1]))
+            w = np.zeros(L.YH_JOB_WORDS, dtype=np.int64)
+            w[0], w[1], w[2], w[3], w[4], w[5] = src_off, dst_off, r, kind, tdim, len(tile_job)
+            w[6:6 + L.YH_MAXR] = 1
+            w[6:6 + r] = shape
+            w[12:12 + r] = ss
+            w[18:18 + r] = ds
+            tile_job.extend([len(rows)] * ntile)
+            rows.append(w)
+        self.njobs = len(rows)
+        self.ntiles = len(tile_job)
+        self.h_jobs = np.ascontiguousarray(np.stack(rows) if rows else np.zeros((0, L.YH_JOB_WORDS), np.int64))
+        self.h_tile_job = np.ascontiguousarray(np.array(tile_job, dtype=np.int32))
+        self.src_numel, self.dst_numel = src_numel, dst_numel
+        # algorithmic element counts (read, written) for roofline accounting
+        self.elems_read = int(sum(math.prod(j[2]) for j in jobs if j[0] >= 0))
+        self.elems_written = int(sum(math.prod(j[2]) for j in jobs))
+        L.call('yastn_hip_check_gather_plan', self.h_jobs.ctypes.data, self.njobs,
+               self.h_tile_job.ctypes.data, self.ntiles, src_numel, dst_numel)
+        self.d_jobs = torch.from_numpy(self.h_jobs).to(device)
+        self.d_tile_job = torch.from_numpy(self.h_tile_job).to(device)
+
+    def args(self):
+        return (self.d_jobs.data_ptr(), self.njobs, self.d_tile_job.data_ptr(), self.ntiles)
+
+
+def _uncovered_boxes(Dn, boxes):
+    """Boxes of the C-ordered block of shape Dn not covered by any of ``boxes`` ((start, stop) per dim)."""
+    r = len(Dn)
+    cuts = []
+    for d in range(r):
+        pts = {0, Dn[d]}
+        for b in boxes:
+            pts.add(b[d][0])
+            pts.add(b[d][1])
+        cuts.append(sorted(pts))
+    covered = np.zeros(tuple(len(c) - 1 for c in cuts), dtype=bool)
+    for b in boxes:
+        idx = tuple(slice(cuts[d].index(b[d][0]), cuts[d].index(b[d][1])) for d in range(r))
+        covered[idx] = True
+    out = []
+    for cell in zip(*np.nonzero(~covered)):
+        out.append(tuple((cuts[d][c], cuts[d][c + 1]) for d, c in enumerate(cell)))
+    return out
+
+
+def _box_in_block(block_shape, box):
+    """(offset, strides, shape) of a sub-box of a C-ordered block."""
+    st = _c_strides(block_shape)
+    off = sum(b[0] * s for b, s in zip(box, st))
+    return off, st, tuple(b[1] - b[0] for b in box)
+
+
+def _merge_groups(Drsh, P):
+    """Group consecutive dims of the permuted source shape P so that group products give Drsh
+    (the reference planner fuses consecutive legs after the permutation, _merging.py:355-372)."""
+    groups, i = [], 0
+    for target in Drsh:
+        g, prod = [], 1
+        while prod < target:
+            if i >= len(P):
+                raise ValueError(f"cannot split {P} into {Drsh}")
+            g.append(i)
+            prod *= P[i]
+            i += 1
+        if prod != target:
+            raise ValueError(f"cannot split {P} into {Drsh}")
+        groups.append(g)
+    if any(P[j] != 1 for j in range(i, len(P))):
+        raise ValueError(f"cannot split {P} into {Drsh}")
+    return groups
+
+
+def _merge_maps(order, Do, Dslc, Drsh, Dn):
+    """For one meta_mrg row: destination offset (inside the block) and destination stride of each
+    permuted source dim P_j = Do[order[j]], using that Drsh groups consecutive permuted dims."""
+    P = tuple(Do[k] for k in order)
+    dst_st = _c_strides(Dn)
+    off = sum(sl[0] * s for sl, s in zip(Dslc, dst_st))
+    groups = _merge_groups(Drsh, P)
+    pst = [0] * len(P)
+    for g, gstride in zip(groups, dst_st):
+        acc = gstride
+        for j in reversed(g):
+            pst[j] = acc
+            acc *= P[j]
+    return off, P, tuple(pst)
+
+
+def _group_meta(meta_new, meta_mrg):
+    runs, i = [], 0
+    for tn, Dn, sln in meta_new:
+        j = i
+        while j < len(meta_mrg) and tuple(meta_mrg[j][0]) == tuple(tn):
+            j += 1
+        runs.append(((tn, tuple(Dn), tuple(sln)), meta_mrg[i:j]))
+        i = j
+    if i != len(meta_mrg):
+        raise ValueError("meta_mrg rows do not follow meta_new")
+    return runs
+
+
+def merge_plan(order, meta_new, meta_mrg, Dsize, src_numel, device):
+    """A1 transpose_and_merge: copy jobs for every meta_mrg row plus zero-fill of uncovered boxes."""
+    def build():
+        jobs = []
+        src_c = {}
+        for (tn, Dn, sln), rows in _group_meta(meta_new, meta_mrg):
+            boxes = []
+            for _, slo, Do, Dslc, Drsh in rows:
+                Do = tuple(Do)
+                off, P, pst = _merge_maps(order, Do, Dslc, Drsh, Dn)
+                sst = src_c.setdefault(Do, _c_strides(Do))
+                jobs.append((slo[0], sln[0] + off, P, tuple(sst[k] for k in order), pst))
+                boxes.append(tuple(tuple(x) for x in Dslc))
+            for box in _uncovered_boxes(Dn, boxes):
+                off, st, shp = _box_in_block(Dn, box)
+                jobs.append((-1, sln[0] + off, shp, (0,) * len(shp), st))
+        return GatherPlan(jobs, src_numel, Dsize, device)
+    return _CACHE.get(('merge', id(meta_mrg), id(meta_new), tuple(order), Dsize, src_numel, _device_key(device)),
+                      (meta_mrg, meta_new), build)
+
+
+def _fill_gaps(covered, total):
+    """1-D zero-fill jobs for [0, total) minus the union of ``covered`` (start, stop) ranges."""
+    jobs, pos = [], 0
+    for a, b in sorted(covered):
+        if a > pos:
+            jobs.append((-1, pos, (a - pos,), (0,), (1,)))
+        pos = max(pos, b)
+    if pos < total:
+        jobs.append((-1, pos, (total - pos,), (0,), (1,)))
+    return jobs
+
+
+def merge_adjoint_plan(order, meta_new, meta_mrg, Dsize, src_numel, device):
+    """A6: grad of the source = gather from the merged gradient under the inverse permutation."""
+    def build():
+        jobs, cov = [], []
+        inv = np.argsort(order)
+        for (tn, Dn, sln), rows in _group_meta(meta_new, meta_mrg):
+            for _, slo, Do, Dslc, Drsh in rows:
+                Do = tuple(Do)
+                off, P, pst = _merge_maps(order, Do, Dslc, Drsh, Dn)
+                # destination = the source block in its own C order; source stride of dim d = pst[j], order[j] = d
+                sst = tuple(pst[int(inv[d])] for d in range(len(Do)))
+                jobs.append((sln[0] + off, slo[0], Do, sst, _c_strides(Do)))
+                cov.append((slo[0], slo[1]))
+        jobs.extend(_fill_gaps(cov, src_numel))
+        return GatherPlan(jobs, Dsize, src_numel, device)
+    return _CACHE.get(('merge_adj', id(meta_mrg), id(meta_new), tuple(order), Dsize, src_numel,
+                       _device_key(device)), (meta_mrg, meta_new), build)
+
+
+def unmerge_plan(meta, numel, device):
+    """A2 unmerge: newdata[sln] = data[slo].view(Do)[sub_slc]."""
+    def build():
+        jobs = []
+        for sln, Dn, slo, Do, sub in meta:
+            off, st, shp = _box_in_block(tuple(Do), sub)
+            if shp != tuple(Dn):
+                raise ValueError(f"unmerge sub-block {shp} != {Dn}")
+            jobs.append((slo[0] + off, sln[0], shp, st, _c_strides(shp)))
+        return GatherPlan(jobs, numel, numel, device)
+    return _CACHE.get(('unmerge', id(meta), numel, _device_key(device)), (meta,), build)
+
+
+def unmerge_adjoint_plan(meta, numel, device):
+    """A8: grad_src[slo].view(Do)[sub_slc] = grad[sln].view(Dn)."""
+    def build():
+        jobs, cov = [], []
+        for sln, Dn, slo, Do, sub in meta:
+            off, st, shp = _box_in_block(tuple(Do), sub)
+            jobs.append((sln[0], slo[0] + off, shp, _c_strides(shp), st))
+            cov.append((slo[0], slo[1]))
+        # the forward covers the whole source (no zero blocks), gaps only if the meta is partial
+        jobs.extend(_fill_gaps(_union(cov), numel))
+        return GatherPlan(jobs, numel, numel, device)
+    return _CACHE.get(('unmerge_adj', id(meta), numel, _device_key(device)), (meta,), build)
+
+
+def _union(ranges):
+    out = []
+    for a, b in sorted(set(ranges)):
+        if out and a <= out[-1][1]:
+            out[-1] = (out[-1][0], max(out[-1][1], b))
+        else:
+            out.append((a, b))
+    return out
+
+
+def transpose_plan(axes, meta, numel, device):
+    """backend.transpose: newdata[sln].view(Dn) = data[slo].view(Do).permute(axes)."""
+    def build():
+        jobs = []
+        for sln, Dn, slo, Do in meta:
+            st = _c_strides(tuple(Do))
+            jobs.append((slo[0], sln[0], tuple(Dn), tuple(st[a] for a in axes), _c_strides(tuple(Dn))))
+        return GatherPlan(jobs, numel, numel, device)
+    return _CACHE.get(('transpose', id(meta), tuple(axes), numel, _device_key(device)), (meta,), build)
+
+
+def _runs(idx):
+    """Split a sorted-or-not index list into runs of consecutive values: (position, start, length)."""
+    out = []
+    idx = [int(x) for x in idx]
+    i = 0
+    while i < len(idx):
+        j = i
+        while j + 1 < len(idx) and idx[j + 1] == idx[j] + 1:
+            j += 1
+        out.append((i, idx[i], j - i + 1))
+        i = j + 1
+    return out
+
+
+def _axis_box(shape, axis, start, length):
+    return tuple((start, start + length) if d == axis else (0, n) for d, n in enumerate(shape))
+
+
+def mask_plan(mask_host, meta, Dsize, src_numel, axis, device, embed=False, mask_key=None):
+    """A12 apply_mask (embed=False) / embed_mask (embed=True) as box-gather jobs over index runs.
+
+    apply: out[sln].view(Dn) = A[sla].view(Da)[.., idx, ..]  (Dn has len(idx) on ``axis``)
+    embed: out[sln].view(Dn)[.., idx, ..] = A[sla].view(Da), zero elsewhere.
+    """
+    def build():
+        jobs = []
+        for sln, Dn, sla, Da, tm in meta:
+            Dn, Da = _as_shape(Dn), _as_shape(Da)
+            idx = mask_host[tuple(tm) if isinstance(tm, (list, tuple)) else tm]
+            runs = _runs(idx)
+            if not embed:
+                for pos, start, ln in runs:
+                    so, sst, shp = _box_in_block(Da, _axis_box(Da, axis, start, ln))
+                    do, dst, _ = _box_in_block(Dn, _axis_box(Dn, axis, pos, ln))
+                    jobs.append((sla[0] + so, sln[0] + do, shp, sst, dst))
+            else:
+                boxes = []
+                for pos, start, ln in runs:
+                    so, sst, shp = _box_in_block(Da, _axis_box(Da, axis, pos, ln))
+                    box = _axis_box(Dn, axis, start, ln)
+                    do, dst, _ = _box_in_block(Dn, box)
+                    jobs.append((sla[0] + so, sln[0] + do, shp, sst, dst))
+                    boxes.append(box)
+                for box in _uncovered_boxes(Dn, boxes):
+                    off, st, shp = _box_in_block(Dn, box)
+                    jobs.append((-1, sln[0] + off, shp, (0,) * len(shp), st))
+        if embed:
+            cov = [tuple(m[0]) for m in meta]
+            jobs.extend(_fill_gaps(cov, Dsize))
+        return GatherPlan(jobs, src_numel, Dsize, device)
+    return _CACHE.get(('mask', embed, id(meta), mask_key, axis, Dsize, src_numel, _device_key(device)),
+                      (meta,), build)
+
+
+def _as_shape(D):
+    return (D,) if isinstance(D, (int, np.integer)) else tuple(D)
+
+
+# --------------------------------------------------------------------------- grouped GEMM
+
+class GemmPlan:
+    """Device work list of the grouped GEMM (see include/yastn_hip.h)."""
+
+    def __init__(self, gemms, terms, in_numel, out_numel, device, cfg=None):
+        self.h_gemms = np.ascontiguousarray(np.array(gemms, dtype=np.int64).reshape(-1, L.YH_GEMM_WORDS))
+        self.h_terms = np.ascontiguousarray(np.array(terms, dtype=np.int64).reshape(-1, L.YH_TERM_WORDS))
+        self.ngemms, self.nterms = len(self.h_gemms), len(self.h_terms)
+        self.in_numel = np.array(list(in_numel) + [0] * (L.YH_MAX_BUFS - len(in_numel)), dtype=np.int64)
+        self.out_numel = np.array(list(out_numel) + [0] * (L.YH_MAX_BUFS - len(out_numel)), dtype=np.int64)
+        self.nin, self.nout = len(in_numel), len(out_numel)
+        # useful flops (real MACs x2) of the plan, independent of tiling
+        k_of = np.zeros(self.ngemms, dtype=np.int64)
+        for g, row in enumerate(self.h_gemms):
+            k_of[g] = int(self.h_terms[row[5]:row[5] + row[6], 6].sum()) if row[6] else 0
+        self.k_of = k_of
+        self.macs = int(np.sum(self.h_gemms[:, 0] * self.h_gemms[:, 1] * k_of)) if self.ngemms else 0
+        self.device = device
+        self._tiles = {}
+        self.cfg = choose_tile(self) if cfg is None else cfg
+        self.d_gemms = torch.from_numpy(self.h_gemms).to(device)
+        self.d_terms = torch.from_numpy(self.h_terms).to(device)
+        self.set_cfg(self.cfg)
+
+    def tiles_for(self, cfg):
+        if cfg not in self._tiles:
+            bm, bn = L.TILE_DIMS[cfg]
+            tl = []
+            for g, row in enumerate(self.h_gemms):
+                M, N = int(row[0]), int(row[1])
+                for m0 in range(0, M, bm):
+                    for n0 in range(0, N, bn):
+                        tl.append((int(self.k_of[g]), g, m0, n0))
+            tl.sort(key=lambda x: -x[0])  # longest K first: big tiles start early
+            h = np.ascontiguousarray(np.array([t[1:] for t in tl], dtype=np.int32).reshape(-1, 3))
+            L.call('yastn_hip_check_gemm_plan', self.h_gemms.ctypes.data, self.ngemms, self.h_terms.ctypes.data,
+                   self.nterms, h.ctypes.data, len(h), cfg, self.in_numel.ctypes.data, self.nin,
+                   self.out_numel.ctypes.data, self.nout)
+            self._tiles[cfg] = (h, torch.from_numpy(h).to(self.device))
+        return self._tiles[cfg]
+
+    def set_cfg(self, cfg):
+        self.cfg = cfg
+        self.h_tiles, self.d_tiles = self.tiles_for(cfg)
+        self.ntiles = len(self.h_tiles)
+
+    def args(self):
+        return (self.d_gemms.data_ptr(), self.ngemms, self.d_terms.data_ptr(), self.nterms,
+                self.d_tiles.data_ptr(), self.ntiles, self.cfg)
+
+
+NUM_CU = 256
+
+
+def choose_tile(plan):
+    """Tile choice from the plan's block-size histogram.
+
+    Estimated makespan = padded MACs / (relative tile efficiency x usable parallelism), where the
+    parallelism is the tile count capped by the CUs times the tiles resident per CU.  Larger tiles
+    reuse operands better (efficiency) but pad small sectors and expose fewer tiles.
+    """
+    if plan.ngemms == 0:
+        return L.YH_TILE_64x64
+    M = plan.h_gemms[:, 0].astype(np.float64)
+    N = plan.h_gemms[:, 1].astype(np.float64)
+    K = np.maximum(plan.k_of.astype(np.float64), 1.0)
+    eff = {L.YH_TILE_32x32: 0.45, L.YH_TILE_64x64: 0.8, L.YH_TILE_128x64: 1.0}
+    resident = {L.YH_TILE_32x32: 4, L.YH_TILE_64x64: 2, L.YH_TILE_128x64: 1}
+    best, best_cost = None, None
+    for cfg, (bm, bn) in L.TILE_DIMS.items():
+        tiles = np.ceil(M / bm) * np.ceil(N / bn)
+        padded = float(np.sum(tiles * bm * bn * K))
+        par = min(float(np.sum(tiles)), NUM_CU * resident[cfg])
+        cost = padded / eff[cfg] / par
+        if best_cost is None or cost < best_cost * 0.999:
+            best, best_cost = cfg, cost
+    return best
+
+
+def dot_plan(meta_dot, a_numel, b_numel, Dsize, device):
+    """A3 dot: one GEMM (one term) per output block."""
+    def build():
+        gemms, terms = [], []
+        for slc, Dc, sla, Da, slb, Db in meta_dot:
+            M, K = Da
+            K2, N = Db
+            if K != K2 or tuple(Dc) != (M, N):
+                raise ValueError(f"dot block shapes {Da} x {Db} -> {Dc}")
+            gemms.append((M, N, slc[0], N, 1, len(terms), 1, 0))
+            terms.append((sla[0], K, 1, slb[0], N, 1, K, (0 << 8) | (1 << 16)))
+        return GemmPlan(gemms, terms, (a_numel, b_numel), (Dsize,), device)
+    return _CACHE.get(('dot', id(meta_dot), a_numel, b_numel, Dsize, _device_key(device)), (meta_dot,), build)
+
+
+def dot_adjoint_plan(meta_dot, a_numel, b_numel, c_numel, device):
+    """A7: gA[sla] = sum gC[slc] B[slb]^H,  gB[slb] = sum A[sla]^H gC[slc]; inputs (A, B, gC), outputs (gA, gB).
+    Returns the plan and whether gA / gB have regions that no GEMM writes (then they start from zeros)."""
+    def build():
+        by_a, by_b = {}, {}
+        for slc, Dc, sla, Da, slb, Db in meta_dot:
+            by_a.setdefault((tuple(sla), tuple(Da)), []).append((slc, Dc, slb, Db))
+            by_b.setdefault((tuple(slb), tuple(Db)), []).append((slc, Dc, sla, Da))
+        gemms, terms = [], []
+        for (sla, Da), lst in by_a.items():
+            M, K = Da
+            gemms.append((M, K, sla[0], K, 1, len(terms), len(lst), 0 << 8))
+            for slc, Dc, slb, Db in lst:
+                N = Dc[1]
+                # op(A) = gC (M x N), op(B)[n][k] = conj(B[k][n])  (B stored K x N)
+                terms.append((slc[0], N, 1, slb[0], 1, N, N, 2 | (2 << 8) | (1 << 16)))
+        for (slb, Db), lst in by_b.items():
+            K, N = Db
+            gemms.append((K, N, slb[0], N, 1, len(terms), len(lst), 1 << 8))
+            for slc, Dc, sla, Da in lst:
+                M = Da[0]
+                # op(A)[k][m] = conj(A[m][k])  (A stored M x K), op(B) = gC (M x N)
+                terms.append((sla[0], 1, K, slc[0], N, 1, M, 1 | (0 << 8) | (2 << 16)))
+        plan = GemmPlan(gemms, terms, (a_numel, b_numel, c_numel), (a_numel, b_numel), device)
+        cov_a = sum(math.prod(Da) for (_, Da) in by_a)
+        cov_b = sum(math.prod(Db) for (_, Db) in by_b)
+        plan.zero_a = cov_a != a_numel
+        plan.zero_b = cov_b != b_numel
+        return plan
+    return _CACHE.get(('dot_adj', id(meta_dot), a_numel, b_numel, c_numel, _device_key(device)), (meta_dot,), build)
+
+
+def gemm_list_plan(gemms, terms, in_numel, out_numel, device, key=None, refs=()):
+    if key is None:
+        return GemmPlan(gemms, terms, in_numel, out_numel, device)
+    return _CACHE.get(key + (_device_key(device),), refs, lambda: GemmPlan(gemms, terms, in_numel, out_numel, device))
+
+
+# --------------------------------------------------------------------------- Jacobi SVD / eigh
+
+class SvdPlan:
+    """Block list of the batched Jacobi kernel: LDS-resident blocks first, then workspace blocks."""
+
+    def __init__(self, blocks, itemsize, mode, a_numel, u_numel, s_numel, v_numel, device):
+        # blocks: list of (a_off, m, n, u_off, s_off, v_off)
+        words, lds_ids, glb_ids = [], [], []
+        work = 0
+        lds_bytes = 0
+        for b, (a_off, m, n, u_off, s_off, v_off) in enumerate(blocks):
+            tr = mode == 0 and m < n
+            p, q = (n, m) if tr else (m, n)
+            need = p * q + q * q
+            if need * itemsize <= L.YH_SVD_LDS_BYTES:
+                lds_ids.append(b)
+                lds_bytes = max(lds_bytes, need * itemsize)
+                w_off = 0
+            else:
+                glb_ids.append(b)
+                w_off = work
+                work += need
+            words.append((a_off, m, n, u_off, s_off, v_off, w_off, 0))
+        size = lambda b: blocks[b][1] * blocks[b][2] * min(blocks[b][1], blocks[b][2])
+        lds_ids.sort(key=lambda b: -size(b))
+        glb_ids.sort(key=lambda b: -size(b))
+        self.h_blocks = np.ascontiguousarray(np.array(words, dtype=np.int64).reshape(-1, L.YH_SVD_WORDS))
+        self.h_order = np.ascontiguousarray(np.array(lds_ids + glb_ids, dtype=np.int32))
+        self.nblocks, self.n_lds, self.lds_bytes, self.work_numel = len(words), len(lds_ids), lds_bytes, work
+        self.flops = sum(4 * m * n * min(m, n) + 8 * min(m, n) ** 3 for (_, m, n, *_r) in blocks)
+        dt = L.YH_C128 if itemsize == 16 else L.YH_F64
+        L.call('yastn_hip_check_svd_plan', self.h_blocks.ctypes.data, self.nblocks, self.h_order.ctypes.data,
+               self.n_lds, self.lds_bytes, mode, dt, a_numel, u_numel, s_numel, v_numel, max(work, 0))
+        self.d_blocks = torch.from_numpy(self.h_blocks).to(device)
+        self.d_order = torch.from_numpy(self.h_order).to(device)
+
+
+def svd_plan(meta, itemsize, a_numel, sizes, device):
+    def build():
+        blocks = [(sl[0], D[0], D[1], slU[0], slS[0], slV[0]) for sl, D, slU, DU, slS, slV, DV in meta]
+        for sl, D, slU, DU, slS, slV, DV in meta:
+            k = min(D)
+            if tuple(DU) != (D[0], k) or tuple(DV) != (k, D[1]) or slS[1] - slS[0] != k:
+                raise ValueError(f"svd meta shapes {D} {DU} {DV} {slS}")
+        return SvdPlan(blocks, itemsize, 0, a_numel, sizes[0], sizes[1], sizes[2], device)
+    return _CACHE.get(('svd', id(meta), itemsize, a_numel, tuple(sizes), _device_key(device)), (meta,), build)
+
+
+def eigh_plan(meta, itemsize, a_numel, sizes, device):
+    def build():
+        blocks = []
+        for sl, D, slU, DU, slS in meta:
+            if D[0] != D[1] or tuple(DU) != tuple(D) or slS[1] - slS[0] != D[0]:
+                raise ValueError(f"eigh meta shapes {D} {DU} {slS}")
+            blocks.append((sl[0], D[0], D[1], slU[0], slS[0], 0))
+        return SvdPlan(blocks, itemsize, 1, a_numel, sizes[1], sizes[0], 1, device)
+    return _CACHE.get(('eigh', id(meta), itemsize, a_numel, tuple(sizes), _device_key(device)), (meta,), build)
+
+
+def as_ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
