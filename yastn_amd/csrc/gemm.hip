@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
     }
   }
 
-  // ---- epilogue: lane holds rows 4*(lane/16)+v, column lane%16 of each fragment
+  // ---- epilogue: v_mfma_f64_16x16x4f64 leaves D[lane/16 + 4 v][lane % 16] in item v of each lane
   T* __restrict__ C = (T*)bufs.out[(gflags >> 8) & 0xff];
   const bool accumulate = gflags & 1;
 #pragma unroll
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
       const int64_t col = n0 + wn * (BN / WN) + j * 16 + (lane & 15);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t row = m0 + wm * (BM / WM) + i * 16 + 4 * (lane >> 4) + v;
+        const int64_t row = m0 + wm * (BM / WM) + i * 16 + (lane >> 4) + 4 * v;
         if (row < M && col < N) {
           T* p = C + c_off + row * c_rs + col * c_cs;
           if constexpr (CPLX) {
