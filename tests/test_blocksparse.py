@@ -1,0 +1,193 @@
+"""The restated planner (yastn_amd.blocksparse) reproduces the reference planner's metas exactly.
+
+Each golden scenario is re-run through the restated tensor layer on a recording numpy backend; the
+sequence of hot-path backend calls (function and every meta argument) must equal the sequence the
+reference planner issued, and the resulting tensors must have the same block structure and data.
+"""
+import types
+
+import numpy as np
+import pytest
+
+import golden_io as G
+import np_test_backend as NB
+from yastn_amd import blocksparse as bs
+
+HOT = ('transpose_and_merge', 'unmerge', 'dot', 'svd', 'eigh', 'fix_svd_signs', 'apply_mask', 'transpose')
+
+
+def recording_backend():
+    rec = []
+    mod = types.ModuleType('rec_backend')
+    for k in dir(NB):
+        if not k.startswith('__'):
+            setattr(mod, k, getattr(NB, k))
+
+    def wrap(name):
+        f = getattr(NB, name)
+
+        def g(*args, **kw):
+            rec.append((name, args))
+            return f(*args, **kw)
+        return g
+    for name in HOT:
+        setattr(mod, name, wrap(name))
+    return mod, rec
+
+
+def _legs_of(t, cfg):
+    nsym = len(t['n'])
+    legs = []
+    for ax, s in enumerate(t['s']):
+        tD = {}
+        for tt, DD in zip(t['t'], t['D']):
+            tD[tuple(tt[ax * nsym:(ax + 1) * nsym])] = DD[ax]
+        ks = sorted(tD)
+        legs.append(bs.leg(cfg, s, ks, [tD[k] for k in ks]))
+    return legs
+
+
+def rebuild(cfg, t):
+    x = bs.from_flat(cfg, _legs_of(t, cfg), tuple(t['n']), np.array(t['data']))
+    assert x.struct.t == t['t'] and x.struct.D == t['D'], 'block order of the restated _fill_tensor'
+    assert [list(s.slcs[0]) for s in x.slices] == t['slices']
+    return x._replace(trans=tuple(t['trans']))
+
+
+def _norm_meta(x):
+    if isinstance(x, dict):
+        return {k: tuple(np.asarray(v).tolist()) for k, v in x.items()}
+    if isinstance(x, np.ndarray):
+        return None
+    if isinstance(x, (list, tuple)):
+        return tuple(_norm_meta(y) for y in x)
+    if isinstance(x, (np.integer,)):
+        return int(x)
+    return x
+
+
+def compare_calls(rec, scenario, skip=()):
+    gold = [c for c in G.calls(scenario) if c[0] not in skip]
+    mine = [c for c in rec if c[0] not in skip]
+    assert [c[0] for c in mine] == [c[0] for c in gold]
+    for (fn, a1), (fn2, a2, kw, outs) in zip(mine, gold):
+        m1 = tuple(_norm_meta(x) for x in a1)
+        m2 = tuple(_norm_meta(x) for x in a2)
+        assert m1 == m2, fn
+
+
+def same_tensor(x, t, atol=None):
+    xt = x.consume_transpose()
+    assert xt.struct.t == t['t'] and xt.struct.D == t['D'] and tuple(xt.struct.s) == tuple(t['s'])
+    assert [list(s.slcs[0]) for s in xt.slices] == t['slices']
+    if atol is None:
+        np.testing.assert_array_equal(np.asarray(xt.data), t['data'])
+    else:
+        np.testing.assert_allclose(np.asarray(xt.data), t['data'], atol=atol)
+
+
+@pytest.mark.parametrize('policy', ['fuse_contracted', 'fuse_to_matrix'])
+def test_mps_transfer_and_truncated_svd(policy):
+    scenario = f'mps_u1_{policy}'
+    be, rec = recording_backend()
+    cfg = bs.make_config(be, 'U1', tensordot_policy=policy)
+    ts = G.tensors(scenario)
+    A = rebuild(cfg, ts['A'])
+    T = bs.tensordot(A, A.conj(), axes=((0, 1), (0, 1)))
+    same_tensor(T, ts['T'], atol=1e-12)
+    AA = A @ A
+    same_tensor(AA, ts['AA'], atol=1e-12)
+    U, S, V = bs.svd_with_truncation(AA, axes=((0, 1), (2, 3)), D_total=64)
+    same_tensor(S, ts['S'], atol=1e-12)
+    U2, S2, V2 = bs.svd_with_truncation(AA, axes=((0, 1), (2, 3)), D_total=64, fix_signs=True)
+    same_tensor(U2, ts['U2'], atol=1e-10)
+    same_tensor(V2, ts['V2'], atol=1e-10)
+    compare_calls(rec, scenario)
+
+
+def test_u1xu1_complex():
+    be, rec = recording_backend()
+    cfg = bs.make_config(be, 'U1xU1')
+    ts = G.tensors('u1xu1_c128')
+    A = rebuild(cfg, ts['A'])
+    B = rebuild(cfg, ts['B'])
+    same_tensor(bs.tensordot(A, A.conj(), axes=((0, 1), (0, 1))), ts['T'], atol=1e-12)
+    TB = bs.tensordot(B, A, axes=((2,), (0,)))
+    same_tensor(TB, ts['TB'], atol=1e-12)
+    U, S, V = bs.svd_with_truncation(TB, axes=((0, 1), (2, 3)), D_total=40, fix_signs=True)
+    same_tensor(S, ts['S'], atol=1e-12)
+    same_tensor(U, ts['U'], atol=1e-10)
+    compare_calls(rec, 'u1xu1_c128')
+
+
+@pytest.mark.parametrize('policy', ['fuse_contracted', 'fuse_to_matrix'])
+def test_fusion_unfusion_transposed_operands(policy):
+    scenario = f'fuse_u1_{policy}'
+    be, rec = recording_backend()
+    cfg = bs.make_config(be, 'U1', tensordot_policy=policy)
+    ts = G.tensors(scenario)
+    a = rebuild(cfg, ts['a'])
+    f = a.fuse_legs(axes=(0, (1, 2), 3))
+    same_tensor(f, ts['f'])
+    g = f.fuse_legs(axes=((0, 2), 1))
+    same_tensor(g, ts['g'])
+    u = g.unfuse_legs(axes=0)
+    same_tensor(u, ts['u'])
+    uu = u.unfuse_legs(axes=2)
+    same_tensor(uu, ts['uu'])
+    b = rebuild(cfg, ts['b'])
+    r1 = bs.tensordot(a.transpose((3, 2, 1, 0)), b, axes=((1, 3), (0, 1)))
+    same_tensor(r1, ts['r1'], atol=1e-12)
+    r2 = bs.tensordot(b, a, axes=((0, 1), (2, 0)))
+    same_tensor(r2, ts['r2'], atol=1e-12)
+    r3 = bs.tensordot(f, f.conj(), axes=((0, 2), (0, 2)))
+    same_tensor(r3, ts['r3'], atol=1e-12)
+    ct = r2.transpose((2, 0, 1)).consume_transpose()
+    same_tensor(ct, ts['ct'], atol=1e-12)
+    compare_calls(rec, scenario)
+
+
+def test_truncation_known_answers():
+    """Kept shapes and charges of the reference's truncation known-answer tests
+    (reference tests/tensor/test_svd.py:189-266) and identical backend call metas."""
+    be, rec = recording_backend()
+    cfg = bs.make_config(be, 'U1')
+    js, _ = G.load('truncation_u1')
+    ts = G.tensors('truncation_u1')
+    a = rebuild(cfg, ts['a'])
+    for k, case in enumerate(js['cases'][:-1]):
+        opts = {kk: (float('inf') if v == 'inf' else v) for kk, v in case['opts'].items()}
+        U, S, V = bs.svd_with_truncation(a, axes=((0, 1), (2, 3)), sU=case['sU'], nU=case['nU'], **opts)
+        assert sum(d[0] for d in S.struct.D) == case['shape'][0]
+        assert [list(t) for t in S.struct.t] == [list(x) for x in case['charges']] or case['shape'][0] == 0
+        same_tensor(S, ts[f'S_{k}'], atol=1e-12)
+    opts = {'D_block': {(0,): 2, (-1,): 3}, 'tol_block': {(0,): 0.6, (-1,): 0.1}}
+    _, S, _ = bs.svd_with_truncation(a, axes=((0, 1), (2, 3)), sU=-1, **opts)
+    assert sum(d[0] for d in S.struct.D) == js['cases'][-1]['shape'][0]
+    compare_calls(rec, 'truncation_u1')
+
+
+def test_eigh_metas():
+    be, rec = recording_backend()
+    cfg = bs.make_config(be, 'U1')
+    ts = G.tensors('eigh_u1_c128')
+    mh = rebuild(cfg, ts['mh'])
+    S, U = bs.eigh(mh, axes=((0, 1), (2, 3)))
+    compare_calls(rec, 'eigh_u1_c128')
+    # reconstruction U S U^H == mh
+    SU = bs.tensordot(U, S, axes=(2, 0))
+    back = bs.tensordot(SU, U.conj(), axes=(2, 2))
+    np.testing.assert_allclose(back.to_dense(), mh.to_dense(), atol=1e-12)
+
+
+def test_fix_signs_scenario():
+    be, rec = recording_backend()
+    cfg = bs.make_config(be, 'U1')
+    ts = G.tensors('fix_signs_kat')
+    USV = bs.Tensor(cfg, bs.Struct(s=tuple(ts['USV']['s']), n=tuple(ts['USV']['n']), t=ts['USV']['t'],
+                                   D=ts['USV']['D'], size=ts['USV']['data'].size),
+                    tuple(bs.Slc((tuple(s),), D, int(np.prod(D))) for s, D in zip(ts['USV']['slices'], ts['USV']['D'])),
+                    np.array(ts['USV']['data']))
+    U, S, V = bs.svd(USV, axes=(0, 1), fix_signs=True)
+    same_tensor(U, ts['U'], atol=1e-12)
+    compare_calls(rec, 'fix_signs_kat')

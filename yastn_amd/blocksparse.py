@@ -1,0 +1,1125 @@
+"""Restated block-sparse planner and the tensor layer the workloads need (abelian U1 / U1xU1).
+
+The backend never sees charges; it receives flat data buffers and integer meta tuples.  On a
+machine without the reference package, this module produces those metas with the same block
+order, offsets and fused-leg decompositions as the reference planner, so that the backend can be
+driven through tensordot / fuse / unfuse / svd / truncation exactly as the reference drives it.
+Golden vectors recorded from the reference pin every meta (tests/test_blocksparse.py).
+
+Restated from (reference file: lines):
+  data model            yastn/tensor/_auxiliary.py:26-50
+  block creation order  yastn/tensor/_initialize.py:158-253
+  merge to matrix       yastn/tensor/_merging.py:76-187 (+ no-change shortcuts 84-134)
+  hard fusion           yastn/tensor/_merging.py:281-377, 488-549, 640-697
+  tensordot             yastn/tensor/_contractions.py:57-346, 453-528 (broadcast)
+  masks                 yastn/tensor/_merging.py:554-583, _contractions.py:529-574
+  svd / truncation      yastn/tensor/linalg.py:49-132, 135-380, 637-735
+  eigh                  yastn/tensor/linalg.py:831-1010
+Meta fusion (mfs), direct-sum fusion, leg-intersection masks and fermionic swap gates are not
+restated: the workloads here do not use them.
+"""
+from __future__ import annotations
+
+from functools import lru_cache
+from itertools import accumulate, groupby, product
+from numbers import Number
+from operator import itemgetter
+from typing import NamedTuple
+
+import numpy as np
+
+
+class BlockSparseError(Exception):
+    """Structural error raised before the backend is called (the reference raises YastnError)."""
+
+
+# ------------------------------------------------------------------ data model
+
+class Struct(NamedTuple):
+    s: tuple = ()
+    n: tuple = ()
+    diag: bool = False
+    t: tuple = ()
+    D: tuple = ()
+    size: int = 0
+
+
+class Slc(NamedTuple):
+    slcs: tuple = ()
+    D: tuple = ()
+    Dp: int = 0
+
+
+class Fusion(NamedTuple):
+    tree: tuple = (1,)
+    op: str = 'o'
+    s: tuple = (1,)
+    t: tuple = ()
+    D: tuple = ()
+
+    def conj(self):
+        return self._replace(s=tuple(-x for x in self.s))
+
+
+class _LegSlices(NamedTuple):
+    t: tuple = ()
+    D: tuple = ()
+    dec: tuple = ()
+
+
+class _Dec(NamedTuple):
+    t: tuple = ()
+    Dslc: tuple = (None, None)
+    Dprod: int = 0
+    Drsh: tuple = ()
+
+
+class SymU1:
+    SYM_ID = 'U1'
+    NSYM = 1
+
+    @classmethod
+    def fuse(cls, charges, signatures, new_signature):
+        return new_signature * np.einsum('kmn,m->kn', charges, np.asarray(signatures, dtype=np.int64))
+
+    @classmethod
+    def zero(cls):
+        return (0,) * cls.NSYM
+
+    @classmethod
+    def add_charges(cls, *charges, signatures=None, new_signature=1):
+        if not charges:
+            return cls.zero()
+        sg = (1,) * len(charges) if signatures is None else signatures
+        c = np.array(charges, dtype=np.int64).reshape((1, len(charges), cls.NSYM))
+        return tuple(cls.fuse(c, sg, new_signature).ravel().tolist())
+
+
+class SymU1xU1(SymU1):
+    SYM_ID = 'U1xU1'
+    NSYM = 2
+
+
+SYMS = {'U1': SymU1, 'U1xU1': SymU1xU1}
+
+
+class Config(NamedTuple):
+    backend: object = None
+    sym: object = SymU1
+    device: str = 'cpu'
+    dtype: str = 'float64'
+    tensordot_policy: str = 'fuse_contracted'
+
+
+def make_config(backend, sym='U1', device='cpu', dtype='float64', tensordot_policy='fuse_contracted'):
+    sym = SYMS[sym] if isinstance(sym, str) else sym
+    return Config(backend=backend, sym=sym, device=device, dtype=dtype, tensordot_policy=tensordot_policy)
+
+
+class Leg(NamedTuple):
+    s: int
+    t: tuple
+    D: tuple
+
+    def conj(self):
+        return Leg(-self.s, self.t, self.D)
+
+
+def leg(config, s, t, D):
+    nsym = config.sym.NSYM
+    t = tuple((x,) if isinstance(x, (int, np.integer)) else tuple(x) for x in t)
+    t = tuple(tuple(int(y) for y in x) for x in t)
+    if any(len(x) != nsym for x in t):
+        raise BlockSparseError('charge length does not match the symmetry')
+    return Leg(int(s), t, tuple(int(x) for x in D))
+
+
+def _slices_of(Dps, Ds):
+    return tuple(Slc(((stop - dp, stop),), ds, dp) for stop, dp, ds in zip(accumulate(Dps), Dps, Ds))
+
+
+# ------------------------------------------------------------------ tensor
+
+class Tensor:
+    """Block-sparse tensor: one flat data buffer of C-ordered blocks in ascending charge order."""
+
+    def __init__(self, config, struct, slices, data, hfs=None, trans=None):
+        self.config = config
+        self.struct = struct
+        self.slices = slices
+        self._data = data
+        self.hfs = tuple(hfs) if hfs is not None else tuple(Fusion(s=(x,)) for x in struct.s)
+        self.trans = tuple(trans) if trans is not None else tuple(range(len(struct.s)))
+
+    # -- bookkeeping
+    def _replace(self, **kw):
+        d = dict(config=self.config, struct=self.struct, slices=self.slices, data=self._data, hfs=self.hfs,
+                 trans=self.trans)
+        d.update(kw)
+        return Tensor(**d)
+
+    @property
+    def data(self):
+        return self._data
+
+    @property
+    def s(self):
+        return tuple(self.struct.s[i] for i in self.trans)
+
+    @property
+    def n(self):
+        return self.struct.n
+
+    @property
+    def isdiag(self):
+        return self.struct.diag
+
+    @property
+    def ndim(self):
+        return len(self.trans)
+
+    @property
+    def ndim_n(self):
+        return len(self.struct.s)
+
+    @property
+    def size(self):
+        return self.struct.size
+
+    @property
+    def yastn_dtype(self):
+        return self.config.backend.get_yastn_dtype(self._data)
+
+    def get_legs(self, axis):
+        """Leg (s, t, D) of a native axis, from the existing blocks."""
+        nsym = self.config.sym.NSYM
+        ax = self.trans[axis]
+        tD = {}
+        for t, D in zip(self.struct.t, self.struct.D):
+            tD[t[ax * nsym:(ax + 1) * nsym]] = D[ax]
+        ks = sorted(tD)
+        return Leg(self.struct.s[ax], tuple(ks), tuple(tD[k] for k in ks))
+
+    def __repr__(self):
+        return f"Tensor(s={self.s}, n={self.n}, blocks={len(self.struct.t)}, size={self.size})"
+
+    # -- algebra
+    def conj(self):
+        sym = self.config.sym
+        n = sym.add_charges(self.struct.n, new_signature=-1)
+        struct = self.struct._replace(s=tuple(-x for x in self.struct.s), n=n)
+        return self._replace(struct=struct, hfs=tuple(h.conj() for h in self.hfs),
+                             data=self.config.backend.conj(self._data))
+
+    def transpose(self, axes):
+        if sorted(axes) != list(range(self.ndim)):
+            raise BlockSparseError('axes do not match tensor ndim')
+        return self._replace(trans=tuple(self.trans[i] for i in axes))
+
+    def moveaxis(self, source, destination):
+        source = (source,) if isinstance(source, int) else tuple(source)
+        destination = (destination,) if isinstance(destination, int) else tuple(destination)
+        source = tuple(s % self.ndim for s in source)
+        destination = tuple(d % self.ndim for d in destination)
+        order = [n for n in range(self.ndim) if n not in source]
+        for d, s in sorted(zip(destination, source)):
+            order.insert(d, s)
+        return self.transpose(tuple(order))
+
+    def __matmul__(self, b):
+        return tensordot(self, b, axes=(self.ndim - 1, 0))
+
+    def __mul__(self, number):
+        return self._replace(data=self._data * number)
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, number):
+        return self._replace(data=self._data / number)
+
+    def __neg__(self):
+        return self._replace(data=-self._data)
+
+    def norm(self):
+        return self.config.backend.norm(self._data, 'fro')
+
+    def to_number(self):
+        """Value of a rank-0 tensor (or the first element)."""
+        if self.size == 0:
+            return self.config.backend.zeros((), dtype='float64', device=self.config.device)
+        return self._data.reshape(-1)[0]
+
+    def rsqrt(self, cutoff=0):
+        if not self.isdiag:
+            raise BlockSparseError('rsqrt is defined for diagonal tensors')
+        return self._replace(data=self.config.backend.rsqrt(self._data, cutoff=cutoff))
+
+    def sqrt(self):
+        return self._replace(data=self.config.backend.sqrt(self._data))
+
+    def consume_transpose(self):
+        return consume_transpose(self)
+
+    def fuse_legs(self, axes, mode='hard'):
+        return fuse_legs(self, axes)
+
+    def unfuse_legs(self, axes):
+        return unfuse_legs(self, axes)
+
+    def svd_with_truncation(self, axes=(0, 1), **kw):
+        return svd_with_truncation(self, axes=axes, **kw)
+
+    def svd(self, axes=(0, 1), **kw):
+        return svd(self, axes=axes, **kw)
+
+    def to_dense(self):
+        """Dense numpy array (legs in logical order, sectors in ascending charge)."""
+        a = consume_transpose(self)
+        nsym = a.config.sym.NSYM
+        data = a.config.backend.to_numpy(a._data)
+        legs = [a.get_legs(i) for i in range(a.ndim)]
+        offs = [dict(zip(l.t, accumulate((0,) + l.D))) for l in legs]
+        out = np.zeros(tuple(sum(l.D) for l in legs), dtype=data.dtype)
+        for t, D, sl in zip(a.struct.t, a.struct.D, a.slices):
+            if a.isdiag:
+                continue
+            idx = tuple(slice(offs[i][t[i * nsym:(i + 1) * nsym]], offs[i][t[i * nsym:(i + 1) * nsym]] + D[i])
+                        for i in range(len(D)))
+            out[idx] = data[sl.slcs[0][0]:sl.slcs[0][1]].reshape(D)
+        return out
+
+
+def _fill_struct(config, legs, n, isdiag=False):
+    """All blocks allowed by the selection rule, sorted by charges (reference _initialize.py:158-253)."""
+    sym, nsym = config.sym, config.sym.NSYM
+    s = tuple(l.s for l in legs)
+    n = sym.zero() if n is None else ((n,) if isinstance(n, (int, np.integer)) else tuple(n))
+    combos_t = list(product(*(l.t for l in legs)))
+    combos_D = list(product(*(l.D for l in legs)))
+    if combos_t:
+        ct = np.array(combos_t, dtype=np.int64).reshape(len(combos_t), len(legs), nsym)
+        ok = np.all(sym.fuse(ct, s, 1) == np.array(n, dtype=np.int64), axis=1)
+    else:
+        ok = np.zeros(0, dtype=bool)
+    meta = []
+    for keep, t, D in zip(ok, combos_t, combos_D):
+        if keep and all(d > 0 for d in D):
+            meta.append((tuple(x for tt in t for x in tt), tuple(D)))
+    meta.sort(key=itemgetter(0))
+    ts = tuple(m[0] for m in meta)
+    Ds = tuple(m[1] for m in meta)
+    Dps = tuple(int(np.prod(D)) for D in Ds)
+    if isdiag:
+        Dps = tuple(D[0] for D in Ds)
+    struct = Struct(s=s, n=tuple(n), diag=isdiag, t=ts, D=Ds, size=sum(Dps))
+    return struct, _slices_of(Dps, Ds)
+
+
+def zeros(config, legs, n=None, dtype=None):
+    struct, slices = _fill_struct(config, legs, n)
+    data = config.backend.zeros((struct.size,), dtype=dtype or config.dtype, device=config.device)
+    return Tensor(config, struct, slices, data)
+
+
+def from_flat(config, legs, n, flat):
+    """Tensor whose block buffer (in the reference's block order) is ``flat`` (numpy or backend array)."""
+    struct, slices = _fill_struct(config, legs, n)
+    if len(flat) != struct.size:
+        raise BlockSparseError(f"data has {len(flat)} elements, structure needs {struct.size}")
+    if isinstance(flat, np.ndarray):
+        dtype = 'complex128' if np.iscomplexobj(flat) else 'float64'
+        flat = config.backend.to_tensor(flat, dtype=dtype, device=config.device)
+    return Tensor(config, struct, slices, flat)
+
+
+def seeded_normal(n, dtype, seed):
+    """Block entries N(0,1) from np.random.default_rng(seed) over the flat buffer (complex: independent
+    real and imaginary parts)."""
+    rng = np.random.default_rng(seed)
+    if dtype == 'complex128':
+        re = rng.standard_normal(n)
+        return re + 1j * rng.standard_normal(n)
+    return rng.standard_normal(n)
+
+
+def rand(config, legs, n=None, dtype='float64', seed=0, scale=1.0):
+    struct, slices = _fill_struct(config, legs, n)
+    return from_flat(config, legs, n, seeded_normal(struct.size, dtype, seed) * scale)
+
+
+# ------------------------------------------------------------------ leg structures
+
+def _leg_structure_merge(teff, tlegs, Deff, Dlegs):
+    """Decomposition of a merged leg: effective charges sorted, sub-charges sorted within
+    (reference _merging.py:681-694)."""
+    rows = sorted(set(zip(teff, tlegs, Deff, Dlegs)))
+    t, D, dec = [], [], []
+    for te, grp in groupby(rows, key=itemgetter(0)):
+        lo, recs = 0, []
+        for _, tl, De, Dl in grp:
+            recs.append(_Dec(tl, (lo, lo + De), De, Dl))
+            lo += De
+        t.append(te)
+        D.append(lo)
+        dec.append(tuple(recs))
+    return _LegSlices(tuple(t), tuple(D), tuple(dec))
+
+
+@lru_cache(maxsize=1024)
+def _leg_structure_combine_charges_prod(sym, t_in, D_in, s_in, t_out, s_out):
+    ct = list(product(*t_in))
+    cD = list(product(*D_in))
+    ct = np.array(ct, dtype=np.int64).reshape((len(ct), len(s_in), sym.NSYM))
+    cD = np.array(cD, dtype=np.int64).reshape((len(cD), len(s_in)))
+    teff = sym.fuse(ct, s_in, s_out)
+    keep = np.array([i for i, te in enumerate(teff.tolist()) if tuple(te) in t_out], dtype=np.int64)
+    cD, ct, teff = cD[keep], ct[keep], teff[keep]
+    Deff = tuple(np.prod(cD, axis=1, dtype=np.int64).tolist())
+    Dlegs = tuple(map(tuple, cD.tolist()))
+    teff = tuple(map(tuple, teff.tolist()))
+    tlegs = tuple(map(tuple, ct.reshape(len(ct), len(s_in) * sym.NSYM).tolist()))
+    return _leg_structure_merge(teff, tlegs, Deff, Dlegs)
+
+
+def _get_tD_legs(struct):
+    lt, nd, nsym = len(struct.t), len(struct.s), len(struct.n)
+    ts = np.array(struct.t, dtype=np.int64).reshape(lt, nd, nsym)
+    Ds = np.array(struct.D, dtype=np.int64).reshape(lt, nd)
+    legs = [sorted(set((tuple(t), D) for t, D in zip(ts[:, n, :].tolist(), Ds[:, n].tolist()))) for n in range(nd)]
+    dicts = [dict(x) for x in legs]
+    if any(len(a) != len(b) for a, b in zip(legs, dicts)):
+        raise BlockSparseError('Bond dimensions related to some charge are not consistent.')
+    return [tuple(d.keys()) for d in dicts], [tuple(d.values()) for d in dicts], dicts
+
+
+def _leg_struct_trivial(struct, axis=0):
+    nsym = len(struct.n)
+    tD = sorted((tt[nsym * axis: nsym * (axis + 1)], DD[axis]) for tt, DD in zip(struct.t, struct.D))
+    t = tuple(x[0] for x in tD)
+    D = tuple(x[1] for x in tD)
+    return _LegSlices(t, D, tuple((_Dec(tt, (0, DD), DD, (DD,)),) for tt, DD in zip(t, D)))
+
+
+# ------------------------------------------------------------------ merge to matrix / fuse / unmerge
+
+@lru_cache(maxsize=1024)
+def _meta_merge_to_matrix(config, struct, slices, axes, inds):
+    """reference _merging.py:137-187."""
+    sym, nsym = config.sym, config.sym.NSYM
+    s_eff = (struct.s[axes[0][0]] if axes[0] else 1, struct.s[axes[1][0]] if axes[1] else -1)
+    t_old = struct.t if inds is None else [struct.t[i] for i in inds]
+    D_old = struct.D if inds is None else [struct.D[i] for i in inds]
+    sl_old = slices if inds is None else [slices[i] for i in inds]
+    tset = np.array(t_old, dtype=np.int64).reshape((len(t_old), len(struct.s), nsym))
+    Dset = np.array(D_old, dtype=np.int64).reshape((len(D_old), len(struct.s)))
+    t, teff, ls = [], [], []
+    for side in (0, 1):
+        ta = tset[:, axes[side], :]
+        Da = Dset[:, axes[side]]
+        Deff = np.prod(Da, axis=1, dtype=np.int64).tolist()
+        Da = [tuple(x) for x in Da.tolist()]
+        s = tuple(struct.s[i] for i in axes[side])
+        te = [tuple(x) for x in sym.fuse(ta, s, s_eff[side]).tolist()]
+        ta = [tuple(x) for x in ta.reshape(len(ta), len(s) * nsym).tolist()]
+        teff.append(te)
+        t.append(ta)
+        ls.append(_leg_structure_merge(te, ta, Deff, Da))
+    rows = sorted((tl, tr, xl, xr, slo.slcs[0], Do) for tl, tr, xl, xr, slo, Do
+                  in zip(teff[0], teff[1], t[0], t[1], sl_old, D_old))
+    meta_mrg, t_new, D_new, sl_new, lo = [], [], [], [], 0
+    for (tel, ter), grp in groupby(rows, key=itemgetter(0, 1)):
+        i0, i1 = ls[0].t.index(tel), ls[1].t.index(ter)
+        tn = tel + ter
+        D0, D1 = ls[0].D[i0], ls[1].D[i1]
+        t_new.append(tn)
+        D_new.append((D0, D1))
+        sl_new.append(Slc(((lo, lo + D0 * D1),), (D0, D1), D0 * D1))
+        lo += D0 * D1
+        grp = list(grp)
+        k = 0
+        for d0, d1 in product(ls[0].dec[i0], ls[1].dec[i1]):
+            if k < len(grp) and d0.t == grp[k][2] and d1.t == grp[k][3]:
+                meta_mrg.append((tn, grp[k][4], grp[k][5], (d0.Dslc, d1.Dslc), (d0.Dprod, d1.Dprod)))
+                k += 1
+    struct_new = struct._replace(t=tuple(t_new), D=tuple(D_new), s=s_eff, size=lo)
+    return struct_new, tuple(sl_new), tuple(meta_mrg), ls[0], ls[1]
+
+
+def _no_change_in_transpose_and_merge(meta_mrg, meta_new, Dsize):
+    """reference _merging.py:92-111 (C-order layouts that coincide need no copy)."""
+    if Dsize in (0, 1):
+        return True
+    lo = 0
+    for _, slo, _, _, _ in meta_mrg:
+        if slo[0] != lo:
+            return False
+        lo = slo[1]
+    if lo != Dsize:
+        return False
+    for (_, Dn, _), (_, grp) in zip(meta_new, groupby(meta_mrg, key=itemgetter(0))):
+        lo = 0
+        for _, _, _, Dslc, _ in grp:
+            if Dslc[0][0] != lo:
+                return False
+            lo = Dslc[0][1]
+        if lo != Dn[0]:
+            return False
+    return True
+
+
+def _transpose_and_merge(config, data, order, struct, slices, meta_mrg, inds=None):
+    meta_new = tuple((x, y, z.slcs[0]) for x, y, z in zip(struct.t, struct.D, slices))
+    if inds is None and tuple(range(len(order))) == tuple(order) and struct.size == config.backend.get_size(data) \
+            and _no_change_in_transpose_and_merge(meta_mrg, meta_new, struct.size):
+        return data
+    return config.backend.transpose_and_merge(data, order, meta_new, meta_mrg, struct.size)
+
+
+def _merge_to_matrix(a, axes, inds=None):
+    order = axes[0] + axes[1]
+    struct, slices, meta_mrg, ls_l, ls_r = _meta_merge_to_matrix(a.config, a.struct, a.slices, axes, inds)
+    data = _transpose_and_merge(a.config, a._data, order, struct, slices, meta_mrg, inds)
+    return data, struct, slices, ls_l, ls_r
+
+
+def _no_change_in_unmerge(meta):
+    """reference _merging.py:122-134."""
+    low, Dn_last, sl_last, sln = 0, 0, (0, 0), (0, 0)
+    for sln, Dn, slo, _, sub in meta:
+        if slo != sl_last:
+            if slo[0] != sl_last[1] or low != Dn_last:
+                return False
+            sl_last, Dn_last, low = slo, Dn[0], 0
+        if low != sub[0][0]:
+            return False
+        low = sub[0][1]
+    return sl_last[1] == sln[1]
+
+
+def _unmerge(config, data, meta):
+    Dsize = meta[-1][0][1] if meta else 0
+    if config.backend.get_size(data) != Dsize:
+        raise BlockSparseError('unmerge size mismatch')
+    if _no_change_in_unmerge(meta):
+        return data
+    return config.backend.unmerge(data, meta)
+
+
+@lru_cache(maxsize=1024)
+def _meta_unmerge_matrix(config, struct, slices, ls0, ls1, snew):
+    """reference _merging.py:528-549."""
+    nsym = config.sym.NSYM
+    rows = []
+    for to, slo, Do in zip(struct.t, slices, struct.D):
+        i0, i1 = ls0.t.index(to[:nsym]), ls1.t.index(to[nsym:])
+        for d0, d1 in product(ls0.dec[i0], ls1.dec[i1]):
+            rows.append((d0.t + d1.t, d0.Drsh + d1.Drsh, d0.Dprod * d1.Dprod, (d0.Dprod, d1.Dprod), slo.slcs[0], Do,
+                         (d0.Dslc, d1.Dslc)))
+    rows.sort(key=itemgetter(0))
+    tnew = tuple(r[0] for r in rows)
+    Dnew = tuple(r[1] for r in rows)
+    Dp = tuple(r[2] for r in rows)
+    struct_new = struct._replace(s=tuple(snew), t=tnew, D=Dnew, size=sum(Dp))
+    sl_new = _slices_of(Dp, Dnew)
+    meta = tuple((x.slcs[0], *r[3:]) for x, r in zip(sl_new, rows))
+    return meta, struct_new, sl_new
+
+
+@lru_cache(maxsize=1024)
+def _meta_fuse_hard(config, struct, slices, axes, inds):
+    """reference _merging.py:304-377."""
+    sym, nsym = config.sym, config.sym.NSYM
+    if inds is None:
+        t_old, D_old, sl_old = struct.t, struct.D, slices
+    else:
+        t_old = tuple(struct.t[i] for i in inds)
+        D_old = tuple(struct.D[i] for i in inds)
+        sl_old = [slices[i] for i in inds]
+        struct = struct._replace(t=t_old, D=D_old)
+    lt, nd = len(t_old), len(struct.s)
+    t_in, D_in, tD_dict = _get_tD_legs(struct)
+    slegs = tuple(tuple(struct.s[n] for n in ax) for ax in axes)
+    s_eff = [struct.s[ax[0]] if ax else -1 for ax in axes]
+    if axes and not axes[0]:
+        s_eff[0] = 1
+    s_eff = tuple(s_eff)
+    tset = np.array(t_old, dtype=np.int64).reshape(lt, nd, nsym)
+    teff = np.zeros((lt, len(s_eff), nsym), dtype=np.int64)
+    for n, ax in enumerate(axes):
+        teff[:, n, :] = sym.fuse(tset[:, ax, :], slegs[n], s_eff[n]) if ax else sym.zero()
+    lls = []
+    for n, ax in enumerate(axes):
+        if len(ax) > 1:
+            tset_n = tuple(set(map(tuple, teff[:, n, :].tolist())))
+            lls.append(_leg_structure_combine_charges_prod(sym, tuple(t_in[i] for i in ax), tuple(D_in[i] for i in ax),
+                                                           slegs[n], tset_n, s_eff[n]))
+        elif len(ax) == 1:
+            t = tuple(tD_dict[ax[0]].keys())
+            D = tuple(tD_dict[ax[0]].values())
+            lls.append(_LegSlices(t, D, tuple((_Dec(tt, (0, DD), DD, (DD,)),) for tt, DD in zip(t, D))))
+        else:
+            lls.append(_LegSlices((sym.zero(),), (1,), ((_Dec((), (0, 1), 1, (1,)),),)))
+    teff_split = [tuple(map(tuple, x)) for x in teff.tolist()]
+    if axes:
+        told = list(zip(*[tset[:, ax, :].reshape(lt, len(ax) * nsym).tolist() for ax in axes]))
+        told = [tuple(map(tuple, x)) for x in told]
+    else:
+        told = list(t_old)
+    teff_flat = list(map(tuple, teff.reshape(lt, len(axes) * nsym).tolist()))
+    rows = sorted((tes, tn, tos, slo.slcs[0], Do) for tes, tn, tos, slo, Do
+                  in zip(teff_split, teff_flat, told, sl_old, D_old))
+    meta_mrg, t_new, D_new = [], [], []
+    for (tes, tn), grp in groupby(rows, key=itemgetter(0, 1)):
+        idx = tuple(ls.t.index(te) for ls, te in zip(lls, tes))
+        decs = tuple(ls.dec[i] for ls, i in zip(lls, idx))
+        t_new.append(tn)
+        D_new.append(tuple(ls.D[i] for ls, i in zip(lls, idx)))
+        grp = list(grp)
+        k = 0
+        for de in product(*decs):
+            if k < len(grp) and tuple(d.t for d in de) == grp[k][2]:
+                meta_mrg.append((tn, grp[k][3], grp[k][4], tuple(d.Dslc for d in de), tuple(d.Dprod for d in de)))
+                k += 1
+    Dp = np.prod(D_new, axis=1, dtype=np.int64).tolist() if D_new else []
+    struct_new = struct._replace(t=tuple(t_new), D=tuple(D_new), s=s_eff, size=sum(Dp))
+    return struct_new, _slices_of(Dp, D_new), meta_mrg, t_in, D_in
+
+
+def _combine_hfs_prod(hfs, t_in, D_in, s_out, axes):
+    """reference _merging.py:697-712."""
+    tfl, Dfl, sfl, op = [], [], [s_out], 'p'
+    tree = [sum(hfs[n].tree[0] for n in axes)]
+    for n in axes:
+        tfl.append(t_in[n])
+        tfl.extend(hfs[n].t)
+        Dfl.append(D_in[n])
+        Dfl.extend(hfs[n].D)
+        sfl.extend(hfs[n].s)
+        tree.extend(hfs[n].tree)
+        op += hfs[n].op
+    return Fusion(tree=tuple(tree), op=op, s=tuple(sfl), t=tuple(tfl), D=tuple(Dfl))
+
+
+def _unfuse_fusion(hf):
+    """One layer of an unfuse (reference _merging.py:981-997)."""
+    tt, DD, ss, hfs = [], [], [], []
+    first, cum = 1, 0
+    for n in range(1, len(hf.tree)):
+        if cum == 0:
+            cum = hf.tree[n]
+        if hf.tree[n] == 1:
+            cum -= 1
+            if cum == 0:
+                tt.append(hf.t[first - 1])
+                DD.append(hf.D[first - 1])
+                ss.append(hf.s[first])
+                hfs.append(Fusion(tree=hf.tree[first:n + 1], op=hf.op[first:n + 1], s=hf.s[first:n + 1],
+                                  t=hf.t[first:n], D=hf.D[first:n]))
+                first = n + 1
+    return tuple(tt), tuple(DD), tuple(ss), hfs
+
+
+def fuse_legs(a, axes):
+    """Hard fusion of groups of legs; e.g. axes=(0, (1, 2), 3) (reference _merging.py:202-301)."""
+    if a.isdiag:
+        raise BlockSparseError('Cannot fuse legs of a diagonal tensor.')
+    axes = tuple((x,) if isinstance(x, int) else tuple(x) for x in axes)
+    order = tuple(x for g in axes for x in g)
+    if sorted(order) != list(range(a.ndim)):
+        raise BlockSparseError('Provided axes do not match tensor ndim.')
+    order = tuple(a.trans[ax] for ax in order)
+    axes = tuple(tuple(a.trans[ax] for ax in g) for g in axes)
+    struct, slices, meta_mrg, t_in, D_in = _meta_fuse_hard(a.config, a.struct, a.slices, axes, None)
+    data = _transpose_and_merge(a.config, a._data, order, struct, slices, meta_mrg)
+    hfs = []
+    for n, g in enumerate(axes):
+        if len(g) > 1:
+            hfs.append(_combine_hfs_prod(a.hfs, t_in, D_in, struct.s[n], g))
+        else:
+            hfs.append(a.hfs[g[0]])
+    return Tensor(a.config, struct, slices, data, hfs=hfs, trans=None)
+
+
+@lru_cache(maxsize=1024)
+def _meta_unfuse_hard(config, struct, slices, axes, hfs):
+    """reference _merging.py:488-525."""
+    sym, nsym = config.sym, config.sym.NSYM
+    t_in, _, tD_dict = _get_tD_legs(struct)
+    lls, hfs_new, snew, nlegs = [], [], [], []
+    for n, hf in enumerate(hfs):
+        if n in axes:
+            t_part, D_part, s_part, hfs_part = _unfuse_fusion(hf)
+            lls.append(_leg_structure_combine_charges_prod(sym, t_part, D_part, s_part, t_in[n], struct.s[n]))
+            hfs_new.extend(hfs_part)
+            nlegs.append(len(hfs_part))
+            snew.extend(s_part)
+        else:
+            t, D = tuple(tD_dict[n].keys()), tuple(tD_dict[n].values())
+            lls.append(_LegSlices(t, D, tuple((_Dec(tt, (0, DD), DD, (DD,)),) for tt, DD in zip(t, D))))
+            hfs_new.append(hf)
+            snew.append(struct.s[n])
+    rows = []
+    for to, slo, Do in zip(struct.t, slices, struct.D):
+        idx = tuple(ls.t.index(to[n * nsym:(n + 1) * nsym]) for n, ls in enumerate(lls))
+        for tt in product(*(ls.dec[i] for ls, i in zip(lls, idx))):
+            rows.append((sum((x.t for x in tt), ()), sum((x.Drsh for x in tt), ()), tuple(x.Dprod for x in tt),
+                         slo.slcs[0], Do, tuple(x.Dslc for x in tt)))
+    rows.sort(key=itemgetter(0))
+    tnew = tuple(r[0] for r in rows)
+    Dnew = tuple(r[1] for r in rows)
+    Dp = np.prod(np.array(Dnew, dtype=np.int64).reshape(len(Dnew), len(snew)), axis=1, dtype=np.int64).tolist()
+    struct_new = struct._replace(s=tuple(snew), t=tnew, D=Dnew, size=sum(Dp))
+    sl_new = _slices_of(Dp, Dnew)
+    meta = tuple((x.slcs[0], *r[2:]) for x, r in zip(sl_new, rows))
+    return meta, struct_new, sl_new, tuple(nlegs), tuple(hfs_new)
+
+
+def unfuse_legs(a, axes):
+    """Revert one layer of hard fusion of the given (logical) legs (reference _merging.py:380-485)."""
+    axes = (axes,) if isinstance(axes, int) else tuple(axes)
+    if a.isdiag:
+        raise BlockSparseError('Cannot unfuse legs of a diagonal tensor.')
+    haxes, uaxes = [], []
+    for mi in range(a.ndim):
+        hi = a.trans[mi]
+        if mi in axes and a.hfs[hi].tree[0] > 1:
+            if a.hfs[hi].op[0] != 'p':
+                raise BlockSparseError('only product fusions can be unfused here')
+            haxes.append(hi)
+            uaxes.append(mi)
+    if not haxes:
+        return a
+    meta, struct, slices, nlegs, hfs = _meta_unfuse_hard(a.config, a.struct, a.slices, tuple(haxes), tuple(a.hfs))
+    data = _unmerge(a.config, a._data, meta)
+    trans = list(a.trans)
+    for unfused, n in zip(nlegs[::-1], uaxes[::-1]):
+        trans = trans[:n] + [trans[n]] * unfused + trans[n + 1:]
+    for ax in range(max(trans), -1, -1):
+        newax = sum(i < ax for i in trans)
+        ii = trans.index(ax)
+        while ii < len(trans) and trans[ii] == ax:
+            trans[ii] = newax
+            newax += 1
+            ii += 1
+    return Tensor(a.config, struct, slices, data, hfs=hfs, trans=trans)
+
+
+def consume_transpose(a):
+    """Apply a pending logical transpose to the data (reference _single.py:316-346)."""
+    if a.trans == tuple(range(a.ndim_n)):
+        return a
+    order = np.array(a.trans, dtype=np.int64)
+    lt, nd, nsym = len(a.struct.t), len(a.struct.s), len(a.struct.n)
+    ts = np.array(a.struct.t, dtype=np.int64).reshape(lt, nd, nsym)
+    Ds = np.array(a.struct.D, dtype=np.int64).reshape(lt, nd)
+    newt = tuple(map(tuple, ts[:, order, :].reshape(lt, nd * nsym).tolist()))
+    newD = tuple(map(tuple, Ds[:, order].tolist()))
+    rows = sorted(zip(newt, newD, a.slices), key=itemgetter(0))
+    Dp = tuple(r[2].Dp for r in rows)
+    slices = _slices_of(Dp, tuple(r[1] for r in rows))
+    struct = a.struct._replace(s=tuple(a.struct.s[i] for i in a.trans), t=tuple(r[0] for r in rows),
+                               D=tuple(r[1] for r in rows))
+    meta = tuple((sn.slcs[0], sn.D, r[2].slcs[0], r[2].D) for sn, r in zip(slices, rows))
+    data = a._data if a.isdiag else a.config.backend.transpose(a._data, a.trans, meta)
+    return Tensor(a.config, struct, slices, data, hfs=tuple(a.hfs[i] for i in a.trans), trans=None)
+
+
+# ------------------------------------------------------------------ tensordot
+
+@lru_cache(maxsize=1024)
+def _common_inds(t_a, t_b, nin_a, nin_b, ndim_a, ndim_b, nsym):
+    """reference _contractions.py:250-278."""
+    ta = np.array(t_a, dtype=np.int64).reshape((len(t_a), ndim_a, nsym))
+    tb = np.array(t_b, dtype=np.int64).reshape((len(t_b), ndim_b, nsym))
+    la = [tuple(x) for x in ta[:, nin_a, :].reshape(len(ta), len(nin_a) * nsym).tolist()]
+    lb = [tuple(x) for x in tb[:, nin_b, :].reshape(len(tb), len(nin_b) * nsym).tolist()]
+    sa, sb = set(la), set(lb)
+    ia = tuple(i for i, x in enumerate(la) if x in sb)
+    ib = tuple(i for i, x in enumerate(lb) if x in sa)
+    return (None if len(ia) == len(la) else ia), (None if len(ib) == len(lb) else ib)
+
+
+@lru_cache(maxsize=1024)
+def _meta_tensordot_f2m(struct_a, slices_a, struct_b, slices_b):
+    """reference _contractions.py:281-298."""
+    nsym = len(struct_a.n)
+    ra = sorted((t[nsym:], t, D, sl.slcs[0]) for t, D, sl in zip(struct_a.t, struct_a.D, slices_a))
+    rb = [(t[:nsym], t, D, sl.slcs[0]) for t, D, sl in zip(struct_b.t, struct_b.D, slices_b)]
+    rows = []
+    for (tar, ta, Da, sla), (tbl, tb, Db, slb) in zip(ra, rb):
+        if tar != tbl:
+            raise BlockSparseError('block charges of the contracted legs do not pair up')
+        rows.append((ta[:nsym] + tb[nsym:], (Da[0], Db[1]), sla, Da, slb, Db))
+    rows.sort()
+    t_c = tuple(r[0] for r in rows)
+    D_c = tuple(r[1] for r in rows)
+    Dp = tuple(D[0] * D[1] for D in D_c)
+    slices_c = _slices_of(Dp, D_c)
+    meta = tuple((sl.slcs[0], *r[1:]) for sl, r in zip(slices_c, rows))
+    return meta, Struct(s=(struct_a.s[0], struct_b.s[1]), t=t_c, D=D_c, size=sum(Dp)), slices_c
+
+
+@lru_cache(maxsize=1024)
+def _meta_tensordot_fc(struct_a, slices_a, struct_b, slices_b):
+    """reference _contractions.py:301-346."""
+    nsym = len(struct_a.n)
+    lta, nda = len(struct_a.t), len(struct_a.s)
+    ta = np.array(struct_a.t, dtype=np.int64).reshape((lta, nda, nsym))
+    Da = np.array(struct_a.D, dtype=np.int64).reshape((lta, nda))
+    ra = sorted((tuple(tc), tuple(to), Dc, Dop, tuple(Do), sl.slcs[0]) for tc, to, Dc, Dop, Do, sl in zip(
+        ta[:, -1, :].tolist(), ta[:, :-1, :].reshape(lta, (nda - 1) * nsym).tolist(), Da[:, -1].tolist(),
+        np.prod(Da[:, :-1], axis=1, dtype=np.int64).tolist(), Da[:, :-1].tolist(), slices_a))
+    ltb, ndb = len(struct_b.t), len(struct_b.s)
+    tb = np.array(struct_b.t, dtype=np.int64).reshape((ltb, ndb, nsym))
+    Db = np.array(struct_b.D, dtype=np.int64).reshape((ltb, ndb))
+    rb = [(tuple(tc), tuple(to), Dc, Dop, tuple(Do), sl.slcs[0]) for tc, to, Dc, Dop, Do, sl in zip(
+        tb[:, 0, :].tolist(), tb[:, 1:, :].reshape(ltb, (ndb - 1) * nsym).tolist(), Db[:, 0].tolist(),
+        np.prod(Db[:, 1:], axis=1, dtype=np.int64).tolist(), Db[:, 1:].tolist(), slices_b)]
+    rows = []
+    for (tar, ga), (tbl, gb) in zip(groupby(ra, key=itemgetter(0)), groupby(rb, key=itemgetter(0))):
+        if tar != tbl:
+            raise BlockSparseError('block charges of the contracted legs do not pair up')
+        for (_, toa, Dca, Dopa, Doa, sla), (_, tob, Dcb, Dopb, Dob, slb) in product(list(ga), list(gb)):
+            rows.append((toa + tob, Doa + Dob, Dopa * Dopb, (Dopa, Dopb), sla, (Dopa, Dca), slb, (Dcb, Dopb)))
+    rows.sort()
+    t_c = tuple(r[0] for r in rows)
+    D_c = tuple(r[1] for r in rows)
+    Dp = tuple(r[2] for r in rows)
+    slices_c = _slices_of(Dp, D_c)
+    meta = tuple((sl.slcs[0], *r[3:]) for sl, r in zip(slices_c, rows))
+    return meta, Struct(s=struct_a.s[:-1] + struct_b.s[1:], t=t_c, D=D_c, size=sum(Dp)), slices_c
+
+
+def tensordot(a, b, axes, conj=(0, 0)):
+    """Contract legs axes[0] of a with axes[1] of b (reference _contractions.py:57-125)."""
+    if conj[0]:
+        a = a.conj()
+    if conj[1]:
+        b = b.conj()
+    in_a, in_b = ((x,) if isinstance(x, int) else tuple(x) for x in axes)
+    in_a = tuple(x % a.ndim for x in in_a)
+    in_b = tuple(x % b.ndim for x in in_b)
+    if len(in_a) != len(in_b):
+        raise BlockSparseError('axes[0] and axes[1] indicate different number of legs.')
+    nin_a = tuple(a.trans[ax] for ax in in_a)
+    nin_b = tuple(b.trans[ax] for ax in in_b)
+    if not all(a.struct.s[i] == -b.struct.s[j] for i, j in zip(nin_a, nin_b)):
+        raise BlockSparseError('Signatures do not match.')
+    if a.isdiag:
+        return _tensordot_diag(a, b, in_b, destination=(0,))
+    if b.isdiag:
+        return _tensordot_diag(b, a, in_a, destination=(-1,))
+    if a.config.backend.BACKEND_ID != b.config.backend.BACKEND_ID:
+        raise BlockSparseError('Two tensors have different backends.')
+    for i, j in zip(nin_a, nin_b):
+        if a.hfs[i].tree != b.hfs[j].tree:
+            raise BlockSparseError('hard-fusion structures of contracted legs differ')
+    nout_a = tuple(i for i in a.trans if i not in nin_a)
+    nout_b = tuple(i for i in b.trans if i not in nin_b)
+    n_c = a.config.sym.add_charges(a.struct.n, b.struct.n)
+    s_c = tuple(a.struct.s[i] for i in nout_a) + tuple(b.struct.s[i] for i in nout_b)
+    hfs_c = tuple(a.hfs[i] for i in nout_a) + tuple(b.hfs[i] for i in nout_b)
+    if a.config.tensordot_policy == 'fuse_to_matrix':
+        data, struct_c, slices_c = _tensordot_f2m(a, b, nout_a, nin_a, nin_b, nout_b, s_c)
+    elif a.config.tensordot_policy == 'fuse_contracted':
+        data, struct_c, slices_c = _tensordot_fc(a, b, nout_a, nin_a, nin_b, nout_b)
+    else:
+        raise BlockSparseError(f"tensordot policy {a.config.tensordot_policy} not restated")
+    return Tensor(a.config, struct_c._replace(n=n_c), slices_c, data, hfs=hfs_c, trans=None)
+
+
+def _tensordot_f2m(a, b, nout_a, nin_a, nin_b, nout_b, s_c):
+    nsym = a.config.sym.NSYM
+    ind_a, ind_b = _common_inds(a.struct.t, b.struct.t, nin_a, nin_b, a.ndim_n, b.ndim_n, nsym)
+    data_a, struct_a, slices_a, ls_l, ls_ac = _merge_to_matrix(a, (nout_a, nin_a), ind_a)
+    data_b, struct_b, slices_b, ls_bc, ls_r = _merge_to_matrix(b, (nin_b, nout_b), ind_b)
+    if ls_ac != ls_bc:
+        raise BlockSparseError('Bond dimensions do not match.')
+    meta_dot, struct_c, slices_c = _meta_tensordot_f2m(struct_a, slices_a, struct_b, slices_b)
+    data = a.config.backend.dot(data_a, data_b, meta_dot, struct_c.size)
+    meta_un, struct_c, slices_c = _meta_unmerge_matrix(a.config, struct_c, slices_c, ls_l, ls_r, s_c)
+    return _unmerge(a.config, data, meta_un), struct_c, slices_c
+
+
+def _tensordot_fc(a, b, nout_a, nin_a, nin_b, nout_b):
+    nsym = a.config.sym.NSYM
+    ind_a, ind_b = _common_inds(a.struct.t, b.struct.t, nin_a, nin_b, a.ndim_n, b.ndim_n, nsym)
+    axes_a = tuple((x,) for x in nout_a) + (nin_a,)
+    order_a = nout_a + nin_a
+    struct_a, slices_a, mrg_a, t_a, D_a = _meta_fuse_hard(a.config, a.struct, a.slices, axes_a, ind_a)
+    data_a = _transpose_and_merge(a.config, a._data, order_a, struct_a, slices_a, mrg_a)
+    axes_b = (nin_b,) + tuple((x,) for x in nout_b)
+    order_b = nin_b + nout_b
+    struct_b, slices_b, mrg_b, t_b, D_b = _meta_fuse_hard(b.config, b.struct, b.slices, axes_b, ind_b)
+    data_b = _transpose_and_merge(b.config, b._data, order_b, struct_b, slices_b, mrg_b)
+    if not all(D_a[i] == D_b[j] for i, j in zip(nin_a, nin_b)):
+        raise BlockSparseError('Bond dimensions do not match.')
+    meta_dot, struct_c, slices_c = _meta_tensordot_fc(struct_a, slices_a, struct_b, slices_b)
+    data = a.config.backend.dot(data_a, data_b, meta_dot, struct_c.size)
+    return data, struct_c, slices_c
+
+
+@lru_cache(maxsize=1024)
+def _meta_broadcast(b_struct, b_slices, a_struct, a_slices, axis):
+    """reference _contractions.py:502-528."""
+    nsym = len(a_struct.n)
+    ind_tb = tuple(x[axis * nsym:(axis + 1) * nsym] for x in b_struct.t)
+    sl_a = {x[:nsym]: s for x, s in zip(a_struct.t, a_slices)}
+    rows = tuple((tb, slb.slcs[0], Db, slb.Dp, sl_a[ib].slcs[0])
+                 for tb, slb, Db, ib in zip(b_struct.t, b_slices, b_struct.D, ind_tb) if ib in sl_a)
+    if any(Db[axis] != sla[1] - sla[0] for _, _, Db, _, sla in rows):
+        raise BlockSparseError('Bond dimensions do not match.')
+    if len(rows) < len(b_struct.t):
+        Dp = tuple(r[3] for r in rows)
+        c_slices = _slices_of(Dp, tuple(r[2] for r in rows))
+        c_struct = b_struct._replace(t=tuple(r[0] for r in rows), D=tuple(r[2] for r in rows), size=sum(Dp))
+    else:
+        c_struct, c_slices = b_struct, b_slices
+    meta = tuple((sn.slcs[0], slb, Db, sla) for (_, slb, Db, _, sla), sn in zip(rows, c_slices))
+    return meta, c_struct, c_slices
+
+
+def broadcast(a, b, axis):
+    """Multiply leg ``axis`` of b by the diagonal tensor a."""
+    ax = b.trans[axis % b.ndim]
+    meta, struct, slices = _meta_broadcast(b.struct, b.slices, a.struct, a.slices, ax)
+    if b.isdiag:
+        meta = tuple((sln, slb, Db[0], sla) for sln, slb, Db, sla in meta)
+        nd, ax = 1, 0
+    else:
+        nd = b.ndim_n
+    data = b.config.backend.dot_diag(a._data, b._data, meta, struct.size, ax, nd)
+    return b._replace(struct=struct, slices=slices, data=data)
+
+
+def _tensordot_diag(a, b, in_b, destination):
+    if len(in_b) != 1:
+        raise BlockSparseError('contraction with a diagonal tensor over more than one leg is not restated')
+    c = broadcast(a, b, in_b[0])
+    return c.moveaxis(source=in_b, destination=destination)
+
+
+# ------------------------------------------------------------------ masks
+
+@lru_cache(maxsize=1024)
+def _meta_mask(a_struct, a_slices, a_isdiag, mask_t, mask_D, axis):
+    """reference _merging.py:554-583."""
+    tD = {t: D for t, D in zip(mask_t, mask_D) if D > 0}
+    nsym = len(a_struct.n)
+    ind = tuple(x[axis * nsym:(axis + 1) * nsym] for x in a_struct.t)
+    rows = tuple((ta, sla.slcs[0], Da, tm, tD[tm]) for ta, sla, Da, tm in zip(a_struct.t, a_slices, a_struct.D, ind)
+                 if tm in tD)
+    c_t = tuple(r[0] for r in rows)
+    if a_isdiag:
+        c_D = tuple((r[4], r[4]) for r in rows)
+        c_Dp = tuple(r[4] for r in rows)
+    else:
+        c_D = tuple(r[2][:axis] + (r[4],) + r[2][axis + 1:] for r in rows)
+        c_Dp = np.prod(c_D, axis=1, dtype=np.int64).tolist() if c_D else ()
+    c_slices = _slices_of(c_Dp, c_D)
+    c_struct = a_struct._replace(t=c_t, D=c_D, size=sum(c_Dp))
+    if a_isdiag:
+        meta = tuple((sn.slcs[0], Dn[0], sla, Da[0], tm) for sn, Dn, (_, sla, Da, tm, _) in zip(c_slices, c_D, rows))
+        return meta, c_struct, c_slices, 0, 1
+    meta = tuple((sn.slcs[0], Dn, sla, Da, tm) for sn, Dn, (_, sla, Da, tm, _) in zip(c_slices, c_D, rows))
+    return meta, c_struct, c_slices, axis, len(a_struct.s)
+
+
+def apply_mask(mask, *args, axes=0):
+    """Keep the entries of legs ``axes`` of args where the diagonal boolean ``mask`` is True
+    (reference _contractions.py:529-574)."""
+    axes = tuple(axes) if hasattr(axes, '__iter__') else (axes,)
+    nsym = mask.config.sym.NSYM
+    be = mask.config.backend
+    m = {t[:nsym]: be.to_mask(mask._data[sl.slcs[0][0]:sl.slcs[0][1]]) for t, sl in zip(mask.struct.t, mask.slices)}
+    mask_t = tuple(m.keys())
+    mask_D = tuple(len(v) for v in m.values())
+    out = []
+    for b, ax in zip(args, axes):
+        ax = b.trans[ax % b.ndim]
+        meta, struct, slices, ax2, nd = _meta_mask(b.struct, b.slices, b.isdiag, mask_t, mask_D, ax)
+        data = be.apply_mask(b._data, m, meta, struct.size, ax2, nd)
+        out.append(b._replace(struct=struct, slices=slices, data=data))
+    return out[0] if len(out) == 1 else out
+
+
+# ------------------------------------------------------------------ decompositions
+
+def _meta_svd(config, struct, slices, minD, sU, nU):
+    """reference linalg.py:323-380 (not cached there either: the backend caches plans by content)."""
+    sym, nsym = config.sym, config.sym.NSYM
+    n0 = sym.zero()
+    if any(D == 0 for D in minD):
+        keep = [i for i, d in enumerate(minD) if d > 0]
+        struct = struct._replace(t=tuple(struct.t[i] for i in keep), D=tuple(struct.D[i] for i in keep))
+        slices = tuple(slices[i] for i in keep)
+        minD = tuple(minD[i] for i in keep)
+    if nU and sU == struct.s[1]:
+        t_con = tuple(x[nsym:] for x in struct.t)
+    elif nU:
+        tc = np.array(struct.t, dtype=np.int64).reshape((len(struct.t), 2, nsym))
+        t_con = tuple(map(tuple, sym.fuse(tc[:, 1:, :], (1,), -1).tolist()))
+    elif sU == -struct.s[0]:
+        t_con = tuple(x[:nsym] for x in struct.t)
+    else:
+        tc = np.array(struct.t, dtype=np.int64).reshape((len(struct.t), 2, nsym))
+        t_con = tuple(map(tuple, sym.fuse(tc[:, :1, :], (1,), -1).tolist()))
+    Un, Vn = (struct.n, n0) if nU else (n0, struct.n)
+    Ut = tuple(x[:nsym] + y for x, y in zip(struct.t, t_con))
+    St = tuple(y + y for y in t_con)
+    Vt = tuple(y + x[nsym:] for y, x in zip(t_con, struct.t))
+    UD = tuple((ds[0], dm) for ds, dm in zip(struct.D, minD))
+    SD = tuple((dm, dm) for dm in minD)
+    VD = tuple((dm, ds[1]) for dm, ds in zip(minD, struct.D))
+    UDp = np.prod(UD, axis=1, dtype=np.int64).tolist() if UD else ()
+    Usl = _slices_of(UDp, UD)
+    rows = tuple(zip(slices, struct.D, Usl, UD, St, Vt, VD))
+    St, Vt, SD, VD = zip(*sorted(zip(St, Vt, SD, VD))) if St else ((), (), (), ())
+    SDp = tuple(d[0] for d in SD)
+    VDp = np.prod(VD, axis=1, dtype=np.int64).tolist() if VD else ()
+    Ssl = _slices_of(SDp, SD)
+    Vsl = _slices_of(VDp, VD)
+    Sd = {x: y.slcs[0] for x, y in zip(St, Ssl)}
+    Vd = {x: y.slcs[0] for x, y in zip(Vt, Vsl)}
+    meta = tuple((sl.slcs[0], d, su.slcs[0], du, Sd[ts], Vd[tv], dv) for sl, d, su, du, ts, tv, dv in rows)
+    Us = Struct(s=(struct.s[0], sU), n=Un, diag=False, t=Ut, D=UD, size=sum(UDp))
+    Ss = Struct(s=(-sU, sU), n=n0, diag=True, t=St, D=SD, size=sum(SDp))
+    Vs = Struct(s=(-sU, struct.s[1]), n=Vn, diag=False, t=Vt, D=VD, size=sum(VDp))
+    return meta, Us, Usl, Ss, Ssl, Vs, Vsl
+
+
+def svd(a, axes=(0, 1), sU=1, nU=True, Uaxis=-1, Vaxis=0, fix_signs=False, **kwargs):
+    """Blockwise thin SVD a = U S V (reference linalg.py:135-320, policy 'fullrank')."""
+    out_l, out_r = ((x,) if isinstance(x, int) else tuple(x) for x in axes)
+    out_hl = tuple(a.trans[ax] for ax in out_l)
+    out_hr = tuple(a.trans[ax] for ax in out_r)
+    data, struct, slices, ls_l, ls_r = _merge_to_matrix(a, (out_hl, out_hr))
+    minD = tuple(min(ds) for ds in struct.D)
+    meta, Ust, Usl, Sst, Ssl, Vst, Vsl = _meta_svd(a.config, struct, slices, minD, sU, nU)
+    sizes = (Ust.size, Sst.size, Vst.size)
+    be = a.config.backend
+    Ud, Sd, Vd = be.svd(data, meta, sizes, diagnostics=kwargs.get('diagnostics', None))
+    if fix_signs:
+        Ud, Vd = be.fix_svd_signs(Ud, Vd, meta)
+    ls_s = _leg_struct_trivial(Sst, axis=0)
+    S = Tensor(a.config, Sst, Ssl, Sd, hfs=(Fusion(s=(-sU,)), Fusion(s=(sU,))))
+    Us = tuple(a.struct.s[i] for i in out_hl) + (sU,)
+    mu, Ust, Usl = _meta_unmerge_matrix(a.config, Ust, Usl, ls_l, ls_s, Us)
+    U = Tensor(a.config, Ust, Usl, _unmerge(a.config, Ud, mu),
+               hfs=tuple(a.hfs[i] for i in out_hl) + (Fusion(s=(sU,)),))
+    Vs = (-sU,) + tuple(a.struct.s[i] for i in out_hr)
+    mv, Vst, Vsl = _meta_unmerge_matrix(a.config, Vst, Vsl, ls_s, ls_r, Vs)
+    V = Tensor(a.config, Vst, Vsl, _unmerge(a.config, Vd, mv),
+               hfs=(Fusion(s=(-sU,)),) + tuple(a.hfs[i] for i in out_hr))
+    return U.moveaxis(-1, Uaxis), S, V.moveaxis(0, Vaxis)
+
+
+def truncation_mask(S, tol=0, tol_block=0, D_block=float('inf'), D_total=float('inf'), truncate_multiplets=False,
+                    mask_f=None, **kwargs):
+    """Boolean diagonal mask of the kept singular values (reference linalg.py:637-735)."""
+    if mask_f is not None:
+        return mask_f(S)
+    be = S.config.backend
+    nsym = S.config.sym.NSYM
+    Sdata = be.detach(S._data)
+    keep = Sdata > -float('inf')
+    if truncate_multiplets:
+        tol_block, D_block = 0, float('inf')
+    tol_null = 0. if isinstance(tol_block, dict) else tol_block
+    D_null = 0 if isinstance(D_block, dict) else D_block
+    for t, sl in zip(S.struct.t, S.slices):
+        t = t[:nsym]
+        lo, hi = sl.slcs[0]
+        tol_rel = tol_block[t] if isinstance(tol_block, dict) and t in tol_block else tol_null
+        blk = Sdata[lo:hi]
+        D_tol = be.sum_elements(blk > tol_rel * be.max_abs(blk)).item()
+        D_bl = D_block[t] if isinstance(D_block, dict) and t in D_block else D_null
+        D_bl = min(D_bl, D_tol)
+        if 0 < D_bl < sl.Dp:
+            inds = be.argsort(blk)
+            keep[lo:hi][inds[:-D_bl]] = False
+        elif D_bl == 0:
+            keep[lo:hi] = False
+    tmp = Sdata * keep
+    D_tol = be.sum_elements(tmp > tol * be.max_abs(tmp)).item()
+    D_total = min(D_total, D_tol)
+    if D_total == 0:
+        keep[:] = False
+        return S._replace(data=keep)
+    inds = be.argsort(tmp)
+    if truncate_multiplets and D_total < len(inds):
+        gap = -1
+        for p in range(D_total, len(inds)):
+            gp = abs(Sdata[inds[-p]] - Sdata[inds[-p - 1]])
+            if gp > gap:
+                D_total, gap = p, gp
+            if gap > abs(Sdata[inds[-p]]):
+                break
+    keep[inds[:-D_total]] = False
+    return S._replace(data=keep)
+
+
+def svd_with_truncation(a, axes=(0, 1), sU=1, nU=True, Uaxis=-1, Vaxis=0, fix_signs=False, tol=0, tol_block=0,
+                        D_block=float('inf'), D_total=float('inf'), truncate_multiplets=False, mask_f=None, **kwargs):
+    """reference linalg.py:49-132."""
+    U, S, V = svd(a, axes=axes, sU=sU, nU=nU, fix_signs=fix_signs, **kwargs)
+    mask = truncation_mask(S, tol=tol, tol_block=tol_block, D_block=D_block, D_total=D_total,
+                           truncate_multiplets=truncate_multiplets, mask_f=mask_f)
+    U, S, V = apply_mask(mask, U, S, V, axes=(-1, 0, 0))
+    return U.moveaxis(-1, Uaxis), S, V.moveaxis(0, Vaxis)
+
+
+def _meta_eigh(config, struct, slices, sU, minD):
+    """reference linalg.py:963-1010."""
+    sym, nsym = config.sym, config.sym.NSYM
+    n0 = sym.zero()
+    if sU == -struct.s[0]:
+        t_con = tuple(x[:nsym] for x in struct.t)
+    else:
+        tc = np.array(struct.t, dtype=np.int64).reshape((len(struct.t), 2, nsym))
+        t_con = tuple(map(tuple, sym.fuse(tc[:, :1, :], (1,), -1).tolist()))
+    Ut = tuple(x[:nsym] + y for x, y in zip(struct.t, t_con))
+    UD = tuple((ds[0], dm) for ds, dm in zip(struct.D, minD))
+    UDp = np.prod(UD, axis=1, dtype=np.int64).tolist() if UD else ()
+    Usl = _slices_of(UDp, UD)
+    SD = tuple((dm, dm) for dm in minD)
+    St = tuple(y + y for y in t_con)
+    rows = tuple(zip(slices, struct.D, Usl, UD, St))
+    St, SD = zip(*sorted(zip(St, SD))) if St else ((), ())
+    SDp = tuple(d[0] for d in SD)
+    Ssl = _slices_of(SDp, SD)
+    Sd = {x: y.slcs[0] for x, y in zip(St, Ssl)}
+    meta = tuple((sl.slcs[0], d, su.slcs[0], du, Sd[ts]) for sl, d, su, du, ts in rows)
+    Us = Struct(s=(struct.s[0], sU), n=n0, diag=False, t=Ut, D=UD, size=sum(UDp))
+    Ss = Struct(s=(-sU, sU), n=n0, diag=True, t=St, D=SD, size=sum(SDp))
+    return meta, Ss, Ssl, Us, Usl
+
+
+def eigh(a, axes, sU=1):
+    """Blockwise Hermitian eigendecomposition a = U S U^H, eigenvalues ascending per block
+    (reference linalg.py:831-960 with the default 'fullrank' policy, without the 'which' reordering)."""
+    out_l, out_r = ((x,) if isinstance(x, int) else tuple(x) for x in axes)
+    out_hl = tuple(a.trans[ax] for ax in out_l)
+    out_hr = tuple(a.trans[ax] for ax in out_r)
+    data, struct, slices, ls_l, ls_r = _merge_to_matrix(a, (out_hl, out_hr))
+    minD = tuple(min(ds) for ds in struct.D)
+    meta, Sst, Ssl, Ust, Usl = _meta_eigh(a.config, struct, slices, sU, minD)
+    Sd, Ud = a.config.backend.eigh(data, meta, (Sst.size, Ust.size))
+    ls_s = _leg_struct_trivial(Sst, axis=1)
+    S = Tensor(a.config, Sst, Ssl, Sd, hfs=(Fusion(s=(-sU,)), Fusion(s=(sU,))))
+    Us = tuple(a.struct.s[i] for i in out_hl) + (sU,)
+    mu, Ust, Usl = _meta_unmerge_matrix(a.config, Ust, Usl, ls_l, ls_s, Us)
+    U = Tensor(a.config, Ust, Usl, _unmerge(a.config, Ud, mu),
+               hfs=tuple(a.hfs[i] for i in out_hl) + (Fusion(s=(sU,)),))
+    return S, U
+
+
+def vdot(a, b):
+    """<a|b> summed over common blocks (a conjugated)."""
+    nsym = a.config.sym.NSYM
+    ib = {t: sl.slcs[0] for t, sl in zip(b.struct.t, b.slices)}
+    meta = tuple((sl.slcs[0], ib[t]) for t, sl in zip(a.struct.t, a.slices) if t in ib)
+    if a.trans != tuple(range(a.ndim_n)) or b.trans != tuple(range(b.ndim_n)):
+        a, b = consume_transpose(a), consume_transpose(b)
+        return vdot(a, b)
+    return a.config.backend.vdot(a.config.backend.conj(a._data), b._data, meta)
