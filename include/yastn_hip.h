@@ -107,6 +107,8 @@ int yastn_hip_unmerge_adjoint(void* grad_src, const void* grad_new, int64_t nume
 #define YH_TILE_32x32 0
 #define YH_TILE_64x64 1
 #define YH_TILE_128x64 2
+#define YH_TILE_16x64 3   /* skinny M (e.g. MPO contractions) */
+#define YH_TILE_64x16 4   /* skinny N */
 
 int yastn_hip_check_gemm_plan(const int64_t* h_gemms, int64_t ngemms, const int64_t* h_terms, int64_t nterms,
                               const int32_t* h_tiles, int64_t ntiles, int tile_config,

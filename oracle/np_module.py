@@ -1,7 +1,8 @@
-"""numpy backend module for CPU tests of the restated tensor layer (yastn_amd.blocksparse).
+"""numpy backend module (oracle) for the restated tensor layer (yastn_amd.blocksparse): CPU checks and
+the CPU baseline leg of bench.py.
 
 Its hot-path functions are the oracle's (oracle/np_backend.py); the rest are numpy one-liners.
-Test infrastructure only.
+Test infrastructure only (oracle package).
 """
 import numpy as np
 

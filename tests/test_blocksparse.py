@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import golden_io as G
-import np_test_backend as NB
+from oracle import np_module as NB
 from yastn_amd import blocksparse as bs
 
 HOT = ('transpose_and_merge', 'unmerge', 'dot', 'svd', 'eigh', 'fix_svd_signs', 'apply_mask', 'transpose')

@@ -236,8 +236,7 @@ class Dot(torch.autograd.Function):
         A = raw(Adata.to(dtype))
         B = raw(Bdata.to(dtype))
         plan = P.dot_plan(meta_dot, A.numel(), B.numel(), Dsize, A.device)
-        covered = sum(int(r[0]) * int(r[1]) for r in plan.h_gemms)
-        out = (torch.empty if covered == Dsize else torch.zeros)(Dsize, dtype=dtype, device=A.device)
+        out = (torch.empty if plan.covered == Dsize else torch.zeros)(Dsize, dtype=dtype, device=A.device)
         L.call('yastn_hip_dot', ptr(out), ptr(A), ptr(B), dtype_code(out), *plan.args(), stream_of(out))
         return out
 

@@ -60,6 +60,10 @@ def _sl(x):
     return slice(x[0], x[1])
 
 
+def _needs_grad(*xs):
+    return torch.is_grad_enabled() and any(torch.is_tensor(x) and x.requires_grad for x in xs)
+
+
 # ------------------------------------------------------------------ dtype / device
 
 def cuda_is_available():
@@ -318,6 +322,10 @@ def bitwise_not(data):
 
 def svd(data, meta, sizes, fullrank_uv=False, ad_decomp_reg=1.0e-12, diagnostics=None, **kwargs):
     """A4: batched one-sided Jacobi SVD of all blocks; adjoint A9 (Lorentzian eps = 1e-12)."""
+    if not _needs_grad(data):
+        if fullrank_uv:
+            raise NotImplementedError("yastn_amd svd computes the thin SVD (fullrank_uv=False)")
+        return F.svd_forward(data, meta, sizes)
     return F.Svd.apply(data, meta, sizes, fullrank_uv, ad_decomp_reg, diagnostics)
 
 
@@ -522,6 +530,8 @@ def vdot(Adata, Bdata, meta):
 
 def dot(Adata, Bdata, meta_dot, Dsize):
     """A3: all sector GEMMs of one tensordot in one grouped FP64-MFMA launch; adjoint A7."""
+    if not _needs_grad(Adata, Bdata):
+        return F.Dot.forward(Adata, Bdata, meta_dot, Dsize)
     return F.Dot.apply(Adata, Bdata, meta_dot, Dsize)
 
 
@@ -567,26 +577,36 @@ def negate_blocks(Adata, slices):
 
 def apply_mask(Adata, mask, meta, Dsize, axis, a_ndim):
     """A12: keep the index lists ``mask[t]`` along ``axis`` of every block (one grouped gather)."""
+    if not _needs_grad(Adata):
+        return F.ApplyMask.forward(Adata, mask, meta, Dsize, axis, a_ndim)
     return F.ApplyMask.apply(Adata, mask, meta, Dsize, axis, a_ndim)
 
 
 def embed_mask(Adata, mask, meta, Dsize, axis, a_ndim):
     """A12 adjoint form: scatter along ``axis`` into zero-filled blocks (one grouped gather)."""
+    if not _needs_grad(Adata):
+        return F.EmbedMask.forward(Adata, mask, meta, Dsize, axis, a_ndim)
     return F.EmbedMask.apply(Adata, mask, meta, Dsize, axis, a_ndim)
 
 
 def transpose(data, axes, meta_transpose):
     """Permute every block into a new block order (one grouped gather)."""
+    if not _needs_grad(data):
+        return F.Transpose.forward(data, axes, meta_transpose)
     return F.Transpose.apply(data, axes, meta_transpose)
 
 
 def transpose_and_merge(data, order, meta_new, meta_mrg, Dsize):
     """A1: gather permuted charge-sector blocks into fused blocks (one grouped launch); adjoint A6."""
+    if not _needs_grad(data):
+        return F.TransposeAndMerge.forward(data, order, meta_new, meta_mrg, Dsize)
     return F.TransposeAndMerge.apply(data, order, meta_new, meta_mrg, Dsize)
 
 
 def unmerge(data, meta):
     """A2: scatter fused blocks back to block layout (one grouped launch); adjoint A8."""
+    if not _needs_grad(data):
+        return F.Unmerge.forward(data, meta)
     return F.Unmerge.apply(data, meta)
 
 

@@ -212,6 +212,12 @@ int launch(const Bufs& b, const int64_t* gemms, const int64_t* terms, const int3
     case YH_TILE_128x64:
       hipLaunchKernelGGL((gemm_kernel<CPLX, 128, 64, 2, 2>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
+    case YH_TILE_16x64:
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 16, 64, 1, 4>), grid, block, 0, s, b, gemms, terms, tiles);
+      break;
+    case YH_TILE_64x16:
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 64, 16, 4, 1>), grid, block, 0, s, b, gemms, terms, tiles);
+      break;
     default:
       yh::set_error("unknown tile config %d", cfg);
       return YH_ERR_ARG;
@@ -220,8 +226,13 @@ int launch(const Bufs& b, const int64_t* gemms, const int64_t* terms, const int3
 }
 
 void tile_dims(int cfg, int* bm, int* bn) {
-  *bm = cfg == YH_TILE_32x32 ? 32 : (cfg == YH_TILE_64x64 ? 64 : 128);
-  *bn = cfg == YH_TILE_32x32 ? 32 : 64;
+  switch (cfg) {
+    case YH_TILE_32x32: *bm = 32; *bn = 32; break;
+    case YH_TILE_64x64: *bm = 64; *bn = 64; break;
+    case YH_TILE_128x64: *bm = 128; *bn = 64; break;
+    case YH_TILE_16x64: *bm = 16; *bn = 64; break;
+    default: *bm = 64; *bn = 16; break;
+  }
 }
 
 }  // namespace
@@ -229,7 +240,7 @@ void tile_dims(int cfg, int* bm, int* bn) {
 extern "C" int yastn_hip_check_gemm_plan(const int64_t* h_gemms, int64_t ngemms, const int64_t* h_terms,
                                          int64_t nterms, const int32_t* h_tiles, int64_t ntiles, int cfg,
                                          const int64_t* in_numel, int nin, const int64_t* out_numel, int nout) {
-  YH_CHECK_ARG(cfg >= YH_TILE_32x32 && cfg <= YH_TILE_128x64, "tile config %d", cfg);
+  YH_CHECK_ARG(cfg >= YH_TILE_32x32 && cfg <= YH_TILE_64x16, "tile config %d", cfg);
   YH_CHECK_ARG(nin <= YH_MAX_BUFS && nout <= YH_MAX_BUFS, "too many buffers");
   int bm, bn;
   tile_dims(cfg, &bm, &bn);

@@ -4,7 +4,7 @@
 // matrix W (p x q, p = max(m,n), q = min(m,n); W = A, or A^H when m < n) and a
 // q x q rotation accumulator V = I.  Each sweep visits all column pairs in the
 // round-robin (circle) order, so the q/2 pairs of a round are disjoint and the
-// 4 waves work on different pairs concurrently; a pair is rotated when
+// 16 waves work on different pairs concurrently; a pair is rotated when
 // |w_i^H w_j| > tol * ||w_i|| ||w_j||.  Converged when a sweep rotates nothing.
 // Then sigma_j = ||w_j||, U = W Sigma^-1 (or V for m < n), sorted descending.
 //
@@ -20,7 +20,8 @@
 
 namespace {
 
-constexpr int kThreads = 256;
+// 16 waves per block: the q/2 disjoint pairs of a round are spread over more waves
+constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
 
 template <bool CPLX>

@@ -21,18 +21,36 @@ from . import _lib as L
 # --------------------------------------------------------------------------- cache
 
 class _Cache:
+    """Plan cache.  Lookup by the identity of the meta objects first (the reference planner returns
+    lru-cached tuples); on a miss, by their content (some metas, e.g. the SVD's, are rebuilt on every
+    call by the reference), so a plan is compiled once per distinct structure."""
+
     def __init__(self, maxsize=4096):
         self.d = {}
+        self.by_content = {}
         self.maxsize = maxsize
 
     def get(self, key, refs, build):
         hit = self.d.get(key)
         if hit is not None and all(a is b for a, b in zip(hit[0], refs)):
             return hit[1]
+        ckey = (key[0],) + tuple(x for x in key[1:] if not isinstance(x, int) or x not in {id(r) for r in refs})
+        try:
+            chash = hash((ckey, refs))
+        except TypeError:
+            chash = None
+        if chash is not None:
+            hit2 = self.by_content.get(chash)
+            if hit2 is not None and hit2[0] == (ckey, refs):
+                self.d[key] = (refs, hit2[1])
+                return hit2[1]
         val = build()
         if len(self.d) >= self.maxsize:
             self.d.clear()
+            self.by_content.clear()
         self.d[key] = (refs, val)
+        if chash is not None:
+            self.by_content[chash] = ((ckey, refs), val)
         return val
 
 
@@ -213,8 +231,10 @@ def merge_plan(order, meta_new, meta_mrg, Dsize, src_numel, device):
                 off, st, shp = _box_in_block(Dn, box)
                 jobs.append((-1, sln[0] + off, shp, (0,) * len(shp), st))
         return GatherPlan(jobs, src_numel, Dsize, device)
-    return _CACHE.get(('merge', id(meta_mrg), id(meta_new), tuple(order), Dsize, src_numel, _device_key(device)),
-                      (meta_mrg, meta_new), build)
+    # meta_new is rebuilt by the caller on every call from the same cached structure that produced
+    # meta_mrg, so meta_mrg's identity (plus cheap consistency fields) identifies the plan
+    return _CACHE.get(('merge', id(meta_mrg), len(meta_new), tuple(meta_new[-1]) if meta_new else None,
+                       tuple(order), Dsize, src_numel, _device_key(device)), (meta_mrg,), build)
 
 
 def _fill_gaps(covered, total):
@@ -244,8 +264,8 @@ def merge_adjoint_plan(order, meta_new, meta_mrg, Dsize, src_numel, device):
                 cov.append((slo[0], slo[1]))
         jobs.extend(_fill_gaps(cov, src_numel))
         return GatherPlan(jobs, Dsize, src_numel, device)
-    return _CACHE.get(('merge_adj', id(meta_mrg), id(meta_new), tuple(order), Dsize, src_numel,
-                       _device_key(device)), (meta_mrg, meta_new), build)
+    return _CACHE.get(('merge_adj', id(meta_mrg), len(meta_new), tuple(meta_new[-1]) if meta_new else None,
+                       tuple(order), Dsize, src_numel, _device_key(device)), (meta_mrg,), build)
 
 
 def unmerge_plan(meta, numel, device):
@@ -372,6 +392,7 @@ class GemmPlan:
             k_of[g] = int(self.h_terms[row[5]:row[5] + row[6], 6].sum()) if row[6] else 0
         self.k_of = k_of
         self.macs = int(np.sum(self.h_gemms[:, 0] * self.h_gemms[:, 1] * k_of)) if self.ngemms else 0
+        self.covered = int(np.sum(self.h_gemms[:, 0] * self.h_gemms[:, 1])) if self.ngemms else 0
         self.device = device
         self._tiles = {}
         self.cfg = choose_tile(self) if cfg is None else cfg
@@ -421,8 +442,10 @@ def choose_tile(plan):
     M = plan.h_gemms[:, 0].astype(np.float64)
     N = plan.h_gemms[:, 1].astype(np.float64)
     K = np.maximum(plan.k_of.astype(np.float64), 1.0)
-    eff = {L.YH_TILE_32x32: 0.45, L.YH_TILE_64x64: 0.8, L.YH_TILE_128x64: 1.0}
-    resident = {L.YH_TILE_32x32: 4, L.YH_TILE_64x64: 2, L.YH_TILE_128x64: 1}
+    eff = {L.YH_TILE_32x32: 0.45, L.YH_TILE_64x64: 0.8, L.YH_TILE_128x64: 1.0, L.YH_TILE_16x64: 0.4,
+           L.YH_TILE_64x16: 0.4}
+    resident = {L.YH_TILE_32x32: 4, L.YH_TILE_64x64: 2, L.YH_TILE_128x64: 1, L.YH_TILE_16x64: 4,
+                L.YH_TILE_64x16: 4}
     best, best_cost = None, None
     for cfg, (bm, bn) in L.TILE_DIMS.items():
         tiles = np.ceil(M / bm) * np.ceil(N / bn)
