@@ -13,7 +13,15 @@ from . import plans as P
 
 SYNC_CHECKS = os.environ.get('YASTN_HIP_SYNC_CHECKS', '1') != '0'
 MAX_SWEEPS = int(os.environ.get('YASTN_HIP_MAX_SWEEPS', '60'))
-AD_EPS = 1e-12  # the reference's Lorentzian broadening is fixed to 1e-12 (torch_svd_gesdd.py:4-6)
+# Lorentzian broadening of the SVD adjoint: the reference fixes it to 1e-12 (torch_svd_gesdd.py:4-6,
+# absolute, whatever ad_decomp_reg says).  set_ad_broadening(eps) changes it, e.g. to 1e-30 for an
+# unbroadened adjoint in finite-difference checks.
+AD_EPS = 1e-12
+
+
+def set_ad_broadening(eps):
+    global AD_EPS
+    AD_EPS = float(eps)
 
 
 def dtype_code(t):

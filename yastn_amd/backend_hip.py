@@ -60,6 +60,11 @@ def _sl(x):
     return slice(x[0], x[1])
 
 
+def set_ad_broadening(eps):
+    """Lorentzian broadening eps of the SVD adjoint (default 1e-12, the reference's fixed value)."""
+    F.set_ad_broadening(eps)
+
+
 def _needs_grad(*xs):
     return torch.is_grad_enabled() and any(torch.is_tensor(x) and x.requires_grad for x in xs)
 

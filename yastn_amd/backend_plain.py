@@ -112,8 +112,16 @@ class _Dot(torch.autograd.Function):
         return gA, gB, None, None
 
 
-def _lor(x, eps=1e-12):
-    return x / (x * x + eps)
+AD_EPS = 1e-12  # the reference's fixed Lorentzian broadening (torch_svd_gesdd.py:4-6)
+
+
+def set_ad_broadening(eps):
+    global AD_EPS
+    AD_EPS = float(eps)
+
+
+def _lor(x):
+    return x / (x * x + AD_EPS)
 
 
 def _svd_block_backward(u, s, vh, gu, gs, gvh):
