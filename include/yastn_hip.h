@@ -35,7 +35,7 @@ extern "C" {
 #define YH_F64 0
 #define YH_C128 1
 
-#define YH_ABI_VERSION 3
+#define YH_ABI_VERSION 4
 
 /* ---- box-gather plan (one job per copied or zero-filled box) -------------
  * int64 words per job (YH_JOB_WORDS):
@@ -46,13 +46,13 @@ extern "C" {
  *              1 = 32x32 LDS-staged transpose of (tdim, rank-1)
  *   4 tdim     kind 1: dimension whose source stride is 1
  *   5 tile_begin  index of the job's first tile
- *   6..11  shape[YH_MAXR]    (destination C order; the last dim has dst stride 1)
- *   12..17 src_stride[YH_MAXR]
- *   18..23 dst_stride[YH_MAXR]
+ *   6 ..  6+YH_MAXR-1    shape[YH_MAXR] (destination C order; the last dim has dst stride 1)
+ *   6+YH_MAXR ..         src_stride[YH_MAXR]
+ *   6+2*YH_MAXR ..       dst_stride[YH_MAXR]   (then padding to YH_JOB_WORDS)
  * tile_job[t] (int32) is the job owning tile t.
  */
-#define YH_MAXR 6
-#define YH_JOB_WORDS 24
+#define YH_MAXR 8
+#define YH_JOB_WORDS 32
 #define YH_GATHER_TILE 2048 /* elements per kind-0 tile */
 #define YH_TRANSPOSE_TILE 32 /* kind-1 tile edge */
 
@@ -156,6 +156,21 @@ int yastn_hip_svd(void* U, void* S, void* Vh, const void* A, int dtype, const in
 int yastn_hip_eigh(void* S, void* U, const void* A, int dtype, const int64_t* d_blocks, int64_t nblocks,
                    const int32_t* d_order, int64_t n_lds, int64_t lds_bytes, void* work, int32_t* d_info,
                    int max_sweeps, hipStream_t stream);
+
+/* Large blocks of A4 / A5 (working matrices beyond one CU): panel Jacobi.  init copies A (or A^H,
+ * or A + mu I with the Gershgorin mu for eigh) into `work` and sets V = I; each round launch takes
+ * (block, panel i, panel j) int32 triples, one workgroup each, stages the panel pair (2b columns) in
+ * LDS, applies one inner sweep of rotations and updates V; flags[job] = 1 if it rotated (the host
+ * stops when a whole sweep rotated nothing); finalize writes U, S, Vh (or S, U) like yastn_hip_svd.
+ * lds_bytes = (2b p + 4 b^2) * itemsize for the largest p of the round. */
+int yastn_hip_device_info(int* lds_per_block, int* num_cu);
+int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, int dtype, int mode, const int64_t* d_blocks,
+                             const int32_t* d_ids, int64_t nb, hipStream_t stream);
+int yastn_hip_svd_panel_round(void* work, int dtype, const int64_t* d_blocks, const int32_t* d_jobs, int64_t njobs,
+                              int b, int64_t lds_bytes, int32_t* d_flags, hipStream_t stream);
+int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const void* work, const double* mu, int dtype, int mode,
+                                 const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, int32_t* d_info,
+                                 int sweeps, hipStream_t stream);
 
 /* A11: replaces backend.fix_svd_signs(Udata, Vdata, meta) (backend_np.py:447-458,
  *      backend_torch.py:353-366): in place on U and Vh; blocks are SVD plan words. */

@@ -5,15 +5,16 @@ indexing; they are independent of the HIP kernels, which the GPU tests check.
 """
 import numpy as np
 
-YH_MAXR = 6
+YH_MAXR = 8
 
 
 def gather(h_jobs, src, dst):
     for w in h_jobs:
         src_off, dst_off, r = int(w[0]), int(w[1]), int(w[2])
+        M = YH_MAXR
         shape = tuple(int(x) for x in w[6:6 + r])
-        ss = np.array(w[12:12 + r], dtype=np.int64)
-        ds = np.array(w[18:18 + r], dtype=np.int64)
+        ss = np.array(w[6 + M:6 + M + r], dtype=np.int64)
+        ds = np.array(w[6 + 2 * M:6 + 2 * M + r], dtype=np.int64)
         idx = np.indices(shape).reshape(r, -1)
         d = dst_off + ds @ idx
         if src_off < 0:

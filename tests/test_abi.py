@@ -38,10 +38,11 @@ def test_abi_version_and_constants():
 def test_plan_checker_rejects_out_of_bounds():
     job = np.zeros((1, L.YH_JOB_WORDS), dtype=np.int64)
     job[0, :6] = (0, 0, 1, 0, 0, 0)
-    job[0, 6:12] = 1
+    M = L.YH_MAXR
+    job[0, 6:6 + M] = 1
     job[0, 6] = 10
-    job[0, 12] = 1
-    job[0, 18] = 1
+    job[0, 6 + M] = 1
+    job[0, 6 + 2 * M] = 1
     tiles = np.zeros(1, dtype=np.int32)
     lib = L.lib()
     ok = lib.yastn_hip_check_gather_plan(job.ctypes.data, 1, tiles.ctypes.data, 1, 10, 10)

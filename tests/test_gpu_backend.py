@@ -176,7 +176,8 @@ def test_eigh(B):
 def test_svd_random_shapes(B):
     """Single blocks of assorted shapes (LDS and workspace instances), real and complex."""
     rng = np.random.default_rng(9)
-    for (m, n) in ((1, 1), (1, 7), (7, 1), (2, 2), (17, 5), (5, 17), (64, 64), (90, 40), (33, 130), (150, 150)):
+    for (m, n) in ((1, 1), (1, 7), (7, 1), (2, 2), (17, 5), (5, 17), (64, 64), (90, 40), (33, 130), (150, 150),
+                   (300, 170), (170, 300), (440, 440)):
         for cplx in (False, True):
             a = rng.standard_normal((m, n)) + (1j * rng.standard_normal((m, n)) if cplx else 0)
             k = min(m, n)
@@ -318,3 +319,43 @@ def test_u1xu1_transfer_vs_oracle(B):
     n = W.u1xu1_transfer(W.u1xu1_transfer_setup(cn, seed=0))
     assert g.struct == n.struct
     assert _rel(_n(g.data), n.data) <= 1e-12
+
+
+def test_svd_panel_path_graded_and_batched(B):
+    """Blocks beyond the LDS kernel (panel Jacobi): a 12-decade graded spectrum and several blocks of
+    different sizes in one call."""
+    rng = np.random.default_rng(13)
+    n = 260
+    q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = np.logspace(0, -12, n)
+    a = (q1 * s) @ q2.T
+    meta = (((0, n * n), (n, n), (0, n * n), (n, n), (0, n), (0, n * n), (n, n)),)
+    U, S, V = B.svd(_t(a.ravel()), meta, (n * n, n, n * n))
+    np.testing.assert_allclose(_n(S), s, rtol=1e-8, atol=1e-13)
+    # batch of mixed blocks (LDS + panel) in one call
+    shapes = [(120, 90), (30, 30), (210, 240), (5, 9), (180, 180)]
+    data, meta, uo, so, vo, off = [], [], 0, 0, 0, 0
+    for m, k2 in shapes:
+        k = min(m, k2)
+        x = rng.standard_normal(m * k2)
+        data.append(x)
+        meta.append(((off, off + m * k2), (m, k2), (uo, uo + m * k), (m, k), (so, so + k), (vo, vo + k * k2), (k, k2)))
+        off, uo, so, vo = off + m * k2, uo + m * k, so + k, vo + k * k2
+    data = np.concatenate(data)
+    meta = tuple(meta)
+    U, S, V = B.svd(_t(data), meta, (uo, so, vo))
+    ref = np.concatenate([np.linalg.svd(data[m[0][0]:m[0][1]].reshape(m[1]), compute_uv=False) for m in meta])
+    _check_svd(data, meta, _n(U), _n(S), _n(V), ref)
+
+
+def test_eigh_panel_path(B):
+    rng = np.random.default_rng(14)
+    for n, cplx in ((200, False), (130, True)):
+        x = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)
+        a = (x + x.conj().T) / 2
+        meta = (((0, n * n), (n, n), (0, n * n), (n, n), (0, n)),)
+        S, U = B.eigh(_t(a.ravel()), meta, (n, n * n))
+        w, u = _n(S), _n(U).reshape(n, n)
+        np.testing.assert_allclose(w, np.linalg.eigvalsh(a), atol=1e-11 * np.abs(w).max())
+        assert np.linalg.norm(u * w @ u.conj().T - a) <= 1e-11 * np.linalg.norm(a)

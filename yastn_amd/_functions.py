@@ -282,18 +282,50 @@ def _check_info(info, what):
             raise L.HipBackendError(f"{what}: Jacobi did not converge in {MAX_SWEEPS} sweeps for {bad} block(s)")
 
 
+def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
+    """LDS-resident blocks in one launch; larger blocks through the panel path (one launch per round of
+    panel pairs, one host check per sweep).  mode 0: SVD (U, S, Vh); mode 1: eigh (S, U)."""
+    dt = dtype_code(data)
+    st = stream_of(data)
+    work = torch.empty(max(plan.work_numel, 1), dtype=data.dtype, device=data.device)
+    info = torch.empty(max(plan.nblocks, 1), dtype=torch.int32, device=data.device)
+    if plan.n_lds:
+        if mode == 0:
+            L.call('yastn_hip_svd', ptr(out_a), ptr(out_s), ptr(out_b), ptr(data), dt, plan.d_blocks.data_ptr(),
+                   plan.n_lds, plan.d_order.data_ptr(), plan.n_lds, plan.lds_bytes, ptr(work), ptr(info),
+                   MAX_SWEEPS, st)
+        else:
+            L.call('yastn_hip_eigh', ptr(out_s), ptr(out_a), ptr(data), dt, plan.d_blocks.data_ptr(), plan.n_lds,
+                   plan.d_order.data_ptr(), plan.n_lds, plan.lds_bytes, ptr(work), ptr(info), MAX_SWEEPS, st)
+    sweeps = 0
+    if plan.n_big:
+        mu = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
+        flags = torch.zeros(max(plan.njobs, 1), dtype=torch.int32, device=data.device)
+        L.call('yastn_hip_svd_panel_init', ptr(data), ptr(work), ptr(mu), dt, mode, plan.d_blocks.data_ptr(),
+               plan.d_big.data_ptr(), plan.n_big, st)
+        jobs0, flags0 = plan.d_jobs.data_ptr(), flags.data_ptr()
+        converged = False
+        while sweeps < MAX_SWEEPS:
+            for off, n in plan.rounds:
+                L.call('yastn_hip_svd_panel_round', ptr(work), dt, plan.d_blocks.data_ptr(), jobs0 + 12 * off, n,
+                       plan.panel_width, plan.panel_lds, flags0 + 4 * off, st)
+            sweeps += 1
+            if not bool(flags.any().item()):
+                converged = True
+                break
+        L.call('yastn_hip_svd_panel_finalize', ptr(out_a if mode == 0 else out_a), ptr(out_s),
+               ptr(out_b) if mode == 0 else None, ptr(work), ptr(mu), dt, mode, plan.d_blocks.data_ptr(),
+               plan.d_big.data_ptr(), plan.n_big, ptr(info), sweeps if converged else -1, st)
+    _check_info(info[:plan.nblocks], 'svd' if mode == 0 else 'eigh')
+
+
 def svd_forward(data, meta, sizes):
     data = raw(data)
     plan = P.svd_plan(meta, data.element_size(), data.numel(), sizes, data.device)
     U = torch.empty(sizes[0], dtype=data.dtype, device=data.device)
     S = torch.empty(sizes[1], dtype=_real_dtype(data), device=data.device)
     V = torch.empty(sizes[2], dtype=data.dtype, device=data.device)
-    work = torch.empty(max(plan.work_numel, 1), dtype=data.dtype, device=data.device)
-    info = torch.empty(max(plan.nblocks, 1), dtype=torch.int32, device=data.device)
-    L.call('yastn_hip_svd', ptr(U), ptr(S), ptr(V), ptr(data), dtype_code(data), plan.d_blocks.data_ptr(),
-           plan.nblocks, plan.d_order.data_ptr(), plan.n_lds, plan.lds_bytes, ptr(work), ptr(info), MAX_SWEEPS,
-           stream_of(data))
-    _check_info(info[:plan.nblocks], 'svd')
+    _run_jacobi(0, plan, data, U, S, V)
     return U, S, V
 
 
@@ -403,11 +435,7 @@ def eigh_forward(data, meta, sizes):
     plan = P.eigh_plan(meta, data.element_size(), data.numel(), sizes, data.device)
     S = torch.zeros(sizes[0], dtype=_real_dtype(data), device=data.device)
     U = torch.zeros(sizes[1], dtype=data.dtype, device=data.device)
-    work = torch.empty(max(plan.work_numel, 1), dtype=data.dtype, device=data.device)
-    info = torch.empty(max(plan.nblocks, 1), dtype=torch.int32, device=data.device)
-    L.call('yastn_hip_eigh', ptr(S), ptr(U), ptr(data), dtype_code(data), plan.d_blocks.data_ptr(), plan.nblocks,
-           plan.d_order.data_ptr(), plan.n_lds, plan.lds_bytes, ptr(work), ptr(info), MAX_SWEEPS, stream_of(data))
-    _check_info(info[:plan.nblocks], 'eigh')
+    _run_jacobi(1, plan, data, U, S, None)
     return S, U
 
 

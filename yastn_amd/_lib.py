@@ -12,9 +12,9 @@ LIB_PATH = os.environ.get('YASTN_HIP_LIB', os.path.join(HERE, 'libyastn_hip.so')
 # constants mirrored from include/yastn_hip.h
 YH_OK, YH_ERR_ARG, YH_ERR_HIP, YH_ERR_NUMERIC = 0, -1, -2, -3
 YH_F64, YH_C128 = 0, 1
-YH_ABI_VERSION = 3
-YH_MAXR = 6
-YH_JOB_WORDS = 24
+YH_ABI_VERSION = 4
+YH_MAXR = 8
+YH_JOB_WORDS = 32
 YH_GATHER_TILE = 2048
 YH_TRANSPOSE_TILE = 32
 YH_GEMM_WORDS = 8
@@ -52,6 +52,10 @@ _SIGS = {
     'yastn_hip_svd': [_p, _p, _p, _p, _i32, _p, _i64, _p, _i64, _i64, _p, _p, _i32, _p],
     'yastn_hip_eigh': [_p, _p, _p, _i32, _p, _i64, _p, _i64, _i64, _p, _p, _i32, _p],
     'yastn_hip_fix_svd_signs': [_p, _p, _i32, _p, _i64, _p],
+    'yastn_hip_device_info': [_p, _p],
+    'yastn_hip_svd_panel_init': [_p, _p, _p, _i32, _i32, _p, _p, _i64, _p],
+    'yastn_hip_svd_panel_round': [_p, _i32, _p, _p, _i64, _i32, _i64, _p, _p],
+    'yastn_hip_svd_panel_finalize': [_p, _p, _p, _p, _p, _i32, _i32, _p, _p, _i64, _p, _i32, _p],
     'yastn_hip_svd_sign_pivots': [_p, _p, _i32, _p, _i64, _p],
     'yastn_hip_svd_adjoint_core': [_p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _i64, _dbl, _i32, _i32, _i32, _p],
     'yastn_hip_eigh_adjoint_core': [_p, _p, _p, _p, _i32, _p, _i64, _i32, _i32, _p],
@@ -96,3 +100,22 @@ def call(name, *args):
         msg = L.yastn_hip_last_error().decode(errors='replace')
         raise HipBackendError(f"{name} failed (status {st}): {msg}")
     return st
+
+
+_DEVICE_LDS = None
+
+
+def device_lds_per_block():
+    """LDS bytes one workgroup may allocate on the current device (64 KiB when no GPU is visible)."""
+    global _DEVICE_LDS
+    if _DEVICE_LDS is None:
+        try:
+            import torch
+            if not torch.cuda.is_available():
+                return 65536
+        except ImportError:
+            return 65536
+        lds, cu = ctypes.c_int(0), ctypes.c_int(0)
+        call('yastn_hip_device_info', ctypes.byref(lds), ctypes.byref(cu))
+        _DEVICE_LDS = int(lds.value)
+    return _DEVICE_LDS

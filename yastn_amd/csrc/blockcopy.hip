@@ -44,6 +44,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kTE = YH_TRANSPOSE_TILE;
+constexpr int kShape = 6, kSrcStride = 6 + YH_MAXR, kDstStride = 6 + 2 * YH_MAXR;
 
 struct JobView {
   int64_t src_off, dst_off;
@@ -61,9 +62,9 @@ __device__ inline void load_job(const int64_t* __restrict__ w, JobView& j) {
   j.tile_begin = w[5];
 #pragma unroll
   for (int d = 0; d < YH_MAXR; ++d) {
-    j.shape[d] = w[6 + d];
-    j.ss[d] = w[12 + d];
-    j.ds[d] = w[18 + d];
+    j.shape[d] = w[kShape + d];
+    j.ss[d] = w[kSrcStride + d];
+    j.ds[d] = w[kDstStride + d];
   }
 }
 
@@ -165,15 +166,15 @@ extern "C" int yastn_hip_check_gather_plan(const int64_t* h_jobs, int64_t njobs,
     YH_CHECK_ARG(kind == 0 || kind == 1, "job %lld: kind %d", (long long)q, kind);
     YH_CHECK_ARG(w[5] == expect_tile, "job %lld: tile_begin %lld != %lld", (long long)q, (long long)w[5],
                  (long long)expect_tile);
-    YH_CHECK_ARG(w[18 + rank - 1] == 1, "job %lld: destination last stride must be 1", (long long)q);
+    YH_CHECK_ARG(w[kDstStride + rank - 1] == 1, "job %lld: destination last stride must be 1", (long long)q);
     int64_t numel = 1, smax = 0, dmax = 0;
     for (int d = 0; d < rank; ++d) {
-      const int64_t n = w[6 + d];
+      const int64_t n = w[kShape + d];
       YH_CHECK_ARG(n >= 1, "job %lld: shape[%d]=%lld", (long long)q, d, (long long)n);
-      YH_CHECK_ARG(w[12 + d] >= 0 && w[18 + d] >= 0, "job %lld: negative stride", (long long)q);
+      YH_CHECK_ARG(w[kSrcStride + d] >= 0 && w[kDstStride + d] >= 0, "job %lld: negative stride", (long long)q);
       numel *= n;
-      smax += (n - 1) * w[12 + d];
-      dmax += (n - 1) * w[18 + d];
+      smax += (n - 1) * w[kSrcStride + d];
+      dmax += (n - 1) * w[kDstStride + d];
     }
     YH_CHECK_ARG(dst_off >= 0 && dst_off + dmax < dst_numel, "job %lld: destination out of bounds", (long long)q);
     YH_CHECK_ARG(src_off == -1 || (src_off >= 0 && src_off + smax < src_numel),
@@ -182,10 +183,10 @@ extern "C" int yastn_hip_check_gather_plan(const int64_t* h_jobs, int64_t njobs,
     if (kind == 0) {
       tiles = yh::cdiv(numel, YH_GATHER_TILE);
     } else {
-      YH_CHECK_ARG(src_off >= 0 && tdim >= 0 && tdim < rank - 1 && w[12 + tdim] == 1,
+      YH_CHECK_ARG(src_off >= 0 && tdim >= 0 && tdim < rank - 1 && w[kSrcStride + tdim] == 1,
                    "job %lld: bad transpose dim", (long long)q);
-      tiles = yh::cdiv(w[6 + tdim], YH_TRANSPOSE_TILE) * yh::cdiv(w[6 + rank - 1], YH_TRANSPOSE_TILE) *
-              (numel / (w[6 + tdim] * w[6 + rank - 1]));
+      tiles = yh::cdiv(w[kShape + tdim], YH_TRANSPOSE_TILE) * yh::cdiv(w[kShape + rank - 1], YH_TRANSPOSE_TILE) *
+              (numel / (w[kShape + tdim] * w[kShape + rank - 1]));
     }
     for (int64_t t = expect_tile; t < expect_tile + tiles; ++t) {
       YH_CHECK_ARG(t < ntiles && h_tile_job[t] == q, "tile %lld does not belong to job %lld", (long long)t,

@@ -74,6 +74,52 @@ __device__ inline void pair_of(int qe, int r, int k, int& i, int& j) {
   }
 }
 
+// One Hestenes rotation of the column pair (w_i, w_j) (length p) and of the matching accumulator
+// columns (v_i, v_j) (length q), by one wave.  Returns true when the pair was rotated, i.e. when
+// |w_i^H w_j| > tol ||w_i|| ||w_j||.  J = [[c, s e^{i phi}], [-s e^{-i phi}, c]], e^{i phi} = g / |g|.
+template <bool CPLX>
+__device__ inline bool jacobi_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>::T* wj, int64_t p,
+                                   typename Num<CPLX>::T* vi_, typename Num<CPLX>::T* vj_, int64_t q, double tol,
+                                   int lane) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;
+  for (int64_t x = lane; x < p; x += 64) {
+    const T u = wi[x], v = wj[x];
+    a += N::abs2(u);
+    b += N::abs2(v);
+    gr += N::re(u) * N::re(v) + N::im(u) * N::im(v);  // conj(u) v
+    gi += N::re(u) * N::im(v) - N::im(u) * N::re(v);
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  gr = wave_sum(gr);
+  gi = wave_sum(gi);
+  const double g = sqrt(gr * gr + gi * gi);
+  if (!(g > tol * sqrt(a * b))) return false;  // also skips zero columns
+  const double zeta = (b - a) / (2.0 * g);
+  const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  const double c = 1.0 / sqrt(1.0 + t * t);
+  const double s = c * t;
+  const double er = gr / g, ei = gi / g;
+  // w_i' = c w_i - s e^{-i phi} w_j ; w_j' = s e^{i phi} w_i + c w_j
+  for (int64_t x = lane; x < p; x += 64) {
+    const T u = wi[x], v = wj[x];
+    const double vr = er * N::re(v) + ei * N::im(v), vi = er * N::im(v) - ei * N::re(v);
+    const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);
+    wi[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vi);
+    wj[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
+  }
+  for (int64_t x = lane; x < q; x += 64) {
+    const T u = vi_[x], v = vj_[x];
+    const double vr = er * N::re(v) + ei * N::im(v), vv = er * N::im(v) - ei * N::re(v);
+    const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);
+    vi_[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vv);
+    vj_[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
+  }
+  return true;
+}
+
 struct BlockDesc {
   int64_t a_off, m, n, u_off, s_off, v_off, work_off;
 };
@@ -88,6 +134,84 @@ __device__ inline BlockDesc load_desc(const int64_t* w) {
   d.v_off = w[5];
   d.work_off = w[6];
   return d;
+}
+
+// Column norms -> singular values (descending, ties by index), vectors written by whole waves.
+// sigma_j is staged in the block's S slice; the ranks are kept in LDS.
+template <bool CPLX, int MODE>
+__device__ void finalize_block(const BlockDesc& d, const typename Num<CPLX>::T* W, const typename Num<CPLX>::T* V,
+                               int64_t p, int64_t q, bool tr, double mu, typename Num<CPLX>::T* __restrict__ U,
+                               double* __restrict__ S, typename Num<CPLX>::T* __restrict__ Vh) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  __shared__ int s_rank_of[YH_SVD_MAX_Q];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // ---- finalize: sigma_j = ||w_j|| staged in the block's S slice, ranks by counting (descending,
+  // ties by index), then vectors written by whole waves, then S in sorted order.
+  for (int64_t j = wave; j < q; j += kWaves) {
+    const T* wj = W + j * p;
+    double a = 0.0;
+    for (int64_t x = lane; x < p; x += 64) a += N::abs2(wj[x]);
+    a = wave_sum(a);
+    if (lane == 0) S[d.s_off + j] = sqrt(a);
+  }
+  __syncthreads();
+  constexpr int kMaxCols = YH_SVD_MAX_Q / kThreads;
+  int rk[kMaxCols];
+  double sg[kMaxCols];
+#pragma unroll
+  for (int z = 0; z < kMaxCols; ++z) {
+    const int64_t j = tid + (int64_t)z * kThreads;
+    rk[z] = -1;
+    sg[z] = 0.0;
+    if (j < q) {
+      const double sj = S[d.s_off + j];
+      int cnt = 0;
+      for (int64_t i = 0; i < q; ++i) {
+        const double si = S[d.s_off + i];
+        cnt += (si > sj) || (si == sj && i < j);
+      }
+      rk[z] = cnt;
+      sg[z] = sj;
+      s_rank_of[j] = cnt;
+    }
+  }
+  __syncthreads();
+  for (int64_t j = wave; j < q; j += kWaves) {
+    const int64_t rkj = s_rank_of[j];
+    const double sj = S[d.s_off + j];
+    const T* wj = W + j * p;
+    const T* vj = V + j * q;
+    if (MODE == 1) {
+      const int64_t pos = q - 1 - rkj;
+      for (int64_t x = lane; x < q; x += 64) U[d.u_off + x * q + pos] = vj[x];
+    } else if (!tr) {
+      // A V = W: U (m x k) column rkj = w_j / sigma_j, Vh (k x n) row rkj = v_j^H
+      const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+      for (int64_t x = lane; x < p; x += 64) {
+        const T w = wj[x];
+        U[d.u_off + x * q + rkj] = N::make(N::re(w) * inv, N::im(w) * inv);
+      }
+      for (int64_t x = lane; x < q; x += 64) Vh[d.v_off + rkj * q + x] = N::conj(vj[x]);
+    } else {
+      // A^H V = W, so A = V Sigma (W Sigma^-1)^H: U (m x m) column rkj = v_j, Vh (m x n) row rkj = w_j^H / sigma_j
+      const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+      for (int64_t x = lane; x < q; x += 64) U[d.u_off + x * q + rkj] = vj[x];
+      for (int64_t x = lane; x < p; x += 64) {
+        const T w = wj[x];
+        Vh[d.v_off + rkj * p + x] = N::make(N::re(w) * inv, -N::im(w) * inv);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int z = 0; z < kMaxCols; ++z) {
+    if (rk[z] < 0) continue;
+    if (MODE == 0)
+      S[d.s_off + rk[z]] = sg[z];
+    else
+      S[d.s_off + (q - 1 - rk[z])] = sg[z] - mu;
+  }
 }
 
 // MODE 0: SVD (U, S, Vh).  MODE 1: Hermitian eigh (S ascending, U).
@@ -158,46 +282,9 @@ __global__ void __launch_bounds__(kThreads)
         int i, j;
         pair_of(qe, r, k, i, j);
         if (j >= q) continue;
-        T* wi = W + (int64_t)i * p;
-        T* wj = W + (int64_t)j * p;
-        double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;
-        for (int64_t x = lane; x < p; x += 64) {
-          const T u = wi[x], v = wj[x];
-          a += N::abs2(u);
-          b += N::abs2(v);
-          // conj(u) * v
-          gr += N::re(u) * N::re(v) + N::im(u) * N::im(v);
-          gi += N::re(u) * N::im(v) - N::im(u) * N::re(v);
-        }
-        a = wave_sum(a);
-        b = wave_sum(b);
-        gr = wave_sum(gr);
-        gi = wave_sum(gi);
-        const double g = sqrt(gr * gr + gi * gi);
-        if (!(g > tol * sqrt(a * b))) continue;  // also skips zero columns
-        // rotation J = [[c, s e^{i phi}], [-s e^{-i phi}, c]] with e^{i phi} = gamma / |gamma|
-        const double zeta = (b - a) / (2.0 * g);
-        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        const double c = 1.0 / sqrt(1.0 + t * t);
-        const double s = c * t;
-        const double er = gr / g, ei = gi / g;  // e^{i phi}
-        // w_i' = c w_i - s e^{-i phi} w_j ; w_j' = s e^{i phi} w_i + c w_j
-        for (int64_t x = lane; x < p; x += 64) {
-          const T u = wi[x], v = wj[x];
-          const double vr = er * N::re(v) + ei * N::im(v), vi = er * N::im(v) - ei * N::re(v);  // e^{-i phi} v
-          const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);  // e^{i phi} u
-          wi[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vi);
-          wj[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
-        }
-        T* vi_ = V + (int64_t)i * q;
-        T* vj_ = V + (int64_t)j * q;
-        for (int64_t x = lane; x < q; x += 64) {
-          const T u = vi_[x], v = vj_[x];
-          const double vr = er * N::re(v) + ei * N::im(v), vv = er * N::im(v) - ei * N::re(v);
-          const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);
-          vi_[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vv);
-          vj_[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
-        }
+        if (!jacobi_pair<CPLX>(W + (int64_t)i * p, W + (int64_t)j * p, p, V + (int64_t)i * q, V + (int64_t)j * q, q,
+                               tol, lane))
+          continue;
         if (lane == 0) s_rotated = 1;
       }
       __syncthreads();
@@ -207,74 +294,147 @@ __global__ void __launch_bounds__(kThreads)
     __syncthreads();
   }
 
-  // ---- finalize: sigma_j = ||w_j|| staged in the block's S slice, ranks by counting (descending,
-  // ties by index), then vectors written by whole waves, then S in sorted order.
-  __shared__ int s_rank_of[YH_SVD_MAX_Q];
-  for (int64_t j = wave; j < q; j += kWaves) {
-    const T* wj = W + j * p;
-    double a = 0.0;
-    for (int64_t x = lane; x < p; x += 64) a += N::abs2(wj[x]);
-    a = wave_sum(a);
-    if (lane == 0) S[d.s_off + j] = sqrt(a);
-  }
-  __syncthreads();
-  constexpr int kMaxCols = YH_SVD_MAX_Q / kThreads;
-  int rk[kMaxCols];
-  double sg[kMaxCols];
-#pragma unroll
-  for (int z = 0; z < kMaxCols; ++z) {
-    const int64_t j = tid + (int64_t)z * kThreads;
-    rk[z] = -1;
-    sg[z] = 0.0;
-    if (j < q) {
-      const double sj = S[d.s_off + j];
-      int cnt = 0;
-      for (int64_t i = 0; i < q; ++i) {
-        const double si = S[d.s_off + i];
-        cnt += (si > sj) || (si == sj && i < j);
-      }
-      rk[z] = cnt;
-      sg[z] = sj;
-      s_rank_of[j] = cnt;
-    }
-  }
-  __syncthreads();
-  for (int64_t j = wave; j < q; j += kWaves) {
-    const int64_t rkj = s_rank_of[j];
-    const double sj = S[d.s_off + j];
-    const T* wj = W + j * p;
-    const T* vj = V + j * q;
-    if (MODE == 1) {
-      const int64_t pos = q - 1 - rkj;
-      for (int64_t x = lane; x < q; x += 64) U[d.u_off + x * q + pos] = vj[x];
-    } else if (!tr) {
-      // A V = W: U (m x k) column rkj = w_j / sigma_j, Vh (k x n) row rkj = v_j^H
-      const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
-      for (int64_t x = lane; x < p; x += 64) {
-        const T w = wj[x];
-        U[d.u_off + x * q + rkj] = N::make(N::re(w) * inv, N::im(w) * inv);
-      }
-      for (int64_t x = lane; x < q; x += 64) Vh[d.v_off + rkj * q + x] = N::conj(vj[x]);
-    } else {
-      // A^H V = W, so A = V Sigma (W Sigma^-1)^H: U (m x m) column rkj = v_j, Vh (m x n) row rkj = w_j^H / sigma_j
-      const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
-      for (int64_t x = lane; x < q; x += 64) U[d.u_off + x * q + rkj] = vj[x];
-      for (int64_t x = lane; x < p; x += 64) {
-        const T w = wj[x];
-        Vh[d.v_off + rkj * p + x] = N::make(N::re(w) * inv, -N::im(w) * inv);
-      }
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int z = 0; z < kMaxCols; ++z) {
-    if (rk[z] < 0) continue;
-    if (MODE == 0)
-      S[d.s_off + rk[z]] = sg[z];
-    else
-      S[d.s_off + (q - 1 - rk[z])] = sg[z] - mu;
-  }
+  finalize_block<CPLX, MODE>(d, W, V, p, q, tr, mu, U, S, Vh);
   if (tid == 0) info[bidx] = converged ? sweeps : -1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Panel (block) Jacobi for blocks whose working matrices do not fit one CU.  The q columns are cut
+// into panels of width b; an outer sweep visits all panel pairs in the circle order, one launch per
+// round with one workgroup per (block, panel pair).  A workgroup stages the p x (2b) panel pair in
+// LDS, applies one inner sweep of Hestenes rotations to it (accumulating them in a 2b x 2b Q), writes
+// it back and updates the matching V columns, V <- V Q.  Pairs of different panels of the same
+// block in one round are disjoint, so all of them run concurrently.
+constexpr int kPanelThreads = 1024;
+
+template <bool CPLX, int MODE>
+__global__ void __launch_bounds__(kPanelThreads)
+    panel_init_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ ids,
+                      const typename Num<CPLX>::T* __restrict__ A, typename Num<CPLX>::T* __restrict__ work,
+                      double* __restrict__ mu_out) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  __shared__ double s_red[kPanelThreads / 64];
+  const int bidx = ids[blockIdx.x];
+  const BlockDesc d = load_desc(blocks + (int64_t)bidx * YH_SVD_WORDS);
+  const bool tr = (MODE == 0) && (d.m < d.n);
+  const int64_t p = tr ? d.n : d.m, q = tr ? d.m : d.n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  T* W = work + d.work_off;
+  T* V = W + p * q;
+  for (int64_t e = tid; e < p * q; e += kPanelThreads) {
+    const int64_t c = e / p, r = e - c * p;
+    W[e] = tr ? N::conj(A[d.a_off + c * d.n + r]) : A[d.a_off + r * d.n + c];
+  }
+  for (int64_t e = tid; e < q * q; e += kPanelThreads) {
+    const int64_t c = e / q, r = e - c * q;
+    V[e] = N::make(r == c ? 1.0 : 0.0, 0.0);
+  }
+  double mu = 0.0;
+  if (MODE == 1) {
+    __syncthreads();
+    double local = 0.0;
+    for (int64_t r = tid; r < p; r += kPanelThreads) {
+      double acc = 0.0;
+      for (int64_t c = 0; c < q; ++c) acc += sqrt(N::abs2(W[r + c * p]));
+      local = fmax(local, acc);
+    }
+    local = wave_max(local);
+    if (lane == 0) s_red[wave] = local;
+    __syncthreads();
+    mu = s_red[0];
+    for (int w = 1; w < kPanelThreads / 64; ++w) mu = fmax(mu, s_red[w]);
+    for (int64_t r = tid; r < p; r += kPanelThreads) {
+      const T x = W[r + r * p];
+      W[r + r * p] = N::make(N::re(x) + mu, N::im(x));
+    }
+  }
+  if (tid == 0) mu_out[bidx] = mu;
+}
+
+template <bool CPLX>
+__global__ void __launch_bounds__(kPanelThreads)
+    panel_round_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
+                       typename Num<CPLX>::T* __restrict__ work, int32_t* __restrict__ flags, int b) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  extern __shared__ double smem_raw[];
+  __shared__ int s_rot;
+  const int32_t* jb = jobs + 3 * (int64_t)blockIdx.x;
+  const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
+  const bool tr = d.m < d.n;
+  const int64_t p = tr ? d.n : d.m, q = tr ? d.m : d.n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t c0i = (int64_t)jb[1] * b, c0j = (int64_t)jb[2] * b;
+  const int64_t ci = min((int64_t)b, q - c0i), cj = min((int64_t)b, q - c0j);
+  const int nc = (int)(ci + cj);
+  const int nq = 2 * b;  // leading dimension of Q
+  T* W = work + d.work_off;
+  T* V = W + p * q;
+  T* Wp = reinterpret_cast<T*>(smem_raw);
+  T* Q = Wp + (int64_t)nq * p;
+  auto gcol = [&](int64_t l) { return l < ci ? c0i + l : c0j + (l - ci); };
+  for (int64_t e = tid; e < (int64_t)nc * p; e += kPanelThreads) {
+    const int64_t l = e / p, r = e - l * p;
+    Wp[e] = W[gcol(l) * p + r];
+  }
+  for (int e = tid; e < nc * nq; e += kPanelThreads) {
+    const int l = e / nq, k = e - l * nq;
+    Q[e] = N::make(l == k ? 1.0 : 0.0, 0.0);
+  }
+  if (tid == 0) s_rot = 0;
+  __syncthreads();
+  const double tol = DBL_EPSILON * sqrt((double)(p > 1 ? p : 1));
+  const int ne = nc + (nc & 1);
+  for (int r = 0; r < ne - 1; ++r) {
+    for (int k = wave; k < ne / 2; k += kPanelThreads / 64) {
+      int i, j;
+      pair_of(ne, r, k, i, j);
+      if (j >= nc) continue;
+      if (jacobi_pair<CPLX>(Wp + (int64_t)i * p, Wp + (int64_t)j * p, p, Q + i * nq, Q + j * nq, nc, tol, lane) &&
+          lane == 0)
+        s_rot = 1;
+    }
+    __syncthreads();
+  }
+  for (int64_t e = tid; e < (int64_t)nc * p; e += kPanelThreads) {
+    const int64_t l = e / p, r = e - l * p;
+    W[gcol(l) * p + r] = Wp[e];
+  }
+  __syncthreads();
+  // V panel columns (q x nc) staged where the W panel was (q <= p), then V <- V Q
+  T* Vp = Wp;
+  for (int64_t e = tid; e < (int64_t)nc * q; e += kPanelThreads) {
+    const int64_t l = e / q, r = e - l * q;
+    Vp[e] = V[gcol(l) * q + r];
+  }
+  __syncthreads();
+  for (int64_t e = tid; e < (int64_t)nc * q; e += kPanelThreads) {
+    const int64_t c = e / q, r = e - c * q;
+    double accr = 0.0, acci = 0.0;
+    for (int k = 0; k < nc; ++k) {
+      const T v = Vp[(int64_t)k * q + r], w = Q[c * nq + k];
+      accr += N::re(v) * N::re(w) - N::im(v) * N::im(w);
+      acci += N::re(v) * N::im(w) + N::im(v) * N::re(w);
+    }
+    V[gcol(c) * q + r] = N::make(accr, acci);
+  }
+  if (tid == 0) flags[blockIdx.x] = s_rot;
+}
+
+template <bool CPLX, int MODE>
+__global__ void __launch_bounds__(kPanelThreads)
+    panel_finalize_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ ids,
+                          const typename Num<CPLX>::T* __restrict__ work, const double* __restrict__ mu,
+                          typename Num<CPLX>::T* __restrict__ U, double* __restrict__ S,
+                          typename Num<CPLX>::T* __restrict__ Vh, int32_t* __restrict__ info, int sweeps) {
+  const int bidx = ids[blockIdx.x];
+  const BlockDesc d = load_desc(blocks + (int64_t)bidx * YH_SVD_WORDS);
+  const bool tr = (MODE == 0) && (d.m < d.n);
+  const int64_t p = tr ? d.n : d.m, q = tr ? d.m : d.n;
+  const typename Num<CPLX>::T* W = work + d.work_off;
+  finalize_block<CPLX, MODE>(d, W, W + p * q, p, q, tr, mu[bidx], U, S, Vh);
+  if (threadIdx.x == 0) info[bidx] = sweeps;
 }
 
 template <bool CPLX, int MODE>
@@ -366,4 +526,83 @@ extern "C" int yastn_hip_eigh(void* S, void* U, const void* A, int dtype, const 
                                                    work, d_info, max_sweeps, stream)
                          : launch_jacobi<true, 1>(d_blocks, d_order, n_lds, lds_bytes, nblocks, A, U, S, nullptr,
                                                   work, d_info, max_sweeps, stream);
+}
+
+extern "C" int yastn_hip_device_info(int* lds_per_block, int* num_cu) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(num_cu, hipDeviceAttributeMultiprocessorCount, dev);
+  return yh::hip_status(e, "device info");
+}
+
+extern "C" int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, int dtype, int mode,
+                                        const int64_t* d_blocks, const int32_t* d_ids, int64_t nb,
+                                        hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(mode == 0 || mode == 1, "mode %d", mode);
+  if (nb == 0) return YH_OK;
+  const dim3 g((unsigned)nb), t(kPanelThreads);
+  if (dtype == YH_F64) {
+    if (mode == 0)
+      hipLaunchKernelGGL((panel_init_kernel<false, 0>), g, t, 0, stream, d_blocks, d_ids, (const double*)A,
+                         (double*)work, mu);
+    else
+      hipLaunchKernelGGL((panel_init_kernel<false, 1>), g, t, 0, stream, d_blocks, d_ids, (const double*)A,
+                         (double*)work, mu);
+  } else {
+    if (mode == 0)
+      hipLaunchKernelGGL((panel_init_kernel<true, 0>), g, t, 0, stream, d_blocks, d_ids, (const double2*)A,
+                         (double2*)work, mu);
+    else
+      hipLaunchKernelGGL((panel_init_kernel<true, 1>), g, t, 0, stream, d_blocks, d_ids, (const double2*)A,
+                         (double2*)work, mu);
+  }
+  return yh::hip_status(hipGetLastError(), "svd panel init launch");
+}
+
+extern "C" int yastn_hip_svd_panel_round(void* work, int dtype, const int64_t* d_blocks, const int32_t* d_jobs,
+                                         int64_t njobs, int b, int64_t lds_bytes, int32_t* d_flags,
+                                         hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(b >= 1 && b <= 64 && lds_bytes > 0, "panel width %d / lds %lld", b, (long long)lds_bytes);
+  if (njobs == 0) return YH_OK;
+  const dim3 g((unsigned)njobs), t(kPanelThreads);
+  if (dtype == YH_F64) {
+    hipFuncSetAttribute((const void*)panel_round_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds_bytes);
+    hipLaunchKernelGGL(panel_round_kernel<false>, g, t, (size_t)lds_bytes, stream, d_blocks, d_jobs,
+                       (double*)work, d_flags, b);
+  } else {
+    hipFuncSetAttribute((const void*)panel_round_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds_bytes);
+    hipLaunchKernelGGL(panel_round_kernel<true>, g, t, (size_t)lds_bytes, stream, d_blocks, d_jobs,
+                       (double2*)work, d_flags, b);
+  }
+  return yh::hip_status(hipGetLastError(), "svd panel round launch");
+}
+
+extern "C" int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const void* work, const double* mu,
+                                            int dtype, int mode, const int64_t* d_blocks, const int32_t* d_ids,
+                                            int64_t nb, int32_t* d_info, int sweeps, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(mode == 0 || mode == 1, "mode %d", mode);
+  if (nb == 0) return YH_OK;
+  const dim3 g((unsigned)nb), t(kPanelThreads);
+  if (dtype == YH_F64) {
+    if (mode == 0)
+      hipLaunchKernelGGL((panel_finalize_kernel<false, 0>), g, t, 0, stream, d_blocks, d_ids, (const double*)work,
+                         mu, (double*)U, (double*)S, (double*)Vh, d_info, sweeps);
+    else
+      hipLaunchKernelGGL((panel_finalize_kernel<false, 1>), g, t, 0, stream, d_blocks, d_ids, (const double*)work,
+                         mu, (double*)U, (double*)S, (double*)Vh, d_info, sweeps);
+  } else {
+    if (mode == 0)
+      hipLaunchKernelGGL((panel_finalize_kernel<true, 0>), g, t, 0, stream, d_blocks, d_ids, (const double2*)work,
+                         mu, (double2*)U, (double*)S, (double2*)Vh, d_info, sweeps);
+    else
+      hipLaunchKernelGGL((panel_finalize_kernel<true, 1>), g, t, 0, stream, d_blocks, d_ids, (const double2*)work,
+                         mu, (double2*)U, (double*)S, (double2*)Vh, d_info, sweeps);
+  }
+  return yh::hip_status(hipGetLastError(), "svd panel finalize launch");
 }

@@ -115,10 +115,11 @@ class GatherPlan:
                 ntile = (-(-shape[tdim] // te)) * (-(-shape[-1] // te)) * (numel // (shape[tdim] * shape[-1]))
             w = np.zeros(L.YH_JOB_WORDS, dtype=np.int64)
             w[0], w[1], w[2], w[3], w[4], w[5] = src_off, dst_off, r, kind, tdim, len(tile_job)
-            w[6:6 + L.YH_MAXR] = 1
+            M = L.YH_MAXR
+            w[6:6 + M] = 1
             w[6:6 + r] = shape
-            w[12:12 + r] = ss
-            w[18:18 + r] = ds
+            w[6 + M:6 + M + r] = ss
+            w[6 + 2 * M:6 + 2 * M + r] = ds
             tile_job.extend([len(rows)] * ntile)
             rows.append(w)
         self.njobs = len(rows)
@@ -512,12 +513,26 @@ def gemm_list_plan(gemms, terms, in_numel, out_numel, device, key=None, refs=())
 
 # --------------------------------------------------------------------------- Jacobi SVD / eigh
 
+def _circle_pair(ne, r, k):
+    n1 = ne - 1
+    if k == 0:
+        i, j = r, n1
+    else:
+        i, j = (r + k) % n1, (r - k + n1) % n1
+    return (i, j) if i < j else (j, i)
+
+
 class SvdPlan:
-    """Block list of the batched Jacobi kernel: LDS-resident blocks first, then workspace blocks."""
+    """Block list of the batched Jacobi SVD / eigh.
+
+    Blocks whose working matrices (W and V, (p q + q^2) elements) fit YH_SVD_LDS_BYTES run in the
+    LDS-resident kernel, one workgroup per block, all in one launch.  Larger blocks use the panel
+    path: W and V in a workspace, panels of width b, one launch per round of panel pairs.
+    """
 
     def __init__(self, blocks, itemsize, mode, a_numel, u_numel, s_numel, v_numel, device):
         # blocks: list of (a_off, m, n, u_off, s_off, v_off)
-        words, lds_ids, glb_ids = [], [], []
+        words, small, big = [], [], []
         work = 0
         lds_bytes = 0
         for b, (a_off, m, n, u_off, s_off, v_off) in enumerate(blocks):
@@ -525,26 +540,60 @@ class SvdPlan:
             p, q = (n, m) if tr else (m, n)
             need = p * q + q * q
             if need * itemsize <= L.YH_SVD_LDS_BYTES:
-                lds_ids.append(b)
+                small.append(b)
                 lds_bytes = max(lds_bytes, need * itemsize)
                 w_off = 0
             else:
-                glb_ids.append(b)
+                big.append(b)
                 w_off = work
                 work += need
             words.append((a_off, m, n, u_off, s_off, v_off, w_off, 0))
         size = lambda b: blocks[b][1] * blocks[b][2] * min(blocks[b][1], blocks[b][2])
-        lds_ids.sort(key=lambda b: -size(b))
-        glb_ids.sort(key=lambda b: -size(b))
+        small.sort(key=lambda b: -size(b))
+        big.sort(key=lambda b: -size(b))
         self.h_blocks = np.ascontiguousarray(np.array(words, dtype=np.int64).reshape(-1, L.YH_SVD_WORDS))
-        self.h_order = np.ascontiguousarray(np.array(lds_ids + glb_ids, dtype=np.int32))
-        self.nblocks, self.n_lds, self.lds_bytes, self.work_numel = len(words), len(lds_ids), lds_bytes, work
+        self.h_order = np.ascontiguousarray(np.array(small + big, dtype=np.int32))
+        self.nblocks, self.n_lds, self.lds_bytes, self.work_numel = len(words), len(small), lds_bytes, work
         self.flops = sum(4 * m * n * min(m, n) + 8 * min(m, n) ** 3 for (_, m, n, *_r) in blocks)
         dt = L.YH_C128 if itemsize == 16 else L.YH_F64
         L.call('yastn_hip_check_svd_plan', self.h_blocks.ctypes.data, self.nblocks, self.h_order.ctypes.data,
                self.n_lds, self.lds_bytes, mode, dt, a_numel, u_numel, s_numel, v_numel, max(work, 0))
         self.d_blocks = torch.from_numpy(self.h_blocks).to(device)
         self.d_order = torch.from_numpy(self.h_order).to(device)
+        # panel path
+        self.n_big = len(big)
+        self.rounds = []
+        if big:
+            limit = L.device_lds_per_block() - 2048
+            pmax = max(max(blocks[b][1], blocks[b][2]) for b in big)
+            width = 16
+            while width > 1 and (2 * width * pmax + 4 * width * width) * itemsize > limit:
+                width //= 2
+            self.panel_width = width
+            self.panel_lds = (2 * width * pmax + 4 * width * width) * itemsize
+            if self.panel_lds > limit:
+                raise ValueError(f"block with p={pmax} does not fit the LDS panel path")
+            per_round = {}
+            for b in big:
+                m, n = blocks[b][1], blocks[b][2]
+                q = min(m, n) if mode == 0 else n
+                npan = -(-q // width)
+                ne = npan + (npan & 1)
+                for r in range(ne - 1):
+                    for k in range(ne // 2):
+                        i, j = _circle_pair(ne, r, k)
+                        if j < npan:
+                            per_round.setdefault(r, []).append((b, i, j))
+            jobs, offs = [], []
+            for r in sorted(per_round):
+                offs.append((len(jobs), len(per_round[r])))
+                jobs.extend(per_round[r])
+            self.rounds = offs
+            self.h_jobs = np.ascontiguousarray(np.array(jobs, dtype=np.int32).reshape(-1, 3))
+            self.d_jobs = torch.from_numpy(self.h_jobs).to(device)
+            self.h_big = np.ascontiguousarray(np.array(big, dtype=np.int32))
+            self.d_big = torch.from_numpy(self.h_big).to(device)
+            self.njobs = len(jobs)
 
 
 def svd_plan(meta, itemsize, a_numel, sizes, device):
