@@ -162,12 +162,15 @@ int yastn_hip_eigh(void* S, void* U, const void* A, int dtype, const int64_t* d_
  * (block, panel i, panel j) int32 triples, one workgroup each, stages the panel pair (2b columns) in
  * LDS, applies one inner sweep of rotations and updates V; flags[job] = 1 if it rotated (the host
  * stops when a whole sweep rotated nothing); finalize writes U, S, Vh (or S, U) like yastn_hip_svd.
+ * Columns whose squared norm is below (eps ||A||_F)^2 (floors[b], set by init) are not rotated, here
+ * and in yastn_hip_svd: they are at the rounding-noise level of the block.
  * lds_bytes = (2b p + 4 b^2) * itemsize for the largest p of the round. */
 int yastn_hip_device_info(int* lds_per_block, int* num_cu);
-int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, int dtype, int mode, const int64_t* d_blocks,
-                             const int32_t* d_ids, int64_t nb, hipStream_t stream);
-int yastn_hip_svd_panel_round(void* work, int dtype, const int64_t* d_blocks, const int32_t* d_jobs, int64_t njobs,
-                              int b, int64_t lds_bytes, int32_t* d_flags, hipStream_t stream);
+int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, double* floors, int dtype, int mode,
+                             const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, hipStream_t stream);
+int yastn_hip_svd_panel_round(void* work, const double* floors, int dtype, const int64_t* d_blocks,
+                              const int32_t* d_jobs, int64_t njobs, int b, int64_t lds_bytes, int32_t* d_flags,
+                              hipStream_t stream);
 int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const void* work, const double* mu, int dtype, int mode,
                                  const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, int32_t* d_info,
                                  int sweeps, hipStream_t stream);

@@ -12,6 +12,13 @@ from . import _lib as L
 from . import plans as P
 
 SYNC_CHECKS = os.environ.get('YASTN_HIP_SYNC_CHECKS', '1') != '0'
+# empirical tile autotuning of every new grouped-GEMM plan (skipped while a HIP graph is captured)
+AUTOTUNE = os.environ.get('YASTN_HIP_AUTOTUNE', '1') != '0'
+
+
+def _maybe_autotune(plan, run):
+    if AUTOTUNE and not getattr(plan, 'tuned', False) and not torch.cuda.is_current_stream_capturing():
+        plan.autotune(run)
 MAX_SWEEPS = int(os.environ.get('YASTN_HIP_MAX_SWEEPS', '60'))
 # Lorentzian broadening of the SVD adjoint: the reference fixes it to 1e-12 (torch_svd_gesdd.py:4-6,
 # absolute, whatever ad_decomp_reg says).  set_ad_broadening(eps) changes it, e.g. to 1e-30 for an
@@ -226,13 +233,16 @@ class EmbedMask(torch.autograd.Function):
 
 # ------------------------------------------------------------------ grouped GEMM
 
-def run_gemm(plan, outs, ins, dtype):
+def run_gemm(plan, outs, ins, dtype, tune=True):
     outs = list(outs) + [None] * (L.YH_MAX_BUFS - len(outs))
     ins = list(ins) + [None] * (L.YH_MAX_BUFS - len(ins))
     po = (L.ctypes.c_void_p * L.YH_MAX_BUFS)(*[ptr(t) for t in outs])
     pi = (L.ctypes.c_void_p * L.YH_MAX_BUFS)(*[ptr(t) for t in ins])
     dev = next(t for t in outs if t is not None)
-    L.call('yastn_hip_grouped_gemm', po, pi, dtype, *plan.args(), stream_of(dev))
+    run = lambda: L.call('yastn_hip_grouped_gemm', po, pi, dtype, *plan.args(), stream_of(dev))
+    if tune and not plan.accumulates:  # re-running an accumulating plan would change its output
+        _maybe_autotune(plan, run)
+    run()
 
 
 class Dot(torch.autograd.Function):
@@ -245,7 +255,10 @@ class Dot(torch.autograd.Function):
         B = raw(Bdata.to(dtype))
         plan = P.dot_plan(meta_dot, A.numel(), B.numel(), Dsize, A.device)
         out = (torch.empty if plan.covered == Dsize else torch.zeros)(Dsize, dtype=dtype, device=A.device)
-        L.call('yastn_hip_dot', ptr(out), ptr(A), ptr(B), dtype_code(out), *plan.args(), stream_of(out))
+        run = lambda: L.call('yastn_hip_dot', ptr(out), ptr(A), ptr(B), dtype_code(out), *plan.args(),
+                             stream_of(out))
+        _maybe_autotune(plan, run)
+        run()
         return out
 
     @staticmethod
@@ -264,8 +277,10 @@ class Dot(torch.autograd.Function):
         plan = P.dot_adjoint_plan(ctx.meta_dot, A.numel(), B.numel(), gC.numel(), A.device)
         gA = (torch.zeros if plan.zero_a else torch.empty)(A.numel(), dtype=dtype, device=A.device)
         gB = (torch.zeros if plan.zero_b else torch.empty)(B.numel(), dtype=dtype, device=A.device)
-        L.call('yastn_hip_dot_adjoint', ptr(gA), ptr(gB), ptr(gC), ptr(A), ptr(B), dtype_code(gA), *plan.args(),
-               stream_of(gA))
+        run = lambda: L.call('yastn_hip_dot_adjoint', ptr(gA), ptr(gB), ptr(gC), ptr(A), ptr(B), dtype_code(gA),
+                             *plan.args(), stream_of(gA))
+        _maybe_autotune(plan, run)
+        run()
         if not Adata.is_complex() and gA.is_complex():
             gA = gA.real.contiguous()
         if not Bdata.is_complex() and gB.is_complex():
@@ -300,15 +315,16 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
     sweeps = 0
     if plan.n_big:
         mu = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
+        floors = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
         flags = torch.zeros(max(plan.njobs, 1), dtype=torch.int32, device=data.device)
-        L.call('yastn_hip_svd_panel_init', ptr(data), ptr(work), ptr(mu), dt, mode, plan.d_blocks.data_ptr(),
-               plan.d_big.data_ptr(), plan.n_big, st)
+        L.call('yastn_hip_svd_panel_init', ptr(data), ptr(work), ptr(mu), ptr(floors), dt, mode,
+               plan.d_blocks.data_ptr(), plan.d_big.data_ptr(), plan.n_big, st)
         jobs0, flags0 = plan.d_jobs.data_ptr(), flags.data_ptr()
         converged = False
         while sweeps < MAX_SWEEPS:
             for off, n in plan.rounds:
-                L.call('yastn_hip_svd_panel_round', ptr(work), dt, plan.d_blocks.data_ptr(), jobs0 + 12 * off, n,
-                       plan.panel_width, plan.panel_lds, flags0 + 4 * off, st)
+                L.call('yastn_hip_svd_panel_round', ptr(work), ptr(floors), dt, plan.d_blocks.data_ptr(),
+                       jobs0 + 12 * off, n, plan.panel_width, plan.panel_lds, flags0 + 4 * off, st)
             sweeps += 1
             if not bool(flags.any().item()):
                 converged = True

@@ -53,8 +53,8 @@ _SIGS = {
     'yastn_hip_eigh': [_p, _p, _p, _i32, _p, _i64, _p, _i64, _i64, _p, _p, _i32, _p],
     'yastn_hip_fix_svd_signs': [_p, _p, _i32, _p, _i64, _p],
     'yastn_hip_device_info': [_p, _p],
-    'yastn_hip_svd_panel_init': [_p, _p, _p, _i32, _i32, _p, _p, _i64, _p],
-    'yastn_hip_svd_panel_round': [_p, _i32, _p, _p, _i64, _i32, _i64, _p, _p],
+    'yastn_hip_svd_panel_init': [_p, _p, _p, _p, _i32, _i32, _p, _p, _i64, _p],
+    'yastn_hip_svd_panel_round': [_p, _p, _i32, _p, _p, _i64, _i32, _i64, _p, _p],
     'yastn_hip_svd_panel_finalize': [_p, _p, _p, _p, _p, _i32, _i32, _p, _p, _i64, _p, _i32, _p],
     'yastn_hip_svd_sign_pivots': [_p, _p, _i32, _p, _i64, _p],
     'yastn_hip_svd_adjoint_core': [_p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _i64, _dbl, _i32, _i32, _i32, _p],

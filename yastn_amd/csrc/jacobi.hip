@@ -76,11 +76,11 @@ __device__ inline void pair_of(int qe, int r, int k, int& i, int& j) {
 
 // One Hestenes rotation of the column pair (w_i, w_j) (length p) and of the matching accumulator
 // columns (v_i, v_j) (length q), by one wave.  Returns true when the pair was rotated, i.e. when
-// |w_i^H w_j| > tol ||w_i|| ||w_j||.  J = [[c, s e^{i phi}], [-s e^{-i phi}, c]], e^{i phi} = g / |g|.
+// |w_i^H w_j| > tol ||w_i|| ||w_j|| and both squared norms exceed floor2.  J = [[c, s e^{i phi}], [-s e^{-i phi}, c]], e^{i phi} = g / |g|.
 template <bool CPLX>
 __device__ inline bool jacobi_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>::T* wj, int64_t p,
                                    typename Num<CPLX>::T* vi_, typename Num<CPLX>::T* vj_, int64_t q, double tol,
-                                   int lane) {
+                                   double floor2, int lane) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
   double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;
@@ -96,7 +96,9 @@ __device__ inline bool jacobi_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>
   gr = wave_sum(gr);
   gi = wave_sum(gi);
   const double g = sqrt(gr * gr + gi * gi);
-  if (!(g > tol * sqrt(a * b))) return false;  // also skips zero columns
+  // columns at the noise level (||w||^2 <= (eps ||A||_F)^2) carry no information beyond absolute
+  // accuracy eps ||A||; rotating them against rounding noise would never converge
+  if (!(g > tol * sqrt(a * b)) || a <= floor2 || b <= floor2) return false;
   const double zeta = (b - a) / (2.0 * g);
   const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
   const double c = 1.0 / sqrt(1.0 + t * t);
@@ -271,6 +273,18 @@ __global__ void __launch_bounds__(kThreads)
   }
 
   const double tol = DBL_EPSILON * sqrt((double)(p > 1 ? p : 1));
+  double floor2;
+  {
+    double local = 0.0;
+    for (int64_t e = tid; e < p * q; e += kThreads) local += N::abs2(W[e]);
+    local = wave_sum(local);
+    __syncthreads();
+    if (lane == 0) s_red[wave] = local;
+    __syncthreads();
+    double fro2 = 0.0;
+    for (int w = 0; w < kWaves; ++w) fro2 += s_red[w];
+    floor2 = DBL_EPSILON * DBL_EPSILON * fro2;
+  }
   const int qe = (int)(q + (q & 1));
   int sweeps = 0;
   bool converged = (q <= 1);
@@ -283,7 +297,7 @@ __global__ void __launch_bounds__(kThreads)
         pair_of(qe, r, k, i, j);
         if (j >= q) continue;
         if (!jacobi_pair<CPLX>(W + (int64_t)i * p, W + (int64_t)j * p, p, V + (int64_t)i * q, V + (int64_t)j * q, q,
-                               tol, lane))
+                               tol, floor2, lane))
           continue;
         if (lane == 0) s_rotated = 1;
       }
@@ -311,7 +325,7 @@ template <bool CPLX, int MODE>
 __global__ void __launch_bounds__(kPanelThreads)
     panel_init_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ ids,
                       const typename Num<CPLX>::T* __restrict__ A, typename Num<CPLX>::T* __restrict__ work,
-                      double* __restrict__ mu_out) {
+                      double* __restrict__ mu_out, double* __restrict__ floor_out) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
   __shared__ double s_red[kPanelThreads / 64];
@@ -349,13 +363,26 @@ __global__ void __launch_bounds__(kPanelThreads)
       W[r + r * p] = N::make(N::re(x) + mu, N::im(x));
     }
   }
-  if (tid == 0) mu_out[bidx] = mu;
+  __syncthreads();
+  double local = 0.0;
+  for (int64_t e = tid; e < p * q; e += kPanelThreads) local += N::abs2(W[e]);
+  local = wave_sum(local);
+  __syncthreads();
+  if (lane == 0) s_red[wave] = local;
+  __syncthreads();
+  if (tid == 0) {
+    double fro2 = 0.0;
+    for (int w = 0; w < kPanelThreads / 64; ++w) fro2 += s_red[w];
+    mu_out[bidx] = mu;
+    floor_out[bidx] = DBL_EPSILON * DBL_EPSILON * fro2;
+  }
 }
 
 template <bool CPLX>
 __global__ void __launch_bounds__(kPanelThreads)
     panel_round_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
-                       typename Num<CPLX>::T* __restrict__ work, int32_t* __restrict__ flags, int b) {
+                       typename Num<CPLX>::T* __restrict__ work, const double* __restrict__ floors,
+                       int32_t* __restrict__ flags, int b) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
   extern __shared__ double smem_raw[];
@@ -385,13 +412,15 @@ __global__ void __launch_bounds__(kPanelThreads)
   if (tid == 0) s_rot = 0;
   __syncthreads();
   const double tol = DBL_EPSILON * sqrt((double)(p > 1 ? p : 1));
+  const double floor2 = floors[jb[0]];
   const int ne = nc + (nc & 1);
   for (int r = 0; r < ne - 1; ++r) {
     for (int k = wave; k < ne / 2; k += kPanelThreads / 64) {
       int i, j;
       pair_of(ne, r, k, i, j);
       if (j >= nc) continue;
-      if (jacobi_pair<CPLX>(Wp + (int64_t)i * p, Wp + (int64_t)j * p, p, Q + i * nq, Q + j * nq, nc, tol, lane) &&
+      if (jacobi_pair<CPLX>(Wp + (int64_t)i * p, Wp + (int64_t)j * p, p, Q + i * nq, Q + j * nq, nc, tol, floor2,
+                            lane) &&
           lane == 0)
         s_rot = 1;
     }
@@ -536,7 +565,7 @@ extern "C" int yastn_hip_device_info(int* lds_per_block, int* num_cu) {
   return yh::hip_status(e, "device info");
 }
 
-extern "C" int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, int dtype, int mode,
+extern "C" int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, double* floors, int dtype, int mode,
                                         const int64_t* d_blocks, const int32_t* d_ids, int64_t nb,
                                         hipStream_t stream) {
   YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
@@ -546,24 +575,24 @@ extern "C" int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, i
   if (dtype == YH_F64) {
     if (mode == 0)
       hipLaunchKernelGGL((panel_init_kernel<false, 0>), g, t, 0, stream, d_blocks, d_ids, (const double*)A,
-                         (double*)work, mu);
+                         (double*)work, mu, floors);
     else
       hipLaunchKernelGGL((panel_init_kernel<false, 1>), g, t, 0, stream, d_blocks, d_ids, (const double*)A,
-                         (double*)work, mu);
+                         (double*)work, mu, floors);
   } else {
     if (mode == 0)
       hipLaunchKernelGGL((panel_init_kernel<true, 0>), g, t, 0, stream, d_blocks, d_ids, (const double2*)A,
-                         (double2*)work, mu);
+                         (double2*)work, mu, floors);
     else
       hipLaunchKernelGGL((panel_init_kernel<true, 1>), g, t, 0, stream, d_blocks, d_ids, (const double2*)A,
-                         (double2*)work, mu);
+                         (double2*)work, mu, floors);
   }
   return yh::hip_status(hipGetLastError(), "svd panel init launch");
 }
 
-extern "C" int yastn_hip_svd_panel_round(void* work, int dtype, const int64_t* d_blocks, const int32_t* d_jobs,
-                                         int64_t njobs, int b, int64_t lds_bytes, int32_t* d_flags,
-                                         hipStream_t stream) {
+extern "C" int yastn_hip_svd_panel_round(void* work, const double* floors, int dtype, const int64_t* d_blocks,
+                                         const int32_t* d_jobs, int64_t njobs, int b, int64_t lds_bytes,
+                                         int32_t* d_flags, hipStream_t stream) {
   YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
   YH_CHECK_ARG(b >= 1 && b <= 64 && lds_bytes > 0, "panel width %d / lds %lld", b, (long long)lds_bytes);
   if (njobs == 0) return YH_OK;
@@ -572,12 +601,12 @@ extern "C" int yastn_hip_svd_panel_round(void* work, int dtype, const int64_t* d
     hipFuncSetAttribute((const void*)panel_round_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds_bytes);
     hipLaunchKernelGGL(panel_round_kernel<false>, g, t, (size_t)lds_bytes, stream, d_blocks, d_jobs,
-                       (double*)work, d_flags, b);
+                       (double*)work, floors, d_flags, b);
   } else {
     hipFuncSetAttribute((const void*)panel_round_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds_bytes);
     hipLaunchKernelGGL(panel_round_kernel<true>, g, t, (size_t)lds_bytes, stream, d_blocks, d_jobs,
-                       (double2*)work, d_flags, b);
+                       (double2*)work, floors, d_flags, b);
   }
   return yh::hip_status(hipGetLastError(), "svd panel round launch");
 }

@@ -394,6 +394,7 @@ class GemmPlan:
         self.k_of = k_of
         self.macs = int(np.sum(self.h_gemms[:, 0] * self.h_gemms[:, 1] * k_of)) if self.ngemms else 0
         self.covered = int(np.sum(self.h_gemms[:, 0] * self.h_gemms[:, 1])) if self.ngemms else 0
+        self.accumulates = bool(np.any(self.h_gemms[:, 7] & 1)) if self.ngemms else False
         self.device = device
         self._tiles = {}
         self.cfg = choose_tile(self) if cfg is None else cfg
@@ -427,32 +428,63 @@ class GemmPlan:
         return (self.d_gemms.data_ptr(), self.ngemms, self.d_terms.data_ptr(), self.nterms,
                 self.d_tiles.data_ptr(), self.ntiles, self.cfg)
 
+    def histogram_key(self):
+        return tuple(sorted(zip(self.h_gemms[:, 0].tolist(), self.h_gemms[:, 1].tolist(), self.k_of.tolist())))
 
+    def autotune(self, run, reps=3):
+        """Pick the fastest tile shape for this plan by timing ``run()`` (one launch of the plan) with
+        every shape; the choice is remembered per block-size histogram, so plans of other operands
+        with the same sector shapes reuse it."""
+        if getattr(self, 'tuned', False):
+            return
+        key = self.histogram_key()
+        if key in _TUNED:
+            self.set_cfg(_TUNED[key])
+            self.tuned = True
+            return
+        times = {}
+        for cfg in L.TILE_DIMS:
+            self.set_cfg(cfg)
+            run()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                run()
+            e1.record()
+            e1.synchronize()
+            times[cfg] = e0.elapsed_time(e1) / reps
+        best = min(times, key=times.get)
+        self.set_cfg(best)
+        self.tune_times = times
+        self.tuned = True
+        _TUNED[key] = best
+
+
+_TUNED = {}
 NUM_CU = 256
 
 
 def choose_tile(plan):
     """Tile choice from the plan's block-size histogram.
 
-    Estimated makespan = padded MACs / (relative tile efficiency x usable parallelism), where the
-    parallelism is the tile count capped by the CUs times the tiles resident per CU.  Larger tiles
-    reuse operands better (efficiency) but pad small sectors and expose fewer tiles.
+    Estimated makespan = max(longest tile, padded MACs / CUs) / relative tile efficiency: tiles of one
+    launch spread over the CUs, and a larger tile reuses operands better (higher efficiency) but pads
+    small sectors and exposes fewer tiles.  Skinny tiles serve sectors with M or N <= 16.
     """
     if plan.ngemms == 0:
         return L.YH_TILE_64x64
     M = plan.h_gemms[:, 0].astype(np.float64)
     N = plan.h_gemms[:, 1].astype(np.float64)
     K = np.maximum(plan.k_of.astype(np.float64), 1.0)
-    eff = {L.YH_TILE_32x32: 0.45, L.YH_TILE_64x64: 0.8, L.YH_TILE_128x64: 1.0, L.YH_TILE_16x64: 0.4,
-           L.YH_TILE_64x16: 0.4}
-    resident = {L.YH_TILE_32x32: 4, L.YH_TILE_64x64: 2, L.YH_TILE_128x64: 1, L.YH_TILE_16x64: 4,
-                L.YH_TILE_64x16: 4}
+    eff = {L.YH_TILE_32x32: 0.35, L.YH_TILE_64x64: 0.8, L.YH_TILE_128x64: 1.0, L.YH_TILE_16x64: 0.3,
+           L.YH_TILE_64x16: 0.3}
     best, best_cost = None, None
     for cfg, (bm, bn) in L.TILE_DIMS.items():
         tiles = np.ceil(M / bm) * np.ceil(N / bn)
-        padded = float(np.sum(tiles * bm * bn * K))
-        par = min(float(np.sum(tiles)), NUM_CU * resident[cfg])
-        cost = padded / eff[cfg] / par
+        work = bm * bn * K
+        padded = float(np.sum(tiles * work))
+        cost = max(float(work.max()), padded / NUM_CU) / eff[cfg]
         if best_cost is None or cost < best_cost * 0.999:
             best, best_cost = cfg, cost
     return best
