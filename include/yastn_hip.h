@@ -35,7 +35,7 @@ extern "C" {
 #define YH_F64 0
 #define YH_C128 1
 
-#define YH_ABI_VERSION 4
+#define YH_ABI_VERSION 5
 
 /* ---- box-gather plan (one job per copied or zero-filled box) -------------
  * int64 words per job (YH_JOB_WORDS):
@@ -109,6 +109,7 @@ int yastn_hip_unmerge_adjoint(void* grad_src, const void* grad_new, int64_t nume
 #define YH_TILE_128x64 2
 #define YH_TILE_16x64 3   /* skinny M (e.g. MPO contractions) */
 #define YH_TILE_64x16 4   /* skinny N */
+#define YH_TILE_128x128 5 /* large sectors */
 
 int yastn_hip_check_gemm_plan(const int64_t* h_gemms, int64_t ngemms, const int64_t* h_terms, int64_t nterms,
                               const int32_t* h_tiles, int64_t ntiles, int tile_config,
@@ -139,7 +140,7 @@ int yastn_hip_dot_adjoint(void* gA, void* gB, const void* gC, const void* A, con
  */
 #define YH_SVD_WORDS 8
 #define YH_SVD_MAX_Q 2048
-#define YH_SVD_LDS_BYTES 49152
+#define YH_SVD_LDS_BYTES 131072
 
 int yastn_hip_check_svd_plan(const int64_t* h_blocks, int64_t nblocks, const int32_t* h_order, int64_t n_lds,
                              int64_t lds_bytes, int mode, int dtype, int64_t a_numel, int64_t u_numel,

@@ -349,6 +349,29 @@ def test_svd_panel_path_graded_and_batched(B):
     _check_svd(data, meta, _n(U), _n(S), _n(V), ref)
 
 
+def test_svd_rank_deficient_and_zero_blocks(B):
+    """Blocks of rank r < min(m, n) and all-zero blocks, LDS and panel sizes, both orientations: the
+    singular vectors of the (numerically) zero singular values are still orthonormal, as LAPACK's are."""
+    rng = np.random.default_rng(15)
+    for cplx in (False, True):
+        cases = [(7, 15, 3), (15, 7, 2), (40, 40, 10), (12, 12, 0), (9, 4, 0), (200, 230, 60), (230, 200, 1),
+                 (180, 180, 0)]
+        data, meta, uo, so, vo, off = [], [], 0, 0, 0, 0
+        for m, n, r in cases:
+            k = min(m, n)
+            x = rng.standard_normal((m, r)) + (1j * rng.standard_normal((m, r)) if cplx else 0)
+            y = rng.standard_normal((r, n)) + (1j * rng.standard_normal((r, n)) if cplx else 0)
+            data.append((x @ y).ravel())
+            meta.append(((off, off + m * n), (m, n), (uo, uo + m * k), (m, k), (so, so + k), (vo, vo + k * n), (k, n)))
+            off, uo, so, vo = off + m * n, uo + m * k, so + k, vo + k * n
+        data = np.concatenate(data)
+        meta = tuple(meta)
+        print('svd rank deficient', cplx, cases, flush=True)
+        U, S, V = B.svd(_t(data), meta, (uo, so, vo))
+        ref = np.concatenate([np.linalg.svd(data[m[0][0]:m[0][1]].reshape(m[1]), compute_uv=False) for m in meta])
+        _check_svd(data, meta, _n(U), _n(S), _n(V), ref)
+
+
 def test_eigh_panel_path(B):
     rng = np.random.default_rng(14)
     for n, cplx in ((200, False), (130, True)):

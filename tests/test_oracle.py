@@ -277,3 +277,28 @@ def test_oracle_eigh_adjoint_fd():
     # the loss symmetrises its argument, so compare the symmetric part
     fd = _num_grad(loss, a)
     np.testing.assert_allclose((g + g.T) / 2, (fd + fd.T) / 2, rtol=1e-6, atol=1e-6)
+
+
+def test_mask_adjoint_and_embed_mask():
+    """apply_mask's adjoint (zero-filled scatter) against the adjoint identity, and embed_mask against
+    its closed form: embedding the selected entries back reproduces the masked tensor with zeros in the
+    dropped rows, and apply_mask of the embedding is the identity."""
+    rng = np.random.default_rng(3)
+    n = 0
+    for fn, args, kw, outs in G.calls('truncation_u1', 'apply_mask'):
+        A, mask, meta, Dsize, axis, ndim = args
+        x = rng.standard_normal(A.size)
+        y = rng.standard_normal(Dsize)
+        mx = O.apply_mask(x, mask, meta, Dsize, axis, ndim)
+        aty = O.apply_mask_adjoint(y, mask, meta, A.size, axis, ndim)
+        assert abs(mx @ y - x @ aty) <= 1e-12 * np.linalg.norm(x) * np.linalg.norm(y)
+        # embed_mask maps the small blocks (sln side) into zero-filled large ones (sla side)
+        emb_meta = tuple((sla, Da, sln, Dn, tm) for sln, Dn, sla, Da, tm in meta)
+        e = O.embed_mask(y, mask, emb_meta, A.size, axis, ndim)
+        np.testing.assert_array_equal(e, O.apply_mask_adjoint(y, mask, meta, A.size, axis, ndim))
+        np.testing.assert_array_equal(O.apply_mask(e, mask, meta, Dsize, axis, ndim), y)
+        kept = np.zeros(A.size, dtype=bool)
+        kept[O.apply_mask_adjoint(np.arange(1, Dsize + 1, dtype=float), mask, meta, A.size, axis, ndim) > 0] = True
+        assert kept.sum() == Dsize and np.all(e[~kept] == 0)
+        n += 1
+    assert n > 0

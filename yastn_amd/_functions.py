@@ -240,7 +240,13 @@ def run_gemm(plan, outs, ins, dtype, tune=True):
     pi = (L.ctypes.c_void_p * L.YH_MAX_BUFS)(*[ptr(t) for t in ins])
     dev = next(t for t in outs if t is not None)
     run = lambda: L.call('yastn_hip_grouped_gemm', po, pi, dtype, *plan.args(), stream_of(dev))
-    if tune and not plan.accumulates:  # re-running an accumulating plan would change its output
+    if tune and plan.accumulates and AUTOTUNE and not getattr(plan, 'tuned', False) \
+            and not torch.cuda.is_current_stream_capturing():
+        # an accumulating plan is timed on copies of its outputs, so the real ones are written once
+        scratch = [t.clone() if t is not None else None for t in outs]
+        ps = (L.ctypes.c_void_p * L.YH_MAX_BUFS)(*[ptr(t) for t in scratch])
+        _maybe_autotune(plan, lambda: L.call('yastn_hip_grouped_gemm', ps, pi, dtype, *plan.args(), stream_of(dev)))
+    elif tune:
         _maybe_autotune(plan, run)
     run()
 

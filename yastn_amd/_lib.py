@@ -12,7 +12,7 @@ LIB_PATH = os.environ.get('YASTN_HIP_LIB', os.path.join(HERE, 'libyastn_hip.so')
 # constants mirrored from include/yastn_hip.h
 YH_OK, YH_ERR_ARG, YH_ERR_HIP, YH_ERR_NUMERIC = 0, -1, -2, -3
 YH_F64, YH_C128 = 0, 1
-YH_ABI_VERSION = 4
+YH_ABI_VERSION = 5
 YH_MAXR = 8
 YH_JOB_WORDS = 32
 YH_GATHER_TILE = 2048
@@ -20,12 +20,12 @@ YH_TRANSPOSE_TILE = 32
 YH_GEMM_WORDS = 8
 YH_TERM_WORDS = 8
 YH_MAX_BUFS = 4
-YH_TILE_32x32, YH_TILE_64x64, YH_TILE_128x64, YH_TILE_16x64, YH_TILE_64x16 = 0, 1, 2, 3, 4
+YH_TILE_32x32, YH_TILE_64x64, YH_TILE_128x64, YH_TILE_16x64, YH_TILE_64x16, YH_TILE_128x128 = 0, 1, 2, 3, 4, 5
 TILE_DIMS = {YH_TILE_32x32: (32, 32), YH_TILE_64x64: (64, 64), YH_TILE_128x64: (128, 64),
-             YH_TILE_16x64: (16, 64), YH_TILE_64x16: (64, 16)}
+             YH_TILE_16x64: (16, 64), YH_TILE_64x16: (64, 16), YH_TILE_128x128: (128, 128)}
 YH_SVD_WORDS = 8
 YH_SVD_MAX_Q = 2048
-YH_SVD_LDS_BYTES = 49152
+YH_SVD_LDS_BYTES = 131072
 
 _p = ctypes.c_void_p
 _i64 = ctypes.c_int64

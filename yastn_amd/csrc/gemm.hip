@@ -7,7 +7,7 @@
 // owned by exactly one workgroup and terms are summed in a fixed order, so results
 // are bitwise reproducible (no atomics).
 //
-// Tile: BM x BN per workgroup of 4 waves (WM x WN), BK = 16, staged through LDS
+// Tile: BM x BN per workgroup of 4 waves (WM x WN); BK-deep slices staged through two LDS buffers
 // k-major; each wave issues v_mfma_f64_16x16x4f64 on FM x FN 16x16 fragments.
 // Complex data is held as separate real / imaginary LDS planes and multiplied
 // with four real MFMAs per fragment.
@@ -39,7 +39,12 @@ struct Elt<true> {
 
 __device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
-template <bool CPLX, int BM, int BN, int WM, int WN>
+// One workgroup per output tile.  Per term, the k loop is software-pipelined: the next BK slice of
+// op(A) and op(B) is loaded into registers while the MFMAs consume the current slice from one of two
+// LDS buffers, then stored into the other buffer; one barrier per slice.  Tiles with a single 16x16
+// fragment per wave keep NS partial accumulators (k sub-steps rotate over them, summed at the end)
+// so that consecutive MFMAs do not wait on each other.
+template <bool CPLX, int BM, int BN, int WM, int WN, int BK>
 __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t* __restrict__ gemms,
                                                         const int64_t* __restrict__ terms,
                                                         const int32_t* __restrict__ tiles) {
@@ -47,8 +52,14 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
   constexpr int FM = BM / WM / 16;
   constexpr int FN = BN / WN / 16;
   constexpr int NPL = CPLX ? 2 : 1;  // LDS planes
-  __shared__ double As[NPL][BK][BM + PAD];
-  __shared__ double Bs[NPL][BK][BN + PAD];
+  constexpr int NS = (FM * FN >= 4) ? 1 : 4 / (FM * FN);
+  constexpr int KSUB = BK / 4;
+  constexpr int NSU = NS < KSUB ? NS : KSUB;  // partial accumulator sets actually used
+  constexpr int A_PER = (BM * BK + kThreads - 1) / kThreads;
+  constexpr int B_PER = (BN * BK + kThreads - 1) / kThreads;
+  static_assert(BK % 4 == 0, "BK");
+  __shared__ double As[2][NPL][BK][BM + PAD];
+  __shared__ double Bs[2][NPL][BK][BN + PAD];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -62,14 +73,16 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
   const int64_t M = gw[0], N = gw[1], c_off = gw[2], c_rs = gw[3], c_cs = gw[4];
   const int64_t tb = gw[5], tc = gw[6], gflags = gw[7];
 
-  d4 accR[FM][FN], accI[FM][FN];
+  d4 accR[NSU][FM][FN], accI[NSU][FM][FN];
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+  for (int s = 0; s < NSU; ++s)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      accR[i][j] = d4{0, 0, 0, 0};
-      accI[i][j] = d4{0, 0, 0, 0};
-    }
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        accR[s][i][j] = d4{0, 0, 0, 0};
+        accI[s][i][j] = d4{0, 0, 0, 0};
+      }
 
   for (int64_t q = tb; q < tb + tc; ++q) {
     const int64_t* tw = terms + q * YH_TERM_WORDS;
@@ -80,90 +93,121 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
     const T* __restrict__ B = (const T*)bufs.in[(tf >> 16) & 0xff];
     const bool a_kfast = (a_cs == 1);
     const bool b_nfast = (b_cs == 1);
+    if (K <= 0) continue;
+
+    // per-thread slice coordinates (fixed for the term); the k coordinate advances by BK per slice
+    int a_i[A_PER], a_k[A_PER], b_j[B_PER], b_k[B_PER];
+    bool a_in[A_PER], b_in[B_PER];
+#pragma unroll
+    for (int p = 0; p < A_PER; ++p) {
+      const int e = tid + p * kThreads;
+      if (a_kfast) {
+        a_k[p] = e % BK;
+        a_i[p] = e / BK;
+      } else {
+        a_i[p] = e % BM;
+        a_k[p] = e / BM;
+      }
+      a_in[p] = (e < BM * BK) && (m0 + a_i[p] < M);
+    }
+#pragma unroll
+    for (int p = 0; p < B_PER; ++p) {
+      const int e = tid + p * kThreads;
+      if (b_nfast) {
+        b_j[p] = e % BN;
+        b_k[p] = e / BN;
+      } else {
+        b_k[p] = e % BK;
+        b_j[p] = e / BK;
+      }
+      b_in[p] = (e < BN * BK) && (n0 + b_j[p] < N);
+    }
+
+    T ra[A_PER], rb[B_PER];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+      for (int p = 0; p < A_PER; ++p) {
+        const int64_t gk = k0 + a_k[p];
+        if (a_in[p] && gk < K)
+          ra[p] = A[a_off + (m0 + a_i[p]) * a_rs + gk * a_cs];
+        else
+          ra[p] = T{};
+      }
+#pragma unroll
+      for (int p = 0; p < B_PER; ++p) {
+        const int64_t gk = k0 + b_k[p];
+        if (b_in[p] && gk < K)
+          rb[p] = B[b_off + gk * b_rs + (n0 + b_j[p]) * b_cs];
+        else
+          rb[p] = T{};
+      }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+      for (int p = 0; p < A_PER; ++p) {
+        if (tid + p * kThreads < BM * BK) {
+          if constexpr (CPLX) {
+            As[buf][0][a_k[p]][a_i[p]] = ra[p].x;
+            As[buf][1][a_k[p]][a_i[p]] = conjA ? -ra[p].y : ra[p].y;
+          } else {
+            As[buf][0][a_k[p]][a_i[p]] = ra[p];
+          }
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < B_PER; ++p) {
+        if (tid + p * kThreads < BN * BK) {
+          if constexpr (CPLX) {
+            Bs[buf][0][b_k[p]][b_j[p]] = rb[p].x;
+            Bs[buf][1][b_k[p]][b_j[p]] = conjB ? -rb[p].y : rb[p].y;
+          } else {
+            Bs[buf][0][b_k[p]][b_j[p]] = rb[p];
+          }
+        }
+      }
+    };
+
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
     for (int64_t k0 = 0; k0 < K; k0 += BK) {
-      // ---- stage op(A)[m0:m0+BM, k0:k0+BK] into As[k][m]
-#pragma unroll
-      for (int p = 0; p < (BM * BK) / kThreads; ++p) {
-        const int e = tid + p * kThreads;
-        int ii, kk;
-        if (a_kfast) {
-          kk = e % BK;
-          ii = e / BK;
-        } else {
-          ii = e % BM;
-          kk = e / BM;
-        }
-        const int64_t gi = m0 + ii, gk = k0 + kk;
-        double re = 0.0, im = 0.0;
-        if (gi < M && gk < K) {
-          const T v = A[a_off + gi * a_rs + gk * a_cs];
-          if constexpr (CPLX) {
-            re = v.x;
-            im = conjA ? -v.y : v.y;
-          } else {
-            re = v;
-          }
-        }
-        As[0][kk][ii] = re;
-        if constexpr (CPLX) As[1][kk][ii] = im;
-      }
-      // ---- stage op(B)[k0:k0+BK, n0:n0+BN] into Bs[k][n]
-#pragma unroll
-      for (int p = 0; p < (BN * BK) / kThreads; ++p) {
-        const int e = tid + p * kThreads;
-        int jj, kk;
-        if (b_nfast) {
-          jj = e % BN;
-          kk = e / BN;
-        } else {
-          kk = e % BK;
-          jj = e / BK;
-        }
-        const int64_t gj = n0 + jj, gk = k0 + kk;
-        double re = 0.0, im = 0.0;
-        if (gj < N && gk < K) {
-          const T v = B[b_off + gk * b_rs + gj * b_cs];
-          if constexpr (CPLX) {
-            re = v.x;
-            im = conjB ? -v.y : v.y;
-          } else {
-            re = v;
-          }
-        }
-        Bs[0][kk][jj] = re;
-        if constexpr (CPLX) Bs[1][kk][jj] = im;
-      }
-      __syncthreads();
+      const bool more = k0 + BK < K;
+      if (more) load(k0 + BK);
       // ---- MFMA over the BK slice, 4 k per instruction
 #pragma unroll
-      for (int ks = 0; ks < BK; ks += 4) {
-        const int kr = ks + (lane >> 4);
+      for (int ks = 0; ks < KSUB; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        constexpr int S = NSU;
+        const int s = ks % S;
         double ar[FM], ai[FM], br[FN], bi[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
           const int col = wm * (BM / WM) + i * 16 + (lane & 15);
-          ar[i] = As[0][kr][col];
-          if constexpr (CPLX) ai[i] = As[1][kr][col];
+          ar[i] = As[buf][0][kr][col];
+          if constexpr (CPLX) ai[i] = As[buf][1][kr][col];
         }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
           const int col = wn * (BN / WN) + j * 16 + (lane & 15);
-          br[j] = Bs[0][kr][col];
-          if constexpr (CPLX) bi[j] = Bs[1][kr][col];
+          br[j] = Bs[buf][0][kr][col];
+          if constexpr (CPLX) bi[j] = Bs[buf][1][kr][col];
         }
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
-            accR[i][j] = mfma(ar[i], br[j], accR[i][j]);
+            accR[s][i][j] = mfma(ar[i], br[j], accR[s][i][j]);
             if constexpr (CPLX) {
-              accR[i][j] = mfma(-ai[i], bi[j], accR[i][j]);
-              accI[i][j] = mfma(ar[i], bi[j], accI[i][j]);
-              accI[i][j] = mfma(ai[i], br[j], accI[i][j]);
+              accR[s][i][j] = mfma(-ai[i], bi[j], accR[s][i][j]);
+              accI[s][i][j] = mfma(ar[i], bi[j], accI[s][i][j]);
+              accI[s][i][j] = mfma(ai[i], br[j], accI[s][i][j]);
             }
           }
       }
+      if (more) store(buf ^ 1);
       __syncthreads();
+      buf ^= 1;
     }
   }
 
@@ -174,6 +218,12 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
+      d4 sr = accR[0][i][j], si = accI[0][i][j];
+#pragma unroll
+      for (int s = 1; s < NSU; ++s) {
+        sr += accR[s][i][j];
+        si += accI[s][i][j];
+      }
       const int64_t col = n0 + wn * (BN / WN) + j * 16 + (lane & 15);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -181,7 +231,7 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
         if (row < M && col < N) {
           T* p = C + c_off + row * c_rs + col * c_cs;
           if constexpr (CPLX) {
-            double2 r = make_double2(accR[i][j][v], accI[i][j][v]);
+            double2 r = make_double2(sr[v], si[v]);
             if (accumulate) {
               const double2 o = *p;
               r.x += o.x;
@@ -189,7 +239,7 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
             }
             *p = r;
           } else {
-            double r = accR[i][j][v];
+            double r = sr[v];
             if (accumulate) r += *p;
             *p = r;
           }
@@ -198,25 +248,31 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
     }
 }
 
+// BK per configuration: the double-buffered LDS tiles stay within 64 KiB.
 template <bool CPLX>
 int launch(const Bufs& b, const int64_t* gemms, const int64_t* terms, const int32_t* tiles, int64_t ntiles,
            int cfg, hipStream_t s) {
   const dim3 grid((unsigned)ntiles), block(kThreads);
+  constexpr int BKW = CPLX ? 8 : 16;  // 64 / 128-wide tiles
+  constexpr int BKH = CPLX ? 4 : 8;   // 128 x 128
   switch (cfg) {
     case YH_TILE_32x32:
-      hipLaunchKernelGGL((gemm_kernel<CPLX, 32, 32, 2, 2>), grid, block, 0, s, b, gemms, terms, tiles);
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 32, 32, 2, 2, 16>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
     case YH_TILE_64x64:
-      hipLaunchKernelGGL((gemm_kernel<CPLX, 64, 64, 2, 2>), grid, block, 0, s, b, gemms, terms, tiles);
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 64, 64, 2, 2, BKW>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
     case YH_TILE_128x64:
-      hipLaunchKernelGGL((gemm_kernel<CPLX, 128, 64, 2, 2>), grid, block, 0, s, b, gemms, terms, tiles);
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 128, 64, 2, 2, BKW>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
     case YH_TILE_16x64:
-      hipLaunchKernelGGL((gemm_kernel<CPLX, 16, 64, 1, 4>), grid, block, 0, s, b, gemms, terms, tiles);
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 16, 64, 1, 4, 16>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
     case YH_TILE_64x16:
-      hipLaunchKernelGGL((gemm_kernel<CPLX, 64, 16, 4, 1>), grid, block, 0, s, b, gemms, terms, tiles);
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 64, 16, 4, 1, 16>), grid, block, 0, s, b, gemms, terms, tiles);
+      break;
+    case YH_TILE_128x128:
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 128, 128, 2, 2, BKH>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
     default:
       yh::set_error("unknown tile config %d", cfg);
@@ -231,6 +287,7 @@ void tile_dims(int cfg, int* bm, int* bn) {
     case YH_TILE_64x64: *bm = 64; *bn = 64; break;
     case YH_TILE_128x64: *bm = 128; *bn = 64; break;
     case YH_TILE_16x64: *bm = 16; *bn = 64; break;
+    case YH_TILE_128x128: *bm = 128; *bn = 128; break;
     default: *bm = 64; *bn = 16; break;
   }
 }
@@ -240,7 +297,7 @@ void tile_dims(int cfg, int* bm, int* bn) {
 extern "C" int yastn_hip_check_gemm_plan(const int64_t* h_gemms, int64_t ngemms, const int64_t* h_terms,
                                          int64_t nterms, const int32_t* h_tiles, int64_t ntiles, int cfg,
                                          const int64_t* in_numel, int nin, const int64_t* out_numel, int nout) {
-  YH_CHECK_ARG(cfg >= YH_TILE_32x32 && cfg <= YH_TILE_64x16, "tile config %d", cfg);
+  YH_CHECK_ARG(cfg >= YH_TILE_32x32 && cfg <= YH_TILE_128x128, "tile config %d", cfg);
   YH_CHECK_ARG(nin <= YH_MAX_BUFS && nout <= YH_MAX_BUFS, "too many buffers");
   int bm, bn;
   tile_dims(cfg, &bm, &bn);

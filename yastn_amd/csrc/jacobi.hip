@@ -12,8 +12,8 @@
 // abs-sum (Gershgorin, so B is positive semidefinite): the right rotations V are
 // the eigenvectors and lambda_j = sigma_j - mu, written ascending.
 //
-// Blocks whose W and V fit in 64 KiB go through an LDS-resident instance of the
-// kernel; larger blocks use the caller's workspace in global memory.
+// Blocks whose W and V fit in YH_SVD_LDS_BYTES go through an LDS-resident instance of the
+// kernel (all sweeps in one launch); larger blocks use the panel path below.
 #include <float.h>
 
 #include "common.h"
@@ -45,16 +45,35 @@ struct Num<true> {
   static __device__ inline double abs2(double2 x) { return x.x * x.x + x.y * x.y; }
 };
 
+// Wave-wide reductions: DPP moves within each row of 16 lanes (quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror, row_mirror: after the four steps every lane holds its row's value),
+// then the four row values are read into scalar registers, so the result is wave-uniform.
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ inline double read_lane(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  v += dpp_move<0x140>(v);
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 __device__ inline double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
+  v = fmax(v, dpp_move<0xB1>(v));
+  v = fmax(v, dpp_move<0x4E>(v));
+  v = fmax(v, dpp_move<0x141>(v));
+  v = fmax(v, dpp_move<0x140>(v));
+  return fmax(fmax(read_lane(v, 0), read_lane(v, 16)), fmax(read_lane(v, 32), read_lane(v, 48)));
 }
 
 // circle-method pairing: qe (even) players, round r in [0, qe-1), pair k in [0, qe/2)
@@ -72,6 +91,51 @@ __device__ inline void pair_of(int qe, int r, int k, int& i, int& j) {
     i = j;
     j = t;
   }
+}
+
+// 1/x and 1/sqrt(x) from the hardware approximations (v_rcp_f64, v_rsq_f64) refined by two
+// Newton-Raphson steps each; the rotation parameters only need c^2 + s^2 = 1 to rounding accuracy.
+__device__ inline double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+__device__ inline double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
+// Hestenes rotation of a column pair from a = ||w_i||^2, b = ||w_j||^2, g = w_i^H w_j = gr + i gi:
+// returns false (no rotation) when |g| <= tol sqrt(a b) or a column is at the noise floor; else
+// c, s, and the phase e^{i phi} = g / |g| = (er, ei).
+__device__ inline bool rotation(double a, double b, double gr, double gi, double tol, double floor2, double& c,
+                                double& s, double& er, double& ei) {
+  const double g2 = gr * gr + gi * gi;
+  // columns at the noise level (||w||^2 <= (eps ||A||_F)^2) carry no information beyond absolute
+  // accuracy eps ||A||; rotating them against rounding noise would never converge
+  if (!(g2 > tol * tol * a * b) || a <= floor2 || b <= floor2) return false;
+  const double rg = rsq_nr(g2);
+  er = gr * rg;
+  ei = gi * rg;
+  const double zeta = 0.5 * (b - a) * rg;  // (b - a) / (2 |g|)
+  const double az = fabs(zeta);
+  double t;
+  if (az < 1e150) {
+    const double x = 1.0 + zeta * zeta;
+    t = rcp_nr(az + x * rsq_nr(x));
+  } else {
+    t = 0.5 / az;
+  }
+  if (zeta < 0) t = -t;
+  c = rsq_nr(1.0 + t * t);
+  s = c * t;
+  return true;
 }
 
 // One Hestenes rotation of the column pair (w_i, w_j) (length p) and of the matching accumulator
@@ -94,16 +158,9 @@ __device__ inline bool jacobi_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>
   a = wave_sum(a);
   b = wave_sum(b);
   gr = wave_sum(gr);
-  gi = wave_sum(gi);
-  const double g = sqrt(gr * gr + gi * gi);
-  // columns at the noise level (||w||^2 <= (eps ||A||_F)^2) carry no information beyond absolute
-  // accuracy eps ||A||; rotating them against rounding noise would never converge
-  if (!(g > tol * sqrt(a * b)) || a <= floor2 || b <= floor2) return false;
-  const double zeta = (b - a) / (2.0 * g);
-  const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  const double c = 1.0 / sqrt(1.0 + t * t);
-  const double s = c * t;
-  const double er = gr / g, ei = gi / g;
+  if (CPLX) gi = wave_sum(gi);
+  double c, s, er, ei;
+  if (!rotation(a, b, gr, gi, tol, floor2, c, s, er, ei)) return false;
   // w_i' = c w_i - s e^{-i phi} w_j ; w_j' = s e^{i phi} w_i + c w_j
   for (int64_t x = lane; x < p; x += 64) {
     const T u = wi[x], v = wj[x];
@@ -120,6 +177,71 @@ __device__ inline bool jacobi_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>
     vj_[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
   }
   return true;
+}
+
+// The same rotation with the two columns held in registers between the inner products and the
+// update (one read and one write per element instead of two reads and one write); p <= 64 * NR.
+template <bool CPLX, int NR>
+__device__ inline bool jacobi_pair_reg(typename Num<CPLX>::T* wi, typename Num<CPLX>::T* wj, int64_t p,
+                                       typename Num<CPLX>::T* vi_, typename Num<CPLX>::T* vj_, int64_t q,
+                                       double tol, double floor2, int lane) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  T xi[NR], xj[NR];
+  double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int64_t x = lane + 64 * r;
+    if (x < p) {
+      xi[r] = wi[x];
+      xj[r] = wj[x];
+    } else {
+      xi[r] = N::make(0.0, 0.0);
+      xj[r] = N::make(0.0, 0.0);
+    }
+    const T u = xi[r], v = xj[r];
+    a += N::abs2(u);
+    b += N::abs2(v);
+    gr += N::re(u) * N::re(v) + N::im(u) * N::im(v);
+    gi += N::re(u) * N::im(v) - N::im(u) * N::re(v);
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  gr = wave_sum(gr);
+  if (CPLX) gi = wave_sum(gi);
+  double c, s, er, ei;
+  if (!rotation(a, b, gr, gi, tol, floor2, c, s, er, ei)) return false;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int64_t x = lane + 64 * r;
+    if (x < p) {
+      const T u = xi[r], v = xj[r];
+      const double vr = er * N::re(v) + ei * N::im(v), vi = er * N::im(v) - ei * N::re(v);
+      const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);
+      wi[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vi);
+      wj[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
+    }
+  }
+  for (int64_t x = lane; x < q; x += 64) {
+    const T u = vi_[x], v = vj_[x];
+    const double vr = er * N::re(v) + ei * N::im(v), vv = er * N::im(v) - ei * N::re(v);
+    const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);
+    vi_[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vv);
+    vj_[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
+  }
+  return true;
+}
+
+// Rotation of one pair with the register-resident variant when the columns are short enough.
+template <bool CPLX>
+__device__ inline bool rotate_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>::T* wj, int64_t p,
+                                   typename Num<CPLX>::T* vi, typename Num<CPLX>::T* vj, int64_t q, double tol,
+                                   double floor2, int lane) {
+  if (p <= 64) return jacobi_pair_reg<CPLX, 1>(wi, wj, p, vi, vj, q, tol, floor2, lane);
+  if (p <= 128) return jacobi_pair_reg<CPLX, 2>(wi, wj, p, vi, vj, q, tol, floor2, lane);
+  if (p <= 256) return jacobi_pair_reg<CPLX, 4>(wi, wj, p, vi, vj, q, tol, floor2, lane);
+  if (p <= 512) return jacobi_pair_reg<CPLX, 8>(wi, wj, p, vi, vj, q, tol, floor2, lane);
+  return jacobi_pair<CPLX>(wi, wj, p, vi, vj, q, tol, floor2, lane);
 }
 
 struct BlockDesc {
@@ -296,7 +418,7 @@ __global__ void __launch_bounds__(kThreads)
         int i, j;
         pair_of(qe, r, k, i, j);
         if (j >= q) continue;
-        if (!jacobi_pair<CPLX>(W + (int64_t)i * p, W + (int64_t)j * p, p, V + (int64_t)i * q, V + (int64_t)j * q, q,
+        if (!rotate_pair<CPLX>(W + (int64_t)i * p, W + (int64_t)j * p, p, V + (int64_t)i * q, V + (int64_t)j * q, q,
                                tol, floor2, lane))
           continue;
         if (lane == 0) s_rotated = 1;
@@ -419,36 +541,54 @@ __global__ void __launch_bounds__(kPanelThreads)
       int i, j;
       pair_of(ne, r, k, i, j);
       if (j >= nc) continue;
-      if (jacobi_pair<CPLX>(Wp + (int64_t)i * p, Wp + (int64_t)j * p, p, Q + i * nq, Q + j * nq, nc, tol, floor2,
+      if (rotate_pair<CPLX>(Wp + (int64_t)i * p, Wp + (int64_t)j * p, p, Q + i * nq, Q + j * nq, nc, tol, floor2,
                             lane) &&
           lane == 0)
         s_rot = 1;
     }
     __syncthreads();
   }
+  if (tid == 0) flags[blockIdx.x] = s_rot;
+  if (!s_rot) return;  // nothing rotated: W and V of the pair are unchanged (uniform branch)
   for (int64_t e = tid; e < (int64_t)nc * p; e += kPanelThreads) {
     const int64_t l = e / p, r = e - l * p;
     W[gcol(l) * p + r] = Wp[e];
   }
   __syncthreads();
-  // V panel columns (q x nc) staged where the W panel was (q <= p), then V <- V Q
+  // V panel columns (q x nc) staged where the W panel was (q <= p), then V <- V Q.  A thread owns
+  // one row r and a group of 8 columns; the Q reads of a wave are (mostly) one address, broadcast.
   T* Vp = Wp;
   for (int64_t e = tid; e < (int64_t)nc * q; e += kPanelThreads) {
     const int64_t l = e / q, r = e - l * q;
     Vp[e] = V[gcol(l) * q + r];
   }
   __syncthreads();
-  for (int64_t e = tid; e < (int64_t)nc * q; e += kPanelThreads) {
-    const int64_t c = e / q, r = e - c * q;
-    double accr = 0.0, acci = 0.0;
-    for (int k = 0; k < nc; ++k) {
-      const T v = Vp[(int64_t)k * q + r], w = Q[c * nq + k];
-      accr += N::re(v) * N::re(w) - N::im(v) * N::im(w);
-      acci += N::re(v) * N::im(w) + N::im(v) * N::re(w);
+  const int ngrp = (nc + 7) / 8;
+  for (int64_t e = tid; e < (int64_t)ngrp * q; e += kPanelThreads) {
+    const int cg = (int)(e / q);
+    const int64_t r = e - (int64_t)cg * q;
+    double accr[8], acci[8];
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc) {
+      accr[cc] = 0.0;
+      acci[cc] = 0.0;
     }
-    V[gcol(c) * q + r] = N::make(accr, acci);
+    for (int k = 0; k < nc; ++k) {
+      const T v = Vp[(int64_t)k * q + r];
+#pragma unroll
+      for (int cc = 0; cc < 8; ++cc) {
+        const int c = cg * 8 + cc;
+        const T w = c < nc ? Q[c * nq + k] : N::make(0.0, 0.0);
+        accr[cc] += N::re(v) * N::re(w) - N::im(v) * N::im(w);
+        acci[cc] += N::re(v) * N::im(w) + N::im(v) * N::re(w);
+      }
+    }
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc) {
+      const int c = cg * 8 + cc;
+      if (c < nc) V[gcol(c) * q + r] = N::make(accr[cc], acci[cc]);
+    }
   }
-  if (tid == 0) flags[blockIdx.x] = s_rot;
 }
 
 template <bool CPLX, int MODE>
@@ -466,12 +606,126 @@ __global__ void __launch_bounds__(kPanelThreads)
   if (threadIdx.x == 0) info[bidx] = sweeps;
 }
 
+// Block-wide sum over kThreads threads (kWaves partials in s_red); every thread receives the total.
+__device__ inline double block_sum(double v, double* s_red, int lane, int wave) {
+  v = wave_sum(v);
+  __syncthreads();  // earlier readers of s_red are done
+  if (lane == 0) s_red[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < kWaves; ++w) t += s_red[w];
+  return t;
+}
+
+// Orthonormal completion of the singular vectors that belong to noise-level singular values.
+// Rotations skip columns whose squared norm is at the noise floor eps^2 ||A||_F^2, so for such a
+// column w_j / sigma_j is not orthogonal to the others.  The W-side vectors (columns of U when
+// m >= n, rows of Vh when m < n) of the trailing singular values with sigma^2 <= 4 eps^2 sum(sigma^2)
+// are replaced, in order, by vectors orthogonalised against every vector before them (classical
+// Gram-Schmidt, two passes): first the scaled column itself, then unit vectors e_r if that leaves
+// less than 1e-6 of the start.  An all-zero block thus gets the leading unit vectors, as LAPACK's
+// SVD returns for a zero matrix.  The accumulated side (V) is orthonormal by construction.
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    complete_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ ids,
+                    typename Num<CPLX>::T* __restrict__ U, const double* __restrict__ S,
+                    typename Num<CPLX>::T* __restrict__ Vh) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  __shared__ double s_red[kWaves];
+  __shared__ double s_cr[YH_SVD_MAX_Q], s_ci[YH_SVD_MAX_Q];
+  const int bidx = ids[blockIdx.x];
+  const BlockDesc d = load_desc(blocks + (int64_t)bidx * YH_SVD_WORDS);
+  const bool tr = d.m < d.n;
+  const int64_t p = tr ? d.n : d.m, k = tr ? d.m : d.n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // vector c, element x, at base[c * cs + x * xs]
+  T* base = tr ? Vh + d.v_off : U + d.u_off;
+  const int64_t cs = tr ? p : 1, xs = tr ? 1 : k;
+  double f = 0.0;
+  for (int64_t j = tid; j < k; j += kThreads) f += S[d.s_off + j] * S[d.s_off + j];
+  const double thr2 = 4.0 * DBL_EPSILON * DBL_EPSILON * block_sum(f, s_red, lane, wave);
+  double cnt = 0.0;
+  for (int64_t j = tid; j < k; j += kThreads) cnt += (S[d.s_off + j] * S[d.s_off + j] > thr2) ? 1.0 : 0.0;
+  const int64_t c0 = (int64_t)block_sum(cnt, s_red, lane, wave);  // S is descending
+  for (int64_t c = c0; c < k; ++c) {
+    T* vc = base + c * cs;
+    for (int64_t t = 0; t <= p; ++t) {
+      if (t > 0) {
+        const int64_t r = (c + t - 1) % p;
+        for (int64_t x = tid; x < p; x += kThreads) vc[x * xs] = N::make(x == r ? 1.0 : 0.0, 0.0);
+      }
+      double n0 = 0.0;
+      for (int64_t x = tid; x < p; x += kThreads) n0 += N::abs2(vc[x * xs]);
+      n0 = block_sum(n0, s_red, lane, wave);
+      if (!(n0 > 0.0)) continue;
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int64_t l = wave; l < c; l += kWaves) {
+          const T* vl = base + l * cs;
+          double hr = 0.0, hi = 0.0;
+          for (int64_t x = lane; x < p; x += 64) {
+            const T a = vl[x * xs], b = vc[x * xs];
+            hr += N::re(a) * N::re(b) + N::im(a) * N::im(b);  // conj(a) b
+            hi += N::re(a) * N::im(b) - N::im(a) * N::re(b);
+          }
+          hr = wave_sum(hr);
+          hi = wave_sum(hi);
+          if (lane == 0) {
+            s_cr[l] = hr;
+            s_ci[l] = hi;
+          }
+        }
+        __syncthreads();
+        for (int64_t x = tid; x < p; x += kThreads) {
+          double ar = 0.0, ai = 0.0;
+          for (int64_t l = 0; l < c; ++l) {
+            const T a = base[l * cs + x * xs];
+            ar += s_cr[l] * N::re(a) - s_ci[l] * N::im(a);
+            ai += s_cr[l] * N::im(a) + s_ci[l] * N::re(a);
+          }
+          const T b = vc[x * xs];
+          vc[x * xs] = N::make(N::re(b) - ar, N::im(b) - ai);
+        }
+        __syncthreads();
+      }
+      double n1 = 0.0;
+      for (int64_t x = tid; x < p; x += kThreads) n1 += N::abs2(vc[x * xs]);
+      n1 = block_sum(n1, s_red, lane, wave);
+      if (n1 > 1e-12 * n0) {
+        const double inv = 1.0 / sqrt(n1);
+        for (int64_t x = tid; x < p; x += kThreads) {
+          const T b = vc[x * xs];
+          vc[x * xs] = N::make(N::re(b) * inv, N::im(b) * inv);
+        }
+        __syncthreads();
+        break;
+      }
+    }
+  }
+}
+
+template <bool CPLX>
+int launch_complete(const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, void* U, const void* S, void* Vh,
+                    hipStream_t stream) {
+  typedef typename Num<CPLX>::T T;
+  if (nb == 0) return YH_OK;
+  hipLaunchKernelGGL(complete_kernel<CPLX>, dim3((unsigned)nb), dim3(kThreads), 0, stream, d_blocks, d_ids, (T*)U,
+                     (const double*)S, (T*)Vh);
+  return yh::hip_status(hipGetLastError(), "svd completion launch");
+}
+
 template <bool CPLX, int MODE>
 int launch_jacobi(const int64_t* d_blocks, const int32_t* d_order, int64_t n_lds, int64_t lds_bytes,
                   int64_t nblocks, const void* A, void* U, void* S, void* Vh, void* work, int32_t* info,
                   int max_sweeps, hipStream_t stream) {
   typedef typename Num<CPLX>::T T;
   if (n_lds > 0) {
+    if (lds_bytes > 65536) {
+      const int st = yh::hip_status(hipFuncSetAttribute((const void*)jacobi_kernel<CPLX, true, MODE>,
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
+                                    "jacobi LDS size");
+      if (st) return st;
+    }
     hipLaunchKernelGGL((jacobi_kernel<CPLX, true, MODE>), dim3((unsigned)n_lds), dim3(kThreads), (size_t)lds_bytes,
                        stream, d_blocks, d_order, (const T*)A, (T*)U, (double*)S, (T*)Vh, (T*)work, info,
                        max_sweeps);
@@ -485,6 +739,7 @@ int launch_jacobi(const int64_t* d_blocks, const int32_t* d_order, int64_t n_lds
     const int st = yh::hip_status(hipGetLastError(), "jacobi (global) launch");
     if (st) return st;
   }
+  if (MODE == 0) return launch_complete<CPLX>(d_blocks, d_order, nblocks, U, S, Vh, stream);
   return YH_OK;
 }
 
@@ -633,5 +888,8 @@ extern "C" int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const vo
       hipLaunchKernelGGL((panel_finalize_kernel<true, 1>), g, t, 0, stream, d_blocks, d_ids, (const double2*)work,
                          mu, (double2*)U, (double*)S, (double2*)Vh, d_info, sweeps);
   }
-  return yh::hip_status(hipGetLastError(), "svd panel finalize launch");
+  const int st = yh::hip_status(hipGetLastError(), "svd panel finalize launch");
+  if (st || mode == 1) return st;
+  return dtype == YH_F64 ? launch_complete<false>(d_blocks, d_ids, nb, U, S, Vh, stream)
+                         : launch_complete<true>(d_blocks, d_ids, nb, U, S, Vh, stream);
 }

@@ -28,10 +28,16 @@ struct Z<true> {
   static __device__ inline double absv(double2 x) { return hypot(x.x, x.y); }
 };
 
-// fix_svd_signs: one wave per (block, column); blocks given as SVD plan words
-// (0 a_off, 1 m, 2 n, 3 u_off, 4 s_off, 5 v_off).  U block is m x k, Vh block k x n.
-// With piv != nullptr only the pivot positions are written: piv[s_off + j] = flat index of the
-// pivot of column j in U (the caller applies the phases with differentiable tensor ops).
+// Work split: kSplit workgroups per block (grid = nblocks * kSplit), so that one large block does not
+// serialise on a single CU; a workgroup with nothing to do exits at once.
+constexpr int kSplit = 16;
+
+// fix_svd_signs.  Blocks given as SVD plan words (0 a_off, 1 m, 2 n, 3 u_off, 4 s_off, 5 v_off);
+// U block m x k, Vh block k x n.  A workgroup takes 64-column chunks of U: lane = column (coalesced
+// row reads), wave = stripe of rows; per column the argmax of int64(|U[r, j]| * 2^40) with ties to the
+// smallest row, then the phase is applied to U[:, j] and Vh[j, :].
+// With piv != nullptr only the pivot positions are written: piv[s_off + j] = flat index of the pivot
+// of column j in U (the caller applies the phases with differentiable tensor ops).
 template <bool CPLX>
 __global__ void __launch_bounds__(kThreads) fix_signs_kernel(typename Z<CPLX>::T* __restrict__ U,
                                                              typename Z<CPLX>::T* __restrict__ Vh,
@@ -39,46 +45,69 @@ __global__ void __launch_bounds__(kThreads) fix_signs_kernel(typename Z<CPLX>::T
                                                              int64_t* __restrict__ piv) {
   typedef Z<CPLX> N;
   typedef typename N::T T;
-  const int64_t* w = blocks + (int64_t)blockIdx.x * YH_SVD_WORDS;
+  constexpr int kW = kThreads / 64;
+  __shared__ int64_t s_best[kW][64];
+  __shared__ int64_t s_row[kW][64];
+  __shared__ double s_ph[2][64];
+  const int64_t* w = blocks + (int64_t)(blockIdx.x / kSplit) * YH_SVD_WORDS;
+  const int split = blockIdx.x % kSplit;
   const int64_t m = w[1], n = w[2], u_off = w[3], s_off = w[4], v_off = w[5];
   const int64_t k = m < n ? m : n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int64_t j = wave; j < k; j += kThreads / 64) {
-    // argmax of int64(|U[r, j]| * 2^40), ties to the smallest row
+  for (int64_t c0 = (int64_t)split * 64; c0 < k; c0 += (int64_t)kSplit * 64) {
+    const int64_t j = c0 + lane;
     int64_t best = -1, brow = 0;
-    for (int64_t r = lane; r < m; r += 64) {
-      const int64_t qv = (int64_t)(N::absv(U[u_off + r * k + j]) * 1099511627776.0);
-      if (qv > best) {
-        best = qv;
-        brow = r;
+    if (j < k) {
+      for (int64_t r = wave; r < m; r += kW) {
+        const int64_t qv = (int64_t)(N::absv(U[u_off + r * k + j]) * 1099511627776.0);
+        if (qv > best) {
+          best = qv;
+          brow = r;
+        }
       }
     }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-      const int64_t ob = __shfl_xor(best, o, 64);
-      const int64_t orow = __shfl_xor(brow, o, 64);
-      if (ob > best || (ob == best && orow < brow)) {
-        best = ob;
-        brow = orow;
+    s_best[wave][lane] = best;
+    s_row[wave][lane] = brow;
+    __syncthreads();
+    if (wave == 0) {
+      for (int x = 1; x < kW; ++x) {
+        const int64_t ob = s_best[x][lane], orow = s_row[x][lane];
+        if (ob > best || (ob == best && orow < brow)) {
+          best = ob;
+          brow = orow;
+        }
+      }
+      if (j < k) {
+        if (piv) {
+          piv[s_off + j] = u_off + brow * k + j;
+        } else {
+          const T pv = U[u_off + brow * k + j];
+          const double a = N::absv(pv);
+          s_ph[0][lane] = N::re(pv) / a;
+          s_ph[1][lane] = N::im(pv) / a;
+        }
       }
     }
-    if (piv) {
-      if (lane == 0) piv[s_off + j] = u_off + brow * k + j;
-      continue;
+    __syncthreads();
+    if (!piv) {
+      const int64_t nc = (k - c0) < 64 ? (k - c0) : 64;
+      if (lane < nc) {
+        const double pr = s_ph[0][lane], pi = s_ph[1][lane];
+        for (int64_t r = wave; r < m; r += kW) {  // u * conj(phase)
+          const T u = U[u_off + r * k + j];
+          U[u_off + r * k + j] = N::mk(N::re(u) * pr + N::im(u) * pi, N::im(u) * pr - N::re(u) * pi);
+        }
+      }
+      for (int64_t l = wave; l < nc; l += kW) {  // Vh row c0 + l times phase
+        const double pr = s_ph[0][l], pi = s_ph[1][l];
+        T* row = Vh + v_off + (c0 + l) * n;
+        for (int64_t c = lane; c < n; c += 64) {
+          const T v = row[c];
+          row[c] = N::mk(N::re(v) * pr - N::im(v) * pi, N::im(v) * pr + N::re(v) * pi);
+        }
+      }
     }
-    const T pv = U[u_off + brow * k + j];
-    const double a = N::absv(pv);
-    const double pr = N::re(pv) / a, pi = N::im(pv) / a;  // phase
-    for (int64_t r = lane; r < m; r += 64) {
-      const T u = U[u_off + r * k + j];
-      // u * conj(phase)
-      U[u_off + r * k + j] = N::mk(N::re(u) * pr + N::im(u) * pi, N::im(u) * pr - N::re(u) * pi);
-    }
-    for (int64_t c = lane; c < n; c += 64) {
-      const T v = Vh[v_off + j * n + c];
-      // v * phase
-      Vh[v_off + j * n + c] = N::mk(N::re(v) * pr - N::im(v) * pi, N::im(v) * pr + N::re(v) * pi);
-    }
+    __syncthreads();
   }
 }
 
@@ -97,10 +126,11 @@ __global__ void __launch_bounds__(kThreads)
                         const int64_t* __restrict__ descs, double eps, int has_gS, int has_gU, int has_gV) {
   typedef Z<CPLX> N;
   typedef typename N::T T;
-  const int64_t* w = descs + (int64_t)blockIdx.x * 8;
+  const int64_t* w = descs + (int64_t)(blockIdx.x / kSplit) * 8;
   const int64_t m = w[0], n = w[1], k = w[2], u_off = w[3], s_off = w[4], v_off = w[5], kk = w[6], t_off = w[7];
   const double* s = S + s_off;
-  for (int64_t e = threadIdx.x; e < k * k; e += kThreads) {
+  const int64_t first = (int64_t)(blockIdx.x % kSplit) * kThreads + threadIdx.x, step = (int64_t)kSplit * kThreads;
+  for (int64_t e = first; e < k * k; e += step) {
     const int64_t i = e / k, j = e - i * k;
     double jr = 0.0, ji = 0.0;
     const double sinv_j = lor(s[j], eps), sinv_i = lor(s[i], eps);
@@ -136,7 +166,7 @@ __global__ void __launch_bounds__(kThreads)
     J[kk + i * k + j] = N::mk(jr, ji);
   }
   if (m >= n) {  // T is m x k
-    for (int64_t e = threadIdx.x; e < m * k; e += kThreads) {
+    for (int64_t e = first; e < m * k; e += step) {
       const int64_t jcol = e % k;
       T v = N::mk(0.0, 0.0);
       if (has_gU && m > k) {
@@ -147,7 +177,7 @@ __global__ void __launch_bounds__(kThreads)
       T_out[t_off + e] = v;
     }
   } else {  // T is k x n
-    for (int64_t e = threadIdx.x; e < k * n; e += kThreads) {
+    for (int64_t e = first; e < k * n; e += step) {
       const int64_t irow = e / n;
       T v = N::mk(0.0, 0.0);
       if (has_gV) {
@@ -168,9 +198,10 @@ __global__ void __launch_bounds__(kThreads)
                          const double* __restrict__ S, const double* __restrict__ gS,
                          const int64_t* __restrict__ descs, int has_gS, int has_gU) {
   typedef Z<CPLX> N;
-  const int64_t* w = descs + (int64_t)blockIdx.x * 8;
+  const int64_t* w = descs + (int64_t)(blockIdx.x / kSplit) * 8;
   const int64_t n = w[0], s_off = w[1], kk = w[2];
-  for (int64_t e = threadIdx.x; e < n * n; e += kThreads) {
+  const int64_t first = (int64_t)(blockIdx.x % kSplit) * kThreads + threadIdx.x, step = (int64_t)kSplit * kThreads;
+  for (int64_t e = first; e < n * n; e += step) {
     const int64_t i = e / n, j = e - i * n;
     double r = 0.0, im = 0.0;
     if (has_gU) {
@@ -196,10 +227,10 @@ extern "C" int yastn_hip_fix_svd_signs(void* U, void* Vh, int dtype, const int64
   YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
   if (nblocks == 0) return YH_OK;
   if (dtype == YH_F64)
-    hipLaunchKernelGGL(fix_signs_kernel<false>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, (double*)U,
+    hipLaunchKernelGGL(fix_signs_kernel<false>, dim3((unsigned)(nblocks * kSplit)), dim3(kThreads), 0, stream, (double*)U,
                        (double*)Vh, d_blocks, (int64_t*)nullptr);
   else
-    hipLaunchKernelGGL(fix_signs_kernel<true>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, (double2*)U,
+    hipLaunchKernelGGL(fix_signs_kernel<true>, dim3((unsigned)(nblocks * kSplit)), dim3(kThreads), 0, stream, (double2*)U,
                        (double2*)Vh, d_blocks, (int64_t*)nullptr);
   return yh::hip_status(hipGetLastError(), "fix_svd_signs launch");
 }
@@ -210,10 +241,10 @@ extern "C" int yastn_hip_svd_sign_pivots(const void* U, int64_t* piv, int dtype,
   YH_CHECK_ARG(piv != nullptr, "null pivot buffer");
   if (nblocks == 0) return YH_OK;
   if (dtype == YH_F64)
-    hipLaunchKernelGGL(fix_signs_kernel<false>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream,
+    hipLaunchKernelGGL(fix_signs_kernel<false>, dim3((unsigned)(nblocks * kSplit)), dim3(kThreads), 0, stream,
                        (double*)const_cast<void*>(U), (double*)nullptr, d_blocks, piv);
   else
-    hipLaunchKernelGGL(fix_signs_kernel<true>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream,
+    hipLaunchKernelGGL(fix_signs_kernel<true>, dim3((unsigned)(nblocks * kSplit)), dim3(kThreads), 0, stream,
                        (double2*)const_cast<void*>(U), (double2*)nullptr, d_blocks, piv);
   return yh::hip_status(hipGetLastError(), "svd_sign_pivots launch");
 }
@@ -225,11 +256,11 @@ extern "C" int yastn_hip_svd_adjoint_core(void* J, void* T, const void* X, const
   YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
   if (nblocks == 0) return YH_OK;
   if (dtype == YH_F64)
-    hipLaunchKernelGGL(svd_adj_core_kernel<false>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, (double*)J,
+    hipLaunchKernelGGL(svd_adj_core_kernel<false>, dim3((unsigned)(nblocks * kSplit)), dim3(kThreads), 0, stream, (double*)J,
                        (double*)T, (const double*)X, (const double*)Y, (const double*)S, (const double*)gS,
                        (const double*)gU, (const double*)gVh, d_descs, eps, has_gS, has_gU, has_gV);
   else
-    hipLaunchKernelGGL(svd_adj_core_kernel<true>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, (double2*)J,
+    hipLaunchKernelGGL(svd_adj_core_kernel<true>, dim3((unsigned)(nblocks * kSplit)), dim3(kThreads), 0, stream, (double2*)J,
                        (double2*)T, (const double2*)X, (const double2*)Y, (const double*)S, (const double*)gS,
                        (const double2*)gU, (const double2*)gVh, d_descs, eps, has_gS, has_gU, has_gV);
   return yh::hip_status(hipGetLastError(), "svd_adjoint_core launch");
@@ -241,10 +272,10 @@ extern "C" int yastn_hip_eigh_adjoint_core(void* Xo, const void* X, const void* 
   YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
   if (nblocks == 0) return YH_OK;
   if (dtype == YH_F64)
-    hipLaunchKernelGGL(eigh_adj_core_kernel<false>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream,
+    hipLaunchKernelGGL(eigh_adj_core_kernel<false>, dim3((unsigned)(nblocks * kSplit)), dim3(kThreads), 0, stream,
                        (double*)Xo, (const double*)X, (const double*)S, (const double*)gS, d_descs, has_gS, has_gU);
   else
-    hipLaunchKernelGGL(eigh_adj_core_kernel<true>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream,
+    hipLaunchKernelGGL(eigh_adj_core_kernel<true>, dim3((unsigned)(nblocks * kSplit)), dim3(kThreads), 0, stream,
                        (double2*)Xo, (const double2*)X, (const double*)S, (const double*)gS, d_descs, has_gS,
                        has_gU);
   return yh::hip_status(hipGetLastError(), "eigh_adjoint_core launch");

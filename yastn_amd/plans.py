@@ -478,7 +478,7 @@ def choose_tile(plan):
     N = plan.h_gemms[:, 1].astype(np.float64)
     K = np.maximum(plan.k_of.astype(np.float64), 1.0)
     eff = {L.YH_TILE_32x32: 0.35, L.YH_TILE_64x64: 0.8, L.YH_TILE_128x64: 1.0, L.YH_TILE_16x64: 0.3,
-           L.YH_TILE_64x16: 0.3}
+           L.YH_TILE_64x16: 0.3, L.YH_TILE_128x128: 1.1}
     best, best_cost = None, None
     for cfg, (bm, bn) in L.TILE_DIMS.items():
         tiles = np.ceil(M / bm) * np.ceil(N / bn)
@@ -567,11 +567,13 @@ class SvdPlan:
         words, small, big = [], [], []
         work = 0
         lds_bytes = 0
+        # the LDS kernel also holds ~8 KiB of static shared memory
+        lds_limit = min(L.YH_SVD_LDS_BYTES, L.device_lds_per_block() - 9 * 1024)
         for b, (a_off, m, n, u_off, s_off, v_off) in enumerate(blocks):
             tr = mode == 0 and m < n
             p, q = (n, m) if tr else (m, n)
             need = p * q + q * q
-            if need * itemsize <= L.YH_SVD_LDS_BYTES:
+            if need * itemsize <= lds_limit:
                 small.append(b)
                 lds_bytes = max(lds_bytes, need * itemsize)
                 w_off = 0
