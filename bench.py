@@ -215,6 +215,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-ctmrg', action='store_true')
     ap.add_argument('--only-heff2', action='store_true', help='headline workload only (for profiling)')
+    ap.add_argument('--copy-probe', action='store_true',
+                    help='also run one 256 MiB device copy, the reference dispatch for scaling HBM counters')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU sharding of "
@@ -228,18 +230,22 @@ def main():
            'config': {'workload': 'DMRG two-site Heff2 apply (4 tensordots), U(1) spin-1/2, chi=1024 over 25 '
                                   'sectors (D(q) ~ exp(-q^2/32)), XXZ MPO bond 5, float64 N(0,1) seed 0',
                       'tensordot_policy': 'fuse_contracted', 'device': torch.cuda.get_device_name(0)}}
+    if args.copy_probe:
+        src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device='cuda:0').fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     print('bench: heff2', flush=True)
     AA, FL, FR, Wm, flops = bench_heff2(args, out)
     if not args.only_heff2:
         print('bench: u1xu1', flush=True)
         bench_u1xu1(args, out)
         if not args.no_ctmrg:
-            try:
-                from yastn_amd import ctmrg
-                print('bench: ctmrg', flush=True)
-                out['ctmrg'] = ctmrg.bench(steps=max(2, args.steps // 5))
-            except ImportError:
-                pass
+            from yastn_amd import ctmrg
+            print('bench: ctmrg', flush=True)
+            out['ctmrg'] = ctmrg.bench(steps=max(2, args.steps // 5))
     if not args.no_cpu:
         print('bench: cpu baseline', flush=True)
         bench_cpu(args, out, flops)

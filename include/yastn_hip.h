@@ -176,6 +176,32 @@ int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const void* work, c
                                  const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, int32_t* d_info,
                                  int sweeps, hipStream_t stream);
 
+/* QR preconditioning of the panel path (SVD blocks only; no reference counterpart: the reference
+ * calls LAPACK gesdd / rocSOLVER gesvd per block, backend_np.py:281-286, torch_svd_gesdd.py:15-21).
+ * W0 = Q R by blocked Householder QR, one-sided Jacobi on X = R^H (q x q), W0 = (Q V_J) Sigma U_J^H.
+ * QR descriptor, YH_QR_WORDS int64 per block (indexed like the SVD block words):
+ *   0 p, 1 q, 2 w_off (W0, p x q col-major), 3 vall_off (Householder vectors, p x q col-major),
+ *   4 tm_off (-T per panel, YH_QR_NB^2 col-major each), 5 x_off (X then V_J, q x q each),
+ *   6 y_off (2 x YH_QR_NB x q scratch), 7 unused.
+ * The trailing-column updates and the final multiplication by Q run through yastn_hip_grouped_gemm
+ * with plans built by the host. */
+#define YH_QR_WORDS 8
+#define YH_QR_NB 16
+int yastn_hip_check_qr_plan(const int64_t* h_qdesc, const int64_t* h_blocks, const int32_t* h_ids, int64_t n,
+                            int dtype, int64_t work_numel, int64_t lds_bytes);
+/* Factor panel `step` (columns step*YH_QR_NB ...) of every listed block (lds_bytes >= p*YH_QR_NB*itemsize). */
+int yastn_hip_svd_qr_panel(void* work, int dtype, const int64_t* d_qdesc, const int32_t* d_ids, int64_t n, int step,
+                           int64_t lds_bytes, hipStream_t stream);
+/* X = R^H, V_J = I. */
+int yastn_hip_svd_precond_init(void* work, int dtype, const int64_t* d_qdesc, const int32_t* d_ids, int64_t n,
+                               hipStream_t stream);
+/* sigma, sorted; q-side vectors x_j / sigma_j (completed where sigma is at the noise level); p-side
+ * vectors initialised to [V_J; 0] (U for m >= n, conjugated rows of Vh for m < n), to be multiplied
+ * by Q.  d_blocks are the block's SVD words (original m, n and output offsets). */
+int yastn_hip_svd_precond_finalize(void* U, void* S, void* Vh, const void* work, int dtype, const int64_t* d_blocks,
+                                   const int64_t* d_qdesc, const int32_t* d_ids, int64_t n, int32_t* d_info,
+                                   int sweeps, hipStream_t stream);
+
 /* A11: replaces backend.fix_svd_signs(Udata, Vdata, meta) (backend_np.py:447-458,
  *      backend_torch.py:353-366): in place on U and Vh; blocks are SVD plan words. */
 int yastn_hip_fix_svd_signs(void* U, void* Vh, int dtype, const int64_t* d_blocks, int64_t nblocks,

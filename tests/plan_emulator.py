@@ -89,3 +89,53 @@ def svd_adjoint_core(descs, J, T, X, Y, S, gS, gU, gVh, eps, has_gS, has_gU, has
             if has_gV:
                 t = sinv[:, None] * gVh[v_off:v_off + k * n].reshape(k, n)
             T[tt:tt + k * n] = t.ravel()
+
+
+QR_NB = 16
+
+
+def _colmajor(buf, off, rows, cols):
+    """View of a column-major rows x cols matrix stored at buf[off:]."""
+    return buf[off:off + rows * cols].reshape(cols, rows).T
+
+
+def qr_panel(work, qd, step):
+    """numpy transliteration of the Householder panel kernel (yastn_amd/csrc/jacobi.hip)."""
+    p, q, w_off, vall_off, tm_off, x_off, y_off, _ = (int(x) for x in qd)
+    k0 = step * QR_NB
+    nb = min(QR_NB, q - k0)
+    Lr = p - k0
+    W0 = _colmajor(work, w_off, p, q)
+    Vall = _colmajor(work, vall_off, p, q)
+    P = W0[k0:, k0:k0 + nb].copy()
+    taus = []
+    for j in range(nb):
+        alpha = P[j, j]
+        xn2 = float(np.sum(np.abs(P[j + 1:, j]) ** 2))
+        if xn2 == 0.0 and np.imag(alpha) == 0.0:
+            tau, beta = 0.0, np.real(alpha)
+        else:
+            nrm = np.sqrt(abs(alpha) ** 2 + xn2)
+            beta = -nrm if np.real(alpha) >= 0 else nrm
+            tau = (beta - alpha) / beta
+            P[j + 1:, j] = P[j + 1:, j] / (alpha - beta)
+        P[j, j] = beta
+        v = np.concatenate([[1.0], P[j + 1:, j]])
+        for c in range(j + 1, nb):
+            w = np.vdot(v, P[j:, c])
+            P[j:, c] = P[j:, c] - np.conj(tau) * v * w
+        taus.append(tau)
+    V = np.zeros((Lr, nb), dtype=work.dtype)
+    for j in range(nb):
+        V[j, j] = 1.0
+        V[j + 1:, j] = P[j + 1:, j]
+    T = np.zeros((nb, nb), dtype=work.dtype)
+    for j in range(nb):
+        T[j, j] = taus[j]
+        z = V[:, :j].conj().T @ V[:, j]
+        T[:j, j] = -taus[j] * (T[:j, :j] @ z)
+    Tm = _colmajor(work, tm_off + step * QR_NB * QR_NB, QR_NB, QR_NB)
+    Tm[:] = 0
+    Tm[:nb, :nb] = -np.triu(T)
+    W0[k0:, k0:k0 + nb] = np.triu(P)
+    Vall[k0:, k0:k0 + nb] = V

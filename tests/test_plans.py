@@ -128,3 +128,77 @@ def test_uncovered_boxes():
 def test_tile_choice_prefers_small_tiles_for_small_sectors():
     small = P.dot_plan(tuple(((0, 64), (8, 8), (0, 64), (8, 8), (0, 64), (8, 8)) for _ in range(1)), 64, 64, 64, CPU)
     assert small.cfg == P.L.YH_TILE_32x32
+
+
+@pytest.mark.parametrize('cplx', [False, True])
+def test_qr_preconditioned_svd_plan_algebra(cplx):
+    """The QR-preconditioned panel path, executed with numpy interpreters of its kernels and of its
+    grouped-GEMM plans (the Jacobi stage replaced by numpy's SVD of X = R^H), reconstructs every
+    block with orthonormal factors, for m >= n and m < n blocks of several panels."""
+    rng = np.random.default_rng(5)
+    shapes = [(100, 80), (70, 120), (130, 130), (90, 37)]
+    dt = np.complex128 if cplx else np.float64
+    data, meta, uo, so, vo, off = [], [], 0, 0, 0, 0
+    for m, n in shapes:
+        k = min(m, n)
+        x = rng.standard_normal(m * n) + (1j * rng.standard_normal(m * n) if cplx else 0)
+        data.append(x.astype(dt))
+        meta.append(((off, off + m * n), (m, n), (uo, uo + m * k), (m, k), (so, so + k), (vo, vo + k * n), (k, n)))
+        off, uo, so, vo = off + m * n, uo + m * k, so + k, vo + k * n
+    data = np.concatenate(data)
+    plan = P.SvdPlan([(sl[0], D[0], D[1], slU[0], slS[0], slV[0]) for sl, D, slU, DU, slS, slV, DV in meta],
+                     data.itemsize, 0, data.size, uo, so, vo, CPU)
+    pre = plan.d_pre.numpy().tolist()
+    assert plan.n_pre >= 3
+    work = np.zeros(plan.work_numel, dtype=dt)
+    U = np.zeros(uo, dtype=dt)
+    S = np.zeros(so)
+    Vh = np.zeros(vo, dtype=dt)
+    for b in pre:  # panel_init: W0 = A or A^H, column-major
+        a_off, m, n = (int(x) for x in plan.h_blocks[b, :3])
+        A = data[a_off:a_off + m * n].reshape(m, n)
+        W0 = A if m >= n else A.conj().T
+        p, q = W0.shape
+        E._colmajor(work, int(plan.h_qdesc[b, 2]), p, q)[:] = W0
+    for step, (d_ids, n_act, trail) in enumerate(plan.qr_steps):
+        for b in d_ids.numpy().tolist():
+            E.qr_panel(work, plan.h_qdesc[b], step)
+        if trail is not None:
+            for gp in trail:
+                E.gemm(gp.h_gemms, gp.h_terms, [work], [work])
+    for b in pre:
+        p, q, w_off, vall_off, tm_off, x_off, y_off, _ = (int(x) for x in plan.h_qdesc[b])
+        R = np.triu(E._colmajor(work, w_off, p, q)[:q, :])
+        a_off, m, n, u_off, s_off, v_off = (int(x) for x in plan.h_blocks[b, :6])
+        A = data[a_off:a_off + m * n].reshape(m, n)
+        W0 = A if m >= n else A.conj().T
+        # R is the R factor of W0 (up to the Householder signs): R^H R = W0^H W0
+        np.testing.assert_allclose(R.conj().T @ R, W0.conj().T @ W0, atol=1e-10 * np.linalg.norm(W0) ** 2)
+        X = R.conj().T
+        Uj, sj, Vjh = np.linalg.svd(X)
+        Vj = Vjh.conj().T
+        S[s_off:s_off + q] = sj
+        if m >= n:  # Vh (q x q) rows u_j^H, U (p x q) columns [v_j; 0]
+            Vh[v_off:v_off + q * q] = Uj.conj().T.ravel()
+            Ub = np.zeros((p, q), dtype=dt)
+            Ub[:q] = Vj
+            U[u_off:u_off + p * q] = Ub.ravel()
+        else:  # U (q x q) columns u_j, Vh (q x p) rows conj([v_j; 0])
+            U[u_off:u_off + q * q] = Uj.ravel()
+            Vb = np.zeros((q, p), dtype=dt)
+            Vb[:, :q] = Vj.conj().T
+            Vh[v_off:v_off + q * p] = Vb.ravel()
+    for h1, h2, h3 in plan.q_apply:
+        E.gemm(h1.h_gemms, h1.h_terms, [work], [work, U, Vh])
+        E.gemm(h2.h_gemms, h2.h_terms, [work], [work])
+        E.gemm(h3.h_gemms, h3.h_terms, [work, U, Vh], [work])
+    for b in pre:  # the blocks of the LDS kernel are not part of this path
+        sl, D, slU, DU, slS, slV, DV = meta[b]
+        a = data[sl[0]:sl[1]].reshape(D)
+        u = U[slU[0]:slU[1]].reshape(DU)
+        v = Vh[slV[0]:slV[1]].reshape(DV)
+        s = S[slS[0]:slS[1]]
+        k = len(s)
+        assert np.linalg.norm(u * s @ v - a) <= 1e-12 * np.linalg.norm(a)
+        assert np.linalg.norm(u.conj().T @ u - np.eye(k)) <= 1e-12 * k
+        assert np.linalg.norm(v @ v.conj().T - np.eye(k)) <= 1e-12 * k

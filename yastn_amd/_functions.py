@@ -325,19 +325,40 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
         flags = torch.zeros(max(plan.njobs, 1), dtype=torch.int32, device=data.device)
         L.call('yastn_hip_svd_panel_init', ptr(data), ptr(work), ptr(mu), ptr(floors), dt, mode,
                plan.d_blocks.data_ptr(), plan.d_big.data_ptr(), plan.n_big, st)
+        if plan.n_pre:
+            # W0 = Q R (Householder, one panel kernel and three grouped GEMMs per step), then X = R^H
+            for step, (d_ids, n, trail) in enumerate(plan.qr_steps):
+                L.call('yastn_hip_svd_qr_panel', ptr(work), dt, plan.d_qdesc.data_ptr(), d_ids.data_ptr(), n, step,
+                       plan.qr_lds, st)
+                if trail is not None:
+                    for gp in trail:
+                        run_gemm(gp, (work,), (work,), dt, tune=False)
+            L.call('yastn_hip_svd_precond_init', ptr(work), dt, plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(),
+                   plan.n_pre, st)
         jobs0, flags0 = plan.d_jobs.data_ptr(), flags.data_ptr()
         converged = False
         while sweeps < MAX_SWEEPS:
             for off, n in plan.rounds:
-                L.call('yastn_hip_svd_panel_round', ptr(work), ptr(floors), dt, plan.d_blocks.data_ptr(),
+                L.call('yastn_hip_svd_panel_round', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
                        jobs0 + 12 * off, n, plan.panel_width, plan.panel_lds, flags0 + 4 * off, st)
             sweeps += 1
             if not bool(flags.any().item()):
                 converged = True
                 break
-        L.call('yastn_hip_svd_panel_finalize', ptr(out_a if mode == 0 else out_a), ptr(out_s),
-               ptr(out_b) if mode == 0 else None, ptr(work), ptr(mu), dt, mode, plan.d_blocks.data_ptr(),
-               plan.d_big.data_ptr(), plan.n_big, ptr(info), sweeps if converged else -1, st)
+        done = sweeps if converged else -1
+        if plan.n_plain:
+            L.call('yastn_hip_svd_panel_finalize', ptr(out_a), ptr(out_s), ptr(out_b) if mode == 0 else None,
+                   ptr(work), ptr(mu), dt, mode, plan.d_blocks.data_ptr(), plan.d_plain.data_ptr(), plan.n_plain,
+                   ptr(info), done, st)
+        if plan.n_pre:
+            L.call('yastn_hip_svd_precond_finalize', ptr(out_a), ptr(out_s), ptr(out_b), ptr(work), dt,
+                   plan.d_blocks.data_ptr(), plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(), plan.n_pre, ptr(info),
+                   done, st)
+            # p-side vectors: [V_J; 0] -> Q [V_J; 0], panels in reverse order
+            for h1, h2, h3 in plan.q_apply:
+                run_gemm(h1, (work,), (work, out_a, out_b), dt, tune=False)
+                run_gemm(h2, (work,), (work,), dt, tune=False)
+                run_gemm(h3, (work, out_a, out_b), (work,), dt, tune=False)
     _check_info(info[:plan.nblocks], 'svd' if mode == 0 else 'eigh')
 
 

@@ -25,6 +25,8 @@ TILE_DIMS = {YH_TILE_32x32: (32, 32), YH_TILE_64x64: (64, 64), YH_TILE_128x64: (
              YH_TILE_16x64: (16, 64), YH_TILE_64x16: (64, 16), YH_TILE_128x128: (128, 128)}
 YH_SVD_WORDS = 8
 YH_SVD_MAX_Q = 2048
+YH_QR_WORDS = 8
+YH_QR_NB = 16
 YH_SVD_LDS_BYTES = 131072
 
 _p = ctypes.c_void_p
@@ -56,6 +58,10 @@ _SIGS = {
     'yastn_hip_svd_panel_init': [_p, _p, _p, _p, _i32, _i32, _p, _p, _i64, _p],
     'yastn_hip_svd_panel_round': [_p, _p, _i32, _p, _p, _i64, _i32, _i64, _p, _p],
     'yastn_hip_svd_panel_finalize': [_p, _p, _p, _p, _p, _i32, _i32, _p, _p, _i64, _p, _i32, _p],
+    'yastn_hip_check_qr_plan': [_p, _p, _p, _i64, _i32, _i64, _i64],
+    'yastn_hip_svd_qr_panel': [_p, _i32, _p, _p, _i64, _i32, _i64, _p],
+    'yastn_hip_svd_precond_init': [_p, _i32, _p, _p, _i64, _p],
+    'yastn_hip_svd_precond_finalize': [_p, _p, _p, _p, _i32, _p, _p, _p, _i64, _p, _i32, _p],
     'yastn_hip_svd_sign_pivots': [_p, _p, _i32, _p, _i64, _p],
     'yastn_hip_svd_adjoint_core': [_p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _i64, _dbl, _i32, _i32, _i32, _p],
     'yastn_hip_eigh_adjoint_core': [_p, _p, _p, _p, _i32, _p, _i64, _i32, _i32, _p],

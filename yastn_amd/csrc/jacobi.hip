@@ -262,10 +262,13 @@ __device__ inline BlockDesc load_desc(const int64_t* w) {
 
 // Column norms -> singular values (descending, ties by index), vectors written by whole waves.
 // sigma_j is staged in the block's S slice; the ranks are kept in LDS.
+// MODE 2 is the QR-preconditioned SVD: W = X (q x q, p == q here), V = V_J, and pf is the long
+// dimension of the block; the q-side vectors x_j / sigma_j go to Vh (m >= n) or U (m < n) and the
+// p-side vectors start as [v_j; 0] in the other output, to be multiplied by Q afterwards.
 template <bool CPLX, int MODE>
 __device__ void finalize_block(const BlockDesc& d, const typename Num<CPLX>::T* W, const typename Num<CPLX>::T* V,
                                int64_t p, int64_t q, bool tr, double mu, typename Num<CPLX>::T* __restrict__ U,
-                               double* __restrict__ S, typename Num<CPLX>::T* __restrict__ Vh) {
+                               double* __restrict__ S, typename Num<CPLX>::T* __restrict__ Vh, int64_t pf = 0) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
   __shared__ int s_rank_of[YH_SVD_MAX_Q];
@@ -309,6 +312,21 @@ __device__ void finalize_block(const BlockDesc& d, const typename Num<CPLX>::T* 
     if (MODE == 1) {
       const int64_t pos = q - 1 - rkj;
       for (int64_t x = lane; x < q; x += 64) U[d.u_off + x * q + pos] = vj[x];
+    } else if (MODE == 2) {
+      const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+      if (!tr) {  // W0 = A = (Q V_J) Sigma U_J^H: Vh (q x q) row rkj = u_j^H, U (pf x q) column rkj = [v_j; 0]
+        for (int64_t x = lane; x < q; x += 64) {
+          const T w = wj[x];
+          Vh[d.v_off + rkj * q + x] = N::make(N::re(w) * inv, -N::im(w) * inv);
+        }
+        for (int64_t x = lane; x < pf; x += 64) U[d.u_off + x * q + rkj] = x < q ? vj[x] : N::make(0.0, 0.0);
+      } else {  // W0 = A^H: U (q x q) column rkj = u_j, Vh (q x pf) row rkj = [v_j; 0]^H
+        for (int64_t x = lane; x < q; x += 64) {
+          const T w = wj[x];
+          U[d.u_off + x * q + rkj] = N::make(N::re(w) * inv, N::im(w) * inv);
+        }
+        for (int64_t x = lane; x < pf; x += 64) Vh[d.v_off + rkj * pf + x] = x < q ? N::conj(vj[x]) : N::make(0.0, 0.0);
+      }
     } else if (!tr) {
       // A V = W: U (m x k) column rkj = w_j / sigma_j, Vh (k x n) row rkj = v_j^H
       const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
@@ -331,7 +349,7 @@ __device__ void finalize_block(const BlockDesc& d, const typename Num<CPLX>::T* 
 #pragma unroll
   for (int z = 0; z < kMaxCols; ++z) {
     if (rk[z] < 0) continue;
-    if (MODE == 0)
+    if (MODE != 1)
       S[d.s_off + rk[z]] = sg[z];
     else
       S[d.s_off + (q - 1 - rk[z])] = sg[z] - mu;
@@ -629,19 +647,22 @@ template <bool CPLX>
 __global__ void __launch_bounds__(kThreads)
     complete_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ ids,
                     typename Num<CPLX>::T* __restrict__ U, const double* __restrict__ S,
-                    typename Num<CPLX>::T* __restrict__ Vh) {
+                    typename Num<CPLX>::T* __restrict__ Vh, int swap) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
   __shared__ double s_red[kWaves];
   __shared__ double s_cr[YH_SVD_MAX_Q], s_ci[YH_SVD_MAX_Q];
   const int bidx = ids[blockIdx.x];
   const BlockDesc d = load_desc(blocks + (int64_t)bidx * YH_SVD_WORDS);
-  const bool tr = d.m < d.n;
-  const int64_t p = tr ? d.n : d.m, k = tr ? d.m : d.n;
+  const int64_t k = d.m < d.n ? d.m : d.n;
+  // the normalised-column side: columns of U (length m) or rows of Vh (length n); without
+  // preconditioning it is U when m >= n, with it (swap) the other one
+  const bool side_vh = (d.m < d.n) != (swap != 0);
+  const int64_t p = side_vh ? d.n : d.m;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // vector c, element x, at base[c * cs + x * xs]
-  T* base = tr ? Vh + d.v_off : U + d.u_off;
-  const int64_t cs = tr ? p : 1, xs = tr ? 1 : k;
+  T* base = side_vh ? Vh + d.v_off : U + d.u_off;
+  const int64_t cs = side_vh ? d.n : 1, xs = side_vh ? 1 : k;
   double f = 0.0;
   for (int64_t j = tid; j < k; j += kThreads) f += S[d.s_off + j] * S[d.s_off + j];
   const double thr2 = 4.0 * DBL_EPSILON * DBL_EPSILON * block_sum(f, s_red, lane, wave);
@@ -706,11 +727,11 @@ __global__ void __launch_bounds__(kThreads)
 
 template <bool CPLX>
 int launch_complete(const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, void* U, const void* S, void* Vh,
-                    hipStream_t stream) {
+                    hipStream_t stream, int swap = 0) {
   typedef typename Num<CPLX>::T T;
   if (nb == 0) return YH_OK;
   hipLaunchKernelGGL(complete_kernel<CPLX>, dim3((unsigned)nb), dim3(kThreads), 0, stream, d_blocks, d_ids, (T*)U,
-                     (const double*)S, (T*)Vh);
+                     (const double*)S, (T*)Vh, swap);
   return yh::hip_status(hipGetLastError(), "svd completion launch");
 }
 
@@ -741,6 +762,199 @@ int launch_jacobi(const int64_t* d_blocks, const int32_t* d_order, int64_t n_lds
   }
   if (MODE == 0) return launch_complete<CPLX>(d_blocks, d_order, nblocks, U, S, Vh, stream);
   return YH_OK;
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// QR preconditioning of the panel path for SVD blocks (Drmac & Veselic, "New fast and accurate
+// Jacobi SVD algorithm I", SIAM J. Matrix Anal. Appl. 29 (2008) 1322-1342): with W0 = Q R
+// (Householder QR of the p x q working matrix), the one-sided iteration runs on X = R^H (q x q),
+// which needs about half the sweeps; X V_J = U_J Sigma gives W0 = (Q V_J) Sigma U_J^H.
+//
+// QR descriptor, YH_QR_WORDS int64 per block: 0 p, 1 q, 2 w_off (W0, p x q column-major; R in its
+// upper triangle afterwards), 3 vall_off (Householder vectors, p x q column-major, unit diagonal;
+// rows above the diagonal are never read), 4 tm_off (-T of each panel, YH_QR_NB x YH_QR_NB
+// column-major, panel after panel), 5 x_off (X, q x q column-major, then V_J, q x q), 6 y_off
+// (two YH_QR_NB x q row-major scratch matrices for the GEMM steps), 7 unused.
+struct QrDesc {
+  int64_t p, q, w_off, vall_off, tm_off, x_off, y_off;
+};
+
+__device__ inline QrDesc load_qr(const int64_t* w) {
+  QrDesc d;
+  d.p = w[0];
+  d.q = w[1];
+  d.w_off = w[2];
+  d.vall_off = w[3];
+  d.tm_off = w[4];
+  d.x_off = w[5];
+  d.y_off = w[6];
+  return d;
+}
+
+// One panel of the blocked Householder QR (LAPACK geqrf / larfg / larft conventions): the
+// (p - k0) x nb panel at column k0 = step * YH_QR_NB is factored in LDS column by column,
+// H_j = I - tau_j v_j v_j^H with H_j^H x = beta e_1 (beta real), applying H_j^H to the panel's
+// later columns.  The vectors go to Vall, the R part back to W0, and -T, with H_1 ... H_nb =
+// I - V T V^H, to Tm.  The columns right of the panel are updated by grouped GEMMs on the host's plan.
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    qr_panel_kernel(const int64_t* __restrict__ qdesc, const int32_t* __restrict__ ids,
+                    typename Num<CPLX>::T* __restrict__ work, int step) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  constexpr int NB = YH_QR_NB;
+  extern __shared__ double smem_raw[];
+  __shared__ double s_red[kWaves];
+  __shared__ double s_tr[NB], s_ti[NB];          // tau_j
+  __shared__ double s_Tr[NB][NB], s_Ti[NB][NB];  // T (upper triangular), [row][col]
+  __shared__ double s_zr[NB], s_zi[NB];
+  const QrDesc d = load_qr(qdesc + (int64_t)ids[blockIdx.x] * YH_QR_WORDS);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t k0 = (int64_t)step * NB;
+  const int nb = (int)((d.q - k0) < NB ? (d.q - k0) : NB);
+  const int64_t L = d.p - k0;
+  const int64_t p = d.p;
+  T* P = reinterpret_cast<T*>(smem_raw);  // L x nb, column-major, leading dimension L
+  T* W0 = work + d.w_off;
+  T* Vall = work + d.vall_off;
+  for (int64_t e = tid; e < L * nb; e += kThreads) {
+    const int64_t c = e / L, r = e - c * L;
+    P[e] = W0[(k0 + r) + (k0 + c) * p];
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    T* x = P + (int64_t)j * L;
+    double loc = 0.0;
+    for (int64_t r = j + 1 + tid; r < L; r += kThreads) loc += N::abs2(x[r]);
+    const double xn2 = block_sum(loc, s_red, lane, wave);
+    const T alpha = x[j];
+    double beta, taur = 0.0, taui = 0.0, scr = 0.0, sci = 0.0;
+    if (xn2 == 0.0 && N::im(alpha) == 0.0) {
+      beta = N::re(alpha);  // H_j = I
+    } else {
+      const double nrm = sqrt(N::abs2(alpha) + xn2);
+      beta = N::re(alpha) >= 0.0 ? -nrm : nrm;
+      taur = (beta - N::re(alpha)) / beta;
+      taui = -N::im(alpha) / beta;
+      const double dr = N::re(alpha) - beta, di = N::im(alpha);  // v = x / (alpha - beta)
+      const double den = dr * dr + di * di;
+      scr = dr / den;
+      sci = -di / den;
+    }
+    __syncthreads();  // every thread has read alpha
+    for (int64_t r = j + 1 + tid; r < L; r += kThreads) {
+      const T v = x[r];
+      x[r] = N::make(N::re(v) * scr - N::im(v) * sci, N::re(v) * sci + N::im(v) * scr);
+    }
+    if (tid == 0) {
+      x[j] = N::make(beta, 0.0);
+      s_tr[j] = taur;
+      s_ti[j] = taui;
+    }
+    __syncthreads();
+    // H_j^H = I - conj(tau) v v^H on the later panel columns, one wave per column; v = [1; x[j+1:]]
+    for (int c = j + 1 + wave; c < nb; c += kWaves) {
+      T* y = P + (int64_t)c * L;
+      double wr = 0.0, wi = 0.0;
+      for (int64_t r = j + 1 + lane; r < L; r += 64) {
+        const T v = x[r], u = y[r];
+        wr += N::re(v) * N::re(u) + N::im(v) * N::im(u);
+        wi += N::re(v) * N::im(u) - N::im(v) * N::re(u);
+      }
+      wr = wave_sum(wr) + N::re(y[j]);
+      if (CPLX) wi = wave_sum(wi);
+      wi += N::im(y[j]);
+      const double cr = taur * wr + taui * wi, ci = taur * wi - taui * wr;  // conj(tau) w
+      for (int64_t r = j + 1 + lane; r < L; r += 64) {
+        const T v = x[r], u = y[r];
+        y[r] = N::make(N::re(u) - (N::re(v) * cr - N::im(v) * ci), N::im(u) - (N::re(v) * ci + N::im(v) * cr));
+      }
+      if (lane == 0) y[j] = N::make(N::re(y[j]) - cr, N::im(y[j]) - ci);
+    }
+    __syncthreads();
+  }
+  // T by columns: T(j, j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) z, z_i = V(:, i)^H v_j
+  for (int j = 0; j < nb; ++j) {
+    const T* vj = P + (int64_t)j * L;
+    for (int i = wave; i < j; i += kWaves) {
+      const T* vi = P + (int64_t)i * L;
+      double zr = 0.0, zi = 0.0;
+      for (int64_t r = j + 1 + lane; r < L; r += 64) {
+        const T a = vi[r], b = vj[r];
+        zr += N::re(a) * N::re(b) + N::im(a) * N::im(b);
+        zi += N::re(a) * N::im(b) - N::im(a) * N::re(b);
+      }
+      zr = wave_sum(zr);
+      if (CPLX) zi = wave_sum(zi);
+      if (lane == 0) {  // row j of v_i times the unit entry of v_j
+        s_zr[i] = zr + N::re(vi[j]);
+        s_zi[i] = zi - N::im(vi[j]);
+      }
+    }
+    __syncthreads();
+    if (tid < j) {
+      double ar = 0.0, ai = 0.0;
+      for (int l = tid; l < j; ++l) {
+        ar += s_Tr[tid][l] * s_zr[l] - s_Ti[tid][l] * s_zi[l];
+        ai += s_Tr[tid][l] * s_zi[l] + s_Ti[tid][l] * s_zr[l];
+      }
+      const double tr_ = s_tr[j], ti_ = s_ti[j];
+      s_Tr[tid][j] = -(tr_ * ar - ti_ * ai);
+      s_Ti[tid][j] = -(tr_ * ai + ti_ * ar);
+    }
+    if (tid == 0) {
+      s_Tr[j][j] = s_tr[j];
+      s_Ti[j][j] = s_ti[j];
+    }
+    __syncthreads();
+  }
+  T* Tm = work + d.tm_off + (int64_t)step * NB * NB;
+  for (int e = tid; e < NB * NB; e += kThreads) {
+    const int c = e / NB, r = e - c * NB;
+    Tm[e] = (r < nb && c < nb && r <= c) ? N::make(-s_Tr[r][c], -s_Ti[r][c]) : N::make(0.0, 0.0);
+  }
+  for (int64_t e = tid; e < L * nb; e += kThreads) {
+    const int64_t c = e / L, r = e - c * L;
+    const T v = P[e];
+    W0[(k0 + r) + (k0 + c) * p] = r <= c ? v : N::make(0.0, 0.0);
+    Vall[(k0 + r) + (k0 + c) * p] = r < c ? N::make(0.0, 0.0) : (r == c ? N::make(1.0, 0.0) : v);
+  }
+}
+
+// X = R^H (R is the q x q upper triangle of W0) and V_J = I.
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    precond_init_kernel(const int64_t* __restrict__ qdesc, const int32_t* __restrict__ ids,
+                        typename Num<CPLX>::T* __restrict__ work) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  const QrDesc d = load_qr(qdesc + (int64_t)ids[blockIdx.x] * YH_QR_WORDS);
+  const T* W0 = work + d.w_off;
+  T* X = work + d.x_off;
+  T* VJ = X + d.q * d.q;
+  for (int64_t e = threadIdx.x; e < d.q * d.q; e += kThreads) {
+    const int64_t c = e / d.q, r = e - c * d.q;
+    X[e] = r >= c ? N::conj(W0[c + r * d.p]) : N::make(0.0, 0.0);
+    VJ[e] = N::make(r == c ? 1.0 : 0.0, 0.0);
+  }
+}
+
+// Finalize of the preconditioned blocks: sigma, ranks, q-side vectors and the [V_J; 0] start of
+// the p-side vectors (jblocks: the Jacobi-stage descriptors, work_off = x_off, m = n = q).
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    precond_finalize_kernel(const int64_t* __restrict__ blocks, const int64_t* __restrict__ qdesc,
+                            const int32_t* __restrict__ ids, const typename Num<CPLX>::T* __restrict__ work,
+                            typename Num<CPLX>::T* __restrict__ U, double* __restrict__ S,
+                            typename Num<CPLX>::T* __restrict__ Vh, int32_t* __restrict__ info, int sweeps) {
+  const int bidx = ids[blockIdx.x];
+  const BlockDesc d = load_desc(blocks + (int64_t)bidx * YH_SVD_WORDS);
+  const QrDesc qd = load_qr(qdesc + (int64_t)bidx * YH_QR_WORDS);
+  const bool tr = d.m < d.n;
+  const typename Num<CPLX>::T* X = work + qd.x_off;
+  finalize_block<CPLX, 2>(d, X, X + qd.q * qd.q, qd.q, qd.q, tr, 0.0, U, S, Vh, qd.p);
+  if (threadIdx.x == 0) info[bidx] = sweeps;
 }
 
 }  // namespace
@@ -892,4 +1106,77 @@ extern "C" int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const vo
   if (st || mode == 1) return st;
   return dtype == YH_F64 ? launch_complete<false>(d_blocks, d_ids, nb, U, S, Vh, stream)
                          : launch_complete<true>(d_blocks, d_ids, nb, U, S, Vh, stream);
+}
+
+extern "C" int yastn_hip_check_qr_plan(const int64_t* h_qdesc, const int64_t* h_blocks, const int32_t* h_ids,
+                                       int64_t n, int dtype, int64_t work_numel, int64_t lds_bytes) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  const int64_t isz = yh::itemsize_of(dtype);
+  for (int64_t t = 0; t < n; ++t) {
+    const int32_t b = h_ids[t];
+    const int64_t* w = h_qdesc + (int64_t)b * YH_QR_WORDS;
+    const int64_t* sw = h_blocks + (int64_t)b * YH_SVD_WORDS;
+    const int64_t p = w[0], q = w[1];
+    const int64_t sm = sw[1], sn = sw[2];
+    const int64_t np = (q + YH_QR_NB - 1) / YH_QR_NB;
+    YH_CHECK_ARG(b >= 0 && q >= 1 && p >= q && q <= YH_SVD_MAX_Q, "qr block %d: %lld x %lld", b, (long long)p,
+                 (long long)q);
+    YH_CHECK_ARG((sm >= sn ? sm : sn) == p && (sm < sn ? sm : sn) == q, "qr block %d: shape mismatch", b);
+    YH_CHECK_ARG(p * YH_QR_NB * isz <= lds_bytes, "qr block %d: panel of %lld rows exceeds LDS", b, (long long)p);
+    const bool ok = w[2] >= 0 && w[2] + p * q <= work_numel && w[3] >= 0 && w[3] + p * q <= work_numel &&
+                    w[4] >= 0 && w[4] + np * YH_QR_NB * YH_QR_NB <= work_numel && w[5] >= 0 &&
+                    w[5] + 2 * q * q <= work_numel && w[6] >= 0 && w[6] + 2 * YH_QR_NB * q <= work_numel;
+    YH_CHECK_ARG(ok, "qr block %d: workspace offsets out of bounds", b);
+  }
+  return YH_OK;
+}
+
+extern "C" int yastn_hip_svd_qr_panel(void* work, int dtype, const int64_t* d_qdesc, const int32_t* d_ids, int64_t n,
+                                      int step, int64_t lds_bytes, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(step >= 0 && lds_bytes > 0 && lds_bytes <= 160 * 1024, "qr panel step %d / lds %lld", step,
+               (long long)lds_bytes);
+  if (n == 0) return YH_OK;
+  const dim3 g((unsigned)n), t(kThreads);
+  if (dtype == YH_F64) {
+    hipFuncSetAttribute((const void*)qr_panel_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds_bytes);
+    hipLaunchKernelGGL(qr_panel_kernel<false>, g, t, (size_t)lds_bytes, stream, d_qdesc, d_ids, (double*)work, step);
+  } else {
+    hipFuncSetAttribute((const void*)qr_panel_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds_bytes);
+    hipLaunchKernelGGL(qr_panel_kernel<true>, g, t, (size_t)lds_bytes, stream, d_qdesc, d_ids, (double2*)work, step);
+  }
+  return yh::hip_status(hipGetLastError(), "svd qr panel launch");
+}
+
+extern "C" int yastn_hip_svd_precond_init(void* work, int dtype, const int64_t* d_qdesc, const int32_t* d_ids,
+                                          int64_t n, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  if (n == 0) return YH_OK;
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(precond_init_kernel<false>, dim3((unsigned)n), dim3(kThreads), 0, stream, d_qdesc, d_ids,
+                       (double*)work);
+  else
+    hipLaunchKernelGGL(precond_init_kernel<true>, dim3((unsigned)n), dim3(kThreads), 0, stream, d_qdesc, d_ids,
+                       (double2*)work);
+  return yh::hip_status(hipGetLastError(), "svd precond init launch");
+}
+
+extern "C" int yastn_hip_svd_precond_finalize(void* U, void* S, void* Vh, const void* work, int dtype,
+                                              const int64_t* d_blocks, const int64_t* d_qdesc, const int32_t* d_ids,
+                                              int64_t n, int32_t* d_info, int sweeps, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  if (n == 0) return YH_OK;
+  const dim3 g((unsigned)n), t(kThreads);
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(precond_finalize_kernel<false>, g, t, 0, stream, d_blocks, d_qdesc, d_ids,
+                       (const double*)work, (double*)U, (double*)S, (double*)Vh, d_info, sweeps);
+  else
+    hipLaunchKernelGGL(precond_finalize_kernel<true>, g, t, 0, stream, d_blocks, d_qdesc, d_ids,
+                       (const double2*)work, (double2*)U, (double*)S, (double2*)Vh, d_info, sweeps);
+  const int st = yh::hip_status(hipGetLastError(), "svd precond finalize launch");
+  if (st) return st;
+  return dtype == YH_F64 ? launch_complete<false>(d_blocks, d_ids, n, U, S, Vh, stream, 1)
+                         : launch_complete<true>(d_blocks, d_ids, n, U, S, Vh, stream, 1);
 }

@@ -554,34 +554,59 @@ def _circle_pair(ne, r, k):
     return (i, j) if i < j else (j, i)
 
 
+PRECOND = True  # QR-precondition the panel path of SVD blocks (off: plain one-sided Jacobi on W0)
+
+
 class SvdPlan:
     """Block list of the batched Jacobi SVD / eigh.
 
-    Blocks whose working matrices (W and V, (p q + q^2) elements) fit YH_SVD_LDS_BYTES run in the
+    Blocks whose working matrices (W and V, (p q + q^2) elements) fit the LDS limit run in the
     LDS-resident kernel, one workgroup per block, all in one launch.  Larger blocks use the panel
-    path: W and V in a workspace, panels of width b, one launch per round of panel pairs.
+    path: W and V in a workspace, panels of width b, one launch per round of panel pairs.  For SVD
+    blocks whose Householder panel (p x YH_QR_NB) fits LDS, the panel path first factors W0 = Q R
+    (blocked Householder QR: a panel kernel plus three grouped GEMMs per step) and iterates on
+    X = R^H; the p-side vectors are multiplied by Q at the end (three grouped GEMMs per step).
     """
 
     def __init__(self, blocks, itemsize, mode, a_numel, u_numel, s_numel, v_numel, device):
         # blocks: list of (a_off, m, n, u_off, s_off, v_off)
-        words, small, big = [], [], []
+        NB = L.YH_QR_NB
+        words, small, big, pre = [], [], [], []
         work = 0
         lds_bytes = 0
+        dev_lds = L.device_lds_per_block()
         # the LDS kernel also holds ~8 KiB of static shared memory
-        lds_limit = min(L.YH_SVD_LDS_BYTES, L.device_lds_per_block() - 9 * 1024)
+        lds_limit = min(L.YH_SVD_LDS_BYTES, dev_lds - 9 * 1024)
+        qr_limit = dev_lds - 8 * 1024
+        qd = np.zeros((len(blocks), L.YH_QR_WORDS), dtype=np.int64)
+        jwords = []
         for b, (a_off, m, n, u_off, s_off, v_off) in enumerate(blocks):
             tr = mode == 0 and m < n
             p, q = (n, m) if tr else (m, n)
             need = p * q + q * q
+            jw = None
             if need * itemsize <= lds_limit:
                 small.append(b)
                 lds_bytes = max(lds_bytes, need * itemsize)
                 w_off = 0
+            elif mode == 0 and PRECOND and p * NB * itemsize <= qr_limit:
+                big.append(b)
+                pre.append(b)
+                nst = -(-q // NB)
+                w_off = work
+                x_off = work + p * q
+                vall_off = x_off + 2 * q * q
+                tm_off = vall_off + p * q
+                y_off = tm_off + nst * NB * NB
+                work = y_off + 2 * NB * q
+                qd[b] = (p, q, w_off, vall_off, tm_off, x_off, y_off, 0)
+                jw = (a_off, q, q, u_off, s_off, v_off, x_off, 0)
             else:
                 big.append(b)
                 w_off = work
                 work += need
             words.append((a_off, m, n, u_off, s_off, v_off, w_off, 0))
+            jwords.append(jw if jw is not None else words[-1])
         size = lambda b: blocks[b][1] * blocks[b][2] * min(blocks[b][1], blocks[b][2])
         small.sort(key=lambda b: -size(b))
         big.sort(key=lambda b: -size(b))
@@ -590,16 +615,24 @@ class SvdPlan:
         self.nblocks, self.n_lds, self.lds_bytes, self.work_numel = len(words), len(small), lds_bytes, work
         self.flops = sum(4 * m * n * min(m, n) + 8 * min(m, n) ** 3 for (_, m, n, *_r) in blocks)
         dt = L.YH_C128 if itemsize == 16 else L.YH_F64
+        self.dt = dt
         L.call('yastn_hip_check_svd_plan', self.h_blocks.ctypes.data, self.nblocks, self.h_order.ctypes.data,
                self.n_lds, self.lds_bytes, mode, dt, a_numel, u_numel, s_numel, v_numel, max(work, 0))
         self.d_blocks = torch.from_numpy(self.h_blocks).to(device)
         self.d_order = torch.from_numpy(self.h_order).to(device)
         # panel path
         self.n_big = len(big)
+        self.n_pre = len(pre)
         self.rounds = []
         if big:
-            limit = L.device_lds_per_block() - 2048
-            pmax = max(max(blocks[b][1], blocks[b][2]) for b in big)
+            self.h_jblocks = np.ascontiguousarray(np.array(jwords, dtype=np.int64).reshape(-1, L.YH_SVD_WORDS))
+            if pre:
+                # the Jacobi stage of the preconditioned blocks works on X (q x q) in its own workspace
+                L.call('yastn_hip_check_svd_plan', self.h_jblocks.ctypes.data, self.nblocks,
+                       self.h_order.ctypes.data, 0, 0, mode, dt, a_numel, u_numel, s_numel, v_numel, max(work, 0))
+            self.d_jblocks = torch.from_numpy(self.h_jblocks).to(device)
+            limit = dev_lds - 2048
+            pmax = max(max(jwords[b][1], jwords[b][2]) for b in big)
             width = 16
             while width > 1 and (2 * width * pmax + 4 * width * width) * itemsize > limit:
                 width //= 2
@@ -627,7 +660,72 @@ class SvdPlan:
             self.d_jobs = torch.from_numpy(self.h_jobs).to(device)
             self.h_big = np.ascontiguousarray(np.array(big, dtype=np.int32))
             self.d_big = torch.from_numpy(self.h_big).to(device)
+            plain = [b for b in big if b not in set(pre)]
+            self.n_plain = len(plain)
+            self.d_plain = torch.from_numpy(np.array(plain or [0], dtype=np.int32)).to(device)
             self.njobs = len(jobs)
+        if pre:
+            self._qr_plans(blocks, pre, qd, itemsize, work, u_numel, v_numel, device)
+
+    def _qr_plans(self, blocks, pre, qd, itemsize, work, u_numel, v_numel, device):
+        """Panel lists and grouped-GEMM plans of the Householder QR (trailing updates) and of the
+        final multiplication of the p-side vectors by Q (buffers: 0 workspace, 1 U, 2 Vh)."""
+        NB = L.YH_QR_NB
+        self.h_qdesc = np.ascontiguousarray(qd)
+        pmax = max(int(qd[b, 0]) for b in pre)
+        self.qr_lds = pmax * NB * itemsize
+        ids = np.array(pre, dtype=np.int32)
+        L.call('yastn_hip_check_qr_plan', self.h_qdesc.ctypes.data, self.h_blocks.ctypes.data, ids.ctypes.data,
+               len(ids), self.dt, max(work, 0), self.qr_lds)
+        self.d_qdesc = torch.from_numpy(self.h_qdesc).to(device)
+        self.d_pre = torch.from_numpy(ids).to(device)
+        nsteps = max(-(-int(qd[b, 1]) // NB) for b in pre)
+        bufs = (max(work, 1), max(u_numel, 1), max(v_numel, 1))
+        self.qr_steps = []   # (d_ids, n, trailing plans)
+        self.q_apply = []    # per step (reverse order): plans H1, H2, H3
+        for s in range(nsteps):
+            k0 = s * NB
+            act = [b for b in pre if int(qd[b, 1]) > k0]
+            g1, t1, g2, t2, g3, t3 = [], [], [], [], [], []
+            h1, u1, h2, u2, h3, u3 = [], [], [], [], [], []
+            for b in act:
+                p, q, w_off, vall_off, tm_off, x_off, y_off, _ = (int(x) for x in qd[b])
+                m, n = blocks[b][1], blocks[b][2]
+                tr = m < n
+                nbs = min(NB, q - k0)
+                Lr = p - k0
+                tm = tm_off + s * NB * NB
+                y2 = y_off + NB * q
+                rest = q - k0 - nbs
+                v0 = vall_off + k0 + k0 * p
+                if rest > 0:
+                    # Y = V^H C ; Y2 = Tm^H Y ; C += V Y2   (Tm = -T: C <- (I - V T^H V^H) C)
+                    g1.append((nbs, rest, y_off, q, 1, len(t1), 1, 0))
+                    t1.append((v0, p, 1, w_off + k0 + (k0 + nbs) * p, 1, p, Lr, 1))
+                    g2.append((nbs, rest, y2, q, 1, len(t2), 1, 0))
+                    t2.append((tm, NB, 1, y_off, q, 1, nbs, 1))
+                    g3.append((Lr, rest, w_off + k0 + (k0 + nbs) * p, 1, p, len(t3), 1, 1))
+                    t3.append((v0, 1, p, y2, q, 1, nbs, 0))
+                # Q application on the p-side vectors M (U for m >= n; conj(M) stored in Vh rows for m < n)
+                if not tr:
+                    mb, m_off, m_rs, m_cs, cj = 1, blocks[b][3], q, 1, 0
+                else:
+                    mb, m_off, m_rs, m_cs, cj = 2, blocks[b][5], 1, p, 1
+                h1.append((nbs, q, y_off, q, 1, len(u1), 1, 0))
+                u1.append((v0, p, 1, m_off + k0 * m_rs, m_rs, m_cs, Lr, (1 - cj) | (0 << 8) | (mb << 16)))
+                h2.append((nbs, q, y2, q, 1, len(u2), 1, 0))
+                u2.append((tm, 1, NB, y_off, q, 1, nbs, cj))
+                h3.append((Lr, q, m_off + k0 * m_rs, m_rs, m_cs, len(u3), 1, 1 | (mb << 8)))
+                u3.append((v0, 1, p, y2, q, 1, nbs, cj))
+            d_ids = torch.from_numpy(np.array(act, dtype=np.int32)).to(device)
+            trail = None
+            if g1:
+                trail = (GemmPlan(g1, t1, bufs[:1], bufs[:1], device), GemmPlan(g2, t2, bufs[:1], bufs[:1], device),
+                         GemmPlan(g3, t3, bufs[:1], bufs[:1], device))
+            self.qr_steps.append((d_ids, len(act), trail))
+            self.q_apply.append((GemmPlan(h1, u1, bufs, bufs[:1], device), GemmPlan(h2, u2, bufs[:1], bufs[:1], device),
+                                 GemmPlan(h3, u3, bufs[:1], bufs, device)))
+        self.q_apply.reverse()
 
 
 def svd_plan(meta, itemsize, a_numel, sizes, device):
