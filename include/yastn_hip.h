@@ -176,6 +176,17 @@ int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const void* work, c
                                  const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, int32_t* d_info,
                                  int sweeps, hipStream_t stream);
 
+/* Split-K for GEMMs with few output tiles and a long K (e.g. a full contraction to a scalar): the
+ * host plan cuts K into S chunks whose GEMMs write partial results (row-major M x N each) to a
+ * scratch buffer, then this reduction sums them in a fixed order, so results stay deterministic.
+ * Record words: 0 M, 1 N, 2 c_off, 3 c_rs, 4 c_cs, 5 p_off, 6 S, 7 flags (bit 0 accumulate,
+ * bits 8-15 output buffer).  Part of A3 dot (backend_np.py:580-587). */
+#define YH_SPLITK_WORDS 8
+int yastn_hip_check_splitk_plan(const int64_t* h_recs, int64_t nrec, const int64_t* out_numel, int nout,
+                                int64_t part_numel);
+int yastn_hip_splitk_reduce(void* const* outs, const void* partials, int dtype, const int64_t* d_recs, int64_t nrec,
+                            hipStream_t stream);
+
 /* QR preconditioning of the panel path (SVD blocks only; no reference counterpart: the reference
  * calls LAPACK gesdd / rocSOLVER gesvd per block, backend_np.py:281-286, torch_svd_gesdd.py:15-21).
  * W0 = Q R by blocked Householder QR, one-sided Jacobi on X = R^H (q x q), W0 = (Q V_J) Sigma U_J^H.

@@ -139,3 +139,17 @@ def qr_panel(work, qd, step):
     Tm[:nb, :nb] = -np.triu(T)
     W0[k0:, k0:k0 + nb] = np.triu(P)
     Vall[k0:, k0:k0 + nb] = V
+
+
+def splitk_reduce(h_red, outs, part):
+    """numpy transliteration of the split-K reduction kernel (yastn_amd/csrc/gemm.hip)."""
+    for M, N, c_off, c_rs, c_cs, p_off, S, fl in (tuple(int(x) for x in r) for r in h_red):
+        acc = part[p_off:p_off + M * N].reshape(M, N).copy()
+        for s in range(1, S):
+            acc = acc + part[p_off + s * M * N:p_off + (s + 1) * M * N].reshape(M, N)
+        C = outs[(fl >> 8) & 0xff]
+        ic = c_off + np.arange(M)[:, None] * c_rs + np.arange(N)[None, :] * c_cs
+        if fl & 1:
+            C[ic] += acc
+        else:
+            C[ic] = acc

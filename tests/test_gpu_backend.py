@@ -382,3 +382,21 @@ def test_eigh_panel_path(B):
         w, u = _n(S), _n(U).reshape(n, n)
         np.testing.assert_allclose(w, np.linalg.eigvalsh(a), atol=1e-11 * np.abs(w).max())
         assert np.linalg.norm(u * w @ u.conj().T - a) <= 1e-11 * np.linalg.norm(a)
+
+
+def test_dot_split_k_long_contractions(B):
+    """Split-K GEMMs (small outputs, long K) against numpy, real and complex, and through autograd."""
+    rng = np.random.default_rng(16)
+    for cplx in (False, True):
+        shapes = [(1, 1, 200000), (3, 5, 20000), (40, 40, 300)]
+        A, Bm, meta, ao, bo, co = [], [], [], 0, 0, 0
+        for M, N, K in shapes:
+            A.append(rng.standard_normal(M * K) + (1j * rng.standard_normal(M * K) if cplx else 0))
+            Bm.append(rng.standard_normal(K * N) + (1j * rng.standard_normal(K * N) if cplx else 0))
+            meta.append(((co, co + M * N), (M, N), (ao, ao + M * K), (M, K), (bo, bo + K * N), (K, N)))
+            ao, bo, co = ao + M * K, bo + K * N, co + M * N
+        A, Bm, meta = np.concatenate(A), np.concatenate(Bm), tuple(meta)
+        print('split-k dot', cplx, shapes, flush=True)
+        got = _n(B.dot(_t(A), _t(Bm), meta, co))
+        ref = O.dot(A, Bm, meta, co)
+        assert _rel(got, ref) <= 1e-12

@@ -202,3 +202,28 @@ def test_qr_preconditioned_svd_plan_algebra(cplx):
         assert np.linalg.norm(u * s @ v - a) <= 1e-12 * np.linalg.norm(a)
         assert np.linalg.norm(u.conj().T @ u - np.eye(k)) <= 1e-12 * k
         assert np.linalg.norm(v @ v.conj().T - np.eye(k)) <= 1e-12 * k
+
+
+def test_split_k_plan_reproduces_long_contractions():
+    """GEMMs with small outputs and long K (a full contraction to a scalar) are cut into K chunks whose
+    partial products are summed in a fixed order; the interpreted plan equals the direct product."""
+    rng = np.random.default_rng(6)
+    shapes = [(1, 1, 50000), (3, 5, 20000), (40, 40, 300)]
+    A, B, meta, ao, bo, co = [], [], [], 0, 0, 0
+    for M, N, K in shapes:
+        A.append(rng.standard_normal(M * K))
+        B.append(rng.standard_normal(K * N))
+        meta.append(((co, co + M * N), (M, N), (ao, ao + M * K), (M, K), (bo, bo + K * N), (K, N)))
+        ao, bo, co = ao + M * K, bo + K * N, co + M * N
+    A, B = np.concatenate(A), np.concatenate(B)
+    plan = P.dot_plan(tuple(meta), A.size, B.size, co, CPU)
+    assert len(plan.h_red) == 2 and plan.part_numel > 0
+    out = np.zeros(co)
+    part = np.zeros(plan.part_numel)
+    outs = [out, None, None, None]
+    outs[plan.part_buf] = part
+    E.gemm(plan.h_gemms, plan.h_terms, outs, [A, B])
+    E.splitk_reduce(plan.h_red, outs, part)
+    for (slc, Dc, sla, Da, slb, Db) in meta:
+        ref = A[sla[0]:sla[1]].reshape(Da) @ B[slb[0]:slb[1]].reshape(Db)
+        np.testing.assert_allclose(out[slc[0]:slc[1]].reshape(Dc), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())

@@ -233,19 +233,32 @@ class EmbedMask(torch.autograd.Function):
 
 # ------------------------------------------------------------------ grouped GEMM
 
-def run_gemm(plan, outs, ins, dtype, tune=True):
-    outs = list(outs) + [None] * (L.YH_MAX_BUFS - len(outs))
+def launch_plan(plan, outs, ins, dtype, stream):
+    """One launch of a grouped-GEMM plan, followed by its split-K reduction when it has one (the
+    partial products then live in a scratch buffer appended to the outputs)."""
+    outs = list(outs)
+    part = None
+    if plan.part_numel:
+        dev = next(t for t in outs if t is not None)
+        part = torch.empty(plan.part_numel, dtype=dev.dtype, device=dev.device)
+        outs = outs + [None] * (plan.part_buf - len(outs)) + [part]
+    outs = outs + [None] * (L.YH_MAX_BUFS - len(outs))
     ins = list(ins) + [None] * (L.YH_MAX_BUFS - len(ins))
     po = (L.ctypes.c_void_p * L.YH_MAX_BUFS)(*[ptr(t) for t in outs])
     pi = (L.ctypes.c_void_p * L.YH_MAX_BUFS)(*[ptr(t) for t in ins])
+    L.call('yastn_hip_grouped_gemm', po, pi, dtype, *plan.args(), stream)
+    if part is not None:
+        L.call('yastn_hip_splitk_reduce', po, ptr(part), dtype, plan.d_red.data_ptr(), len(plan.h_red), stream)
+
+
+def run_gemm(plan, outs, ins, dtype, tune=True):
     dev = next(t for t in outs if t is not None)
-    run = lambda: L.call('yastn_hip_grouped_gemm', po, pi, dtype, *plan.args(), stream_of(dev))
+    run = lambda: launch_plan(plan, outs, ins, dtype, stream_of(dev))
     if tune and plan.accumulates and AUTOTUNE and not getattr(plan, 'tuned', False) \
             and not torch.cuda.is_current_stream_capturing():
         # an accumulating plan is timed on copies of its outputs, so the real ones are written once
         scratch = [t.clone() if t is not None else None for t in outs]
-        ps = (L.ctypes.c_void_p * L.YH_MAX_BUFS)(*[ptr(t) for t in scratch])
-        _maybe_autotune(plan, lambda: L.call('yastn_hip_grouped_gemm', ps, pi, dtype, *plan.args(), stream_of(dev)))
+        _maybe_autotune(plan, lambda: launch_plan(plan, scratch, ins, dtype, stream_of(dev)))
     elif tune:
         _maybe_autotune(plan, run)
     run()
@@ -261,8 +274,7 @@ class Dot(torch.autograd.Function):
         B = raw(Bdata.to(dtype))
         plan = P.dot_plan(meta_dot, A.numel(), B.numel(), Dsize, A.device)
         out = (torch.empty if plan.covered == Dsize else torch.zeros)(Dsize, dtype=dtype, device=A.device)
-        run = lambda: L.call('yastn_hip_dot', ptr(out), ptr(A), ptr(B), dtype_code(out), *plan.args(),
-                             stream_of(out))
+        run = lambda: launch_plan(plan, (out,), (A, B), dtype_code(out), stream_of(out))
         _maybe_autotune(plan, run)
         run()
         return out
@@ -283,8 +295,7 @@ class Dot(torch.autograd.Function):
         plan = P.dot_adjoint_plan(ctx.meta_dot, A.numel(), B.numel(), gC.numel(), A.device)
         gA = (torch.zeros if plan.zero_a else torch.empty)(A.numel(), dtype=dtype, device=A.device)
         gB = (torch.zeros if plan.zero_b else torch.empty)(B.numel(), dtype=dtype, device=A.device)
-        run = lambda: L.call('yastn_hip_dot_adjoint', ptr(gA), ptr(gB), ptr(gC), ptr(A), ptr(B), dtype_code(gA),
-                             *plan.args(), stream_of(gA))
+        run = lambda: launch_plan(plan, (gA, gB), (A, B, gC), dtype_code(gA), stream_of(gA))
         _maybe_autotune(plan, run)
         run()
         if not Adata.is_complex() and gA.is_complex():
@@ -303,6 +314,17 @@ def _check_info(info, what):
             raise L.HipBackendError(f"{what}: Jacobi did not converge in {MAX_SWEEPS} sweeps for {bad} block(s)")
 
 
+_SIDE = {}
+
+
+def _side_stream(device):
+    """A second stream per device (default priority), for work that overlaps the current stream."""
+    key = torch.device(device).index
+    if key not in _SIDE:
+        _SIDE[key] = torch.cuda.Stream(device=device)
+    return _SIDE[key]
+
+
 def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
     """LDS-resident blocks in one launch; larger blocks through the panel path (one launch per round of
     panel pairs, one host check per sweep).  mode 0: SVD (U, S, Vh); mode 1: eigh (S, U)."""
@@ -310,14 +332,24 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
     st = stream_of(data)
     work = torch.empty(max(plan.work_numel, 1), dtype=data.dtype, device=data.device)
     info = torch.empty(max(plan.nblocks, 1), dtype=torch.int32, device=data.device)
+    joined = None
     if plan.n_lds:
+        # the LDS-resident kernel (one workgroup per small block) runs on a side stream, overlapping
+        # the panel path of the large blocks, which occupies only part of the GPU; joined below
+        lds_st = st
+        if plan.n_big and not torch.cuda.is_current_stream_capturing():
+            cur = torch.cuda.current_stream(data.device)
+            side = _side_stream(data.device)
+            side.wait_stream(cur)
+            lds_st = side.cuda_stream
+            joined = (cur, side)
         if mode == 0:
             L.call('yastn_hip_svd', ptr(out_a), ptr(out_s), ptr(out_b), ptr(data), dt, plan.d_blocks.data_ptr(),
                    plan.n_lds, plan.d_order.data_ptr(), plan.n_lds, plan.lds_bytes, ptr(work), ptr(info),
-                   MAX_SWEEPS, st)
+                   MAX_SWEEPS, lds_st)
         else:
             L.call('yastn_hip_eigh', ptr(out_s), ptr(out_a), ptr(data), dt, plan.d_blocks.data_ptr(), plan.n_lds,
-                   plan.d_order.data_ptr(), plan.n_lds, plan.lds_bytes, ptr(work), ptr(info), MAX_SWEEPS, st)
+                   plan.d_order.data_ptr(), plan.n_lds, plan.lds_bytes, ptr(work), ptr(info), MAX_SWEEPS, lds_st)
     sweeps = 0
     if plan.n_big:
         mu = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
@@ -342,7 +374,10 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
                 L.call('yastn_hip_svd_panel_round', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
                        jobs0 + 12 * off, n, plan.panel_width, plan.panel_lds, flags0 + 4 * off, st)
             sweeps += 1
-            if not bool(flags.any().item()):
+            fmin, fmax = torch.stack([flags.min(), flags.max()]).tolist()
+            if fmin < 0:
+                raise RuntimeError("yastn_amd: svd panel round: LDS plan smaller than a panel pair")
+            if fmax == 0:
                 converged = True
                 break
         done = sweeps if converged else -1
@@ -359,6 +394,8 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
                 run_gemm(h1, (work,), (work, out_a, out_b), dt, tune=False)
                 run_gemm(h2, (work,), (work,), dt, tune=False)
                 run_gemm(h3, (work, out_a, out_b), (work,), dt, tune=False)
+    if joined is not None:
+        joined[0].wait_stream(joined[1])
     _check_info(info[:plan.nblocks], 'svd' if mode == 0 else 'eigh')
 
 

@@ -26,6 +26,7 @@ TILE_DIMS = {YH_TILE_32x32: (32, 32), YH_TILE_64x64: (64, 64), YH_TILE_128x64: (
 YH_SVD_WORDS = 8
 YH_SVD_MAX_Q = 2048
 YH_QR_WORDS = 8
+YH_SPLITK_WORDS = 8
 YH_QR_NB = 16
 YH_SVD_LDS_BYTES = 131072
 
@@ -48,6 +49,8 @@ _SIGS = {
     'yastn_hip_embed_mask': [_p, _i64, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
     'yastn_hip_check_gemm_plan': [_p, _i64, _p, _i64, _p, _i64, _i32, _p, _i32, _p, _i32],
     'yastn_hip_grouped_gemm': [_p, _p, _i32, _p, _i64, _p, _i64, _p, _i64, _i32, _p],
+    'yastn_hip_check_splitk_plan': [_p, _i64, _p, _i32, _i64],
+    'yastn_hip_splitk_reduce': [_p, _p, _i32, _p, _i64, _p],
     'yastn_hip_dot': [_p, _p, _p, _i32, _p, _i64, _p, _i64, _p, _i64, _i32, _p],
     'yastn_hip_dot_adjoint': [_p, _p, _p, _p, _p, _i32, _p, _i64, _p, _i64, _p, _i64, _i32, _p],
     'yastn_hip_check_svd_plan': [_p, _i64, _p, _i64, _i64, _i32, _i32, _i64, _i64, _i64, _i64, _i64],

@@ -248,6 +248,42 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
     }
 }
 
+
+// Split-K reduction: for each record, C (=|+=) sum_s P_s in the order s = 0, 1, ... (deterministic).
+// Record words (YH_SPLITK_WORDS): 0 M, 1 N, 2 c_off, 3 c_rs, 4 c_cs, 5 p_off (partial s at
+// p_off + s M N, row-major M x N), 6 S, 7 flags (bit 0 accumulate, bits 8-15 output buffer).
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads) splitk_reduce_kernel(Bufs bufs, const typename Elt<CPLX>::T* __restrict__ P,
+                                                                 const int64_t* __restrict__ recs) {
+  typedef typename Elt<CPLX>::T T;
+  const int64_t* w = recs + (int64_t)blockIdx.x * YH_SPLITK_WORDS;
+  const int64_t M = w[0], N = w[1], c_off = w[2], c_rs = w[3], c_cs = w[4], p_off = w[5], S = w[6], fl = w[7];
+  T* __restrict__ C = (T*)bufs.out[(fl >> 8) & 0xff];
+  for (int64_t e = threadIdx.x; e < M * N; e += kThreads) {
+    const int64_t i = e / N, j = e - i * N;
+    T acc = P[p_off + e];
+    for (int64_t s = 1; s < S; ++s) {
+      const T v = P[p_off + s * M * N + e];
+      if constexpr (CPLX) {
+        acc.x += v.x;
+        acc.y += v.y;
+      } else {
+        acc += v;
+      }
+    }
+    T* c = C + c_off + i * c_rs + j * c_cs;
+    if (fl & 1) {
+      if constexpr (CPLX) {
+        acc.x += (*c).x;
+        acc.y += (*c).y;
+      } else {
+        acc += *c;
+      }
+    }
+    *c = acc;
+  }
+}
+
 // BK per configuration: the double-buffered LDS tiles stay within 64 KiB.
 template <bool CPLX>
 int launch(const Bufs& b, const int64_t* gemms, const int64_t* terms, const int32_t* tiles, int64_t ntiles,
@@ -367,4 +403,39 @@ extern "C" int yastn_hip_dot_adjoint(void* gA, void* gB, const void* gC, const v
   void* outs[YH_MAX_BUFS] = {gA, gB, nullptr, nullptr};
   const void* ins[YH_MAX_BUFS] = {A, B, gC, nullptr};
   return yastn_hip_grouped_gemm(outs, ins, dtype, d_gemms, ngemms, d_terms, nterms, d_tiles, ntiles, cfg, stream);
+}
+
+extern "C" int yastn_hip_check_splitk_plan(const int64_t* h_recs, int64_t nrec, const int64_t* out_numel, int nout,
+                                           int64_t part_numel) {
+  YH_CHECK_ARG(nout <= YH_MAX_BUFS, "too many buffers");
+  for (int64_t r = 0; r < nrec; ++r) {
+    const int64_t* w = h_recs + r * YH_SPLITK_WORDS;
+    const int64_t M = w[0], N = w[1], c_off = w[2], c_rs = w[3], c_cs = w[4], p_off = w[5], S = w[6];
+    const int out = (int)((w[7] >> 8) & 0xff);
+    YH_CHECK_ARG(M >= 1 && N >= 1 && S >= 1 && c_rs >= 0 && c_cs >= 0 && out < nout, "split-k record %lld",
+                 (long long)r);
+    YH_CHECK_ARG(c_off >= 0 && c_off + (M - 1) * c_rs + (N - 1) * c_cs < out_numel[out],
+                 "split-k record %lld: output out of bounds", (long long)r);
+    YH_CHECK_ARG(p_off >= 0 && p_off + S * M * N <= part_numel, "split-k record %lld: partials out of bounds",
+                 (long long)r);
+  }
+  return YH_OK;
+}
+
+extern "C" int yastn_hip_splitk_reduce(void* const* outs, const void* partials, int dtype, const int64_t* d_recs,
+                                       int64_t nrec, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  if (nrec == 0) return YH_OK;
+  Bufs b;
+  for (int i = 0; i < YH_MAX_BUFS; ++i) {
+    b.in[i] = nullptr;
+    b.out[i] = outs[i];
+  }
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(splitk_reduce_kernel<false>, dim3((unsigned)nrec), dim3(kThreads), 0, stream, b,
+                       (const double*)partials, d_recs);
+  else
+    hipLaunchKernelGGL(splitk_reduce_kernel<true>, dim3((unsigned)nrec), dim3(kThreads), 0, stream, b,
+                       (const double2*)partials, d_recs);
+  return yh::hip_status(hipGetLastError(), "split-k reduce launch");
 }

@@ -20,6 +20,8 @@
 
 namespace {
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+
 // 16 waves per block: the q/2 disjoint pairs of a round are spread over more waves
 constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
@@ -138,6 +140,25 @@ __device__ inline bool rotation(double a, double b, double gr, double gi, double
   return true;
 }
 
+// (u, v) <- (c u - s e^{-i phi} v, s e^{i phi} u + c v); for real data e^{i phi} = +-1 is folded into s.
+template <bool CPLX>
+__device__ inline void apply_rot(typename Num<CPLX>::T& u, typename Num<CPLX>::T& v, double c, double s, double er,
+                                 double ei) {
+  typedef Num<CPLX> N;
+  if constexpr (CPLX) {
+    const double vr = er * N::re(v) + ei * N::im(v), vi = er * N::im(v) - ei * N::re(v);
+    const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);
+    const typename N::T u2 = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vi);
+    v = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
+    u = u2;
+  } else {
+    const double se = s * er;
+    const double u2 = c * u - se * v;
+    v = se * u + c * v;
+    u = u2;
+  }
+}
+
 // One Hestenes rotation of the column pair (w_i, w_j) (length p) and of the matching accumulator
 // columns (v_i, v_j) (length q), by one wave.  Returns true when the pair was rotated, i.e. when
 // |w_i^H w_j| > tol ||w_i|| ||w_j|| and both squared norms exceed floor2.  J = [[c, s e^{i phi}], [-s e^{-i phi}, c]], e^{i phi} = g / |g|.
@@ -163,18 +184,16 @@ __device__ inline bool jacobi_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>
   if (!rotation(a, b, gr, gi, tol, floor2, c, s, er, ei)) return false;
   // w_i' = c w_i - s e^{-i phi} w_j ; w_j' = s e^{i phi} w_i + c w_j
   for (int64_t x = lane; x < p; x += 64) {
-    const T u = wi[x], v = wj[x];
-    const double vr = er * N::re(v) + ei * N::im(v), vi = er * N::im(v) - ei * N::re(v);
-    const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);
-    wi[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vi);
-    wj[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
+    T u = wi[x], v = wj[x];
+    apply_rot<CPLX>(u, v, c, s, er, ei);
+    wi[x] = u;
+    wj[x] = v;
   }
   for (int64_t x = lane; x < q; x += 64) {
-    const T u = vi_[x], v = vj_[x];
-    const double vr = er * N::re(v) + ei * N::im(v), vv = er * N::im(v) - ei * N::re(v);
-    const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);
-    vi_[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vv);
-    vj_[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
+    T u = vi_[x], v = vj_[x];
+    apply_rot<CPLX>(u, v, c, s, er, ei);
+    vi_[x] = u;
+    vj_[x] = v;
   }
   return true;
 }
@@ -215,19 +234,17 @@ __device__ inline bool jacobi_pair_reg(typename Num<CPLX>::T* wi, typename Num<C
   for (int r = 0; r < NR; ++r) {
     const int64_t x = lane + 64 * r;
     if (x < p) {
-      const T u = xi[r], v = xj[r];
-      const double vr = er * N::re(v) + ei * N::im(v), vi = er * N::im(v) - ei * N::re(v);
-      const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);
-      wi[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vi);
-      wj[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
+      T u = xi[r], v = xj[r];
+      apply_rot<CPLX>(u, v, c, s, er, ei);
+      wi[x] = u;
+      wj[x] = v;
     }
   }
   for (int64_t x = lane; x < q; x += 64) {
-    const T u = vi_[x], v = vj_[x];
-    const double vr = er * N::re(v) + ei * N::im(v), vv = er * N::im(v) - ei * N::re(v);
-    const double ur = er * N::re(u) - ei * N::im(u), ui = er * N::im(u) + ei * N::re(u);
-    vi_[x] = N::make(c * N::re(u) - s * vr, c * N::im(u) - s * vv);
-    vj_[x] = N::make(s * ur + c * N::re(v), s * ui + c * N::im(v));
+    T u = vi_[x], v = vj_[x];
+    apply_rot<CPLX>(u, v, c, s, er, ei);
+    vi_[x] = u;
+    vj_[x] = v;
   }
   return true;
 }
@@ -522,7 +539,7 @@ template <bool CPLX>
 __global__ void __launch_bounds__(kPanelThreads)
     panel_round_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
                        typename Num<CPLX>::T* __restrict__ work, const double* __restrict__ floors,
-                       int32_t* __restrict__ flags, int b) {
+                       int32_t* __restrict__ flags, int b, int64_t lds_cap) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
   extern __shared__ double smem_raw[];
@@ -535,15 +552,21 @@ __global__ void __launch_bounds__(kPanelThreads)
   const int64_t c0i = (int64_t)jb[1] * b, c0j = (int64_t)jb[2] * b;
   const int64_t ci = min((int64_t)b, q - c0i), cj = min((int64_t)b, q - c0j);
   const int nc = (int)(ci + cj);
-  const int nq = 2 * b;  // leading dimension of Q
+  const int nq = 2 * b + 1;  // leading dimension of Q (odd: the column reads of the V update spread over banks)
   T* W = work + d.work_off;
   T* V = W + p * q;
   T* Wp = reinterpret_cast<T*>(smem_raw);
-  T* Q = Wp + (int64_t)nq * p;
+  T* Q = Wp + (int64_t)(2 * b) * p;  // after the 2b panel columns
+  if ((int64_t)((2 * b) * p + (int64_t)nc * nq) * (int64_t)sizeof(T) > lds_cap) {  // uniform: plan too small
+    if (tid == 0) flags[blockIdx.x] = -1;
+    return;
+  }
   auto gcol = [&](int64_t l) { return l < ci ? c0i + l : c0j + (l - ci); };
-  for (int64_t e = tid; e < (int64_t)nc * p; e += kPanelThreads) {
-    const int64_t l = e / p, r = e - l * p;
-    Wp[e] = W[gcol(l) * p + r];
+  constexpr int kPW = kPanelThreads / 64;
+  for (int l = wave; l < nc; l += kPW) {  // one wave per column, lanes along it
+    const T* src = W + gcol(l) * p;
+    T* dst = Wp + (int64_t)l * p;
+    for (int64_t r = lane; r < p; r += 64) dst[r] = src[r];
   }
   for (int e = tid; e < nc * nq; e += kPanelThreads) {
     const int l = e / nq, k = e - l * nq;
@@ -568,43 +591,45 @@ __global__ void __launch_bounds__(kPanelThreads)
   }
   if (tid == 0) flags[blockIdx.x] = s_rot;
   if (!s_rot) return;  // nothing rotated: W and V of the pair are unchanged (uniform branch)
-  for (int64_t e = tid; e < (int64_t)nc * p; e += kPanelThreads) {
-    const int64_t l = e / p, r = e - l * p;
-    W[gcol(l) * p + r] = Wp[e];
+  for (int l = wave; l < nc; l += kPW) {
+    const T* src = Wp + (int64_t)l * p;
+    T* dst = W + gcol(l) * p;
+    for (int64_t r = lane; r < p; r += 64) dst[r] = src[r];
   }
   __syncthreads();
-  // V panel columns (q x nc) staged where the W panel was (q <= p), then V <- V Q.  A thread owns
-  // one row r and a group of 8 columns; the Q reads of a wave are (mostly) one address, broadcast.
+  // V panel columns (q x nc) staged where the W panel was (q <= p), then V <- V Q on the FP64
+  // matrix cores: one wave per 16 x 16 output tile, v_mfma_f64_16x16x4f64 over k in steps of 4
+  // (lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16); item v of the result is
+  // D(l / 16 + 4 v, l % 16)).
   T* Vp = Wp;
-  for (int64_t e = tid; e < (int64_t)nc * q; e += kPanelThreads) {
-    const int64_t l = e / q, r = e - l * q;
-    Vp[e] = V[gcol(l) * q + r];
+  for (int l = wave; l < nc; l += kPW) {
+    const T* src = V + gcol(l) * q;
+    T* dst = Vp + (int64_t)l * q;
+    for (int64_t r = lane; r < q; r += 64) dst[r] = src[r];
   }
   __syncthreads();
-  const int ngrp = (nc + 7) / 8;
-  for (int64_t e = tid; e < (int64_t)ngrp * q; e += kPanelThreads) {
-    const int cg = (int)(e / q);
-    const int64_t r = e - (int64_t)cg * q;
-    double accr[8], acci[8];
-#pragma unroll
-    for (int cc = 0; cc < 8; ++cc) {
-      accr[cc] = 0.0;
-      acci[cc] = 0.0;
-    }
-    for (int k = 0; k < nc; ++k) {
-      const T v = Vp[(int64_t)k * q + r];
-#pragma unroll
-      for (int cc = 0; cc < 8; ++cc) {
-        const int c = cg * 8 + cc;
-        const T w = c < nc ? Q[c * nq + k] : N::make(0.0, 0.0);
-        accr[cc] += N::re(v) * N::re(w) - N::im(v) * N::im(w);
-        acci[cc] += N::re(v) * N::im(w) + N::im(v) * N::re(w);
+  const int qi = (int)q;  // q <= YH_SVD_MAX_Q
+  const int tq = (qi + 15) / 16, tcn = (nc + 15) / 16;
+  for (int t = wave; t < tq * tcn; t += kPW) {
+    const int ti = t / tcn, tj = t - ti * tcn;
+    const int ra = ti * 16 + (lane & 15), cb = tj * 16 + (lane & 15);
+    d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < nc; k0 += 4) {
+      const int k = k0 + (lane >> 4);
+      const T a = (ra < qi && k < nc) ? Vp[(int64_t)k * q + ra] : N::make(0.0, 0.0);
+      const T bq = (cb < nc && k < nc) ? Q[cb * nq + k] : N::make(0.0, 0.0);
+      accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::re(bq), accR, 0, 0, 0);
+      if constexpr (CPLX) {
+        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(a), N::im(bq), accR, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::im(bq), accI, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(a), N::re(bq), accI, 0, 0, 0);
       }
     }
+    const int col = tj * 16 + (lane & 15);
 #pragma unroll
-    for (int cc = 0; cc < 8; ++cc) {
-      const int c = cg * 8 + cc;
-      if (c < nc) V[gcol(c) * q + r] = N::make(accr[cc], acci[cc]);
+    for (int v = 0; v < 4; ++v) {
+      const int row = ti * 16 + (lane >> 4) + 4 * v;
+      if (row < qi && col < nc) V[gcol(col) * q + row] = N::make(accR[v], accI[v]);
     }
   }
 }
@@ -1070,12 +1095,12 @@ extern "C" int yastn_hip_svd_panel_round(void* work, const double* floors, int d
     hipFuncSetAttribute((const void*)panel_round_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds_bytes);
     hipLaunchKernelGGL(panel_round_kernel<false>, g, t, (size_t)lds_bytes, stream, d_blocks, d_jobs,
-                       (double*)work, floors, d_flags, b);
+                       (double*)work, floors, d_flags, b, lds_bytes);
   } else {
     hipFuncSetAttribute((const void*)panel_round_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds_bytes);
     hipLaunchKernelGGL(panel_round_kernel<true>, g, t, (size_t)lds_bytes, stream, d_blocks, d_jobs,
-                       (double2*)work, floors, d_flags, b);
+                       (double2*)work, floors, d_flags, b, lds_bytes);
   }
   return yh::hip_status(hipGetLastError(), "svd panel round launch");
 }

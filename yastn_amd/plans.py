@@ -381,20 +381,32 @@ class GemmPlan:
     """Device work list of the grouped GEMM (see include/yastn_hip.h)."""
 
     def __init__(self, gemms, terms, in_numel, out_numel, device, cfg=None):
-        self.h_gemms = np.ascontiguousarray(np.array(gemms, dtype=np.int64).reshape(-1, L.YH_GEMM_WORDS))
-        self.h_terms = np.ascontiguousarray(np.array(terms, dtype=np.int64).reshape(-1, L.YH_TERM_WORDS))
+        g0 = np.array(gemms, dtype=np.int64).reshape(-1, L.YH_GEMM_WORDS)
+        t0 = np.array(terms, dtype=np.int64).reshape(-1, L.YH_TERM_WORDS)
+        # useful flops (real MACs x2) and coverage of the plan, independent of tiling and splitting
+        k0 = np.array([int(t0[r[5]:r[5] + r[6], 6].sum()) if r[6] else 0 for r in g0], dtype=np.int64)
+        self.macs = int(np.sum(g0[:, 0] * g0[:, 1] * k0)) if len(g0) else 0
+        self.covered = int(np.sum(g0[:, 0] * g0[:, 1])) if len(g0) else 0
+        self.accumulates = bool(np.any(g0[:, 7] & 1)) if len(g0) else False
+        out_numel = list(out_numel)
+        g0, t0, self.h_red, self.part_numel = _split_k(g0, t0, len(out_numel))
+        self.part_buf = len(out_numel) if self.part_numel else None
+        if self.part_numel:
+            out_numel.append(self.part_numel)
+            L.call('yastn_hip_check_splitk_plan', self.h_red.ctypes.data, len(self.h_red),
+                   np.array(out_numel + [0] * (L.YH_MAX_BUFS - len(out_numel)), dtype=np.int64).ctypes.data,
+                   len(out_numel), self.part_numel)
+            self.d_red = torch.from_numpy(self.h_red).to(device)
+        self.h_gemms = np.ascontiguousarray(g0)
+        self.h_terms = np.ascontiguousarray(t0)
         self.ngemms, self.nterms = len(self.h_gemms), len(self.h_terms)
         self.in_numel = np.array(list(in_numel) + [0] * (L.YH_MAX_BUFS - len(in_numel)), dtype=np.int64)
-        self.out_numel = np.array(list(out_numel) + [0] * (L.YH_MAX_BUFS - len(out_numel)), dtype=np.int64)
+        self.out_numel = np.array(out_numel + [0] * (L.YH_MAX_BUFS - len(out_numel)), dtype=np.int64)
         self.nin, self.nout = len(in_numel), len(out_numel)
-        # useful flops (real MACs x2) of the plan, independent of tiling
         k_of = np.zeros(self.ngemms, dtype=np.int64)
         for g, row in enumerate(self.h_gemms):
             k_of[g] = int(self.h_terms[row[5]:row[5] + row[6], 6].sum()) if row[6] else 0
         self.k_of = k_of
-        self.macs = int(np.sum(self.h_gemms[:, 0] * self.h_gemms[:, 1] * k_of)) if self.ngemms else 0
-        self.covered = int(np.sum(self.h_gemms[:, 0] * self.h_gemms[:, 1])) if self.ngemms else 0
-        self.accumulates = bool(np.any(self.h_gemms[:, 7] & 1)) if self.ngemms else False
         self.device = device
         self._tiles = {}
         self.cfg = choose_tile(self) if cfg is None else cfg
@@ -463,6 +475,36 @@ class GemmPlan:
 
 _TUNED = {}
 NUM_CU = 256
+SPLITK_MIN_K = 8192       # split the K of a single-term GEMM with few output tiles beyond this length
+SPLITK_MAX_MN = 64 * 64   # ... when its output has at most this many elements
+SPLITK_MAX_S = 512
+
+
+def _split_k(g0, t0, out_buf):
+    """Cut long-K single-term GEMMs with small outputs into S chunks writing partial products to the
+    buffer `out_buf` (row-major M x N per chunk); returns the new gemm / term lists, the reduction
+    records and the partials' size.  Other GEMMs are kept as they are."""
+    gl, tl, red, part = [], [], [], 0
+    for row in g0:
+        M, N, c_off, c_rs, c_cs, tb, tc, fl = (int(x) for x in row)
+        if tc == 1 and out_buf < L.YH_MAX_BUFS and M * N <= SPLITK_MAX_MN and int(t0[tb, 6]) >= SPLITK_MIN_K:
+            a_off, a_rs, a_cs, b_off, b_rs, b_cs, K, tf = (int(x) for x in t0[tb])
+            kc = max(2048, -(-K // SPLITK_MAX_S))
+            kc = -(-kc // 16) * 16
+            S = -(-K // kc)
+            for sidx in range(S):
+                ks = min(kc, K - sidx * kc)
+                gl.append((M, N, part + sidx * M * N, N, 1, len(tl), 1, out_buf << 8))
+                tl.append((a_off + sidx * kc * a_cs, a_rs, a_cs, b_off + sidx * kc * b_rs, b_rs, b_cs, ks, tf))
+            red.append((M, N, c_off, c_rs, c_cs, part, S, fl))
+            part += S * M * N
+        else:
+            gl.append((M, N, c_off, c_rs, c_cs, len(tl), tc, fl))
+            tl.extend(tuple(int(x) for x in t) for t in t0[tb:tb + tc])
+    g = np.array(gl, dtype=np.int64).reshape(-1, L.YH_GEMM_WORDS)
+    t = np.array(tl, dtype=np.int64).reshape(-1, L.YH_TERM_WORDS)
+    r = np.ascontiguousarray(np.array(red, dtype=np.int64).reshape(-1, L.YH_SPLITK_WORDS))
+    return g, t, r, part
 
 
 def choose_tile(plan):
@@ -634,10 +676,12 @@ class SvdPlan:
             limit = dev_lds - 2048
             pmax = max(max(jwords[b][1], jwords[b][2]) for b in big)
             width = 16
-            while width > 1 and (2 * width * pmax + 4 * width * width) * itemsize > limit:
+            # LDS of a panel pair: W columns (p x 2b) and the rotation accumulator (2b x (2b + 1))
+            lds = lambda w: (2 * w * pmax + 2 * w * (2 * w + 1)) * itemsize
+            while width > 1 and lds(width) > limit:
                 width //= 2
             self.panel_width = width
-            self.panel_lds = (2 * width * pmax + 4 * width * width) * itemsize
+            self.panel_lds = lds(width)
             if self.panel_lds > limit:
                 raise ValueError(f"block with p={pmax} does not fit the LDS panel path")
             per_round = {}
