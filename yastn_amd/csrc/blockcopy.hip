@@ -46,28 +46,6 @@ constexpr int kThreads = 256;
 constexpr int kTE = YH_TRANSPOSE_TILE;
 constexpr int kShape = 6, kSrcStride = 6 + YH_MAXR, kDstStride = 6 + 2 * YH_MAXR;
 
-struct JobView {
-  int64_t src_off, dst_off;
-  int rank, kind, tdim;
-  int64_t tile_begin;
-  int64_t shape[YH_MAXR], ss[YH_MAXR], ds[YH_MAXR];
-};
-
-__device__ inline void load_job(const int64_t* __restrict__ w, JobView& j) {
-  j.src_off = w[0];
-  j.dst_off = w[1];
-  j.rank = (int)w[2];
-  j.kind = (int)w[3];
-  j.tdim = (int)w[4];
-  j.tile_begin = w[5];
-#pragma unroll
-  for (int d = 0; d < YH_MAXR; ++d) {
-    j.shape[d] = w[kShape + d];
-    j.ss[d] = w[kSrcStride + d];
-    j.ds[d] = w[kDstStride + d];
-  }
-}
-
 template <typename T>
 __device__ inline T zero_of();
 template <>
@@ -75,6 +53,10 @@ __device__ inline double zero_of<double>() { return 0.0; }
 template <>
 __device__ inline double2 zero_of<double2>() { return make_double2(0.0, 0.0); }
 
+// The job words are read in place (wave-uniform, so they live in scalar registers), with the loops
+// over dimensions unrolled to YH_MAXR so that no per-thread array is indexed at run time; the
+// multi-index decoding uses 32-bit arithmetic (a box holds fewer than 2^31 elements, checked on the
+// host), the offsets 64-bit.
 template <typename T>
 __global__ void __launch_bounds__(kThreads) gather_kernel(T* __restrict__ dst, const T* __restrict__ src,
                                                           const int64_t* __restrict__ jobs,
@@ -82,72 +64,83 @@ __global__ void __launch_bounds__(kThreads) gather_kernel(T* __restrict__ dst, c
   __shared__ T tile[kTE][kTE + 1];
   const int64_t t = blockIdx.x;
   const int jid = tile_job[t];
-  JobView j;
-  load_job(jobs + (int64_t)jid * YH_JOB_WORDS, j);
-  const int64_t local = t - j.tile_begin;
-  const int r = j.rank;
+  const int64_t* __restrict__ w = jobs + (int64_t)jid * YH_JOB_WORDS;
+  const int64_t src_off = w[0], dst_off = w[1];
+  const int r = (int)w[2], kind = (int)w[3], tdim = (int)w[4];
+  const int64_t local = t - w[5];
 
-  if (j.kind == 0) {
-    int64_t numel = 1;
-    for (int d = 0; d < r; ++d) numel *= j.shape[d];
-    const int64_t e0 = local * YH_GATHER_TILE;
-    const int64_t e1 = min(numel, e0 + (int64_t)YH_GATHER_TILE);
-    const int64_t L = j.shape[r - 1];
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += kThreads) {
+  if (kind == 0) {
+    uint32_t numel = 1;
+#pragma unroll
+    for (int d = 0; d < YH_MAXR; ++d)
+      if (d < r) numel *= (uint32_t)w[kShape + d];
+    const uint32_t e0 = (uint32_t)local * YH_GATHER_TILE;
+    const uint32_t e1 = min(numel, e0 + (uint32_t)YH_GATHER_TILE);
+    const uint32_t L = (uint32_t)w[kShape + r - 1];
+    const int64_t ssl = w[kSrcStride + r - 1];
+    for (uint32_t e = e0 + threadIdx.x; e < e1; e += kThreads) {
       // decode the destination-order multi-index of e
-      int64_t rem = e / L;
-      const int64_t c = e - rem * L;
-      int64_t so = c * j.ss[r - 1];
+      uint32_t rem = e / L;
+      const uint32_t c = e - rem * L;
+      int64_t so = (int64_t)c * ssl;
       int64_t dof = c;  // dst stride of the last dim is 1
-      for (int d = r - 2; d >= 0; --d) {
-        const int64_t q = rem / j.shape[d];
-        const int64_t i = rem - q * j.shape[d];
-        so += i * j.ss[d];
-        dof += i * j.ds[d];
-        rem = q;
+#pragma unroll
+      for (int d = YH_MAXR - 2; d >= 0; --d) {
+        if (d <= r - 2) {
+          const uint32_t n = (uint32_t)w[kShape + d];
+          const uint32_t q = rem / n;
+          const uint32_t i = rem - q * n;
+          so += (int64_t)i * w[kSrcStride + d];
+          dof += (int64_t)i * w[kDstStride + d];
+          rem = q;
+        }
       }
-      dst[j.dst_off + dof] = (j.src_off >= 0) ? src[j.src_off + so] : zero_of<T>();
+      dst[dst_off + dof] = (src_off >= 0) ? src[src_off + so] : zero_of<T>();
     }
     return;
   }
 
   // kind 1: 32x32 transpose tile over dims (k = tdim, jd = r-1)
-  const int k = j.tdim;
+  const int k = tdim;
   const int jd = r - 1;
-  const int64_t nk = yh::cdiv(j.shape[k], kTE);
-  const int64_t nj = yh::cdiv(j.shape[jd], kTE);
-  const int64_t per_outer = nk * nj;
-  int64_t outer = local / per_outer;
-  const int64_t in_outer = local - outer * per_outer;
-  const int64_t kt = in_outer / nj;
-  const int64_t jt = in_outer - kt * nj;
-  int64_t sbase = j.src_off, dbase = j.dst_off;
-  for (int d = r - 2; d >= 0; --d) {
-    if (d == k) continue;
-    const int64_t q = outer / j.shape[d];
-    const int64_t i = outer - q * j.shape[d];
-    sbase += i * j.ss[d];
-    dbase += i * j.ds[d];
-    outer = q;
+  const int64_t shk = w[kShape + k], shj = w[kShape + jd];
+  const uint32_t nk = (uint32_t)yh::cdiv(shk, kTE);
+  const uint32_t nj = (uint32_t)yh::cdiv(shj, kTE);
+  const uint32_t per_outer = nk * nj;
+  uint32_t outer = (uint32_t)local / per_outer;
+  const uint32_t in_outer = (uint32_t)local - outer * per_outer;
+  const uint32_t kt = in_outer / nj;
+  const uint32_t jt = in_outer - kt * nj;
+  int64_t sbase = src_off, dbase = dst_off;
+#pragma unroll
+  for (int d = YH_MAXR - 2; d >= 0; --d) {
+    if (d <= r - 2 && d != k) {
+      const uint32_t n = (uint32_t)w[kShape + d];
+      const uint32_t q = outer / n;
+      const uint32_t i = outer - q * n;
+      sbase += (int64_t)i * w[kSrcStride + d];
+      dbase += (int64_t)i * w[kDstStride + d];
+      outer = q;
+    }
   }
-  const int64_t k0 = kt * kTE, j0 = jt * kTE;
-  const int64_t kk_lim = min((int64_t)kTE, j.shape[k] - k0);
-  const int64_t jj_lim = min((int64_t)kTE, j.shape[jd] - j0);
+  const int64_t k0 = (int64_t)kt * kTE, j0 = (int64_t)jt * kTE;
+  const int64_t kk_lim = min((int64_t)kTE, shk - k0);
+  const int64_t jj_lim = min((int64_t)kTE, shj - j0);
+  const int64_t ssj = w[kSrcStride + jd], dsk = w[kDstStride + k];
   const int lane32 = threadIdx.x & (kTE - 1);
   const int row8 = threadIdx.x >> 5;  // 0..7
   // read: consecutive lanes walk the source-contiguous dim k
 #pragma unroll
   for (int p = 0; p < kTE / 8; ++p) {
     const int jj = row8 + 8 * p;
-    if (lane32 < kk_lim && jj < jj_lim)
-      tile[jj][lane32] = src[sbase + (k0 + lane32) /* ss[k]==1 */ + (j0 + jj) * j.ss[jd]];
+    if (lane32 < kk_lim && jj < jj_lim) tile[jj][lane32] = src[sbase + (k0 + lane32) /* ss[k]==1 */ + (j0 + jj) * ssj];
   }
   __syncthreads();
   // write: consecutive lanes walk the destination-contiguous dim
 #pragma unroll
   for (int p = 0; p < kTE / 8; ++p) {
     const int kk = row8 + 8 * p;
-    if (lane32 < jj_lim && kk < kk_lim) dst[dbase + (j0 + lane32) + (k0 + kk) * j.ds[k]] = tile[lane32][kk];
+    if (lane32 < jj_lim && kk < kk_lim) dst[dbase + (j0 + lane32) + (k0 + kk) * dsk] = tile[lane32][kk];
   }
 }
 
@@ -176,6 +169,7 @@ extern "C" int yastn_hip_check_gather_plan(const int64_t* h_jobs, int64_t njobs,
       smax += (n - 1) * w[kSrcStride + d];
       dmax += (n - 1) * w[kDstStride + d];
     }
+    YH_CHECK_ARG(numel < ((int64_t)1 << 31), "job %lld: box of %lld elements", (long long)q, (long long)numel);
     YH_CHECK_ARG(dst_off >= 0 && dst_off + dmax < dst_numel, "job %lld: destination out of bounds", (long long)q);
     YH_CHECK_ARG(src_off == -1 || (src_off >= 0 && src_off + smax < src_numel),
                  "job %lld: source out of bounds", (long long)q);

@@ -51,18 +51,18 @@ struct Num<true> {
 // [2,3,0,1], row_half_mirror, row_mirror: after the four steps every lane holds its row's value),
 // then the four row values are read into scalar registers, so the result is wave-uniform.
 template <int CTRL>
-__device__ inline double dpp_move(double v) {
+__device__ __attribute__((always_inline)) inline double dpp_move(double v) {
   const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
   return __hiloint2double(hi, lo);
 }
 
-__device__ inline double read_lane(double v, int l) {
+__device__ __attribute__((always_inline)) inline double read_lane(double v, int l) {
   return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
                           __builtin_amdgcn_readlane(__double2loint(v), l));
 }
 
-__device__ inline double wave_sum(double v) {
+__device__ __attribute__((always_inline)) inline double wave_sum(double v) {
   v += dpp_move<0xB1>(v);
   v += dpp_move<0x4E>(v);
   v += dpp_move<0x141>(v);
@@ -70,7 +70,7 @@ __device__ inline double wave_sum(double v) {
   return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
-__device__ inline double wave_max(double v) {
+__device__ __attribute__((always_inline)) inline double wave_max(double v) {
   v = fmax(v, dpp_move<0xB1>(v));
   v = fmax(v, dpp_move<0x4E>(v));
   v = fmax(v, dpp_move<0x141>(v));
@@ -97,7 +97,7 @@ __device__ inline void pair_of(int qe, int r, int k, int& i, int& j) {
 
 // 1/x and 1/sqrt(x) from the hardware approximations (v_rcp_f64, v_rsq_f64) refined by two
 // Newton-Raphson steps each; the rotation parameters only need c^2 + s^2 = 1 to rounding accuracy.
-__device__ inline double rcp_nr(double x) {
+__device__ __attribute__((always_inline)) inline double rcp_nr(double x) {
   double y = __builtin_amdgcn_rcp(x);
   double e = fma(-x, y, 1.0);
   y = fma(y, e, y);
@@ -105,7 +105,7 @@ __device__ inline double rcp_nr(double x) {
   return fma(y, e, y);
 }
 
-__device__ inline double rsq_nr(double x) {
+__device__ __attribute__((always_inline)) inline double rsq_nr(double x) {
   double y = __builtin_amdgcn_rsq(x);
   double e = fma(-x * y, y, 1.0);
   y = fma(0.5 * y, e, y);
@@ -116,7 +116,7 @@ __device__ inline double rsq_nr(double x) {
 // Hestenes rotation of a column pair from a = ||w_i||^2, b = ||w_j||^2, g = w_i^H w_j = gr + i gi:
 // returns false (no rotation) when |g| <= tol sqrt(a b) or a column is at the noise floor; else
 // c, s, and the phase e^{i phi} = g / |g| = (er, ei).
-__device__ inline bool rotation(double a, double b, double gr, double gi, double tol, double floor2, double& c,
+__device__ __attribute__((always_inline)) inline bool rotation(double a, double b, double gr, double gi, double tol, double floor2, double& c,
                                 double& s, double& er, double& ei) {
   const double g2 = gr * gr + gi * gi;
   // columns at the noise level (||w||^2 <= (eps ||A||_F)^2) carry no information beyond absolute
@@ -142,7 +142,7 @@ __device__ inline bool rotation(double a, double b, double gr, double gi, double
 
 // (u, v) <- (c u - s e^{-i phi} v, s e^{i phi} u + c v); for real data e^{i phi} = +-1 is folded into s.
 template <bool CPLX>
-__device__ inline void apply_rot(typename Num<CPLX>::T& u, typename Num<CPLX>::T& v, double c, double s, double er,
+__device__ __attribute__((always_inline)) inline void apply_rot(typename Num<CPLX>::T& u, typename Num<CPLX>::T& v, double c, double s, double er,
                                  double ei) {
   typedef Num<CPLX> N;
   if constexpr (CPLX) {
@@ -163,7 +163,7 @@ __device__ inline void apply_rot(typename Num<CPLX>::T& u, typename Num<CPLX>::T
 // columns (v_i, v_j) (length q), by one wave.  Returns true when the pair was rotated, i.e. when
 // |w_i^H w_j| > tol ||w_i|| ||w_j|| and both squared norms exceed floor2.  J = [[c, s e^{i phi}], [-s e^{-i phi}, c]], e^{i phi} = g / |g|.
 template <bool CPLX>
-__device__ inline bool jacobi_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>::T* wj, int64_t p,
+__device__ __attribute__((always_inline)) inline bool jacobi_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>::T* wj, int64_t p,
                                    typename Num<CPLX>::T* vi_, typename Num<CPLX>::T* vj_, int64_t q, double tol,
                                    double floor2, int lane) {
   typedef Num<CPLX> N;
@@ -201,7 +201,7 @@ __device__ inline bool jacobi_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>
 // The same rotation with the two columns held in registers between the inner products and the
 // update (one read and one write per element instead of two reads and one write); p <= 64 * NR.
 template <bool CPLX, int NR>
-__device__ inline bool jacobi_pair_reg(typename Num<CPLX>::T* wi, typename Num<CPLX>::T* wj, int64_t p,
+__device__ __attribute__((always_inline)) inline bool jacobi_pair_reg(typename Num<CPLX>::T* wi, typename Num<CPLX>::T* wj, int64_t p,
                                        typename Num<CPLX>::T* vi_, typename Num<CPLX>::T* vj_, int64_t q,
                                        double tol, double floor2, int lane) {
   typedef Num<CPLX> N;
@@ -251,7 +251,7 @@ __device__ inline bool jacobi_pair_reg(typename Num<CPLX>::T* wi, typename Num<C
 
 // Rotation of one pair with the register-resident variant when the columns are short enough.
 template <bool CPLX>
-__device__ inline bool rotate_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>::T* wj, int64_t p,
+__device__ __attribute__((always_inline)) inline bool rotate_pair(typename Num<CPLX>::T* wi, typename Num<CPLX>::T* wj, int64_t p,
                                    typename Num<CPLX>::T* vi, typename Num<CPLX>::T* vj, int64_t q, double tol,
                                    double floor2, int lane) {
   if (p <= 64) return jacobi_pair_reg<CPLX, 1>(wi, wj, p, vi, vj, q, tol, floor2, lane);
