@@ -99,6 +99,8 @@ def _dot_launch_stats(backend, workload_fn, reps=50):
     for A, B, meta, Dsize in calls:
         cplx = A.is_complex() or B.is_complex()
         fl = sum((8 if cplx else 2) * Da[0] * Da[1] * Db[1] for _, _, _, Da, _, Db in meta)
+        isz = 16 if cplx else 8
+        by = sum(isz * (Da[0] * Da[1] + Db[0] * Db[1] + Da[0] * Db[1]) for _, _, _, Da, _, Db in meta)
         for _ in range(3):
             backend.dot(A, B, meta, Dsize)
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
@@ -110,7 +112,7 @@ def _dot_launch_stats(backend, workload_fn, reps=50):
         ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / reps
         dt = torch.promote_types(A.dtype, B.dtype)
         plan = P.dot_plan(meta, A.numel(), B.numel(), Dsize, A.device)
-        out.append((fl, ms, plan, dt))
+        out.append((fl, ms, plan, dt, by))
     return out
 
 
@@ -135,29 +137,40 @@ def bench_heff2(args, out):
     out['config']['useful_gflop_per_step'] = flops / 1e9
     out['config']['execution'] = 'HIP graph replay of the 4-tensordot step (captured once; same kernels per replay)'
     out['eager'] = {'ms_per_step': t_eager * 1e3, 'value': flops / t_eager / 1e12, 'graph_vs_eager_maxdiff': graph_err}
-    # plain per-block path on the same inputs
     cfgp = bs.make_config(backend_plain, 'U1', device='cuda:0')
-    AAp, FLp, FRp, Wp = (x._replace(config=cfgp) for x in (AA, FL, FR, Wm))
-    with torch.no_grad():
-        tp = _timeit(lambda: W.heff2(AAp, FLp, FRp, Wp), max(3, args.steps // 4), 1, sync)
-    out['plain_gpu'] = {'ms_per_step': tp * 1e3, 'value': flops / tp / 1e12, 'speedup_batched': tp / t}
+    if not args.no_plain:  # plain per-block path on the same inputs
+        AAp, FLp, FRp, Wp = (x._replace(config=cfgp) for x in (AA, FL, FR, Wm))
+        with torch.no_grad():
+            tp = _timeit(lambda: W.heff2(AAp, FLp, FRp, Wp), max(3, args.steps // 4), 1, sync)
+        out['plain_gpu'] = {'ms_per_step': tp * 1e3, 'value': flops / tp / 1e12, 'speedup_batched': tp / t}
     # truncated block SVD back to chi (two-site update)
     with torch.no_grad():
         res = W.heff2(AA, FL, FR, Wm)
         ts = _timeit(lambda: W.two_site_truncation(res, 1024), 3, 1, sync)
-        resp = res._replace(config=cfgp)
-        tsp = _timeit(lambda: W.two_site_truncation(resp, 1024), 2, 1, sync)
-    out['svd_truncation'] = {'workload': 'svd_with_truncation of the Heff2 output to D_total=1024 '
-                                         '(27 blocks, max 198x198)', 'ms_batched': ts * 1e3, 'ms_plain': tsp * 1e3}
+        out['svd_truncation'] = {'workload': 'svd_with_truncation of the Heff2 output to D_total=1024 '
+                                             '(27 blocks, max 198x198)', 'ms_batched': ts * 1e3}
+        if not args.no_plain:
+            resp = res._replace(config=cfgp)
+            tsp = _timeit(lambda: W.two_site_truncation(resp, 1024), 2, 1, sync)
+            out['svd_truncation']['ms_plain'] = tsp * 1e3
     # roofline of the dominant kernel (the launch with the largest duration)
     stats = _dot_launch_stats(backend_hip, lambda: W.heff2(AA, FL, FR, Wm))
-    fl, ms, plan, dt = max(stats, key=lambda x: x[1])
-    ach = fl / (ms * 1e-3) / 1e12
-    out['roofline'] = {'bound': 'mfma', 'achieved': ach, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
-                       'frac': ach / PEAK_FP64_TFLOPS, 'traffic': None,
-                       'kernel': f'gemm_kernel tile {plan.cfg} ({plan.ngemms} sector GEMMs, {plan.ntiles} tiles)',
-                       'avg_launch_ms': ms, 'flops_per_launch': fl,
-                       'all_dot_launches_ms': [round(x[1], 5) for x in stats]}
+    fl, ms, plan, dt, by = max(stats, key=lambda x: x[1])
+    # the bound follows the launch's arithmetic intensity against the ridge point of FP64 MFMA / HBM
+    ridge = PEAK_FP64_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9)
+    kname = {0: '32x32', 1: '64x64', 2: '128x64', 3: '16x64', 4: '64x16', 5: '128x128', 6: 'small-M streaming'}
+    common = {'kernel': f'grouped GEMM, {kname.get(plan.cfg, plan.cfg)} tiles ({plan.ngemms} sector GEMMs, '
+                        f'{plan.ntiles} tiles)',
+              'avg_launch_ms': ms, 'flops_per_launch': fl, 'algorithmic_bytes_per_launch': by,
+              'arithmetic_intensity': fl / by, 'all_dot_launches_ms': [round(x[1], 5) for x in stats]}
+    if fl / by < ridge:
+        ach = by / (ms * 1e-3) / 1e9
+        out['roofline'] = {'bound': 'hbm', 'achieved': ach, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+                           'frac': ach / PEAK_HBM_GBS, 'traffic': None, **common}
+    else:
+        ach = fl / (ms * 1e-3) / 1e12
+        out['roofline'] = {'bound': 'mfma', 'achieved': ach, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
+                           'frac': ach / PEAK_FP64_TFLOPS, 'traffic': None, **common}
     return AA, FL, FR, Wm, flops
 
 
@@ -215,6 +228,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-ctmrg', action='store_true')
     ap.add_argument('--only-heff2', action='store_true', help='headline workload only (for profiling)')
+    ap.add_argument('--no-plain', action='store_true', help='skip the plain per-block baselines (for profiling)')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run one 256 MiB device copy, the reference dispatch for scaling HBM counters')
     args = ap.parse_args()

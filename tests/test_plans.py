@@ -58,7 +58,11 @@ def test_dot_plans_reproduce_golden(scenario):
         E.gemm(plan.h_gemms, plan.h_terms, [C], [A, B])
         assert np.linalg.norm(C - outs[0]) <= 1e-13 * np.linalg.norm(outs[0])
         for cfg in P.L.TILE_DIMS:
-            plan.tiles_for(cfg)  # host-checked for every tile shape
+            if plan.applicable(cfg):
+                plan.tiles_for(cfg)  # host-checked for every tile shape
+            else:  # the checker refuses the small-M kernel for a plan with M > 16
+                with pytest.raises(RuntimeError):
+                    plan.tiles_for(cfg)
         # adjoint
         rng = np.random.default_rng(2)
         gC = rng.standard_normal(Dsize).astype(dt)
@@ -126,7 +130,10 @@ def test_uncovered_boxes():
 
 
 def test_tile_choice_prefers_small_tiles_for_small_sectors():
-    small = P.dot_plan(tuple(((0, 64), (8, 8), (0, 64), (8, 8), (0, 64), (8, 8)) for _ in range(1)), 64, 64, 64, CPU)
+    tiny = P.dot_plan(tuple(((0, 64), (8, 8), (0, 64), (8, 8), (0, 64), (8, 8)) for _ in range(1)), 64, 64, 64, CPU)
+    assert tiny.cfg == P.L.YH_TILE_SMALLM  # a few rows: the streaming kernel
+    small = P.dot_plan(tuple(((0, 576), (24, 24), (0, 576), (24, 24), (0, 576), (24, 24)) for _ in range(1)),
+                       576, 576, 576, CPU)
     assert small.cfg == P.L.YH_TILE_32x32
 
 

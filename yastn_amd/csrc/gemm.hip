@@ -284,6 +284,81 @@ __global__ void __launch_bounds__(kThreads) splitk_reduce_kernel(Bufs bufs, cons
   }
 }
 
+
+// Streaming kernel for GEMMs whose M is at most 16 (e.g. contractions with a small MPO leg): a tile
+// is all M rows times 256 columns, one thread per column; the A entries are wave-uniform (scalar
+// loads), B and C are read / written along the columns, so the kernel runs at the HBM rate.
+constexpr int kSmallM = 16, kSmallN = 256;
+
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads) gemm_smallm_kernel(Bufs bufs, const int64_t* __restrict__ gemms,
+                                                               const int64_t* __restrict__ terms,
+                                                               const int32_t* __restrict__ tiles) {
+  typedef typename Elt<CPLX>::T T;
+  const int32_t* tl = tiles + 3 * (int64_t)blockIdx.x;
+  const int64_t* gw = gemms + (int64_t)tl[0] * YH_GEMM_WORDS;
+  const int64_t M = gw[0], N = gw[1], c_off = gw[2], c_rs = gw[3], c_cs = gw[4];
+  const int64_t tb = gw[5], tc = gw[6], gflags = gw[7];
+  const int64_t j = (int64_t)tl[2] + threadIdx.x;
+  double accr[kSmallM], acci[kSmallM];
+#pragma unroll
+  for (int i = 0; i < kSmallM; ++i) {
+    accr[i] = 0.0;
+    acci[i] = 0.0;
+  }
+  const bool live = j < N;
+  for (int64_t q = tb; q < tb + tc; ++q) {
+    const int64_t* tw = terms + q * YH_TERM_WORDS;
+    const int64_t a_off = tw[0], a_rs = tw[1], a_cs = tw[2];
+    const int64_t b_off = tw[3], b_rs = tw[4], b_cs = tw[5], K = tw[6], tf = tw[7];
+    const double sa = (tf & 1) ? -1.0 : 1.0, sb = (tf & 2) ? -1.0 : 1.0;  // conjugation signs
+    const T* __restrict__ A = (const T*)bufs.in[(tf >> 8) & 0xff];
+    const T* __restrict__ B = (const T*)bufs.in[(tf >> 16) & 0xff];
+    for (int64_t k = 0; k < K; ++k) {
+      T b;
+      if (live)
+        b = B[b_off + k * b_rs + j * b_cs];
+      else
+        b = T{};
+#pragma unroll
+      for (int i = 0; i < kSmallM; ++i) {
+        if (i < M) {
+          const T a = A[a_off + i * a_rs + k * a_cs];
+          if constexpr (CPLX) {
+            const double ar = a.x, ai = sa * a.y, br = b.x, bi = sb * b.y;
+            accr[i] += ar * br - ai * bi;
+            acci[i] += ar * bi + ai * br;
+          } else {
+            accr[i] += a * b;
+          }
+        }
+      }
+    }
+  }
+  if (!live) return;
+  T* __restrict__ C = (T*)bufs.out[(gflags >> 8) & 0xff];
+  const bool accumulate = gflags & 1;
+#pragma unroll
+  for (int i = 0; i < kSmallM; ++i) {
+    if (i < M) {
+      T* p = C + c_off + i * c_rs + j * c_cs;
+      if constexpr (CPLX) {
+        double2 r = make_double2(accr[i], acci[i]);
+        if (accumulate) {
+          const double2 o = *p;
+          r.x += o.x;
+          r.y += o.y;
+        }
+        *p = r;
+      } else {
+        double r = accr[i];
+        if (accumulate) r += *p;
+        *p = r;
+      }
+    }
+  }
+}
+
 // BK per configuration: the double-buffered LDS tiles stay within 64 KiB.
 template <bool CPLX>
 int launch(const Bufs& b, const int64_t* gemms, const int64_t* terms, const int32_t* tiles, int64_t ntiles,
@@ -310,6 +385,9 @@ int launch(const Bufs& b, const int64_t* gemms, const int64_t* terms, const int3
     case YH_TILE_128x128:
       hipLaunchKernelGGL((gemm_kernel<CPLX, 128, 128, 2, 2, BKH>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
+    case YH_TILE_SMALLM:
+      hipLaunchKernelGGL((gemm_smallm_kernel<CPLX>), grid, block, 0, s, b, gemms, terms, tiles);
+      break;
     default:
       yh::set_error("unknown tile config %d", cfg);
       return YH_ERR_ARG;
@@ -324,6 +402,7 @@ void tile_dims(int cfg, int* bm, int* bn) {
     case YH_TILE_128x64: *bm = 128; *bn = 64; break;
     case YH_TILE_16x64: *bm = 16; *bn = 64; break;
     case YH_TILE_128x128: *bm = 128; *bn = 128; break;
+    case YH_TILE_SMALLM: *bm = kSmallM; *bn = kSmallN; break;
     default: *bm = 64; *bn = 16; break;
   }
 }
@@ -333,7 +412,7 @@ void tile_dims(int cfg, int* bm, int* bn) {
 extern "C" int yastn_hip_check_gemm_plan(const int64_t* h_gemms, int64_t ngemms, const int64_t* h_terms,
                                          int64_t nterms, const int32_t* h_tiles, int64_t ntiles, int cfg,
                                          const int64_t* in_numel, int nin, const int64_t* out_numel, int nout) {
-  YH_CHECK_ARG(cfg >= YH_TILE_32x32 && cfg <= YH_TILE_128x128, "tile config %d", cfg);
+  YH_CHECK_ARG(cfg >= YH_TILE_32x32 && cfg <= YH_TILE_SMALLM, "tile config %d", cfg);
   YH_CHECK_ARG(nin <= YH_MAX_BUFS && nout <= YH_MAX_BUFS, "too many buffers");
   int bm, bn;
   tile_dims(cfg, &bm, &bn);
@@ -342,6 +421,8 @@ extern "C" int yastn_hip_check_gemm_plan(const int64_t* h_gemms, int64_t ngemms,
     const int64_t M = w[0], N = w[1], c_off = w[2], c_rs = w[3], c_cs = w[4], tb = w[5], tc = w[6];
     const int out = (int)((w[7] >> 8) & 0xff);
     YH_CHECK_ARG(M >= 0 && N >= 0 && c_rs >= 0 && c_cs >= 0, "gemm %lld: negative dims", (long long)g);
+    YH_CHECK_ARG(cfg != YH_TILE_SMALLM || M <= bm, "gemm %lld: M = %lld exceeds the small-M tile", (long long)g,
+                 (long long)M);
     YH_CHECK_ARG(out < nout, "gemm %lld: output buffer %d", (long long)g, out);
     if (M > 0 && N > 0)
       YH_CHECK_ARG(c_off >= 0 && c_off + (M - 1) * c_rs + (N - 1) * c_cs < out_numel[out],

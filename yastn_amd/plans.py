@@ -431,6 +431,12 @@ class GemmPlan:
             self._tiles[cfg] = (h, torch.from_numpy(h).to(self.device))
         return self._tiles[cfg]
 
+    def applicable(self, cfg):
+        """The streaming small-M kernel takes only plans whose GEMMs all have M <= 16."""
+        if cfg == L.YH_TILE_SMALLM:
+            return self.ngemms == 0 or int(self.h_gemms[:, 0].max()) <= L.TILE_DIMS[cfg][0]
+        return True
+
     def set_cfg(self, cfg):
         self.cfg = cfg
         self.h_tiles, self.d_tiles = self.tiles_for(cfg)
@@ -456,6 +462,8 @@ class GemmPlan:
             return
         times = {}
         for cfg in L.TILE_DIMS:
+            if not self.applicable(cfg):
+                continue
             self.set_cfg(cfg)
             run()
             e0 = torch.cuda.Event(enable_timing=True)
@@ -519,10 +527,14 @@ def choose_tile(plan):
     M = plan.h_gemms[:, 0].astype(np.float64)
     N = plan.h_gemms[:, 1].astype(np.float64)
     K = np.maximum(plan.k_of.astype(np.float64), 1.0)
+    if plan.applicable(L.YH_TILE_SMALLM) and M.max() <= 8:
+        return L.YH_TILE_SMALLM  # a few rows: streaming beats any MFMA tile, whose rows would be padding
     eff = {L.YH_TILE_32x32: 0.35, L.YH_TILE_64x64: 0.8, L.YH_TILE_128x64: 1.0, L.YH_TILE_16x64: 0.3,
-           L.YH_TILE_64x16: 0.3, L.YH_TILE_128x128: 1.1}
+           L.YH_TILE_64x16: 0.3, L.YH_TILE_128x128: 1.1, L.YH_TILE_SMALLM: 0.3}
     best, best_cost = None, None
     for cfg, (bm, bn) in L.TILE_DIMS.items():
+        if not plan.applicable(cfg):
+            continue
         tiles = np.ceil(M / bm) * np.ceil(N / bn)
         work = bm * bn * K
         padded = float(np.sum(tiles * work))
