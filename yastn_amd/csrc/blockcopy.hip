@@ -45,6 +45,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kTE = YH_TRANSPOSE_TILE;
 constexpr int kShape = 6, kSrcStride = 6 + YH_MAXR, kDstStride = 6 + 2 * YH_MAXR;
+constexpr int kRun = YH_GATHER_TILE / kThreads;  // elements per thread in a kind-0 tile
 
 template <typename T>
 __device__ inline T zero_of();
@@ -78,25 +79,37 @@ __global__ void __launch_bounds__(kThreads) gather_kernel(T* __restrict__ dst, c
     const uint32_t e1 = min(numel, e0 + (uint32_t)YH_GATHER_TILE);
     const uint32_t L = (uint32_t)w[kShape + r - 1];
     const int64_t ssl = w[kSrcStride + r - 1];
-    for (uint32_t e = e0 + threadIdx.x; e < e1; e += kThreads) {
-      // decode the destination-order multi-index of e
-      uint32_t rem = e / L;
-      const uint32_t c = e - rem * L;
-      int64_t so = (int64_t)c * ssl;
-      int64_t dof = c;  // dst stride of the last dim is 1
+    // lanes take consecutive destination elements (coalesced); each thread's kRun elements are
+    // decoded and loaded first, then stored, so that all its loads are in flight together
+    T v[kRun];
+    int64_t dofs[kRun];
 #pragma unroll
-      for (int d = YH_MAXR - 2; d >= 0; --d) {
-        if (d <= r - 2) {
-          const uint32_t n = (uint32_t)w[kShape + d];
-          const uint32_t q = rem / n;
-          const uint32_t i = rem - q * n;
-          so += (int64_t)i * w[kSrcStride + d];
-          dof += (int64_t)i * w[kDstStride + d];
-          rem = q;
+    for (int i = 0; i < kRun; ++i) {
+      const uint32_t e = e0 + threadIdx.x + i * kThreads;
+      dofs[i] = -1;
+      if (e < e1) {
+        uint32_t rem = e / L;
+        const uint32_t c = e - rem * L;
+        int64_t so = (int64_t)c * ssl;
+        int64_t dof = c;  // dst stride of the last dim is 1
+#pragma unroll
+        for (int d = YH_MAXR - 2; d >= 0; --d) {
+          if (d <= r - 2) {
+            const uint32_t n = (uint32_t)w[kShape + d];
+            const uint32_t q = rem / n;
+            const uint32_t ii = rem - q * n;
+            so += (int64_t)ii * w[kSrcStride + d];
+            dof += (int64_t)ii * w[kDstStride + d];
+            rem = q;
+          }
         }
+        v[i] = (src_off >= 0) ? src[src_off + so] : zero_of<T>();
+        dofs[i] = dof;
       }
-      dst[dst_off + dof] = (src_off >= 0) ? src[src_off + so] : zero_of<T>();
     }
+#pragma unroll
+    for (int i = 0; i < kRun; ++i)
+      if (dofs[i] >= 0) dst[dst_off + dofs[i]] = v[i];
     return;
   }
 

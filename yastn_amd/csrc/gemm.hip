@@ -123,8 +123,11 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
       b_in[p] = (e < BN * BK) && (n0 + b_j[p] < N);
     }
 
-    T ra[A_PER], rb[B_PER];
-    auto load = [&](int64_t k0) {
+    // Two register sets: while slice s is consumed from LDS, slice s+1 waits in one set (loaded an
+    // iteration earlier) and slice s+2 is being loaded into the other, so two slices of global loads
+    // are in flight; slice s+1 goes to the other LDS buffer at the end of the iteration.
+    T ra0[A_PER], rb0[B_PER], ra1[A_PER], rb1[B_PER];
+    auto load = [&](int64_t k0, T* ra, T* rb) {
 #pragma unroll
       for (int p = 0; p < A_PER; ++p) {
         const int64_t gk = k0 + a_k[p];
@@ -142,7 +145,7 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
           rb[p] = T{};
       }
     };
-    auto store = [&](int buf) {
+    auto store = [&](int buf, const T* ra, const T* rb) {
 #pragma unroll
       for (int p = 0; p < A_PER; ++p) {
         if (tid + p * kThreads < BM * BK) {
@@ -166,15 +169,8 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
         }
       }
     };
-
-    load(0);
-    store(0);
-    __syncthreads();
-    int buf = 0;
-    for (int64_t k0 = 0; k0 < K; k0 += BK) {
-      const bool more = k0 + BK < K;
-      if (more) load(k0 + BK);
-      // ---- MFMA over the BK slice, 4 k per instruction
+    // ---- MFMA over one BK slice of LDS buffer `buf`, 4 k per instruction
+    auto compute = [&](int buf) {
 #pragma unroll
       for (int ks = 0; ks < KSUB; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
@@ -205,9 +201,26 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
             }
           }
       }
-      if (more) store(buf ^ 1);
+    };
+
+    const int64_t ns = (K + BK - 1) / BK;  // slices
+    load(0, ra0, rb0);
+    store(0, ra0, rb0);
+    if (ns > 1) load(BK, ra0, rb0);
+    if (ns > 2) load(2 * BK, ra1, rb1);
+    __syncthreads();
+    for (int64_t s = 0; s < ns; s += 2) {
+      // even slice s in buffer 0; set 0 holds slice s+1, set 1 slice s+2
+      compute(0);
+      if (s + 1 < ns) store(1, ra0, rb0);
+      if (s + 3 < ns) load((s + 3) * BK, ra0, rb0);
       __syncthreads();
-      buf ^= 1;
+      if (s + 1 >= ns) break;
+      // odd slice s+1 in buffer 1; set 1 holds slice s+2, set 0 slice s+3
+      compute(1);
+      if (s + 2 < ns) store(0, ra1, rb1);
+      if (s + 4 < ns) load((s + 4) * BK, ra1, rb1);
+      __syncthreads();
     }
   }
 
@@ -366,9 +379,10 @@ int launch(const Bufs& b, const int64_t* gemms, const int64_t* terms, const int3
   const dim3 grid((unsigned)ntiles), block(kThreads);
   constexpr int BKW = CPLX ? 8 : 16;  // 64 / 128-wide tiles
   constexpr int BKH = CPLX ? 4 : 8;   // 128 x 128
+  constexpr int BKS = CPLX ? 16 : 32; // 32-wide and skinny tiles: deeper slices, more loads in flight
   switch (cfg) {
     case YH_TILE_32x32:
-      hipLaunchKernelGGL((gemm_kernel<CPLX, 32, 32, 2, 2, 16>), grid, block, 0, s, b, gemms, terms, tiles);
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 32, 32, 2, 2, BKS>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
     case YH_TILE_64x64:
       hipLaunchKernelGGL((gemm_kernel<CPLX, 64, 64, 2, 2, BKW>), grid, block, 0, s, b, gemms, terms, tiles);
@@ -377,10 +391,10 @@ int launch(const Bufs& b, const int64_t* gemms, const int64_t* terms, const int3
       hipLaunchKernelGGL((gemm_kernel<CPLX, 128, 64, 2, 2, BKW>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
     case YH_TILE_16x64:
-      hipLaunchKernelGGL((gemm_kernel<CPLX, 16, 64, 1, 4, 16>), grid, block, 0, s, b, gemms, terms, tiles);
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 16, 64, 1, 4, BKS>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
     case YH_TILE_64x16:
-      hipLaunchKernelGGL((gemm_kernel<CPLX, 64, 16, 4, 1, 16>), grid, block, 0, s, b, gemms, terms, tiles);
+      hipLaunchKernelGGL((gemm_kernel<CPLX, 64, 16, 4, 1, BKS>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
     case YH_TILE_128x128:
       hipLaunchKernelGGL((gemm_kernel<CPLX, 128, 128, 2, 2, BKH>), grid, block, 0, s, b, gemms, terms, tiles);
