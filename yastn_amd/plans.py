@@ -423,8 +423,7 @@ class GemmPlan:
                 for m0 in range(0, M, bm):
                     for n0 in range(0, N, bn):
                         tl.append((int(self.k_of[g]), g, m0, n0))
-            tl.sort(key=lambda x: -x[0])  # longest K first: big tiles start early
-            h = np.ascontiguousarray(np.array([t[1:] for t in tl], dtype=np.int32).reshape(-1, 3))
+            h = np.ascontiguousarray(np.array(_xcd_order(tl), dtype=np.int32).reshape(-1, 3))
             L.call('yastn_hip_check_gemm_plan', self.h_gemms.ctypes.data, self.ngemms, self.h_terms.ctypes.data,
                    self.nterms, h.ctypes.data, len(h), cfg, self.in_numel.ctypes.data, self.nin,
                    self.out_numel.ctypes.data, self.nout)
@@ -483,6 +482,33 @@ class GemmPlan:
 
 _TUNED = {}
 NUM_CU = 256
+NUM_XCD = 8  # workgroup i of a launch runs on XCD i % 8, each XCD with its own L2
+
+
+def _xcd_order(tl):
+    """Dispatch order of the tiles (k, g, m0, n0) of one launch: the tiles of one row panel (g, m0)
+    share their A rows, so panels are dealt to the XCDs as units (largest work first, to the least
+    loaded XCD), and position i of the list holds a tile of XCD i % NUM_XCD; each XCD's tiles go
+    longest K first.  An XCD that runs out takes tiles from the fullest one."""
+    units = {}
+    for t in tl:
+        units.setdefault((t[1], t[2]), []).append(t)
+    load = [0] * NUM_XCD
+    queues = [[] for _ in range(NUM_XCD)]
+    for key, ts in sorted(units.items(), key=lambda kv: -sum(t[0] + 1 for t in kv[1])):
+        x = min(range(NUM_XCD), key=lambda i: load[i])
+        queues[x].extend(ts)
+        load[x] += sum(t[0] + 1 for t in ts)
+    for q in queues:
+        q.sort(key=lambda t: (-t[0], t[1], t[2], t[3]))
+        q.reverse()  # pop() from the end takes the longest first
+    out = []
+    while any(queues):
+        for x in range(NUM_XCD):
+            q = queues[x] if queues[x] else max(queues, key=len)
+            if q:
+                out.append(q.pop()[1:])
+    return out
 SPLITK_MIN_K = 8192       # split the K of a single-term GEMM with few output tiles beyond this length
 SPLITK_MAX_MN = 64 * 64   # ... when its output has at most this many elements
 SPLITK_MAX_S = 512
