@@ -77,6 +77,22 @@ def capture_graph(fn, warm=3):
     return g, out
 
 
+def _pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary (rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE, scaled by a copy of known size), or None when the summary is of another launch."""
+    path = os.path.join(ROOT, 'profiles', 'heff2_dominant_gemm_pmc.txt')
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        if ':' in line and not line.startswith('#'):
+            k, v = line.split(':', 1)
+            kv[k.strip()] = v.strip()
+    if kv.get('kernel') != kernel:
+        return None
+    return int(kv['traffic_bytes_per_launch'])
+
+
 def _dot_launch_stats(backend, workload_fn, reps=50):
     """Per-launch duration (HIP events on the launch stream) of every grouped-GEMM launch of one
     workload pass; returns [(flops, avg_ms, plan)] in launch order."""
@@ -163,14 +179,15 @@ def bench_heff2(args, out):
                         f'{plan.ntiles} tiles)',
               'avg_launch_ms': ms, 'flops_per_launch': fl, 'algorithmic_bytes_per_launch': by,
               'arithmetic_intensity': fl / by, 'all_dot_launches_ms': [round(x[1], 5) for x in stats]}
+    common['traffic'] = _pmc_traffic(common['kernel'])
     if fl / by < ridge:
         ach = by / (ms * 1e-3) / 1e9
         out['roofline'] = {'bound': 'hbm', 'achieved': ach, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
-                           'frac': ach / PEAK_HBM_GBS, 'traffic': None, **common}
+                           'frac': ach / PEAK_HBM_GBS, **common}
     else:
         ach = fl / (ms * 1e-3) / 1e12
         out['roofline'] = {'bound': 'mfma', 'achieved': ach, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
-                           'frac': ach / PEAK_FP64_TFLOPS, 'traffic': None, **common}
+                           'frac': ach / PEAK_FP64_TFLOPS, **common}
     return AA, FL, FR, Wm, flops
 
 
