@@ -535,88 +535,24 @@ __global__ void __launch_bounds__(kPanelThreads)
   }
 }
 
-template <bool CPLX>
-__global__ void __launch_bounds__(kPanelThreads)
-    panel_round_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
-                       typename Num<CPLX>::T* __restrict__ work, const double* __restrict__ floors,
-                       int32_t* __restrict__ flags, int b, int64_t lds_cap) {
+// out (rows x nc) = In (rows x nc, column-major, leading dimension ldi, in LDS) * Q (nc x nc, column
+// c at Q + c nq) on the FP64 matrix cores, written to global columns dst(c)[row]: one wave per 16 x 16
+// output tile, v_mfma_f64_16x16x4f64 over k in steps of 4 (lane l supplies A(l % 16, l / 16) and
+// B(l / 16, l % 16); item v of the result is D(l / 16 + 4 v, l % 16)).
+template <bool CPLX, typename DST>
+__device__ __attribute__((always_inline)) inline void times_q(const typename Num<CPLX>::T* In, int64_t ldi,
+                                                              int rows, const typename Num<CPLX>::T* Q, int nq,
+                                                              int nc, DST dst, int wave, int lane, int nwaves) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
-  extern __shared__ double smem_raw[];
-  __shared__ int s_rot;
-  const int32_t* jb = jobs + 3 * (int64_t)blockIdx.x;
-  const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
-  const bool tr = d.m < d.n;
-  const int64_t p = tr ? d.n : d.m, q = tr ? d.m : d.n;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t c0i = (int64_t)jb[1] * b, c0j = (int64_t)jb[2] * b;
-  const int64_t ci = min((int64_t)b, q - c0i), cj = min((int64_t)b, q - c0j);
-  const int nc = (int)(ci + cj);
-  const int nq = 2 * b + 1;  // leading dimension of Q (odd: the column reads of the V update spread over banks)
-  T* W = work + d.work_off;
-  T* V = W + p * q;
-  T* Wp = reinterpret_cast<T*>(smem_raw);
-  T* Q = Wp + (int64_t)(2 * b) * p;  // after the 2b panel columns
-  if ((int64_t)((2 * b) * p + (int64_t)nc * nq) * (int64_t)sizeof(T) > lds_cap) {  // uniform: plan too small
-    if (tid == 0) flags[blockIdx.x] = -1;
-    return;
-  }
-  auto gcol = [&](int64_t l) { return l < ci ? c0i + l : c0j + (l - ci); };
-  constexpr int kPW = kPanelThreads / 64;
-  for (int l = wave; l < nc; l += kPW) {  // one wave per column, lanes along it
-    const T* src = W + gcol(l) * p;
-    T* dst = Wp + (int64_t)l * p;
-    for (int64_t r = lane; r < p; r += 64) dst[r] = src[r];
-  }
-  for (int e = tid; e < nc * nq; e += kPanelThreads) {
-    const int l = e / nq, k = e - l * nq;
-    Q[e] = N::make(l == k ? 1.0 : 0.0, 0.0);
-  }
-  if (tid == 0) s_rot = 0;
-  __syncthreads();
-  const double tol = DBL_EPSILON * sqrt((double)(p > 1 ? p : 1));
-  const double floor2 = floors[jb[0]];
-  const int ne = nc + (nc & 1);
-  for (int r = 0; r < ne - 1; ++r) {
-    for (int k = wave; k < ne / 2; k += kPanelThreads / 64) {
-      int i, j;
-      pair_of(ne, r, k, i, j);
-      if (j >= nc) continue;
-      if (rotate_pair<CPLX>(Wp + (int64_t)i * p, Wp + (int64_t)j * p, p, Q + i * nq, Q + j * nq, nc, tol, floor2,
-                            lane) &&
-          lane == 0)
-        s_rot = 1;
-    }
-    __syncthreads();
-  }
-  if (tid == 0) flags[blockIdx.x] = s_rot;
-  if (!s_rot) return;  // nothing rotated: W and V of the pair are unchanged (uniform branch)
-  for (int l = wave; l < nc; l += kPW) {
-    const T* src = Wp + (int64_t)l * p;
-    T* dst = W + gcol(l) * p;
-    for (int64_t r = lane; r < p; r += 64) dst[r] = src[r];
-  }
-  __syncthreads();
-  // V panel columns (q x nc) staged where the W panel was (q <= p), then V <- V Q on the FP64
-  // matrix cores: one wave per 16 x 16 output tile, v_mfma_f64_16x16x4f64 over k in steps of 4
-  // (lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16); item v of the result is
-  // D(l / 16 + 4 v, l % 16)).
-  T* Vp = Wp;
-  for (int l = wave; l < nc; l += kPW) {
-    const T* src = V + gcol(l) * q;
-    T* dst = Vp + (int64_t)l * q;
-    for (int64_t r = lane; r < q; r += 64) dst[r] = src[r];
-  }
-  __syncthreads();
-  const int qi = (int)q;  // q <= YH_SVD_MAX_Q
-  const int tq = (qi + 15) / 16, tcn = (nc + 15) / 16;
-  for (int t = wave; t < tq * tcn; t += kPW) {
+  const int tq = (rows + 15) / 16, tcn = (nc + 15) / 16;
+  for (int t = wave; t < tq * tcn; t += nwaves) {
     const int ti = t / tcn, tj = t - ti * tcn;
     const int ra = ti * 16 + (lane & 15), cb = tj * 16 + (lane & 15);
     d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < nc; k0 += 4) {
       const int k = k0 + (lane >> 4);
-      const T a = (ra < qi && k < nc) ? Vp[(int64_t)k * q + ra] : N::make(0.0, 0.0);
+      const T a = (ra < rows && k < nc) ? In[(int64_t)k * ldi + ra] : N::make(0.0, 0.0);
       const T bq = (cb < nc && k < nc) ? Q[cb * nq + k] : N::make(0.0, 0.0);
       accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::re(bq), accR, 0, 0, 0);
       if constexpr (CPLX) {
@@ -629,9 +565,209 @@ __global__ void __launch_bounds__(kPanelThreads)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = ti * 16 + (lane >> 4) + 4 * v;
-      if (row < qi && col < nc) V[gcol(col) * q + row] = N::make(accR[v], accI[v]);
+      if (row < rows && col < nc) dst(col)[row] = N::make(accR[v], accI[v]);
     }
   }
+}
+
+// One round of the panel Jacobi for one (block, panel pair): the 2b columns of panels I and J
+// (Wp, p x nc) are staged in LDS and their Gram matrix G = Wp^H Wp (nc x nc) is formed by wave-wide
+// inner products.  The workgroup then runs an inner sweep on G: per sub-round of the circle order it
+// takes the rotation of each disjoint pair from (G_ii, G_jj, G_ij) -- the same quantities and
+// criterion as a Hestenes rotation of the columns -- and applies G <- J^H G J and Q <- Q J for the
+// accumulator Q.  Afterwards W's columns become Wp Q and V's V Q
+// (matrix cores); a pair without rotations writes nothing.  Forming G once per round replaces 2b-1
+// sub-rounds of passes over columns of length p by passes over rows of length 2b.
+template <bool CPLX>
+__global__ void __launch_bounds__(kPanelThreads)
+    panel_round_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
+                       typename Num<CPLX>::T* __restrict__ work, const double* __restrict__ floors,
+                       int32_t* __restrict__ flags, int b, int64_t lds_cap) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  constexpr int kPW = kPanelThreads / 64;
+  constexpr int kMaxPairs = 64;  // nc <= 128
+  extern __shared__ double smem_raw[];
+  __shared__ int s_rot;
+  __shared__ int s_pi[kMaxPairs], s_pj[kMaxPairs], s_do[kMaxPairs];
+  __shared__ double s_c[kMaxPairs], s_s[kMaxPairs], s_er[kMaxPairs], s_ei[kMaxPairs];
+  const int32_t* jb = jobs + 3 * (int64_t)blockIdx.x;
+  const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
+  const bool tr = d.m < d.n;
+  const int64_t p = tr ? d.n : d.m, q = tr ? d.m : d.n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t c0i = (int64_t)jb[1] * b, c0j = (int64_t)jb[2] * b;
+  const int64_t ci = min((int64_t)b, q - c0i), cj = min((int64_t)b, q - c0j);
+  const int nc = (int)(ci + cj);
+  const int nq = 2 * b + 1;  // leading dimension of G and Q (odd: column reads spread over banks)
+  T* W = work + d.work_off;
+  T* V = W + p * q;
+  T* Wp = reinterpret_cast<T*>(smem_raw);
+  T* G = Wp + (int64_t)(2 * b) * p;  // after the 2b panel columns
+  T* Q = G + (int64_t)(2 * b) * nq;
+  // Gram tiles of 16 x 16 (upper triangle), each summed over kChunk partial products
+  const int T16 = (2 * b + 15) / 16;
+  const int ntile = T16 * (T16 + 1) / 2;
+  const int kChunk = kPW / ntile >= 4 ? 4 : (kPW / ntile >= 1 ? kPW / ntile : 1);
+  double* part = reinterpret_cast<double*>(Q + (int64_t)(2 * b) * nq);  // ntile * kChunk * 256 (x2) doubles
+  const int64_t need = (int64_t)((2 * b) * p + (int64_t)2 * (2 * b) * nq) * (int64_t)sizeof(T) +
+                       (int64_t)ntile * kChunk * 256 * (CPLX ? 2 : 1) * (int64_t)sizeof(double);
+  if (need > lds_cap || nc > 2 * kMaxPairs) {
+    if (tid == 0) flags[blockIdx.x] = -1;  // uniform: plan too small for this panel pair
+    return;
+  }
+  auto gcol = [&](int64_t l) { return l < ci ? c0i + l : c0j + (l - ci); };
+  for (int l = wave; l < nc; l += kPW) {  // one wave per column, lanes along it
+    const T* src = W + gcol(l) * p;
+    T* dst = Wp + (int64_t)l * p;
+    for (int64_t r = lane; r < p; r += 64) dst[r] = src[r];
+  }
+  if (tid == 0) s_rot = 0;
+  __syncthreads();
+  // ---- G = Wp^H Wp on the matrix cores: the 16 x 16 tiles (ti <= tj) of G, each split over kChunk
+  // waves along p (lane l supplies A(l % 16, k) = conj(Wp[k, ti 16 + l % 16]) and B(k, l % 16) =
+  // Wp[k, tj 16 + l % 16] with k = k0 + l / 16); the partial tiles are summed through LDS and mirrored.
+  {
+    for (int wv = wave; wv < ntile * kChunk; wv += kPW) {
+      const int tl = wv / kChunk, ch = wv - tl * kChunk;
+      int ti = 0, rem = tl;
+      while (rem >= T16 - ti) {
+        rem -= T16 - ti;
+        ++ti;
+      }
+      const int tj = ti + rem;
+      const int ca = ti * 16 + (lane & 15), cb = tj * 16 + (lane & 15);
+      const int64_t klen = (p + kChunk - 1) / kChunk;
+      const int64_t kb = ch * klen, ke = min(p, kb + klen);
+      d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
+      for (int64_t k0 = kb; k0 < ke; k0 += 4) {
+        const int64_t k = k0 + (lane >> 4);
+        const T a = (ca < nc && k < ke) ? Wp[(int64_t)ca * p + k] : N::make(0.0, 0.0);
+        const T bb = (cb < nc && k < ke) ? Wp[(int64_t)cb * p + k] : N::make(0.0, 0.0);
+        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::re(bb), accR, 0, 0, 0);
+        if constexpr (CPLX) {  // conj(a) b
+          accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(a), N::im(bb), accR, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::im(bb), accI, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(a), N::re(bb), accI, 0, 0, 0);
+        }
+      }
+      double* dst = part + (int64_t)wv * 256 * (CPLX ? 2 : 1);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        dst[v * 64 + lane] = accR[v];
+        if constexpr (CPLX) dst[256 + v * 64 + lane] = accI[v];
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < ntile * 256; e += kPanelThreads) {
+      const int tl = e >> 8, idx = e & 255;
+      const int v = idx >> 6, ln = idx & 63;
+      int ti = 0, rem = tl;
+      while (rem >= T16 - ti) {
+        rem -= T16 - ti;
+        ++ti;
+      }
+      const int tj = ti + rem;
+      const int i = ti * 16 + (ln >> 4) + 4 * v, j = tj * 16 + (ln & 15);  // D(l / 16 + 4 v, l % 16)
+      double gr = 0.0, gi = 0.0;
+      for (int ch = 0; ch < kChunk; ++ch) {
+        const double* src = part + (int64_t)(tl * kChunk + ch) * 256 * (CPLX ? 2 : 1);
+        gr += src[idx];
+        if constexpr (CPLX) gi += src[256 + idx];
+      }
+      if (i < nc && j < nc) {
+        G[j * nq + i] = N::make(gr, gi);  // G(i, j)
+        if (ti != tj) G[i * nq + j] = N::make(gr, -gi);  // G(j, i) of the mirrored tile
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < nc * nq; e += kPanelThreads) {
+    const int l = e / nq, k = e - l * nq;
+    Q[e] = N::make(l == k ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  // ---- inner sweep on G.  Per sub-round: the rotation of each disjoint pair from the current G, then
+  // G <- J^H G J in one pass over 2 x 2 blocks (rows of pair p, columns of pair p2: each output block
+  // depends only on the same input block and the two rotations) and Q <- Q J, all threads.
+  {
+    const double tol = DBL_EPSILON * sqrt((double)(p > 1 ? p : 1));
+    const double floor2 = floors[jb[0]];
+    const int ne = nc + (nc & 1);
+    const int npr = ne / 2;
+    for (int r = 0; r < ne - 1; ++r) {
+      if (tid < npr) {
+        int i, j;
+        pair_of(ne, r, tid, i, j);
+        int go = 0;
+        double c = 1.0, sn = 0.0, er = 1.0, ei = 0.0;
+        if (j < nc) {
+          const T gij = G[j * nq + i];
+          go = rotation(N::re(G[i * nq + i]), N::re(G[j * nq + j]), N::re(gij), N::im(gij), tol, floor2, c, sn, er,
+                        ei)
+                   ? 1
+                   : 0;
+        }
+        s_pi[tid] = i;
+        s_pj[tid] = j;
+        s_do[tid] = go;
+        s_c[tid] = c;
+        s_s[tid] = sn;
+        s_er[tid] = er;
+        s_ei[tid] = ei;
+        if (go) s_rot = 1;
+      }
+      __syncthreads();
+      for (int e = tid; e < npr * npr + npr * nc; e += kPanelThreads) {
+        if (e < npr * npr) {
+          const int pa = e / npr, pb = e - pa * npr;
+          if (!s_do[pa] && !s_do[pb]) continue;
+          const int i = s_pi[pa], j = s_pj[pa], i2 = s_pi[pb], j2 = s_pj[pb];
+          const bool hj = j < nc, hj2 = j2 < nc;
+          T b00 = G[i2 * nq + i];
+          T b01 = hj2 ? G[j2 * nq + i] : N::make(0.0, 0.0);
+          T b10 = hj ? G[i2 * nq + j] : N::make(0.0, 0.0);
+          T b11 = (hj && hj2) ? G[j2 * nq + j] : N::make(0.0, 0.0);
+          if (s_do[pa]) {  // rows: J^H
+            apply_rot<CPLX>(b00, b10, s_c[pa], s_s[pa], s_er[pa], -s_ei[pa]);
+            apply_rot<CPLX>(b01, b11, s_c[pa], s_s[pa], s_er[pa], -s_ei[pa]);
+          }
+          if (s_do[pb]) {  // columns: J
+            apply_rot<CPLX>(b00, b01, s_c[pb], s_s[pb], s_er[pb], s_ei[pb]);
+            apply_rot<CPLX>(b10, b11, s_c[pb], s_s[pb], s_er[pb], s_ei[pb]);
+          }
+          G[i2 * nq + i] = b00;
+          if (hj2) G[j2 * nq + i] = b01;
+          if (hj) G[i2 * nq + j] = b10;
+          if (hj && hj2) G[j2 * nq + j] = b11;
+        } else {
+          const int ee = e - npr * npr;
+          const int pb = ee / nc, row = ee - pb * nc;
+          if (!s_do[pb]) continue;
+          const int i2 = s_pi[pb], j2 = s_pj[pb];
+          T u = Q[i2 * nq + row], v = Q[j2 * nq + row];
+          apply_rot<CPLX>(u, v, s_c[pb], s_s[pb], s_er[pb], s_ei[pb]);
+          Q[i2 * nq + row] = u;
+          Q[j2 * nq + row] = v;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) flags[blockIdx.x] = s_rot;
+  if (!s_rot) return;  // nothing rotated: W and V of the pair are unchanged (uniform branch)
+  // ---- W columns <- Wp Q (straight from LDS to global)
+  times_q<CPLX>(Wp, p, (int)p, Q, nq, nc, [&](int c) { return W + gcol(c) * p; }, wave, lane, kPW);
+  __syncthreads();
+  // ---- V panel columns (q x nc) staged where Wp was, then V <- V Q
+  T* Vp = Wp;
+  for (int l = wave; l < nc; l += kPW) {
+    const T* src = V + gcol(l) * q;
+    T* dst = Vp + (int64_t)l * q;
+    for (int64_t r = lane; r < q; r += 64) dst[r] = src[r];
+  }
+  __syncthreads();
+  times_q<CPLX>(Vp, q, (int)q, Q, nq, nc, [&](int c) { return V + gcol(c) * q; }, wave, lane, kPW);
 }
 
 template <bool CPLX, int MODE>
@@ -1092,13 +1228,17 @@ extern "C" int yastn_hip_svd_panel_round(void* work, const double* floors, int d
   if (njobs == 0) return YH_OK;
   const dim3 g((unsigned)njobs), t(kPanelThreads);
   if (dtype == YH_F64) {
-    hipFuncSetAttribute((const void*)panel_round_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds_bytes);
+    const int st = yh::hip_status(hipFuncSetAttribute((const void*)panel_round_kernel<false>,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
+                                  "svd panel round LDS size");
+    if (st) return st;
     hipLaunchKernelGGL(panel_round_kernel<false>, g, t, (size_t)lds_bytes, stream, d_blocks, d_jobs,
                        (double*)work, floors, d_flags, b, lds_bytes);
   } else {
-    hipFuncSetAttribute((const void*)panel_round_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds_bytes);
+    const int st = yh::hip_status(hipFuncSetAttribute((const void*)panel_round_kernel<true>,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
+                                  "svd panel round LDS size");
+    if (st) return st;
     hipLaunchKernelGGL(panel_round_kernel<true>, g, t, (size_t)lds_bytes, stream, d_blocks, d_jobs,
                        (double2*)work, floors, d_flags, b, lds_bytes);
   }
@@ -1164,12 +1304,16 @@ extern "C" int yastn_hip_svd_qr_panel(void* work, int dtype, const int64_t* d_qd
   if (n == 0) return YH_OK;
   const dim3 g((unsigned)n), t(kThreads);
   if (dtype == YH_F64) {
-    hipFuncSetAttribute((const void*)qr_panel_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds_bytes);
+    const int st = yh::hip_status(hipFuncSetAttribute((const void*)qr_panel_kernel<false>,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
+                                  "svd qr panel LDS size");
+    if (st) return st;
     hipLaunchKernelGGL(qr_panel_kernel<false>, g, t, (size_t)lds_bytes, stream, d_qdesc, d_ids, (double*)work, step);
   } else {
-    hipFuncSetAttribute((const void*)qr_panel_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds_bytes);
+    const int st = yh::hip_status(hipFuncSetAttribute((const void*)qr_panel_kernel<true>,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
+                                  "svd qr panel LDS size");
+    if (st) return st;
     hipLaunchKernelGGL(qr_panel_kernel<true>, g, t, (size_t)lds_bytes, stream, d_qdesc, d_ids, (double2*)work, step);
   }
   return yh::hip_status(hipGetLastError(), "svd qr panel launch");

@@ -9,6 +9,7 @@ scalar arguments.  Each plan is validated once on the host by the library's
 with an out-of-bounds work list.
 """
 import ctypes
+import os
 import math
 from itertools import product
 
@@ -635,6 +636,9 @@ def _circle_pair(ne, r, k):
 
 
 PRECOND = True  # QR-precondition the panel path of SVD blocks (off: plain one-sided Jacobi on W0)
+# Panel width b of the panel Jacobi (a workgroup rotates 2b columns per round): smaller panels give more
+# workgroups per round but more rounds per sweep.  Tuning knob; the LDS limit may lower it further.
+PANEL_WIDTH = int(os.environ.get('YASTN_HIP_PANEL_WIDTH', '16'))
 
 
 class SvdPlan:
@@ -713,9 +717,14 @@ class SvdPlan:
             self.d_jblocks = torch.from_numpy(self.h_jblocks).to(device)
             limit = dev_lds - 2048
             pmax = max(max(jwords[b][1], jwords[b][2]) for b in big)
-            width = 16
-            # LDS of a panel pair: W columns (p x 2b) and the rotation accumulator (2b x (2b + 1))
-            lds = lambda w: (2 * w * pmax + 2 * w * (2 * w + 1)) * itemsize
+            width = PANEL_WIDTH
+            # LDS of a panel pair: W columns (p x 2b), its Gram matrix and the rotation accumulator
+            # (2b x (2b + 1) each), and the partial 16 x 16 Gram tiles (kChunk per tile)
+            def lds(w):
+                t16 = -(-2 * w // 16)
+                ntile = t16 * (t16 + 1) // 2
+                kch = min(4, max(1, 16 // ntile))
+                return (2 * w * pmax + 2 * (2 * w) * (2 * w + 1)) * itemsize + ntile * kch * 256 * itemsize
             while width > 1 and lds(width) > limit:
                 width //= 2
             self.panel_width = width
