@@ -596,8 +596,9 @@ __global__ void __launch_bounds__(kPanelThreads)
   const bool tr = d.m < d.n;
   const int64_t p = tr ? d.n : d.m, q = tr ? d.m : d.n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool intra = jb[1] == jb[2];  // one panel against itself, else two panels (cross pairs only)
   const int64_t c0i = (int64_t)jb[1] * b, c0j = (int64_t)jb[2] * b;
-  const int64_t ci = min((int64_t)b, q - c0i), cj = min((int64_t)b, q - c0j);
+  const int64_t ci = min((int64_t)b, q - c0i), cj = intra ? 0 : min((int64_t)b, q - c0j);
   const int nc = (int)(ci + cj);
   const int nq = 2 * b + 1;  // leading dimension of G and Q (odd: column reads spread over banks)
   T* W = work + d.work_off;
@@ -688,17 +689,45 @@ __global__ void __launch_bounds__(kPanelThreads)
   }
   __syncthreads();
   // ---- inner sweep on G.  Per sub-round: the rotation of each disjoint pair from the current G, then
-  // G <- J^H G J in one pass over 2 x 2 blocks (rows of pair p, columns of pair p2: each output block
-  // depends only on the same input block and the two rotations) and Q <- Q J, all threads.
+  // G <- J^H G J in one pass over 2 x 2 blocks (rows of slot p, columns of slot p2: each output block
+  // depends only on the same input block and the two rotations) and Q <- Q J, all threads.  A slot is
+  // a pair or, where the ordering leaves a column idle, a single column (j >= nc).
+  //  - one panel (intra): all pairs of its columns, circle order (nc - 1 sub-rounds of nc / 2 pairs);
+  //  - two panels: only the cross pairs (i in I, j in J), max(ci, cj) sub-rounds, sub-round s pairing
+  //    column k of the smaller panel with column (k + s) mod (size) of the larger one.
+  // Every pair of a block's columns is thus rotated once per sweep (panels against themselves in the
+  // first round of a sweep, then the circle order of panel pairs).
   {
     const double tol = DBL_EPSILON * sqrt((double)(p > 1 ? p : 1));
     const double floor2 = floors[jb[0]];
     const int ne = nc + (nc & 1);
-    const int npr = ne / 2;
-    for (int r = 0; r < ne - 1; ++r) {
+    const int ci_ = (int)ci, cj_ = (int)cj;
+    const int npr = intra ? ne / 2 : (ci_ > cj_ ? ci_ : cj_);  // slots per sub-round
+    const int nsub = intra ? ne - 1 : npr;
+    for (int r = 0; r < nsub; ++r) {
       if (tid < npr) {
         int i, j;
-        pair_of(ne, r, tid, i, j);
+        if (intra) {
+          pair_of(ne, r, tid, i, j);
+        } else if (ci_ <= cj_) {
+          const int l = ci_ + (tid + r) % cj_;
+          if (tid < ci_) {
+            i = tid;
+            j = l;
+          } else {
+            i = l;  // idle column of panel J
+            j = nc;
+          }
+        } else {
+          const int k = (tid + r) % ci_;
+          if (tid < cj_) {
+            i = k;
+            j = ci_ + tid;
+          } else {
+            i = k;  // idle column of panel I
+            j = nc;
+          }
+        }
         int go = 0;
         double c = 1.0, sn = 0.0, er = 1.0, ei = 0.0;
         if (j < nc) {

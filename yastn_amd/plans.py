@@ -731,12 +731,15 @@ class SvdPlan:
             self.panel_lds = lds(width)
             if self.panel_lds > limit:
                 raise ValueError(f"block with p={pmax} does not fit the LDS panel path")
+            # a sweep: every panel against itself (round -1), then the circle order of panel pairs
             per_round = {}
             for b in big:
                 m, n = blocks[b][1], blocks[b][2]
                 q = min(m, n) if mode == 0 else n
                 npan = -(-q // width)
                 ne = npan + (npan & 1)
+                for i in range(npan):
+                    per_round.setdefault(-1, []).append((b, i, i))
                 for r in range(ne - 1):
                     for k in range(ne // 2):
                         i, j = _circle_pair(ne, r, k)
