@@ -367,12 +367,13 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
                         run_gemm(gp, (work,), (work,), dt, tune=False)
             L.call('yastn_hip_svd_precond_init', ptr(work), dt, plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(),
                    plan.n_pre, st)
-        jobs0, flags0 = plan.d_jobs.data_ptr(), flags.data_ptr()
+        part = torch.empty(max(plan.part_numel, 1), dtype=torch.float64, device=data.device)
+        qbuf = torch.empty(max(plan.qbuf_numel, 1), dtype=data.dtype, device=data.device)
         converged = False
         while sweeps < MAX_SWEEPS:
-            for off, n in plan.rounds:
-                L.call('yastn_hip_svd_panel_round', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
-                       jobs0 + 12 * off, n, plan.panel_width, plan.panel_lds, flags0 + 4 * off, st)
+            L.call('yastn_hip_svd_panel_sweep', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
+                   plan.d_jobs.data_ptr(), plan.h_rounds.ctypes.data, len(plan.rounds), plan.panel_width,
+                   plan.maxch_p, plan.maxch_q, ptr(part), part.numel(), ptr(qbuf), qbuf.numel(), ptr(flags), st)
             sweeps += 1
             fmin, fmax = torch.stack([flags.min(), flags.max()]).tolist()
             if fmin < 0:

@@ -16,6 +16,8 @@
 // kernel (all sweeps in one launch); larger blocks use the panel path below.
 #include <float.h>
 
+#include <algorithm>
+
 #include "common.h"
 
 namespace {
@@ -535,144 +537,200 @@ __global__ void __launch_bounds__(kPanelThreads)
   }
 }
 
-// out (rows x nc) = In (rows x nc, column-major, leading dimension ldi, in LDS) * Q (nc x nc, column
-// c at Q + c nq) on the FP64 matrix cores, written to global columns dst(c)[row]: one wave per 16 x 16
-// output tile, v_mfma_f64_16x16x4f64 over k in steps of 4 (lane l supplies A(l % 16, l / 16) and
-// B(l / 16, l % 16); item v of the result is D(l / 16 + 4 v, l % 16)).
-template <bool CPLX, typename DST>
-__device__ __attribute__((always_inline)) inline void times_q(const typename Num<CPLX>::T* In, int64_t ldi,
-                                                              int rows, const typename Num<CPLX>::T* Q, int nq,
-                                                              int nc, DST dst, int wave, int lane, int nwaves) {
+// Dst[:, c] <- sum_k Src[:, k] Q[k, c] for the nc panel columns (rows x nc; column k of Src at src(k),
+// column c of Dst at dst(c); the two may be the same memory), on the FP64 matrix cores.  One wave per
+// 16-row stripe: it reads the stripe's nc source columns into registers (lane l holds A(l % 16,
+// 4 s + l / 16), s = 0 .. nc / 4 - 1), then multiplies by Q (up to 4 output tiles of 16 columns, item v
+// of a tile is D(l / 16 + 4 v, l % 16)) and writes the stripe; stripes are disjoint rows, so an update
+// in place never reads what another wave writes.  nc <= 64.
+template <bool CPLX, typename SRC, typename DST>
+__device__ __attribute__((always_inline)) inline void stripes_times_q(int rows, SRC src, DST dst,
+                                                                      const typename Num<CPLX>::T* Q, int nq,
+                                                                      int nc, int wave, int lane, int nwaves) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
-  const int tq = (rows + 15) / 16, tcn = (nc + 15) / 16;
-  for (int t = wave; t < tq * tcn; t += nwaves) {
-    const int ti = t / tcn, tj = t - ti * tcn;
-    const int ra = ti * 16 + (lane & 15), cb = tj * 16 + (lane & 15);
-    d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < nc; k0 += 4) {
-      const int k = k0 + (lane >> 4);
-      const T a = (ra < rows && k < nc) ? In[(int64_t)k * ldi + ra] : N::make(0.0, 0.0);
-      const T bq = (cb < nc && k < nc) ? Q[cb * nq + k] : N::make(0.0, 0.0);
-      accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::re(bq), accR, 0, 0, 0);
-      if constexpr (CPLX) {
-        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(a), N::im(bq), accR, 0, 0, 0);
-        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::im(bq), accI, 0, 0, 0);
-        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(a), N::re(bq), accI, 0, 0, 0);
-      }
-    }
-    const int col = tj * 16 + (lane & 15);
+  constexpr int kS = 16, kT = 4;
+  const int nk4 = (nc + 3) / 4, tcn = (nc + 15) / 16;
+  for (int st = wave; st < (rows + 15) / 16; st += nwaves) {
+    const int ra = st * 16 + (lane & 15);
+    T a[kS];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = ti * 16 + (lane >> 4) + 4 * v;
-      if (row < rows && col < nc) dst(col)[row] = N::make(accR[v], accI[v]);
+    for (int sidx = 0; sidx < kS; ++sidx) {
+      const int k = 4 * sidx + (lane >> 4);
+      a[sidx] = (sidx < nk4 && k < nc && ra < rows) ? src(k)[ra] : N::make(0.0, 0.0);
+    }
+#pragma unroll
+    for (int tj = 0; tj < kT; ++tj) {
+      if (tj < tcn) {
+        const int cb = tj * 16 + (lane & 15);
+        d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int sidx = 0; sidx < kS; ++sidx) {
+          if (sidx < nk4) {
+            const int k = 4 * sidx + (lane >> 4);
+            const T bq = (cb < nc && k < nc) ? Q[cb * nq + k] : N::make(0.0, 0.0);
+            accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a[sidx]), N::re(bq), accR, 0, 0, 0);
+            if constexpr (CPLX) {
+              accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(a[sidx]), N::im(bq), accR, 0, 0, 0);
+              accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a[sidx]), N::im(bq), accI, 0, 0, 0);
+              accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(a[sidx]), N::re(bq), accI, 0, 0, 0);
+            }
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = st * 16 + (lane >> 4) + 4 * v;
+          if (row < rows && cb < nc) dst(cb)[row] = N::make(accR[v], accI[v]);
+        }
+      }
     }
   }
 }
 
-// One round of the panel Jacobi for one (block, panel pair): the 2b columns of panels I and J
-// (Wp, p x nc) are staged in LDS and their Gram matrix G = Wp^H Wp (nc x nc) is formed by wave-wide
-// inner products.  The workgroup then runs an inner sweep on G: per sub-round of the circle order it
-// takes the rotation of each disjoint pair from (G_ii, G_jj, G_ij) -- the same quantities and
-// criterion as a Hestenes rotation of the columns -- and applies G <- J^H G J and Q <- Q J for the
-// accumulator Q.  Afterwards W's columns become Wp Q and V's V Q
-// (matrix cores); a pair without rotations writes nothing.  Forming G once per round replaces 2b-1
-// sub-rounds of passes over columns of length p by passes over rows of length 2b.
+// A round of the panel Jacobi runs three kernels over the (block, panel i, panel j) jobs of the round:
+//  1. panel_gram_kernel: the Gram matrix G = Wp^H Wp (nc x nc) of the pair's nc = |I| + |J| columns
+//     Wp, as partial sums over chunks of kChunkRows rows, one workgroup per (chunk, job);
+//  2. panel_inner_kernel: one workgroup per job sums the partials, runs one inner sweep of rotations on
+//     G (the same quantities and criterion as Hestenes rotations of the columns) and writes the
+//     accumulated rotation Q (nc x nc) and whether anything rotated;
+//  3. panel_apply_kernel: W's and V's pair columns <- (.) Q in place, one workgroup per (chunk, job).
+// The first and last kernels spread a pair over many CUs; the sequential inner sweep touches only
+// nc x nc numbers.  Workgroups of jobs that did not rotate exit at once in kernel 3.
+constexpr int kChunkRows = 64;     // rows of W (or V) per workgroup of kernels 1 and 3
+constexpr int kChunkThreads = 256;  // 4 waves: one 16-row stripe each
+
+struct PanelPair {
+  int64_t p, q, ci, cj, c0i, c0j;
+  int nc;
+  bool intra;
+  __device__ int64_t col(int64_t l) const { return l < ci ? c0i + l : c0j + (l - ci); }
+};
+
+__device__ inline PanelPair panel_pair(const BlockDesc& d, const int32_t* jb, int b) {
+  PanelPair pp;
+  const bool tr = d.m < d.n;
+  pp.p = tr ? d.n : d.m;
+  pp.q = tr ? d.m : d.n;
+  pp.intra = jb[1] == jb[2];  // one panel against itself, else two panels (cross pairs only)
+  pp.c0i = (int64_t)jb[1] * b;
+  pp.c0j = (int64_t)jb[2] * b;
+  pp.ci = min((int64_t)b, pp.q - pp.c0i);
+  pp.cj = pp.intra ? 0 : min((int64_t)b, pp.q - pp.c0j);
+  pp.nc = (int)(pp.ci + pp.cj);
+  return pp;
+}
+
+// Upper-triangle 16 x 16 tile tl of a T16 x T16 tile grid -> (ti, tj), ti <= tj.
+__device__ inline void tile_of(int tl, int T16, int& ti, int& tj) {
+  int rem = tl;
+  ti = 0;
+  while (rem >= T16 - ti) {
+    rem -= T16 - ti;
+    ++ti;
+  }
+  tj = ti + rem;
+}
+
+// Partial Gram tiles of one (row chunk, job): the chunk's nc columns are staged in LDS (odd column
+// stride, so the 16 columns of a tile fall in different banks); each wave forms upper-triangle tiles
+// on the matrix cores (lane l supplies A(l % 16, k) = conj(Wp[k, ti 16 + l % 16]) and B(k, l % 16) =
+// Wp[k, tj 16 + l % 16], k = k0 + l / 16) and stores them in the MFMA result order (item v at v 64 + l).
 template <bool CPLX>
-__global__ void __launch_bounds__(kPanelThreads)
-    panel_round_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
-                       typename Num<CPLX>::T* __restrict__ work, const double* __restrict__ floors,
-                       int32_t* __restrict__ flags, int b, int64_t lds_cap) {
+__global__ void __launch_bounds__(kChunkThreads)
+    panel_gram_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
+                      const typename Num<CPLX>::T* __restrict__ work, double* __restrict__ part, int b, int maxch) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
-  constexpr int kPW = kPanelThreads / 64;
-  constexpr int kMaxPairs = 64;  // nc <= 128
+  extern __shared__ double smem_raw[];
+  constexpr int ld = kChunkRows + 1;
+  const int32_t* jb = jobs + 3 * (int64_t)blockIdx.y;
+  const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
+  const PanelPair pp = panel_pair(d, jb, b);
+  const int64_t r0 = (int64_t)blockIdx.x * kChunkRows;
+  if (r0 >= pp.p) return;  // uniform: beyond this block's rows
+  const int rows = (int)min((int64_t)kChunkRows, pp.p - r0);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const T* W = work + d.work_off;
+  T* S = reinterpret_cast<T*>(smem_raw);
+  for (int e = tid; e < pp.nc * kChunkRows; e += kChunkThreads) {
+    const int l = e / kChunkRows, r = e - l * kChunkRows;
+    S[l * ld + r] = r < rows ? W[pp.col(l) * pp.p + r0 + r] : N::make(0.0, 0.0);
+  }
+  __syncthreads();
+  const int T16 = (pp.nc + 15) / 16, ntile = T16 * (T16 + 1) / 2;
+  const int T16m = (2 * b + 15) / 16, ntmax = T16m * (T16m + 1) / 2;
+  for (int tl = wave; tl < ntile; tl += kChunkThreads / 64) {
+    int ti, tj;
+    tile_of(tl, T16, ti, tj);
+    const int ca = ti * 16 + (lane & 15), cb = tj * 16 + (lane & 15);
+    d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k0 = 0; k0 < kChunkRows; k0 += 4) {
+      const int k = k0 + (lane >> 4);
+      const T a = ca < pp.nc ? S[ca * ld + k] : N::make(0.0, 0.0);
+      const T bb = cb < pp.nc ? S[cb * ld + k] : N::make(0.0, 0.0);
+      accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::re(bb), accR, 0, 0, 0);
+      if constexpr (CPLX) {  // conj(a) b
+        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(a), N::im(bb), accR, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::im(bb), accI, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(a), N::re(bb), accI, 0, 0, 0);
+      }
+    }
+    double* dst = part + (((int64_t)blockIdx.y * maxch + blockIdx.x) * ntmax + tl) * 256 * (CPLX ? 2 : 1);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      dst[v * 64 + lane] = accR[v];
+      if constexpr (CPLX) dst[256 + v * 64 + lane] = accI[v];
+    }
+  }
+}
+
+// Inner sweep of one job on its Gram matrix (LDS: G and the rotation accumulator Q, nc x (2b + 1)
+// each).  flags[job] = 1 if anything rotated (then Q goes to qbuf, column c at qbuf + (job 2b + c) 2b),
+// 0 if not, -1 if lds_cap is too small for the pair.
+template <bool CPLX>
+__global__ void __launch_bounds__(kPanelThreads)
+    panel_inner_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
+                       const double* __restrict__ floors, const double* __restrict__ part,
+                       typename Num<CPLX>::T* __restrict__ qbuf, int32_t* __restrict__ flags, int b, int maxch,
+                       int64_t lds_cap) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  constexpr int kMaxPairs = 32;  // nc <= 64
   extern __shared__ double smem_raw[];
   __shared__ int s_rot;
   __shared__ int s_pi[kMaxPairs], s_pj[kMaxPairs], s_do[kMaxPairs];
   __shared__ double s_c[kMaxPairs], s_s[kMaxPairs], s_er[kMaxPairs], s_ei[kMaxPairs];
   const int32_t* jb = jobs + 3 * (int64_t)blockIdx.x;
   const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
-  const bool tr = d.m < d.n;
-  const int64_t p = tr ? d.n : d.m, q = tr ? d.m : d.n;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const bool intra = jb[1] == jb[2];  // one panel against itself, else two panels (cross pairs only)
-  const int64_t c0i = (int64_t)jb[1] * b, c0j = (int64_t)jb[2] * b;
-  const int64_t ci = min((int64_t)b, q - c0i), cj = intra ? 0 : min((int64_t)b, q - c0j);
-  const int nc = (int)(ci + cj);
+  const PanelPair pp = panel_pair(d, jb, b);
+  const int64_t p = pp.p, ci = pp.ci, cj = pp.cj;
+  const int nc = pp.nc;
+  const bool intra = pp.intra;
+  const int tid = threadIdx.x;
   const int nq = 2 * b + 1;  // leading dimension of G and Q (odd: column reads spread over banks)
-  T* W = work + d.work_off;
-  T* V = W + p * q;
-  T* Wp = reinterpret_cast<T*>(smem_raw);
-  T* G = Wp + (int64_t)(2 * b) * p;  // after the 2b panel columns
+  T* G = reinterpret_cast<T*>(smem_raw);
   T* Q = G + (int64_t)(2 * b) * nq;
-  // Gram tiles of 16 x 16 (upper triangle), each summed over kChunk partial products
-  const int T16 = (2 * b + 15) / 16;
-  const int ntile = T16 * (T16 + 1) / 2;
-  const int kChunk = kPW / ntile >= 4 ? 4 : (kPW / ntile >= 1 ? kPW / ntile : 1);
-  double* part = reinterpret_cast<double*>(Q + (int64_t)(2 * b) * nq);  // ntile * kChunk * 256 (x2) doubles
-  const int64_t need = (int64_t)((2 * b) * p + (int64_t)2 * (2 * b) * nq) * (int64_t)sizeof(T) +
-                       (int64_t)ntile * kChunk * 256 * (CPLX ? 2 : 1) * (int64_t)sizeof(double);
+  const int64_t need = (int64_t)2 * (2 * b) * nq * (int64_t)sizeof(T);
   if (need > lds_cap || nc > 2 * kMaxPairs) {
     if (tid == 0) flags[blockIdx.x] = -1;  // uniform: plan too small for this panel pair
     return;
   }
-  auto gcol = [&](int64_t l) { return l < ci ? c0i + l : c0j + (l - ci); };
-  for (int l = wave; l < nc; l += kPW) {  // one wave per column, lanes along it
-    const T* src = W + gcol(l) * p;
-    T* dst = Wp + (int64_t)l * p;
-    for (int64_t r = lane; r < p; r += 64) dst[r] = src[r];
-  }
   if (tid == 0) s_rot = 0;
-  __syncthreads();
-  // ---- G = Wp^H Wp on the matrix cores: the 16 x 16 tiles (ti <= tj) of G, each split over kChunk
-  // waves along p (lane l supplies A(l % 16, k) = conj(Wp[k, ti 16 + l % 16]) and B(k, l % 16) =
-  // Wp[k, tj 16 + l % 16] with k = k0 + l / 16); the partial tiles are summed through LDS and mirrored.
+  // ---- G: sum of the chunks' partial tiles, mirrored to the lower triangle
   {
-    for (int wv = wave; wv < ntile * kChunk; wv += kPW) {
-      const int tl = wv / kChunk, ch = wv - tl * kChunk;
-      int ti = 0, rem = tl;
-      while (rem >= T16 - ti) {
-        rem -= T16 - ti;
-        ++ti;
-      }
-      const int tj = ti + rem;
-      const int ca = ti * 16 + (lane & 15), cb = tj * 16 + (lane & 15);
-      const int64_t klen = (p + kChunk - 1) / kChunk;
-      const int64_t kb = ch * klen, ke = min(p, kb + klen);
-      d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
-      for (int64_t k0 = kb; k0 < ke; k0 += 4) {
-        const int64_t k = k0 + (lane >> 4);
-        const T a = (ca < nc && k < ke) ? Wp[(int64_t)ca * p + k] : N::make(0.0, 0.0);
-        const T bb = (cb < nc && k < ke) ? Wp[(int64_t)cb * p + k] : N::make(0.0, 0.0);
-        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::re(bb), accR, 0, 0, 0);
-        if constexpr (CPLX) {  // conj(a) b
-          accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(a), N::im(bb), accR, 0, 0, 0);
-          accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::im(bb), accI, 0, 0, 0);
-          accI = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(a), N::re(bb), accI, 0, 0, 0);
-        }
-      }
-      double* dst = part + (int64_t)wv * 256 * (CPLX ? 2 : 1);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        dst[v * 64 + lane] = accR[v];
-        if constexpr (CPLX) dst[256 + v * 64 + lane] = accI[v];
-      }
-    }
-    __syncthreads();
+    const int T16 = (nc + 15) / 16, ntile = T16 * (T16 + 1) / 2;
+    const int T16m = (2 * b + 15) / 16, ntmax = T16m * (T16m + 1) / 2;
+    const int nch = (int)((p + kChunkRows - 1) / kChunkRows);
     for (int e = tid; e < ntile * 256; e += kPanelThreads) {
       const int tl = e >> 8, idx = e & 255;
       const int v = idx >> 6, ln = idx & 63;
-      int ti = 0, rem = tl;
-      while (rem >= T16 - ti) {
-        rem -= T16 - ti;
-        ++ti;
-      }
-      const int tj = ti + rem;
+      int ti, tj;
+      tile_of(tl, T16, ti, tj);
       const int i = ti * 16 + (ln >> 4) + 4 * v, j = tj * 16 + (ln & 15);  // D(l / 16 + 4 v, l % 16)
       double gr = 0.0, gi = 0.0;
-      for (int ch = 0; ch < kChunk; ++ch) {
-        const double* src = part + (int64_t)(tl * kChunk + ch) * 256 * (CPLX ? 2 : 1);
+      for (int ch = 0; ch < nch; ++ch) {
+        const double* src = part + (((int64_t)blockIdx.x * maxch + ch) * ntmax + tl) * 256 * (CPLX ? 2 : 1);
         gr += src[idx];
         if constexpr (CPLX) gi += src[256 + idx];
       }
@@ -681,7 +739,6 @@ __global__ void __launch_bounds__(kPanelThreads)
         if (ti != tj) G[i * nq + j] = N::make(gr, -gi);  // G(j, i) of the mirrored tile
       }
     }
-    __syncthreads();
   }
   for (int e = tid; e < nc * nq; e += kPanelThreads) {
     const int l = e / nq, k = e - l * nq;
@@ -784,19 +841,45 @@ __global__ void __launch_bounds__(kPanelThreads)
     }
   }
   if (tid == 0) flags[blockIdx.x] = s_rot;
-  if (!s_rot) return;  // nothing rotated: W and V of the pair are unchanged (uniform branch)
-  // ---- W columns <- Wp Q (straight from LDS to global)
-  times_q<CPLX>(Wp, p, (int)p, Q, nq, nc, [&](int c) { return W + gcol(c) * p; }, wave, lane, kPW);
-  __syncthreads();
-  // ---- V panel columns (q x nc) staged where Wp was, then V <- V Q
-  T* Vp = Wp;
-  for (int l = wave; l < nc; l += kPW) {
-    const T* src = V + gcol(l) * q;
-    T* dst = Vp + (int64_t)l * q;
-    for (int64_t r = lane; r < q; r += 64) dst[r] = src[r];
+  if (!s_rot) return;  // uniform
+  T* dst = qbuf + (int64_t)blockIdx.x * (2 * b) * (2 * b);
+  for (int e = tid; e < nc * nc; e += kPanelThreads) {
+    const int c = e / nc, k = e - c * nc;
+    dst[c * (2 * b) + k] = Q[c * nq + k];
+  }
+}
+
+// X[:, pair columns] <- X[:, pair columns] Q in place for one (row chunk, job), X = W for the first
+// maxch_p chunks of the grid and X = V after them; jobs that did not rotate exit at once.
+template <bool CPLX>
+__global__ void __launch_bounds__(kChunkThreads)
+    panel_apply_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
+                       typename Num<CPLX>::T* __restrict__ work, const typename Num<CPLX>::T* __restrict__ qbuf,
+                       const int32_t* __restrict__ flags, int b, int maxch_p) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  extern __shared__ double smem_raw[];
+  if (flags[blockIdx.y] <= 0) return;  // uniform: nothing rotated (or no plan)
+  const int32_t* jb = jobs + 3 * (int64_t)blockIdx.y;
+  const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
+  const PanelPair pp = panel_pair(d, jb, b);
+  const bool isv = (int)blockIdx.x >= maxch_p;
+  const int64_t ldx = isv ? pp.q : pp.p;
+  const int64_t r0 = (int64_t)(isv ? blockIdx.x - maxch_p : blockIdx.x) * kChunkRows;
+  if (r0 >= ldx) return;  // uniform: beyond this block's rows
+  const int rows = (int)min((int64_t)kChunkRows, ldx - r0);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nc = pp.nc, nq = 2 * b + 1;
+  T* Qs = reinterpret_cast<T*>(smem_raw);
+  const T* src = qbuf + (int64_t)blockIdx.y * (2 * b) * (2 * b);
+  for (int e = tid; e < nc * nc; e += kChunkThreads) {
+    const int c = e / nc, k = e - c * nc;
+    Qs[c * nq + k] = src[c * (2 * b) + k];
   }
   __syncthreads();
-  times_q<CPLX>(Vp, q, (int)q, Q, nq, nc, [&](int c) { return V + gcol(c) * q; }, wave, lane, kPW);
+  T* X = work + d.work_off + (isv ? pp.p * pp.q : 0) + r0;
+  auto col = [&](int c) { return X + pp.col(c) * ldx; };
+  stripes_times_q<CPLX>(rows, col, col, Qs, nq, nc, wave, lane, kChunkThreads / 64);
 }
 
 template <bool CPLX, int MODE>
@@ -1249,29 +1332,90 @@ extern "C" int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, d
   return yh::hip_status(hipGetLastError(), "svd panel init launch");
 }
 
-extern "C" int yastn_hip_svd_panel_round(void* work, const double* floors, int dtype, const int64_t* d_blocks,
-                                         const int32_t* d_jobs, int64_t njobs, int b, int64_t lds_bytes,
-                                         int32_t* d_flags, hipStream_t stream) {
+extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int dtype, const int64_t* d_blocks,
+                                         const int32_t* d_jobs, const int64_t* h_rounds, int64_t nrounds, int b,
+                                         int maxch_p, int maxch_q, void* part, int64_t part_numel, void* qbuf,
+                                         int64_t qbuf_numel, int32_t* d_flags, hipStream_t stream) {
   YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
-  YH_CHECK_ARG(b >= 1 && b <= 64 && lds_bytes > 0, "panel width %d / lds %lld", b, (long long)lds_bytes);
-  if (njobs == 0) return YH_OK;
-  const dim3 g((unsigned)njobs), t(kPanelThreads);
-  if (dtype == YH_F64) {
-    const int st = yh::hip_status(hipFuncSetAttribute((const void*)panel_round_kernel<false>,
-                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
-                                  "svd panel round LDS size");
-    if (st) return st;
-    hipLaunchKernelGGL(panel_round_kernel<false>, g, t, (size_t)lds_bytes, stream, d_blocks, d_jobs,
-                       (double*)work, floors, d_flags, b, lds_bytes);
-  } else {
-    const int st = yh::hip_status(hipFuncSetAttribute((const void*)panel_round_kernel<true>,
-                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
-                                  "svd panel round LDS size");
-    if (st) return st;
-    hipLaunchKernelGGL(panel_round_kernel<true>, g, t, (size_t)lds_bytes, stream, d_blocks, d_jobs,
-                       (double2*)work, floors, d_flags, b, lds_bytes);
+  YH_CHECK_ARG(b >= 1 && b <= 32, "panel width %d", b);
+  YH_CHECK_ARG(maxch_p >= 1 && maxch_q >= 1 && maxch_p <= 65535 && maxch_q <= 65535, "chunk counts");
+  const bool cplx = dtype == YH_C128;
+  const int T16m = (2 * b + 15) / 16, ntmax = T16m * (T16m + 1) / 2;
+  const int64_t isz = cplx ? 16 : 8;
+  int64_t maxn = 0, end = 0;
+  for (int64_t r = 0; r < nrounds; ++r) {
+    YH_CHECK_ARG(h_rounds[2 * r] >= end && h_rounds[2 * r + 1] >= 0, "round %lld", (long long)r);
+    end = h_rounds[2 * r] + h_rounds[2 * r + 1];
+    maxn = std::max(maxn, h_rounds[2 * r + 1]);
   }
-  return yh::hip_status(hipGetLastError(), "svd panel round launch");
+  YH_CHECK_ARG(maxn <= 65535, "jobs per round %lld", (long long)maxn);
+  YH_CHECK_ARG(part_numel >= maxn * maxch_p * ntmax * 256 * (cplx ? 2 : 1), "partial Gram buffer too small");
+  YH_CHECK_ARG(qbuf_numel >= maxn * (2 * b) * (2 * b), "rotation buffer too small");
+  const int64_t lds_inner = 2 * (2 * b) * (2 * b + 1) * isz;
+  const int64_t lds_gram = (2 * b) * (kChunkRows + 1) * isz;
+  const int64_t lds_apply = (2 * b) * (2 * b + 1) * isz;
+  int st = YH_OK;
+  if (cplx) {
+    st = yh::hip_status(hipFuncSetAttribute((const void*)panel_inner_kernel<true>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_inner),
+                        "svd panel inner LDS size");
+    if (!st)
+      st = yh::hip_status(hipFuncSetAttribute((const void*)panel_gram_kernel<true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_gram),
+                          "svd panel gram LDS size");
+    if (!st)
+      st = yh::hip_status(hipFuncSetAttribute((const void*)panel_apply_kernel<true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_apply),
+                          "svd panel apply LDS size");
+  } else {
+    st = yh::hip_status(hipFuncSetAttribute((const void*)panel_inner_kernel<false>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_inner),
+                        "svd panel inner LDS size");
+  }
+  if (st) return st;
+  for (int64_t r = 0; r < nrounds; ++r) {
+    const int64_t off = h_rounds[2 * r], n = h_rounds[2 * r + 1];
+    if (n == 0) continue;
+    const int32_t* jobs = d_jobs + 3 * off;
+    int32_t* flags = d_flags + off;
+    const dim3 gg((unsigned)maxch_p, (unsigned)n), ga((unsigned)(maxch_p + maxch_q), (unsigned)n);
+    if (cplx) {
+      hipLaunchKernelGGL(panel_gram_kernel<true>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
+                         (const double2*)work, (double*)part, b, maxch_p);
+      hipLaunchKernelGGL(panel_inner_kernel<true>, dim3((unsigned)n), dim3(kPanelThreads), (size_t)lds_inner, stream,
+                         d_blocks, jobs, floors, (const double*)part, (double2*)qbuf, flags, b, maxch_p, lds_inner);
+      hipLaunchKernelGGL(panel_apply_kernel<true>, ga, dim3(kChunkThreads), (size_t)lds_apply, stream, d_blocks,
+                         jobs, (double2*)work, (const double2*)qbuf, (const int32_t*)flags, b, maxch_p);
+    } else {
+      hipLaunchKernelGGL(panel_gram_kernel<false>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
+                         (const double*)work, (double*)part, b, maxch_p);
+      hipLaunchKernelGGL(panel_inner_kernel<false>, dim3((unsigned)n), dim3(kPanelThreads), (size_t)lds_inner, stream,
+                         d_blocks, jobs, floors, (const double*)part, (double*)qbuf, flags, b, maxch_p, lds_inner);
+      hipLaunchKernelGGL(panel_apply_kernel<false>, ga, dim3(kChunkThreads), (size_t)lds_apply, stream, d_blocks,
+                         jobs, (double*)work, (const double*)qbuf, (const int32_t*)flags, b, maxch_p);
+    }
+    st = yh::hip_status(hipGetLastError(), "svd panel round launch");
+    if (st) return st;
+  }
+  return YH_OK;
+}
+
+extern "C" int yastn_hip_check_panel_jobs(const int64_t* h_blocks, int64_t nblocks, const int32_t* h_jobs,
+                                          int64_t njobs, int b, int mode, int maxch_p, int maxch_q) {
+  YH_CHECK_ARG(b >= 1 && b <= 32, "panel width %d", b);
+  YH_CHECK_ARG(mode == 0 || mode == 1, "mode %d", mode);
+  for (int64_t j = 0; j < njobs; ++j) {
+    const int32_t blk = h_jobs[3 * j], pi = h_jobs[3 * j + 1], pj = h_jobs[3 * j + 2];
+    YH_CHECK_ARG(blk >= 0 && blk < nblocks, "job %lld: block %d", (long long)j, blk);
+    const int64_t m = h_blocks[blk * YH_SVD_WORDS + 1], n = h_blocks[blk * YH_SVD_WORDS + 2];
+    const int64_t p = m < n ? n : m, q = m < n ? m : n;
+    const int64_t npan = (q + b - 1) / b;
+    YH_CHECK_ARG(m >= 1 && n >= 1 && pi >= 0 && pi <= pj && pj < npan, "job %lld: panels %d, %d of %lld",
+                 (long long)j, pi, pj, (long long)npan);
+    YH_CHECK_ARG((p + kChunkRows - 1) / kChunkRows <= maxch_p && (q + kChunkRows - 1) / kChunkRows <= maxch_q,
+                 "job %lld: %lld x %lld beyond the chunk grid", (long long)j, (long long)p, (long long)q);
+  }
+  return YH_OK;
 }
 
 extern "C" int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const void* work, const double* mu,

@@ -639,6 +639,7 @@ PRECOND = True  # QR-precondition the panel path of SVD blocks (off: plain one-s
 # Panel width b of the panel Jacobi (a workgroup rotates 2b columns per round): smaller panels give more
 # workgroups per round but more rounds per sweep.  Tuning knob; the LDS limit may lower it further.
 PANEL_WIDTH = int(os.environ.get('YASTN_HIP_PANEL_WIDTH', '16'))
+PANEL_CHUNK_ROWS = 64  # rows per workgroup of the panel Gram and update kernels
 
 
 class SvdPlan:
@@ -716,21 +717,16 @@ class SvdPlan:
                        self.h_order.ctypes.data, 0, 0, mode, dt, a_numel, u_numel, s_numel, v_numel, max(work, 0))
             self.d_jblocks = torch.from_numpy(self.h_jblocks).to(device)
             limit = dev_lds - 2048
-            pmax = max(max(jwords[b][1], jwords[b][2]) for b in big)
-            width = PANEL_WIDTH
-            # LDS of a panel pair: W columns (p x 2b), its Gram matrix and the rotation accumulator
-            # (2b x (2b + 1) each), and the partial 16 x 16 Gram tiles (kChunk per tile)
-            def lds(w):
-                t16 = -(-2 * w // 16)
-                ntile = t16 * (t16 + 1) // 2
-                kch = min(4, max(1, 16 // ntile))
-                return (2 * w * pmax + 2 * (2 * w) * (2 * w + 1)) * itemsize + ntile * kch * 256 * itemsize
-            while width > 1 and lds(width) > limit:
+            width = max(1, min(PANEL_WIDTH, 32))
+            # LDS of the inner sweep of a panel pair: its Gram matrix and the rotation accumulator,
+            # 2b x (2b + 1) each; the columns themselves stay in the workspace
+            while width > 1 and 2 * (2 * width) * (2 * width + 1) * itemsize > limit:
                 width //= 2
             self.panel_width = width
-            self.panel_lds = lds(width)
-            if self.panel_lds > limit:
-                raise ValueError(f"block with p={pmax} does not fit the LDS panel path")
+            pmax = max(max(jwords[b][1], jwords[b][2]) for b in big)
+            qmax = max(min(jwords[b][1], jwords[b][2]) for b in big)
+            self.maxch_p = -(-pmax // PANEL_CHUNK_ROWS)
+            self.maxch_q = -(-qmax // PANEL_CHUNK_ROWS)
             # a sweep: every panel against itself (round -1), then the circle order of panel pairs
             per_round = {}
             for b in big:
@@ -750,7 +746,14 @@ class SvdPlan:
                 offs.append((len(jobs), len(per_round[r])))
                 jobs.extend(per_round[r])
             self.rounds = offs
+            self.h_rounds = np.ascontiguousarray(np.array(offs, dtype=np.int64).reshape(-1, 2))
             self.h_jobs = np.ascontiguousarray(np.array(jobs, dtype=np.int32).reshape(-1, 3))
+            L.call('yastn_hip_check_panel_jobs', self.h_jblocks.ctypes.data, self.nblocks, self.h_jobs.ctypes.data,
+                   len(jobs), width, mode, self.maxch_p, self.maxch_q)
+            maxn = max(n for _, n in offs)
+            t16 = -(-2 * width // 16)
+            self.part_numel = maxn * self.maxch_p * (t16 * (t16 + 1) // 2) * 256 * (itemsize // 8)
+            self.qbuf_numel = maxn * (2 * width) ** 2
             self.d_jobs = torch.from_numpy(self.h_jobs).to(device)
             self.h_big = np.ascontiguousarray(np.array(big, dtype=np.int32))
             self.d_big = torch.from_numpy(self.h_big).to(device)
