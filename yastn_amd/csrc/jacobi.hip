@@ -598,6 +598,7 @@ __device__ __attribute__((always_inline)) inline void stripes_times_q(int rows, 
 // nc x nc numbers.  Workgroups of jobs that did not rotate exit at once in kernel 3.
 constexpr int kChunkRows = 64;     // rows of W (or V) per workgroup of kernels 1 and 3
 constexpr int kChunkThreads = 256;  // 4 waves: one 16-row stripe each
+constexpr int kInnerThreads = 1024;  // inner sweep: one thread per 2 x 2 block of G or (pair, row) of Q
 
 struct PanelPair {
   int64_t p, q, ci, cj, c0i, c0j;
@@ -689,7 +690,7 @@ __global__ void __launch_bounds__(kChunkThreads)
 // each).  flags[job] = 1 if anything rotated (then Q goes to qbuf, column c at qbuf + (job 2b + c) 2b),
 // 0 if not, -1 if lds_cap is too small for the pair.
 template <bool CPLX>
-__global__ void __launch_bounds__(kPanelThreads)
+__global__ void __launch_bounds__(kInnerThreads)
     panel_inner_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
                        const double* __restrict__ floors, const double* __restrict__ part,
                        typename Num<CPLX>::T* __restrict__ qbuf, int32_t* __restrict__ flags, int b, int maxch,
@@ -699,8 +700,9 @@ __global__ void __launch_bounds__(kPanelThreads)
   constexpr int kMaxPairs = 32;  // nc <= 64
   extern __shared__ double smem_raw[];
   __shared__ int s_rot;
-  __shared__ int s_pi[kMaxPairs], s_pj[kMaxPairs], s_do[kMaxPairs];
-  __shared__ double s_c[kMaxPairs], s_s[kMaxPairs], s_er[kMaxPairs], s_ei[kMaxPairs];
+  __shared__ int4 s_ij[kMaxPairs];
+  __shared__ double2 s_cs[kMaxPairs];
+  __shared__ double s_si[kMaxPairs];
   const int32_t* jb = jobs + 3 * (int64_t)blockIdx.x;
   const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
   const PanelPair pp = panel_pair(d, jb, b);
@@ -722,7 +724,7 @@ __global__ void __launch_bounds__(kPanelThreads)
     const int T16 = (nc + 15) / 16, ntile = T16 * (T16 + 1) / 2;
     const int T16m = (2 * b + 15) / 16, ntmax = T16m * (T16m + 1) / 2;
     const int nch = (int)((p + kChunkRows - 1) / kChunkRows);
-    for (int e = tid; e < ntile * 256; e += kPanelThreads) {
+    for (int e = tid; e < ntile * 256; e += kInnerThreads) {
       const int tl = e >> 8, idx = e & 255;
       const int v = idx >> 6, ln = idx & 63;
       int ti, tj;
@@ -740,7 +742,7 @@ __global__ void __launch_bounds__(kPanelThreads)
       }
     }
   }
-  for (int e = tid; e < nc * nq; e += kPanelThreads) {
+  for (int e = tid; e < nc * nq; e += kInnerThreads) {
     const int l = e / nq, k = e - l * nq;
     Q[e] = N::make(l == k ? 1.0 : 0.0, 0.0);
   }
@@ -754,36 +756,64 @@ __global__ void __launch_bounds__(kPanelThreads)
   //    column k of the smaller panel with column (k + s) mod (size) of the larger one.
   // Every pair of a block's columns is thus rotated once per sweep (panels against themselves in the
   // first round of a sweep, then the circle order of panel pairs).
+  // Slot k's rotation is published as s_ij[k] = (i, j, rotated) and s_cs[k] = (c, s Re e^{i phi}) (+ s_si[k] =
+  // s Im e^{i phi}); each thread's items (2 x 2 blocks of G or (slot, row) pieces of Q) are decoded once.
   {
     const double tol = DBL_EPSILON * sqrt((double)(p > 1 ? p : 1));
     const double floor2 = floors[jb[0]];
-    const int ne = nc + (nc & 1);
+    const int ne = nc + (nc & 1), n1 = ne - 1;
     const int ci_ = (int)ci, cj_ = (int)cj;
     const int npr = intra ? ne / 2 : (ci_ > cj_ ? ci_ : cj_);  // slots per sub-round
     const int nsub = intra ? ne - 1 : npr;
+    const int ng = npr * npr, nitems = ng + npr * nc;
+    constexpr int kItems = (3 * kMaxPairs * kMaxPairs + kInnerThreads - 1) / kInnerThreads;
+    int ia[kItems], ib[kItems];  // G item: (slot pa, slot pb); Q item: (-1 - row, slot pb); none: ib = -1
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      const int e = tid + k * kInnerThreads;
+      if (e < ng) {
+        ia[k] = e / npr;
+        ib[k] = e - ia[k] * npr;
+      } else if (e < nitems) {
+        const int ee = e - ng;
+        ib[k] = ee / nc;
+        ia[k] = -1 - (ee - ib[k] * nc);
+      } else {
+        ia[k] = 0;
+        ib[k] = -1;
+      }
+    }
+    // slot state of the parameter lanes (tid < npr), advanced by one per sub-round
+    int sa = 0, sb = 0;
+    if (intra) {
+      sa = tid % (n1 > 0 ? n1 : 1);             // (r + k) mod n1 at r = 0
+      sb = (n1 - tid % (n1 > 0 ? n1 : 1)) % (n1 > 0 ? n1 : 1);  // (r - k + n1) mod n1 at r = 0
+    } else if (ci_ <= cj_) {
+      sa = tid % cj_;  // (k + r) mod cj
+    } else {
+      sa = tid % ci_;  // (k + r) mod ci
+    }
     for (int r = 0; r < nsub; ++r) {
       if (tid < npr) {
         int i, j;
         if (intra) {
-          pair_of(ne, r, tid, i, j);
+          if (tid == 0) {
+            i = r;
+            j = n1;
+          } else {
+            i = min(sa, sb);
+            j = max(sa, sb);
+          }
+          sa = sa + 1 == n1 ? 0 : sa + 1;
+          sb = sb + 1 == n1 ? 0 : sb + 1;
         } else if (ci_ <= cj_) {
-          const int l = ci_ + (tid + r) % cj_;
-          if (tid < ci_) {
-            i = tid;
-            j = l;
-          } else {
-            i = l;  // idle column of panel J
-            j = nc;
-          }
+          i = tid < ci_ ? tid : ci_ + sa;
+          j = tid < ci_ ? ci_ + sa : nc;  // tid >= ci: idle column of panel J
+          sa = sa + 1 == cj_ ? 0 : sa + 1;
         } else {
-          const int k = (tid + r) % ci_;
-          if (tid < cj_) {
-            i = k;
-            j = ci_ + tid;
-          } else {
-            i = k;  // idle column of panel I
-            j = nc;
-          }
+          i = sa;
+          j = tid < cj_ ? ci_ + tid : nc;  // tid >= cj: idle column of panel I
+          sa = sa + 1 == ci_ ? 0 : sa + 1;
         }
         int go = 0;
         double c = 1.0, sn = 0.0, er = 1.0, ei = 0.0;
@@ -794,47 +824,51 @@ __global__ void __launch_bounds__(kPanelThreads)
                    ? 1
                    : 0;
         }
-        s_pi[tid] = i;
-        s_pj[tid] = j;
-        s_do[tid] = go;
-        s_c[tid] = c;
-        s_s[tid] = sn;
-        s_er[tid] = er;
-        s_ei[tid] = ei;
+        s_ij[tid] = make_int4(i, j, go, 0);
+        s_cs[tid] = make_double2(c, sn * er);
+        if constexpr (CPLX) s_si[tid] = sn * ei;
         if (go) s_rot = 1;
       }
       __syncthreads();
-      for (int e = tid; e < npr * npr + npr * nc; e += kPanelThreads) {
-        if (e < npr * npr) {
-          const int pa = e / npr, pb = e - pa * npr;
-          if (!s_do[pa] && !s_do[pb]) continue;
-          const int i = s_pi[pa], j = s_pj[pa], i2 = s_pi[pb], j2 = s_pj[pb];
+#pragma unroll
+      for (int k = 0; k < kItems; ++k) {
+        if (ib[k] < 0) continue;
+        const int pb = ib[k];
+        const int4 qb = s_ij[pb];
+        if (ia[k] >= 0) {
+          const int4 qa = s_ij[ia[k]];
+          if (!qa.z && !qb.z) continue;
+          const int i = qa.x, j = qa.y, i2 = qb.x, j2 = qb.y;
           const bool hj = j < nc, hj2 = j2 < nc;
           T b00 = G[i2 * nq + i];
           T b01 = hj2 ? G[j2 * nq + i] : N::make(0.0, 0.0);
           T b10 = hj ? G[i2 * nq + j] : N::make(0.0, 0.0);
           T b11 = (hj && hj2) ? G[j2 * nq + j] : N::make(0.0, 0.0);
-          if (s_do[pa]) {  // rows: J^H
-            apply_rot<CPLX>(b00, b10, s_c[pa], s_s[pa], s_er[pa], -s_ei[pa]);
-            apply_rot<CPLX>(b01, b11, s_c[pa], s_s[pa], s_er[pa], -s_ei[pa]);
+          if (qa.z) {  // rows: J^H
+            const double2 cs = s_cs[ia[k]];
+            const double si = CPLX ? s_si[ia[k]] : 0.0;
+            apply_rot<CPLX>(b00, b10, cs.x, 1.0, cs.y, -si);
+            apply_rot<CPLX>(b01, b11, cs.x, 1.0, cs.y, -si);
           }
-          if (s_do[pb]) {  // columns: J
-            apply_rot<CPLX>(b00, b01, s_c[pb], s_s[pb], s_er[pb], s_ei[pb]);
-            apply_rot<CPLX>(b10, b11, s_c[pb], s_s[pb], s_er[pb], s_ei[pb]);
+          if (qb.z) {  // columns: J
+            const double2 cs = s_cs[pb];
+            const double si = CPLX ? s_si[pb] : 0.0;
+            apply_rot<CPLX>(b00, b01, cs.x, 1.0, cs.y, si);
+            apply_rot<CPLX>(b10, b11, cs.x, 1.0, cs.y, si);
           }
           G[i2 * nq + i] = b00;
           if (hj2) G[j2 * nq + i] = b01;
           if (hj) G[i2 * nq + j] = b10;
           if (hj && hj2) G[j2 * nq + j] = b11;
         } else {
-          const int ee = e - npr * npr;
-          const int pb = ee / nc, row = ee - pb * nc;
-          if (!s_do[pb]) continue;
-          const int i2 = s_pi[pb], j2 = s_pj[pb];
-          T u = Q[i2 * nq + row], v = Q[j2 * nq + row];
-          apply_rot<CPLX>(u, v, s_c[pb], s_s[pb], s_er[pb], s_ei[pb]);
-          Q[i2 * nq + row] = u;
-          Q[j2 * nq + row] = v;
+          if (!qb.z) continue;
+          const int row = -1 - ia[k];
+          const double2 cs = s_cs[pb];
+          const double si = CPLX ? s_si[pb] : 0.0;
+          T u = Q[qb.x * nq + row], v = Q[qb.y * nq + row];
+          apply_rot<CPLX>(u, v, cs.x, 1.0, cs.y, si);
+          Q[qb.x * nq + row] = u;
+          Q[qb.y * nq + row] = v;
         }
       }
       __syncthreads();
@@ -843,7 +877,7 @@ __global__ void __launch_bounds__(kPanelThreads)
   if (tid == 0) flags[blockIdx.x] = s_rot;
   if (!s_rot) return;  // uniform
   T* dst = qbuf + (int64_t)blockIdx.x * (2 * b) * (2 * b);
-  for (int e = tid; e < nc * nc; e += kPanelThreads) {
+  for (int e = tid; e < nc * nc; e += kInnerThreads) {
     const int c = e / nc, k = e - c * nc;
     dst[c * (2 * b) + k] = Q[c * nq + k];
   }
@@ -942,6 +976,15 @@ __global__ void __launch_bounds__(kThreads)
   double cnt = 0.0;
   for (int64_t j = tid; j < k; j += kThreads) cnt += (S[d.s_off + j] * S[d.s_off + j] > thr2) ? 1.0 : 0.0;
   const int64_t c0 = (int64_t)block_sum(cnt, s_red, lane, wave);  // S is descending
+  if (c0 == 0) {
+    // an all-noise (e.g. all-zero) block: the procedure below would accept e_c for every c, so write
+    // the leading unit vectors directly instead of k sequential orthogonalisations
+    for (int64_t e = tid; e < k * p; e += kThreads) {
+      const int64_t c = e / p, x = e - c * p;
+      base[c * cs + x * xs] = N::make(x == c ? 1.0 : 0.0, 0.0);
+    }
+    return;
+  }
   for (int64_t c = c0; c < k; ++c) {
     T* vc = base + c * cs;
     for (int64_t t = 0; t <= p; ++t) {
@@ -1382,14 +1425,14 @@ extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int d
     if (cplx) {
       hipLaunchKernelGGL(panel_gram_kernel<true>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
                          (const double2*)work, (double*)part, b, maxch_p);
-      hipLaunchKernelGGL(panel_inner_kernel<true>, dim3((unsigned)n), dim3(kPanelThreads), (size_t)lds_inner, stream,
+      hipLaunchKernelGGL(panel_inner_kernel<true>, dim3((unsigned)n), dim3(kInnerThreads), (size_t)lds_inner, stream,
                          d_blocks, jobs, floors, (const double*)part, (double2*)qbuf, flags, b, maxch_p, lds_inner);
       hipLaunchKernelGGL(panel_apply_kernel<true>, ga, dim3(kChunkThreads), (size_t)lds_apply, stream, d_blocks,
                          jobs, (double2*)work, (const double2*)qbuf, (const int32_t*)flags, b, maxch_p);
     } else {
       hipLaunchKernelGGL(panel_gram_kernel<false>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
                          (const double*)work, (double*)part, b, maxch_p);
-      hipLaunchKernelGGL(panel_inner_kernel<false>, dim3((unsigned)n), dim3(kPanelThreads), (size_t)lds_inner, stream,
+      hipLaunchKernelGGL(panel_inner_kernel<false>, dim3((unsigned)n), dim3(kInnerThreads), (size_t)lds_inner, stream,
                          d_blocks, jobs, floors, (const double*)part, (double*)qbuf, flags, b, maxch_p, lds_inner);
       hipLaunchKernelGGL(panel_apply_kernel<false>, ga, dim3(kChunkThreads), (size_t)lds_apply, stream, d_blocks,
                          jobs, (double*)work, (const double*)qbuf, (const int32_t*)flags, b, maxch_p);
