@@ -537,51 +537,56 @@ __global__ void __launch_bounds__(kPanelThreads)
   }
 }
 
-// Dst[:, c] <- sum_k Src[:, k] Q[k, c] for the nc panel columns (rows x nc; column k of Src at src(k),
-// column c of Dst at dst(c); the two may be the same memory), on the FP64 matrix cores.  One wave per
-// 16-row stripe: it reads the stripe's nc source columns into registers (lane l holds A(l % 16,
-// 4 s + l / 16), s = 0 .. nc / 4 - 1), then multiplies by Q (up to 4 output tiles of 16 columns, item v
-// of a tile is D(l / 16 + 4 v, l % 16)) and writes the stripe; stripes are disjoint rows, so an update
-// in place never reads what another wave writes.  nc <= 64.
-template <bool CPLX, typename SRC, typename DST>
-__device__ __attribute__((always_inline)) inline void stripes_times_q(int rows, SRC src, DST dst,
-                                                                      const typename Num<CPLX>::T* Q, int nq,
-                                                                      int nc, int wave, int lane, int nwaves) {
+// One 16-row stripe of X (rows x nc, column c at col(c)) times Q (nc x nc, column c at Q + c nq) on
+// the FP64 matrix cores, in two steps: stripe_load reads the stripe into registers (lane l holds
+// A(l % 16, 4 s + l / 16), s = 0 .. nc / 4 - 1), stripe_mul_store multiplies (up to 4 output tiles of
+// 16 columns; item v of a tile is D(l / 16 + 4 v, l % 16)) and writes the stripe back, so an update in
+// place never overwrites what the wave has not read, and the loads can be issued before Q is ready.
+// nc <= 64.
+constexpr int kStripeK = 16;  // k-steps of 4: nc <= 64
+
+template <bool CPLX, typename COL>
+__device__ __attribute__((always_inline)) inline void stripe_load(typename Num<CPLX>::T (&a)[kStripeK], COL col,
+                                                                  int rows, int st, int nc, int lane) {
+  typedef Num<CPLX> N;
+  const int nk4 = (nc + 3) / 4, ra = st * 16 + (lane & 15);
+#pragma unroll
+  for (int sidx = 0; sidx < kStripeK; ++sidx) {
+    const int k = 4 * sidx + (lane >> 4);
+    a[sidx] = (sidx < nk4 && k < nc && ra < rows) ? col(k)[ra] : N::make(0.0, 0.0);
+  }
+}
+
+template <bool CPLX, typename COL>
+__device__ __attribute__((always_inline)) inline void stripe_mul_store(const typename Num<CPLX>::T (&a)[kStripeK],
+                                                                       COL col, int rows, int st,
+                                                                       const typename Num<CPLX>::T* Q, int nq,
+                                                                       int nc, int lane) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
-  constexpr int kS = 16, kT = 4;
   const int nk4 = (nc + 3) / 4, tcn = (nc + 15) / 16;
-  for (int st = wave; st < (rows + 15) / 16; st += nwaves) {
-    const int ra = st * 16 + (lane & 15);
-    T a[kS];
 #pragma unroll
-    for (int sidx = 0; sidx < kS; ++sidx) {
-      const int k = 4 * sidx + (lane >> 4);
-      a[sidx] = (sidx < nk4 && k < nc && ra < rows) ? src(k)[ra] : N::make(0.0, 0.0);
-    }
+  for (int tj = 0; tj < 4; ++tj) {
+    if (tj < tcn) {
+      const int cb = tj * 16 + (lane & 15);
+      d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int tj = 0; tj < kT; ++tj) {
-      if (tj < tcn) {
-        const int cb = tj * 16 + (lane & 15);
-        d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int sidx = 0; sidx < kS; ++sidx) {
-          if (sidx < nk4) {
-            const int k = 4 * sidx + (lane >> 4);
-            const T bq = (cb < nc && k < nc) ? Q[cb * nq + k] : N::make(0.0, 0.0);
-            accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a[sidx]), N::re(bq), accR, 0, 0, 0);
-            if constexpr (CPLX) {
-              accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(a[sidx]), N::im(bq), accR, 0, 0, 0);
-              accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a[sidx]), N::im(bq), accI, 0, 0, 0);
-              accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(a[sidx]), N::re(bq), accI, 0, 0, 0);
-            }
+      for (int sidx = 0; sidx < kStripeK; ++sidx) {
+        if (sidx < nk4) {
+          const int k = 4 * sidx + (lane >> 4);
+          const T bq = (cb < nc && k < nc) ? Q[cb * nq + k] : N::make(0.0, 0.0);
+          accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a[sidx]), N::re(bq), accR, 0, 0, 0);
+          if constexpr (CPLX) {
+            accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(a[sidx]), N::im(bq), accR, 0, 0, 0);
+            accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a[sidx]), N::im(bq), accI, 0, 0, 0);
+            accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(a[sidx]), N::re(bq), accI, 0, 0, 0);
           }
         }
+      }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = st * 16 + (lane >> 4) + 4 * v;
-          if (row < rows && cb < nc) dst(cb)[row] = N::make(accR[v], accI[v]);
-        }
+      for (int v = 0; v < 4; ++v) {
+        const int row = st * 16 + (lane >> 4) + 4 * v;
+        if (row < rows && cb < nc) col(cb)[row] = N::make(accR[v], accI[v]);
       }
     }
   }
@@ -598,6 +603,7 @@ __device__ __attribute__((always_inline)) inline void stripes_times_q(int rows, 
 // nc x nc numbers.  Workgroups of jobs that did not rotate exit at once in kernel 3.
 constexpr int kChunkRows = 64;     // rows of W (or V) per workgroup of kernels 1 and 3
 constexpr int kChunkThreads = 256;  // 4 waves: one 16-row stripe each
+static_assert(kChunkRows == 16 * (kChunkThreads / 64), "one 16-row stripe per wave");
 constexpr int kInnerThreads = 1024;  // inner sweep: one thread per 2 x 2 block of G or (pair, row) of Q
 
 struct PanelPair {
@@ -904,6 +910,10 @@ __global__ void __launch_bounds__(kChunkThreads)
   const int rows = (int)min((int64_t)kChunkRows, ldx - r0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nc = pp.nc, nq = 2 * b + 1;
+  T* X = work + d.work_off + (isv ? pp.p * pp.q : 0) + r0;
+  auto col = [&](int c) { return X + pp.col(c) * ldx; };
+  T a[kStripeK];  // this wave's 16-row stripe of the chunk, loaded before Q is staged
+  stripe_load<CPLX>(a, col, rows, wave, nc, lane);
   T* Qs = reinterpret_cast<T*>(smem_raw);
   const T* src = qbuf + (int64_t)blockIdx.y * (2 * b) * (2 * b);
   for (int e = tid; e < nc * nc; e += kChunkThreads) {
@@ -911,9 +921,7 @@ __global__ void __launch_bounds__(kChunkThreads)
     Qs[c * nq + k] = src[c * (2 * b) + k];
   }
   __syncthreads();
-  T* X = work + d.work_off + (isv ? pp.p * pp.q : 0) + r0;
-  auto col = [&](int c) { return X + pp.col(c) * ldx; };
-  stripes_times_q<CPLX>(rows, col, col, Qs, nq, nc, wave, lane, kChunkThreads / 64);
+  stripe_mul_store<CPLX>(a, col, rows, wave, Qs, nq, nc, lane);
 }
 
 template <bool CPLX, int MODE>
@@ -959,6 +967,7 @@ __global__ void __launch_bounds__(kThreads)
   typedef typename N::T T;
   __shared__ double s_red[kWaves];
   __shared__ double s_cr[YH_SVD_MAX_Q], s_ci[YH_SVD_MAX_Q];
+  __shared__ double s_pr[kThreads], s_pim[CPLX ? kThreads : 1];  // partial sums of the update
   const int bidx = ids[blockIdx.x];
   const BlockDesc d = load_desc(blocks + (int64_t)bidx * YH_SVD_WORDS);
   const int64_t k = d.m < d.n ? d.m : d.n;
@@ -997,31 +1006,61 @@ __global__ void __launch_bounds__(kThreads)
       n0 = block_sum(n0, s_red, lane, wave);
       if (!(n0 > 0.0)) continue;
       for (int pass = 0; pass < 2; ++pass) {
-        for (int64_t l = wave; l < c; l += kWaves) {
-          const T* vl = base + l * cs;
-          double hr = 0.0, hi = 0.0;
+        // h_l = <v_l, v_c>, l < c: a wave takes four l at a time, so their loads are in flight together
+        for (int64_t l0 = 4 * wave; l0 < c; l0 += 4 * kWaves) {
+          double hr[4] = {0.0, 0.0, 0.0, 0.0}, hi[4] = {0.0, 0.0, 0.0, 0.0};
           for (int64_t x = lane; x < p; x += 64) {
-            const T a = vl[x * xs], b = vc[x * xs];
-            hr += N::re(a) * N::re(b) + N::im(a) * N::im(b);  // conj(a) b
-            hi += N::re(a) * N::im(b) - N::im(a) * N::re(b);
+            const T b = vc[x * xs];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              if (l0 + u < c) {
+                const T a = base[(l0 + u) * cs + x * xs];
+                hr[u] += N::re(a) * N::re(b) + N::im(a) * N::im(b);  // conj(a) b
+                hi[u] += N::re(a) * N::im(b) - N::im(a) * N::re(b);
+              }
+            }
           }
-          hr = wave_sum(hr);
-          hi = wave_sum(hi);
-          if (lane == 0) {
-            s_cr[l] = hr;
-            s_ci[l] = hi;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const double r = wave_sum(hr[u]), i = wave_sum(hi[u]);
+            if (lane == 0 && l0 + u < c) {
+              s_cr[l0 + u] = r;
+              s_ci[l0 + u] = i;
+            }
           }
         }
         __syncthreads();
-        for (int64_t x = tid; x < p; x += kThreads) {
+        // v_c -= sum_l h_l v_l.  With ng = kThreads / p >= 2 groups of p threads, thread (g, x) sums the
+        // terms l = g mod ng and the partials meet in LDS; otherwise one thread per x does all l.
+        const int64_t ng = p * 2 <= kThreads ? kThreads / p : 1;
+        for (int64_t e = tid; e < ng * p; e += kThreads) {
+          const int64_t g = e / p, x = e - g * p;
           double ar = 0.0, ai = 0.0;
-          for (int64_t l = 0; l < c; ++l) {
+#pragma unroll 8
+          for (int64_t l = g; l < c; l += ng) {
             const T a = base[l * cs + x * xs];
             ar += s_cr[l] * N::re(a) - s_ci[l] * N::im(a);
             ai += s_cr[l] * N::im(a) + s_ci[l] * N::re(a);
           }
-          const T b = vc[x * xs];
-          vc[x * xs] = N::make(N::re(b) - ar, N::im(b) - ai);
+          if (ng > 1) {
+            s_pr[e] = ar;
+            if constexpr (CPLX) s_pim[e] = ai;
+          } else {
+            const T b = vc[x * xs];
+            vc[x * xs] = N::make(N::re(b) - ar, N::im(b) - ai);
+          }
+        }
+        if (ng > 1) {
+          __syncthreads();
+          for (int64_t x = tid; x < p; x += kThreads) {
+            double ar = 0.0, ai = 0.0;
+            for (int64_t g = 0; g < ng; ++g) {
+              ar += s_pr[g * p + x];
+              if constexpr (CPLX) ai += s_pim[g * p + x];
+            }
+            const T b = vc[x * xs];
+            vc[x * xs] = N::make(N::re(b) - ar, N::im(b) - ai);
+          }
         }
         __syncthreads();
       }
