@@ -79,18 +79,16 @@ def capture_graph(fn, warm=3):
 
 def _pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary (rocprofv3 --pmc FETCH_SIZE /
-    WRITE_SIZE, scaled by a copy of known size), or None when the summary is of another launch."""
+    WRITE_SIZE, scaled by a copy of known size), or None when the summary has no such launch."""
     path = os.path.join(ROOT, 'profiles', 'heff2_dominant_gemm_pmc.txt')
     if not os.path.exists(path):
         return None
-    kv = {}
     for line in open(path):
-        if ':' in line and not line.startswith('#'):
-            k, v = line.split(':', 1)
-            kv[k.strip()] = v.strip()
-    if kv.get('kernel') != kernel:
-        return None
-    return int(kv['traffic_bytes_per_launch'])
+        if line.startswith('traffic | '):
+            _, k, v = (x.strip() for x in line.split(' | '))
+            if k == kernel:
+                return int(v)
+    return None
 
 
 def _dot_launch_stats(backend, workload_fn, reps=50):
@@ -128,8 +126,40 @@ def _dot_launch_stats(backend, workload_fn, reps=50):
         ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / reps
         dt = torch.promote_types(A.dtype, B.dtype)
         plan = P.dot_plan(meta, A.numel(), B.numel(), Dsize, A.device)
-        out.append((fl, ms, plan, dt, by))
+        out.append((fl, ms, plan, dt, by, (A, B, meta, Dsize)))
     return out
+
+
+KNAME = {0: '32x32', 1: '64x64', 2: '128x64', 3: '16x64', 4: '64x16', 5: '128x128', 6: 'small-M streaming'}
+
+
+def _kernel_desc(plan):
+    return f'grouped GEMM, {KNAME.get(plan.cfg, plan.cfg)} tiles ({plan.ngemms} sector GEMMs, {plan.ntiles} tiles)'
+
+
+def _pmc_sweep(backend, stats, reps=3):
+    """For counter runs: after a marker dispatch, launch every dot of the step with every applicable
+    tile shape (reps launches each, in the printed order), so the HBM counters of whichever shape the
+    autotuner picks in a timing run are on record."""
+    import torch
+    from yastn_amd import _lib as L
+    torch.cuda.synchronize()
+    torch.cuda._sleep(1000)  # marker: the sweep's dispatches follow it
+    order = []
+    for _, _, plan, _, _, (A, B, meta, Dsize) in stats:
+        if getattr(plan, 'part_buf', None) is not None:
+            continue  # split-K launches are two kernels; not swept
+        tuned = plan.cfg
+        for cfg in L.TILE_DIMS:
+            if not plan.applicable(cfg):
+                continue
+            plan.set_cfg(cfg)
+            for _ in range(reps):
+                backend.dot(A, B, meta, Dsize)
+            order.append(_kernel_desc(plan))
+        plan.set_cfg(tuned)
+    torch.cuda.synchronize()
+    print('pmc-sweep: ' + json.dumps({'reps': reps, 'launches': order}), flush=True)
 
 
 def bench_heff2(args, out):
@@ -171,15 +201,15 @@ def bench_heff2(args, out):
             out['svd_truncation']['ms_plain'] = tsp * 1e3
     # roofline of the dominant kernel (the launch with the largest duration)
     stats = _dot_launch_stats(backend_hip, lambda: W.heff2(AA, FL, FR, Wm))
-    fl, ms, plan, dt, by = max(stats, key=lambda x: x[1])
+    fl, ms, plan, dt, by, _ = max(stats, key=lambda x: x[1])
     # the bound follows the launch's arithmetic intensity against the ridge point of FP64 MFMA / HBM
     ridge = PEAK_FP64_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9)
-    kname = {0: '32x32', 1: '64x64', 2: '128x64', 3: '16x64', 4: '64x16', 5: '128x128', 6: 'small-M streaming'}
-    common = {'kernel': f'grouped GEMM, {kname.get(plan.cfg, plan.cfg)} tiles ({plan.ngemms} sector GEMMs, '
-                        f'{plan.ntiles} tiles)',
+    common = {'kernel': _kernel_desc(plan),
               'avg_launch_ms': ms, 'flops_per_launch': fl, 'algorithmic_bytes_per_launch': by,
               'arithmetic_intensity': fl / by, 'all_dot_launches_ms': [round(x[1], 5) for x in stats]}
     common['traffic'] = _pmc_traffic(common['kernel'])
+    if args.copy_probe:
+        _pmc_sweep(backend_hip, stats)
     if fl / by < ridge:
         ach = by / (ms * 1e-3) / 1e9
         out['roofline'] = {'bound': 'hbm', 'achieved': ach, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
