@@ -35,7 +35,7 @@ extern "C" {
 #define YH_F64 0
 #define YH_C128 1
 
-#define YH_ABI_VERSION 6
+#define YH_ABI_VERSION 7
 
 /* ---- box-gather plan (one job per copied or zero-filled box) -------------
  * int64 words per job (YH_JOB_WORDS):
@@ -162,25 +162,23 @@ int yastn_hip_eigh(void* S, void* U, const void* A, int dtype, const int64_t* d_
 /* Large blocks of A4 / A5 (working matrices beyond one CU): panel Jacobi.  init copies A (or A^H,
  * or A + mu I with the Gershgorin mu for eigh) into `work` and sets V = I.  A sweep visits the
  * (block, panel i, panel j) int32 jobs of its rounds (h_rounds: nrounds (offset, count) pairs into
- * d_jobs); per round, three launches: partial Gram matrices of each job's 2b columns over chunks of
- * 64 rows (part), one inner sweep of rotations per job on its Gram matrix (rotation -> qbuf,
- * d_flags[job] = 1 if it rotated, 0 if not, -1 if the LDS cannot hold the pair), and the in-place
- * update of the pair's columns of W and V.  The host stops when a whole sweep rotated nothing;
- * finalize writes U, S, Vh (or S, U) like yastn_hip_svd.  Columns whose squared norm is below
- * (eps ||A||_F)^2 (floors[b], set by init) are not rotated, here and in yastn_hip_svd: they are at the
- * rounding-noise level of the block.  maxch_p / maxch_q = ceil(max p / 64) / ceil(max q / 64);
- * part holds >= (max jobs per round) maxch_p T (T + 1) / 2 256 (x2 complex) doubles, T = ceil(2b / 16),
- * qbuf >= (max jobs per round) (2b)^2 elements.  b <= 32.
+ * d_jobs); per round, two launches: partial Gram matrices of each job's 2b columns over chunks of 64
+ * rows (part), then per job `slices` workgroups that each run the job's inner sweep of rotations on
+ * its Gram matrix and update their share of the pair's rows of W and V in place (d_flags[job] = 1 if
+ * it rotated, 0 if not, -1 if the LDS cannot hold the pair).  The host stops when a whole sweep
+ * rotated nothing; finalize writes U, S, Vh (or S, U) like yastn_hip_svd.  Columns whose squared norm
+ * is below (eps ||A||_F)^2 (floors[b], set by init) are not rotated, here and in yastn_hip_svd: they are
+ * at the rounding-noise level of the block.  maxch_p = ceil(max p / 64); part holds >= (max jobs per
+ * round) maxch_p T (T + 1) / 2 256 (x2 complex) doubles, T = ceil(2b / 16).  b <= 32.
  * yastn_hip_check_panel_jobs validates a job list against the blocks on the host. */
 int yastn_hip_device_info(int* lds_per_block, int* num_cu);
 int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, double* floors, int dtype, int mode,
                              const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, hipStream_t stream);
 int yastn_hip_svd_panel_sweep(void* work, const double* floors, int dtype, const int64_t* d_blocks,
                               const int32_t* d_jobs, const int64_t* h_rounds, int64_t nrounds, int b, int maxch_p,
-                              int maxch_q, void* part, int64_t part_numel, void* qbuf, int64_t qbuf_numel,
-                              int32_t* d_flags, hipStream_t stream);
+                              int slices, void* part, int64_t part_numel, int32_t* d_flags, hipStream_t stream);
 int yastn_hip_check_panel_jobs(const int64_t* h_blocks, int64_t nblocks, const int32_t* h_jobs, int64_t njobs, int b,
-                               int mode, int maxch_p, int maxch_q);
+                               int mode, int maxch_p);
 int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const void* work, const double* mu, int dtype, int mode,
                                  const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, int32_t* d_info,
                                  int sweeps, hipStream_t stream);

@@ -368,12 +368,11 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
             L.call('yastn_hip_svd_precond_init', ptr(work), dt, plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(),
                    plan.n_pre, st)
         part = torch.empty(max(plan.part_numel, 1), dtype=torch.float64, device=data.device)
-        qbuf = torch.empty(max(plan.qbuf_numel, 1), dtype=data.dtype, device=data.device)
         converged = False
         while sweeps < MAX_SWEEPS:
             L.call('yastn_hip_svd_panel_sweep', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
                    plan.d_jobs.data_ptr(), plan.h_rounds.ctypes.data, len(plan.rounds), plan.panel_width,
-                   plan.maxch_p, plan.maxch_q, ptr(part), part.numel(), ptr(qbuf), qbuf.numel(), ptr(flags), st)
+                   plan.maxch_p, plan.panel_slices, ptr(part), part.numel(), ptr(flags), st)
             sweeps += 1
             fmin, fmax = torch.stack([flags.min(), flags.max()]).tolist()
             if fmin < 0:

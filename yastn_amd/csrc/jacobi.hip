@@ -540,7 +540,8 @@ __global__ void __launch_bounds__(kPanelThreads)
 // One 16-row stripe of X (rows x nc, column c at col(c)) times Q (nc x nc, column c at Q + c nq) on
 // the FP64 matrix cores, in two steps: stripe_load reads the stripe into registers (lane l holds
 // A(l % 16, 4 s + l / 16), s = 0 .. nc / 4 - 1), stripe_mul_store multiplies (up to 4 output tiles of
-// 16 columns; item v of a tile is D(l / 16 + 4 v, l % 16)) and writes the stripe back, so an update in
+// 16 columns, computed as Q^T X^T so that item v of a tile is D(l % 16, l / 16 + 4 v)) and writes the
+// stripe back, so an update in
 // place never overwrites what the wave has not read, and the loads can be issued before Q is ready.
 // nc <= 64.
 constexpr int kStripeK = 16;  // k-steps of 4: nc <= 64
@@ -575,35 +576,39 @@ __device__ __attribute__((always_inline)) inline void stripe_mul_store(const typ
         if (sidx < nk4) {
           const int k = 4 * sidx + (lane >> 4);
           const T bq = (cb < nc && k < nc) ? Q[cb * nq + k] : N::make(0.0, 0.0);
-          accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a[sidx]), N::re(bq), accR, 0, 0, 0);
+          // transposed product (Q^T X^T): the result tile is D^T, so a lane's items run down a column
+          // of X and each store covers 16 consecutive rows of 4 columns
+          accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(bq), N::re(a[sidx]), accR, 0, 0, 0);
           if constexpr (CPLX) {
-            accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(a[sidx]), N::im(bq), accR, 0, 0, 0);
-            accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a[sidx]), N::im(bq), accI, 0, 0, 0);
-            accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(a[sidx]), N::re(bq), accI, 0, 0, 0);
+            accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(bq), N::im(a[sidx]), accR, 0, 0, 0);
+            accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(bq), N::re(a[sidx]), accI, 0, 0, 0);
+            accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(bq), N::im(a[sidx]), accI, 0, 0, 0);
           }
         }
       }
+      const int row = st * 16 + (lane & 15);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = st * 16 + (lane >> 4) + 4 * v;
-        if (row < rows && cb < nc) col(cb)[row] = N::make(accR[v], accI[v]);
+        const int c = tj * 16 + (lane >> 4) + 4 * v;  // item v: D^T(l / 16 + 4 v, l % 16) = D(row, c)
+        if (row < rows && c < nc) col(c)[row] = N::make(accR[v], accI[v]);
       }
     }
   }
 }
 
-// A round of the panel Jacobi runs three kernels over the (block, panel i, panel j) jobs of the round:
+// A round of the panel Jacobi runs two kernels over the (block, panel i, panel j) jobs of the round:
 //  1. panel_gram_kernel: the Gram matrix G = Wp^H Wp (nc x nc) of the pair's nc = |I| + |J| columns
-//     Wp, as partial sums over chunks of kChunkRows rows, one workgroup per (chunk, job);
-//  2. panel_inner_kernel: one workgroup per job sums the partials, runs one inner sweep of rotations on
-//     G (the same quantities and criterion as Hestenes rotations of the columns) and writes the
-//     accumulated rotation Q (nc x nc) and whether anything rotated;
-//  3. panel_apply_kernel: W's and V's pair columns <- (.) Q in place, one workgroup per (chunk, job).
-// The first and last kernels spread a pair over many CUs; the sequential inner sweep touches only
-// nc x nc numbers.  Workgroups of jobs that did not rotate exit at once in kernel 3.
-constexpr int kChunkRows = 64;     // rows of W (or V) per workgroup of kernels 1 and 3
-constexpr int kChunkThreads = 256;  // 4 waves: one 16-row stripe each
-static_assert(kChunkRows == 16 * (kChunkThreads / 64), "one 16-row stripe per wave");
+//     Wp, as partial sums over chunks of kGramRows rows, one workgroup per (chunk, job);
+//  2. panel_inner_kernel: per job, `slices` workgroups each sum the partials and run the same inner
+//     sweep of rotations on G (the same quantities and criterion as Hestenes rotations of the
+//     columns; identical arithmetic, so identical results), accumulating the rotation Q (nc x nc), and
+//     then update their slice of the pair's rows of W and V by Q in place.
+// The Gram and update work, products over all p (and q) rows, is spread over many CUs; the inner
+// sweep is repeated per slice instead of being followed by another launch.  A pair that did not
+// rotate writes nothing.
+constexpr int kGramRows = 64;      // rows of W per workgroup of kernel 1
+constexpr int kChunkRows = 64;     // unit of the row-chunk count maxch_p passed by the host
+constexpr int kChunkThreads = 256;  // 4 waves
 constexpr int kInnerThreads = 1024;  // inner sweep: one thread per 2 x 2 block of G or (pair, row) of Q
 
 struct PanelPair {
@@ -649,18 +654,18 @@ __global__ void __launch_bounds__(kChunkThreads)
   typedef Num<CPLX> N;
   typedef typename N::T T;
   extern __shared__ double smem_raw[];
-  constexpr int ld = kChunkRows + 1;
+  constexpr int ld = kGramRows + 1;
   const int32_t* jb = jobs + 3 * (int64_t)blockIdx.y;
   const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
   const PanelPair pp = panel_pair(d, jb, b);
-  const int64_t r0 = (int64_t)blockIdx.x * kChunkRows;
+  const int64_t r0 = (int64_t)blockIdx.x * kGramRows;
   if (r0 >= pp.p) return;  // uniform: beyond this block's rows
-  const int rows = (int)min((int64_t)kChunkRows, pp.p - r0);
+  const int rows = (int)min((int64_t)kGramRows, pp.p - r0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const T* W = work + d.work_off;
   T* S = reinterpret_cast<T*>(smem_raw);
-  for (int e = tid; e < pp.nc * kChunkRows; e += kChunkThreads) {
-    const int l = e / kChunkRows, r = e - l * kChunkRows;
+  for (int e = tid; e < pp.nc * kGramRows; e += kChunkThreads) {
+    const int l = e / kGramRows, r = e - l * kGramRows;
     S[l * ld + r] = r < rows ? W[pp.col(l) * pp.p + r0 + r] : N::make(0.0, 0.0);
   }
   __syncthreads();
@@ -672,7 +677,7 @@ __global__ void __launch_bounds__(kChunkThreads)
     const int ca = ti * 16 + (lane & 15), cb = tj * 16 + (lane & 15);
     d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k0 = 0; k0 < kChunkRows; k0 += 4) {
+    for (int k0 = 0; k0 < kGramRows; k0 += 4) {
       const int k = k0 + (lane >> 4);
       const T a = ca < pp.nc ? S[ca * ld + k] : N::make(0.0, 0.0);
       const T bb = cb < pp.nc ? S[cb * ld + k] : N::make(0.0, 0.0);
@@ -693,13 +698,13 @@ __global__ void __launch_bounds__(kChunkThreads)
 }
 
 // Inner sweep of one job on its Gram matrix (LDS: G and the rotation accumulator Q, nc x (2b + 1)
-// each).  flags[job] = 1 if anything rotated (then Q goes to qbuf, column c at qbuf + (job 2b + c) 2b),
-// 0 if not, -1 if lds_cap is too small for the pair.
+// each), then the update of slice blockIdx.y of the pair's rows of W and V.  flags[job] = 1 if
+// anything rotated, 0 if not, -1 if lds_cap is too small for the pair.
 template <bool CPLX>
 __global__ void __launch_bounds__(kInnerThreads)
     panel_inner_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
-                       const double* __restrict__ floors, const double* __restrict__ part,
-                       typename Num<CPLX>::T* __restrict__ qbuf, int32_t* __restrict__ flags, int b, int maxch,
+                       typename Num<CPLX>::T* __restrict__ work, const double* __restrict__ floors,
+                       const double* __restrict__ part, int32_t* __restrict__ flags, int b, int maxch,
                        int64_t lds_cap) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
@@ -721,7 +726,7 @@ __global__ void __launch_bounds__(kInnerThreads)
   T* Q = G + (int64_t)(2 * b) * nq;
   const int64_t need = (int64_t)2 * (2 * b) * nq * (int64_t)sizeof(T);
   if (need > lds_cap || nc > 2 * kMaxPairs) {
-    if (tid == 0) flags[blockIdx.x] = -1;  // uniform: plan too small for this panel pair
+    if (tid == 0 && blockIdx.y == 0) flags[blockIdx.x] = -1;  // uniform: plan too small for this panel pair
     return;
   }
   if (tid == 0) s_rot = 0;
@@ -729,7 +734,7 @@ __global__ void __launch_bounds__(kInnerThreads)
   {
     const int T16 = (nc + 15) / 16, ntile = T16 * (T16 + 1) / 2;
     const int T16m = (2 * b + 15) / 16, ntmax = T16m * (T16m + 1) / 2;
-    const int nch = (int)((p + kChunkRows - 1) / kChunkRows);
+    const int nch = (int)((p + kGramRows - 1) / kGramRows);
     for (int e = tid; e < ntile * 256; e += kInnerThreads) {
       const int tl = e >> 8, idx = e & 255;
       const int v = idx >> 6, ln = idx & 63;
@@ -880,48 +885,29 @@ __global__ void __launch_bounds__(kInnerThreads)
       __syncthreads();
     }
   }
-  if (tid == 0) flags[blockIdx.x] = s_rot;
-  if (!s_rot) return;  // uniform
-  T* dst = qbuf + (int64_t)blockIdx.x * (2 * b) * (2 * b);
-  for (int e = tid; e < nc * nc; e += kInnerThreads) {
-    const int c = e / nc, k = e - c * nc;
-    dst[c * (2 * b) + k] = Q[c * nq + k];
+  if (tid == 0 && blockIdx.y == 0) flags[blockIdx.x] = s_rot;
+  if (!s_rot) return;  // uniform: nothing rotated, W and V of the pair are unchanged
+  // ---- this workgroup's share of the update: the 16-row stripes of W (p rows) then V (q rows), cut
+  // into gridDim.y equal slices; the pair's columns <- (.) Q in place, one stripe per wave at a time
+  // (stripes are disjoint rows, and a wave reads its whole stripe before writing it)
+  {
+    const int lane = tid & 63, wave = tid >> 6;
+    const int64_t q = pp.q;
+    T* W = work + d.work_off;
+    T* V = W + p * q;
+    const int sw = (int)((p + 15) / 16), sv = (int)((q + 15) / 16);
+    const int per = (sw + sv + (int)gridDim.y - 1) / (int)gridDim.y;
+    const int s0 = (int)blockIdx.y * per, s1 = min(sw + sv, s0 + per);
+    for (int st = s0 + wave; st < s1; st += kInnerThreads / 64) {
+      const bool isv = st >= sw;
+      T* X = isv ? V : W;
+      const int64_t ldx = isv ? q : p;
+      auto col = [&](int c) { return X + pp.col(c) * ldx; };
+      T a[kStripeK];
+      stripe_load<CPLX>(a, col, (int)ldx, isv ? st - sw : st, nc, lane);
+      stripe_mul_store<CPLX>(a, col, (int)ldx, isv ? st - sw : st, Q, nq, nc, lane);
+    }
   }
-}
-
-// X[:, pair columns] <- X[:, pair columns] Q in place for one (row chunk, job), X = W for the first
-// maxch_p chunks of the grid and X = V after them; jobs that did not rotate exit at once.
-template <bool CPLX>
-__global__ void __launch_bounds__(kChunkThreads)
-    panel_apply_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
-                       typename Num<CPLX>::T* __restrict__ work, const typename Num<CPLX>::T* __restrict__ qbuf,
-                       const int32_t* __restrict__ flags, int b, int maxch_p) {
-  typedef Num<CPLX> N;
-  typedef typename N::T T;
-  extern __shared__ double smem_raw[];
-  if (flags[blockIdx.y] <= 0) return;  // uniform: nothing rotated (or no plan)
-  const int32_t* jb = jobs + 3 * (int64_t)blockIdx.y;
-  const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
-  const PanelPair pp = panel_pair(d, jb, b);
-  const bool isv = (int)blockIdx.x >= maxch_p;
-  const int64_t ldx = isv ? pp.q : pp.p;
-  const int64_t r0 = (int64_t)(isv ? blockIdx.x - maxch_p : blockIdx.x) * kChunkRows;
-  if (r0 >= ldx) return;  // uniform: beyond this block's rows
-  const int rows = (int)min((int64_t)kChunkRows, ldx - r0);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nc = pp.nc, nq = 2 * b + 1;
-  T* X = work + d.work_off + (isv ? pp.p * pp.q : 0) + r0;
-  auto col = [&](int c) { return X + pp.col(c) * ldx; };
-  T a[kStripeK];  // this wave's 16-row stripe of the chunk, loaded before Q is staged
-  stripe_load<CPLX>(a, col, rows, wave, nc, lane);
-  T* Qs = reinterpret_cast<T*>(smem_raw);
-  const T* src = qbuf + (int64_t)blockIdx.y * (2 * b) * (2 * b);
-  for (int e = tid; e < nc * nc; e += kChunkThreads) {
-    const int c = e / nc, k = e - c * nc;
-    Qs[c * nq + k] = src[c * (2 * b) + k];
-  }
-  __syncthreads();
-  stripe_mul_store<CPLX>(a, col, rows, wave, Qs, nq, nc, lane);
 }
 
 template <bool CPLX, int MODE>
@@ -1416,11 +1402,12 @@ extern "C" int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, d
 
 extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int dtype, const int64_t* d_blocks,
                                          const int32_t* d_jobs, const int64_t* h_rounds, int64_t nrounds, int b,
-                                         int maxch_p, int maxch_q, void* part, int64_t part_numel, void* qbuf,
-                                         int64_t qbuf_numel, int32_t* d_flags, hipStream_t stream) {
+                                         int maxch_p, int slices, void* part, int64_t part_numel, int32_t* d_flags,
+                                         hipStream_t stream) {
   YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
   YH_CHECK_ARG(b >= 1 && b <= 32, "panel width %d", b);
-  YH_CHECK_ARG(maxch_p >= 1 && maxch_q >= 1 && maxch_p <= 65535 && maxch_q <= 65535, "chunk counts");
+  YH_CHECK_ARG(maxch_p >= 1 && maxch_p <= 65535, "chunk count %d", maxch_p);
+  YH_CHECK_ARG(slices >= 1 && slices <= 64, "update slices %d", slices);
   const bool cplx = dtype == YH_C128;
   const int T16m = (2 * b + 15) / 16, ntmax = T16m * (T16m + 1) / 2;
   const int64_t isz = cplx ? 16 : 8;
@@ -1431,50 +1418,35 @@ extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int d
     maxn = std::max(maxn, h_rounds[2 * r + 1]);
   }
   YH_CHECK_ARG(maxn <= 65535, "jobs per round %lld", (long long)maxn);
-  YH_CHECK_ARG(part_numel >= maxn * maxch_p * ntmax * 256 * (cplx ? 2 : 1), "partial Gram buffer too small");
-  YH_CHECK_ARG(qbuf_numel >= maxn * (2 * b) * (2 * b), "rotation buffer too small");
+  const int gch = (int)((maxch_p * (int64_t)kChunkRows + kGramRows - 1) / kGramRows);  // Gram chunks
+  YH_CHECK_ARG(part_numel >= maxn * gch * ntmax * 256 * (cplx ? 2 : 1), "partial Gram buffer too small");
   const int64_t lds_inner = 2 * (2 * b) * (2 * b + 1) * isz;
-  const int64_t lds_gram = (2 * b) * (kChunkRows + 1) * isz;
-  const int64_t lds_apply = (2 * b) * (2 * b + 1) * isz;
-  int st = YH_OK;
-  if (cplx) {
-    st = yh::hip_status(hipFuncSetAttribute((const void*)panel_inner_kernel<true>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_inner),
-                        "svd panel inner LDS size");
-    if (!st)
-      st = yh::hip_status(hipFuncSetAttribute((const void*)panel_gram_kernel<true>,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_gram),
-                          "svd panel gram LDS size");
-    if (!st)
-      st = yh::hip_status(hipFuncSetAttribute((const void*)panel_apply_kernel<true>,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_apply),
-                          "svd panel apply LDS size");
-  } else {
-    st = yh::hip_status(hipFuncSetAttribute((const void*)panel_inner_kernel<false>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_inner),
-                        "svd panel inner LDS size");
-  }
+  const int64_t lds_gram = (2 * b) * (kGramRows + 1) * isz;
+  int st = yh::hip_status(hipFuncSetAttribute(cplx ? (const void*)panel_inner_kernel<true>
+                                                   : (const void*)panel_inner_kernel<false>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_inner),
+                          "svd panel inner LDS size");
+  if (!st && cplx)
+    st = yh::hip_status(hipFuncSetAttribute((const void*)panel_gram_kernel<true>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_gram),
+                        "svd panel gram LDS size");
   if (st) return st;
   for (int64_t r = 0; r < nrounds; ++r) {
     const int64_t off = h_rounds[2 * r], n = h_rounds[2 * r + 1];
     if (n == 0) continue;
     const int32_t* jobs = d_jobs + 3 * off;
     int32_t* flags = d_flags + off;
-    const dim3 gg((unsigned)maxch_p, (unsigned)n), ga((unsigned)(maxch_p + maxch_q), (unsigned)n);
+    const dim3 gg((unsigned)gch, (unsigned)n), gi((unsigned)n, (unsigned)slices);
     if (cplx) {
       hipLaunchKernelGGL(panel_gram_kernel<true>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
-                         (const double2*)work, (double*)part, b, maxch_p);
-      hipLaunchKernelGGL(panel_inner_kernel<true>, dim3((unsigned)n), dim3(kInnerThreads), (size_t)lds_inner, stream,
-                         d_blocks, jobs, floors, (const double*)part, (double2*)qbuf, flags, b, maxch_p, lds_inner);
-      hipLaunchKernelGGL(panel_apply_kernel<true>, ga, dim3(kChunkThreads), (size_t)lds_apply, stream, d_blocks,
-                         jobs, (double2*)work, (const double2*)qbuf, (const int32_t*)flags, b, maxch_p);
+                         (const double2*)work, (double*)part, b, gch);
+      hipLaunchKernelGGL(panel_inner_kernel<true>, gi, dim3(kInnerThreads), (size_t)lds_inner, stream, d_blocks,
+                         jobs, (double2*)work, floors, (const double*)part, flags, b, gch, lds_inner);
     } else {
       hipLaunchKernelGGL(panel_gram_kernel<false>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
-                         (const double*)work, (double*)part, b, maxch_p);
-      hipLaunchKernelGGL(panel_inner_kernel<false>, dim3((unsigned)n), dim3(kInnerThreads), (size_t)lds_inner, stream,
-                         d_blocks, jobs, floors, (const double*)part, (double*)qbuf, flags, b, maxch_p, lds_inner);
-      hipLaunchKernelGGL(panel_apply_kernel<false>, ga, dim3(kChunkThreads), (size_t)lds_apply, stream, d_blocks,
-                         jobs, (double*)work, (const double*)qbuf, (const int32_t*)flags, b, maxch_p);
+                         (const double*)work, (double*)part, b, gch);
+      hipLaunchKernelGGL(panel_inner_kernel<false>, gi, dim3(kInnerThreads), (size_t)lds_inner, stream, d_blocks,
+                         jobs, (double*)work, floors, (const double*)part, flags, b, gch, lds_inner);
     }
     st = yh::hip_status(hipGetLastError(), "svd panel round launch");
     if (st) return st;
@@ -1483,7 +1455,7 @@ extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int d
 }
 
 extern "C" int yastn_hip_check_panel_jobs(const int64_t* h_blocks, int64_t nblocks, const int32_t* h_jobs,
-                                          int64_t njobs, int b, int mode, int maxch_p, int maxch_q) {
+                                          int64_t njobs, int b, int mode, int maxch_p) {
   YH_CHECK_ARG(b >= 1 && b <= 32, "panel width %d", b);
   YH_CHECK_ARG(mode == 0 || mode == 1, "mode %d", mode);
   for (int64_t j = 0; j < njobs; ++j) {
@@ -1494,8 +1466,8 @@ extern "C" int yastn_hip_check_panel_jobs(const int64_t* h_blocks, int64_t nbloc
     const int64_t npan = (q + b - 1) / b;
     YH_CHECK_ARG(m >= 1 && n >= 1 && pi >= 0 && pi <= pj && pj < npan, "job %lld: panels %d, %d of %lld",
                  (long long)j, pi, pj, (long long)npan);
-    YH_CHECK_ARG((p + kChunkRows - 1) / kChunkRows <= maxch_p && (q + kChunkRows - 1) / kChunkRows <= maxch_q,
-                 "job %lld: %lld x %lld beyond the chunk grid", (long long)j, (long long)p, (long long)q);
+    YH_CHECK_ARG((p + kChunkRows - 1) / kChunkRows <= maxch_p, "job %lld: %lld rows beyond the chunk grid",
+                 (long long)j, (long long)p);
   }
   return YH_OK;
 }

@@ -639,7 +639,9 @@ PRECOND = True  # QR-precondition the panel path of SVD blocks (off: plain one-s
 # Panel width b of the panel Jacobi (a workgroup rotates 2b columns per round): smaller panels give more
 # workgroups per round but more rounds per sweep.  Tuning knob; the LDS limit may lower it further.
 PANEL_WIDTH = int(os.environ.get('YASTN_HIP_PANEL_WIDTH', '16'))
-PANEL_CHUNK_ROWS = 64  # rows per workgroup of the panel Gram and update kernels
+PANEL_CHUNK_ROWS = 64  # rows per workgroup of the panel Gram kernel
+# workgroups per panel pair in the inner kernel (0: about one 16-row stripe of W and V per wave); tuning knob
+PANEL_SLICES = max(0, min(64, int(os.environ.get('YASTN_HIP_PANEL_SLICES', '0'))))
 
 
 class SvdPlan:
@@ -726,7 +728,9 @@ class SvdPlan:
             pmax = max(max(jwords[b][1], jwords[b][2]) for b in big)
             qmax = max(min(jwords[b][1], jwords[b][2]) for b in big)
             self.maxch_p = -(-pmax // PANEL_CHUNK_ROWS)
-            self.maxch_q = -(-qmax // PANEL_CHUNK_ROWS)
+            # workgroups per panel pair in the inner kernel: each updates a slice of the 16-row stripes
+            # of W and V, about one stripe per wave (16 waves)
+            self.panel_slices = PANEL_SLICES or max(1, min(64, -(-(-(-pmax // 16) - (-qmax // 16)) // 16)))
             # a sweep: every panel against itself (round -1), then the circle order of panel pairs
             per_round = {}
             for b in big:
@@ -749,11 +753,10 @@ class SvdPlan:
             self.h_rounds = np.ascontiguousarray(np.array(offs, dtype=np.int64).reshape(-1, 2))
             self.h_jobs = np.ascontiguousarray(np.array(jobs, dtype=np.int32).reshape(-1, 3))
             L.call('yastn_hip_check_panel_jobs', self.h_jblocks.ctypes.data, self.nblocks, self.h_jobs.ctypes.data,
-                   len(jobs), width, mode, self.maxch_p, self.maxch_q)
+                   len(jobs), width, mode, self.maxch_p)
             maxn = max(n for _, n in offs)
             t16 = -(-2 * width // 16)
             self.part_numel = maxn * self.maxch_p * (t16 * (t16 + 1) // 2) * 256 * (itemsize // 8)
-            self.qbuf_numel = maxn * (2 * width) ** 2
             self.d_jobs = torch.from_numpy(self.h_jobs).to(device)
             self.h_big = np.ascontiguousarray(np.array(big, dtype=np.int32))
             self.d_big = torch.from_numpy(self.h_big).to(device)
