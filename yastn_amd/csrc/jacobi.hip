@@ -544,35 +544,34 @@ __global__ void __launch_bounds__(kPanelThreads)
 // stripe back, so an update in
 // place never overwrites what the wave has not read, and the loads can be issued before Q is ready.
 // nc <= 64.
-constexpr int kStripeK = 16;  // k-steps of 4: nc <= 64
-
-template <bool CPLX, typename COL>
-__device__ __attribute__((always_inline)) inline void stripe_load(typename Num<CPLX>::T (&a)[kStripeK], COL col,
-                                                                  int rows, int st, int nc, int lane) {
+// KS k-steps of 4: nc <= 4 KS.
+template <bool CPLX, int KS, typename COL>
+__device__ __attribute__((always_inline)) inline void stripe_load(typename Num<CPLX>::T (&a)[KS], COL col, int rows,
+                                                                  int st, int nc, int lane) {
   typedef Num<CPLX> N;
   const int nk4 = (nc + 3) / 4, ra = st * 16 + (lane & 15);
 #pragma unroll
-  for (int sidx = 0; sidx < kStripeK; ++sidx) {
+  for (int sidx = 0; sidx < KS; ++sidx) {
     const int k = 4 * sidx + (lane >> 4);
     a[sidx] = (sidx < nk4 && k < nc && ra < rows) ? col(k)[ra] : N::make(0.0, 0.0);
   }
 }
 
-template <bool CPLX, typename COL>
-__device__ __attribute__((always_inline)) inline void stripe_mul_store(const typename Num<CPLX>::T (&a)[kStripeK],
-                                                                       COL col, int rows, int st,
+template <bool CPLX, int KS, typename COL>
+__device__ __attribute__((always_inline)) inline void stripe_mul_store(const typename Num<CPLX>::T (&a)[KS], COL col,
+                                                                       int rows, int st,
                                                                        const typename Num<CPLX>::T* Q, int nq,
                                                                        int nc, int lane) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
   const int nk4 = (nc + 3) / 4, tcn = (nc + 15) / 16;
 #pragma unroll
-  for (int tj = 0; tj < 4; ++tj) {
+  for (int tj = 0; tj < KS / 4; ++tj) {
     if (tj < tcn) {
       const int cb = tj * 16 + (lane & 15);
       d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int sidx = 0; sidx < kStripeK; ++sidx) {
+      for (int sidx = 0; sidx < KS; ++sidx) {
         if (sidx < nk4) {
           const int k = 4 * sidx + (lane >> 4);
           const T bq = (cb < nc && k < nc) ? Q[cb * nq + k] : N::make(0.0, 0.0);
@@ -700,7 +699,7 @@ __global__ void __launch_bounds__(kChunkThreads)
 // Inner sweep of one job on its Gram matrix (LDS: G and the rotation accumulator Q, nc x (2b + 1)
 // each), then the update of slice blockIdx.y of the pair's rows of W and V.  flags[job] = 1 if
 // anything rotated, 0 if not, -1 if lds_cap is too small for the pair.
-template <bool CPLX>
+template <bool CPLX, int KS>
 __global__ void __launch_bounds__(kInnerThreads)
     panel_inner_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
                        typename Num<CPLX>::T* __restrict__ work, const double* __restrict__ floors,
@@ -725,11 +724,30 @@ __global__ void __launch_bounds__(kInnerThreads)
   T* G = reinterpret_cast<T*>(smem_raw);
   T* Q = G + (int64_t)(2 * b) * nq;
   const int64_t need = (int64_t)2 * (2 * b) * nq * (int64_t)sizeof(T);
-  if (need > lds_cap || nc > 2 * kMaxPairs) {
+  if (need > lds_cap || nc > 2 * kMaxPairs || nc > 4 * KS) {
     if (tid == 0 && blockIdx.y == 0) flags[blockIdx.x] = -1;  // uniform: plan too small for this panel pair
     return;
   }
   if (tid == 0) s_rot = 0;
+  // ---- this workgroup's share of the update: the 16-row stripes of W (p rows) then V (q rows), cut
+  // into gridDim.y equal slices, one stripe per wave at a time.  Each wave's first stripe is read now,
+  // so that its loads overlap the inner sweep (nothing writes W or V of this pair before the update).
+  const int lane = tid & 63, wave = tid >> 6;
+  T* const Wd = work + d.work_off;
+  const int sw = (int)((p + 15) / 16), sv = (int)((pp.q + 15) / 16);
+  const int per = (sw + sv + (int)gridDim.y - 1) / (int)gridDim.y;
+  const int s0 = (int)blockIdx.y * per, s1 = min(sw + sv, s0 + per);
+  auto stripe_col = [&](int st) {  // column c of stripe st's matrix (W, or V after sw stripes)
+    const bool isv = st >= sw;
+    T* X = isv ? Wd + p * pp.q : Wd;
+    const int64_t ldx = isv ? pp.q : p;
+    return [=](int c) { return X + pp.col(c) * ldx; };
+  };
+  auto stripe_rows = [&](int st) { return st >= sw ? (int)pp.q : (int)p; };
+  auto stripe_idx = [&](int st) { return st >= sw ? st - sw : st; };
+  T a[KS];  // KS k-steps of 4 cover the pair's nc <= 4 KS columns
+  const int st0 = s0 + wave;
+  if (st0 < s1) stripe_load<CPLX, KS>(a, stripe_col(st0), stripe_rows(st0), stripe_idx(st0), nc, lane);
   // ---- G: sum of the chunks' partial tiles, mirrored to the lower triangle
   {
     const int T16 = (nc + 15) / 16, ntile = T16 * (T16 + 1) / 2;
@@ -887,26 +905,11 @@ __global__ void __launch_bounds__(kInnerThreads)
   }
   if (tid == 0 && blockIdx.y == 0) flags[blockIdx.x] = s_rot;
   if (!s_rot) return;  // uniform: nothing rotated, W and V of the pair are unchanged
-  // ---- this workgroup's share of the update: the 16-row stripes of W (p rows) then V (q rows), cut
-  // into gridDim.y equal slices; the pair's columns <- (.) Q in place, one stripe per wave at a time
-  // (stripes are disjoint rows, and a wave reads its whole stripe before writing it)
-  {
-    const int lane = tid & 63, wave = tid >> 6;
-    const int64_t q = pp.q;
-    T* W = work + d.work_off;
-    T* V = W + p * q;
-    const int sw = (int)((p + 15) / 16), sv = (int)((q + 15) / 16);
-    const int per = (sw + sv + (int)gridDim.y - 1) / (int)gridDim.y;
-    const int s0 = (int)blockIdx.y * per, s1 = min(sw + sv, s0 + per);
-    for (int st = s0 + wave; st < s1; st += kInnerThreads / 64) {
-      const bool isv = st >= sw;
-      T* X = isv ? V : W;
-      const int64_t ldx = isv ? q : p;
-      auto col = [&](int c) { return X + pp.col(c) * ldx; };
-      T a[kStripeK];
-      stripe_load<CPLX>(a, col, (int)ldx, isv ? st - sw : st, nc, lane);
-      stripe_mul_store<CPLX>(a, col, (int)ldx, isv ? st - sw : st, Q, nq, nc, lane);
-    }
+  // ---- the pair's columns of this workgroup's stripes <- (.) Q in place (stripes are disjoint rows,
+  // and a wave reads its whole stripe before writing it)
+  for (int st = st0; st < s1; st += kInnerThreads / 64) {
+    if (st != st0) stripe_load<CPLX, KS>(a, stripe_col(st), stripe_rows(st), stripe_idx(st), nc, lane);
+    stripe_mul_store<CPLX, KS>(a, stripe_col(st), stripe_rows(st), stripe_idx(st), Q, nq, nc, lane);
   }
 }
 
@@ -1422,9 +1425,11 @@ extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int d
   YH_CHECK_ARG(part_numel >= maxn * gch * ntmax * 256 * (cplx ? 2 : 1), "partial Gram buffer too small");
   const int64_t lds_inner = 2 * (2 * b) * (2 * b + 1) * isz;
   const int64_t lds_gram = (2 * b) * (kGramRows + 1) * isz;
-  int st = yh::hip_status(hipFuncSetAttribute(cplx ? (const void*)panel_inner_kernel<true>
-                                                   : (const void*)panel_inner_kernel<false>,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_inner),
+  // the inner kernel's stripe registers cover 4 KS columns: 32 (KS = 8) for b <= 16, else 64
+  const bool wide = 2 * b > 32;
+  const void* inner = cplx ? (wide ? (const void*)panel_inner_kernel<true, 16> : (const void*)panel_inner_kernel<true, 8>)
+                           : (wide ? (const void*)panel_inner_kernel<false, 16> : (const void*)panel_inner_kernel<false, 8>);
+  int st = yh::hip_status(hipFuncSetAttribute(inner, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_inner),
                           "svd panel inner LDS size");
   if (!st && cplx)
     st = yh::hip_status(hipFuncSetAttribute((const void*)panel_gram_kernel<true>,
@@ -1440,13 +1445,21 @@ extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int d
     if (cplx) {
       hipLaunchKernelGGL(panel_gram_kernel<true>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
                          (const double2*)work, (double*)part, b, gch);
-      hipLaunchKernelGGL(panel_inner_kernel<true>, gi, dim3(kInnerThreads), (size_t)lds_inner, stream, d_blocks,
-                         jobs, (double2*)work, floors, (const double*)part, flags, b, gch, lds_inner);
+      if (wide)
+        hipLaunchKernelGGL((panel_inner_kernel<true, 16>), gi, dim3(kInnerThreads), (size_t)lds_inner, stream,
+                           d_blocks, jobs, (double2*)work, floors, (const double*)part, flags, b, gch, lds_inner);
+      else
+        hipLaunchKernelGGL((panel_inner_kernel<true, 8>), gi, dim3(kInnerThreads), (size_t)lds_inner, stream,
+                           d_blocks, jobs, (double2*)work, floors, (const double*)part, flags, b, gch, lds_inner);
     } else {
       hipLaunchKernelGGL(panel_gram_kernel<false>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
                          (const double*)work, (double*)part, b, gch);
-      hipLaunchKernelGGL(panel_inner_kernel<false>, gi, dim3(kInnerThreads), (size_t)lds_inner, stream, d_blocks,
-                         jobs, (double*)work, floors, (const double*)part, flags, b, gch, lds_inner);
+      if (wide)
+        hipLaunchKernelGGL((panel_inner_kernel<false, 16>), gi, dim3(kInnerThreads), (size_t)lds_inner, stream,
+                           d_blocks, jobs, (double*)work, floors, (const double*)part, flags, b, gch, lds_inner);
+      else
+        hipLaunchKernelGGL((panel_inner_kernel<false, 8>), gi, dim3(kInnerThreads), (size_t)lds_inner, stream,
+                           d_blocks, jobs, (double*)work, floors, (const double*)part, flags, b, gch, lds_inner);
     }
     st = yh::hip_status(hipGetLastError(), "svd panel round launch");
     if (st) return st;
