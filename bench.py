@@ -241,6 +241,41 @@ def bench_u1xu1(args, out):
                              'ms_batched': t * 1e3, 'ms_plain': tp * 1e3, 'speedup': tp / t}
 
 
+def bench_qr(args, out):
+    """backend.qr (batched Householder QR, hand-written adjoint) against the plain per-block
+    torch.linalg.qr path of the same GPU, on 13 square blocks with the CTMRG chi D^2 sector sizes."""
+    import numpy as np
+    import torch
+    from yastn_amd import backend_hip, backend_plain
+    sync = torch.cuda.synchronize
+    dims = (4, 20, 68, 152, 280, 396, 440, 396, 280, 152, 68, 20, 4)
+    meta, off = [], 0
+    for d in dims:
+        meta.append(((off, off + d * d), (d, d), (off, off + d * d), (d, d), (off, off + d * d), (d, d)))
+        off += d * d
+    meta, sizes = tuple(meta), (off, off)
+    a = torch.as_tensor(np.random.default_rng(0).standard_normal(off), device='cuda:0')
+    w = torch.as_tensor(np.random.default_rng(1).standard_normal(off), device='cuda:0')
+
+    def fwd(B):
+        with torch.no_grad():
+            B.qr(a, meta, sizes)
+
+    def fwd_bwd(B):
+        x = a.detach().requires_grad_(True)
+        Q, R = B.qr(x, meta, sizes)
+        ((Q * w).sum() + (R * w).sum()).backward()
+
+    res = {'workload': 'backend.qr of 13 square float64 blocks (4 ... 440, the CTMRG chi D^2 sector sizes)'}
+    for name, fn in (('fwd', fwd), ('fwd_bwd', fwd_bwd)):
+        tb = _timeit(lambda: fn(backend_hip), 5, 2, sync)
+        tp = _timeit(lambda: fn(backend_plain), 3, 1, sync)
+        res[f'ms_{name}_batched'] = tb * 1e3
+        res[f'ms_{name}_plain'] = tp * 1e3
+        res[f'speedup_{name}'] = tp / tb
+    out['qr'] = res
+
+
 def bench_cpu(args, out, flops):
     """The numpy oracle on the host (per-block numpy.dot through OpenBLAS), bounded sample of the same
     Heff2 workload."""
@@ -303,6 +338,8 @@ def main():
     if not args.only_heff2:
         print('bench: u1xu1', flush=True)
         bench_u1xu1(args, out)
+        print('bench: qr', flush=True)
+        bench_qr(args, out)
         if not args.no_ctmrg:
             from yastn_amd import ctmrg
             print('bench: ctmrg', flush=True)

@@ -35,7 +35,7 @@ extern "C" {
 #define YH_F64 0
 #define YH_C128 1
 
-#define YH_ABI_VERSION 7
+#define YH_ABI_VERSION 8
 
 /* ---- box-gather plan (one job per copied or zero-filled box) -------------
  * int64 words per job (YH_JOB_WORDS):
@@ -200,7 +200,7 @@ int yastn_hip_splitk_reduce(void* const* outs, const void* partials, int dtype, 
  * QR descriptor, YH_QR_WORDS int64 per block (indexed like the SVD block words):
  *   0 p, 1 q, 2 w_off (W0, p x q col-major), 3 vall_off (Householder vectors, p x q col-major),
  *   4 tm_off (-T per panel, YH_QR_NB^2 col-major each), 5 x_off (X then V_J, q x q each),
- *   6 y_off (2 x YH_QR_NB x q scratch), 7 unused.
+ *   6 y_off (2 x YH_QR_NB x q scratch), 7 number of reflectors k (0: q).
  * The trailing-column updates and the final multiplication by Q run through yastn_hip_grouped_gemm
  * with plans built by the host. */
 #define YH_QR_WORDS 8
@@ -219,6 +219,40 @@ int yastn_hip_svd_precond_init(void* work, int dtype, const int64_t* d_qdesc, co
 int yastn_hip_svd_precond_finalize(void* U, void* S, void* Vh, const void* work, int dtype, const int64_t* d_blocks,
                                    const int64_t* d_qdesc, const int32_t* d_ids, int64_t n, int32_t* d_info,
                                    int sweeps, hipStream_t stream);
+
+/* Batched QR: replaces backend.qr(data, meta, sizes) (backend_np.py:511-520, backend_torch.py:475-484):
+ * per block A (m x n) = Q (m x k) R (k x n), k = min(m, n), economic, with the sign of each R row and Q
+ * column chosen so that diag(R) is real and >= 0.  Blocked Householder QR on the same panel kernel and
+ * grouped-GEMM steps as the preconditioning (QR descriptor as above, with 0 p = m, 1 q = n, 5 x_off:
+ * E (m x k col-major, [I_k; 0], then Q = H_1 ... H_k E), 7 k); block words (YH_QROP_WORDS int64):
+ * 0 a_off, 1 q_off, 2 r_off, element offsets of the row-major blocks. */
+#define YH_QROP_WORDS 3
+int yastn_hip_check_qr_op_plan(const int64_t* h_qdesc, const int64_t* h_qblocks, int64_t nblocks, int dtype,
+                               int64_t a_numel, int64_t q_numel, int64_t r_numel, int64_t work_numel,
+                               int64_t lds_bytes);
+/* W0 = A (column-major), E = [I_k; 0]. */
+int yastn_hip_qr_init(const void* A, void* work, int dtype, const int64_t* d_qdesc, const int64_t* d_qblocks,
+                      int64_t nblocks, hipStream_t stream);
+/* Q, R from E and the upper triangle of W0, signs fixed. */
+int yastn_hip_qr_finalize(void* Q, void* R, const void* work, int dtype, const int64_t* d_qdesc,
+                          const int64_t* d_qblocks, int64_t nblocks, hipStream_t stream);
+
+/* Adjoint of the batched QR (the reference differentiates torch.linalg.qr through torch's autograd,
+ * backend_torch.py:475-484).  Per block (m x n, k = min(m, n), R1 = R(:, :k)):
+ *   gQ' = gQ (+ A(:, k:) gR(:, k:)^H for m < n),  M1 = R1 gR1^H,  M2 = gQ'^H Q,
+ *   C = copyltu(M1 - M2) (lower part, its conjugate transpose above, real part on the diagonal),
+ *   B = gQ' + Q C,  gA(:, :k) = B R1^{-H}  (+ gA(:, k:) = Q gR(:, k:) for m < n);
+ * the products run as grouped GEMMs of the host's plan, these two entry points do the rest.
+ * Words (YH_QRADJ_WORDS int64 per block): 0 m, 1 n, 2 k, 3 m_off (M1, M2, C: k x k row-major), 4 b_off
+ * (B, m x k row-major), 5 ga_off (block of gA, m x n row-major), 6 r_off (R, k x n row-major), 7 unused.
+ * trsm: one 64-thread workgroup per (row, block), k <= 65536 / itemsize. */
+#define YH_QRADJ_WORDS 8
+int yastn_hip_check_qr_adjoint_plan(const int64_t* h_words, int64_t nblocks, int dtype, int64_t m_numel,
+                                    int64_t b_numel, int64_t ga_numel, int64_t r_numel);
+int yastn_hip_qr_adjoint_core(void* C, const void* M1, const void* M2, int dtype, const int64_t* d_words,
+                              int64_t nblocks, hipStream_t stream);
+int yastn_hip_qr_adjoint_trsm(void* gA, const void* B, const void* R, int dtype, const int64_t* d_words,
+                              int64_t nblocks, int64_t max_rows, int64_t max_k, hipStream_t stream);
 
 /* A11: replaces backend.fix_svd_signs(Udata, Vdata, meta) (backend_np.py:447-458,
  *      backend_torch.py:353-366): in place on U and Vh; blocks are SVD plan words. */

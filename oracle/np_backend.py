@@ -122,6 +122,20 @@ def svd(data, meta, sizes):
     return U, S, V
 
 
+def qr(data, meta, sizes):
+    """backend_np.py:511-520: economic QR of every block (scipy.linalg.qr there, numpy's LAPACK geqrf
+    here), with the signs of R's rows (and Q's columns) chosen so that diag(R) is >= 0."""
+    Q = np.zeros(sizes[0], dtype=data.dtype)
+    R = np.zeros(sizes[1], dtype=data.dtype)
+    for sl, D, slQ, DQ, slR, DR in meta:
+        q, r = np.linalg.qr(data[_sl(sl)].reshape(D), mode='reduced')
+        s = np.sign(np.real(np.diag(r)))
+        s[s == 0] = 1
+        Q[_sl(slQ)] = (q * s).ravel()
+        R[_sl(slR)] = (s.reshape(-1, 1) * r).ravel()
+    return Q, R
+
+
 def fix_svd_signs(U, V, meta):
     """backend_np.py:447-458: make the entry of largest (2^40-quantised) modulus in each U column real positive.
 

@@ -302,3 +302,38 @@ def test_mask_adjoint_and_embed_mask():
         assert kept.sum() == Dsize and np.all(e[~kept] == 0)
         n += 1
     assert n > 0
+
+
+def test_oracle_qr_invariants_and_gram_schmidt():
+    """Oracle qr: A = Q R, Q^H Q = I, R upper triangular with a real non-negative diagonal, and, for
+    full-rank blocks, equal to modified Gram-Schmidt on the columns (the positive-diagonal QR is unique)."""
+    rng = np.random.default_rng(21)
+    shapes = [(5, 3), (3, 5), (4, 4), (7, 2), (1, 3)]
+    for cplx in (False, True):
+        meta, off, oq, orr = [], 0, 0, 0
+        blocks = []
+        for m, n in shapes:
+            k = min(m, n)
+            a = rng.standard_normal((m, n)) + (1j * rng.standard_normal((m, n)) if cplx else 0)
+            blocks.append(a)
+            meta.append(((off, off + m * n), (m, n), (oq, oq + m * k), (m, k), (orr, orr + k * n), (k, n)))
+            off, oq, orr = off + m * n, oq + m * k, orr + k * n
+        data = np.concatenate([b.ravel() for b in blocks])
+        Q, R = O.qr(data, tuple(meta), (oq, orr))
+        for (sl, D, slQ, DQ, slR, DR), a in zip(meta, blocks):
+            q = Q[slQ[0]:slQ[1]].reshape(DQ)
+            r = R[slR[0]:slR[1]].reshape(DR)
+            k = DQ[1]
+            assert np.linalg.norm(q @ r - a) <= 1e-13 * np.linalg.norm(a)
+            assert np.linalg.norm(q.conj().T @ q - np.eye(k)) <= 1e-13
+            assert np.allclose(np.tril(r, -1), 0) and np.all(np.real(np.diag(r)) >= 0)
+            assert np.allclose(np.imag(np.diag(r)), 0)
+            # modified Gram-Schmidt of the first k columns; the rest of R is Q^H A
+            g = np.zeros((D[0], k), dtype=a.dtype)
+            for j in range(k):
+                v = a[:, j].copy()
+                for i in range(j):
+                    v = v - g[:, i] * (g[:, i].conj() @ v)
+                g[:, j] = v / np.linalg.norm(v)
+            assert np.linalg.norm(g - q) <= 1e-12
+            assert np.linalg.norm(g.conj().T @ a - r) <= 1e-12 * np.linalg.norm(a)

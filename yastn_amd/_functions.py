@@ -631,3 +631,87 @@ def fix_svd_signs(Udata, Vdata, meta):
     ph = Udata[piv[:s_numel]]
     ph = ph / ph.abs()
     return Udata * ph.conj()[col], Vdata * ph[row]
+
+def qr_forward(data, meta, sizes, plan=None):
+    """Batched Householder QR of every block (plan: plans.QrPlan with native=True)."""
+    data = raw(data)
+    if plan is None:
+        plan = P.qr_plan(meta, data.element_size(), data.numel(), sizes, data.device)
+    dt = dtype_code(data)
+    st = stream_of(data)
+    Q = torch.zeros(sizes[0], dtype=data.dtype, device=data.device)
+    R = torch.zeros(sizes[1], dtype=data.dtype, device=data.device)
+    if plan.nblocks == 0:
+        return Q, R
+    work = torch.empty(plan.work_numel, dtype=data.dtype, device=data.device)
+    L.call('yastn_hip_qr_init', ptr(data), ptr(work), dt, plan.d_qdesc.data_ptr(), plan.d_qblocks.data_ptr(),
+           plan.nblocks, st)
+    for step, (d_ids, n, trail) in enumerate(plan.steps):
+        L.call('yastn_hip_svd_qr_panel', ptr(work), dt, plan.d_qdesc.data_ptr(), d_ids.data_ptr(), n, step,
+               plan.lds, st)
+        if trail is not None:
+            for gp in trail:
+                run_gemm(gp, (work,), (work,), dt, tune=False)
+    for h1, h2, h3 in plan.q_apply:
+        run_gemm(h1, (work,), (work,), dt, tune=False)
+        run_gemm(h2, (work,), (work,), dt, tune=False)
+        run_gemm(h3, (work,), (work,), dt, tune=False)
+    L.call('yastn_hip_qr_finalize', ptr(Q), ptr(R), ptr(work), dt, plan.d_qdesc.data_ptr(),
+           plan.d_qblocks.data_ptr(), plan.nblocks, st)
+    return Q, R
+
+
+def qr_backward(data, Q, R, gQ, gR, meta, sizes):
+    """Adjoint of the batched QR: with R1 = R(:, :k), gQ' = gQ (+ A(:, k:) gR(:, k:)^H when m < n),
+    C = copyltu(R1 gR1^H - gQ'^H Q), gA(:, :k) = (gQ' + Q C) R1^{-H} and gA(:, k:) = Q gR(:, k:)."""
+    data = raw(data)
+    if gQ is None:
+        gQ = torch.zeros_like(Q)
+    if gR is None:
+        gR = torch.zeros_like(R)
+    gQ, gR = gQ.contiguous(), gR.contiguous()
+    plan = P.qr_adjoint_plan(meta, data.element_size(), data.numel(), sizes, data.device)
+    dt = dtype_code(data)
+    st = stream_of(data)
+    gA = torch.zeros(data.numel(), dtype=data.dtype, device=data.device)
+    if plan.nblocks == 0:
+        return gA
+    gQp = gQ.clone()
+    if plan.gq is not None:
+        run_gemm(plan.gq, (gQp,), (data, gR), dt, tune=False)
+        run_gemm(plan.gy, (gA,), (Q, gR), dt, tune=False)
+    M1 = torch.empty(plan.m_numel, dtype=data.dtype, device=data.device)
+    M2 = torch.empty_like(M1)
+    run_gemm(plan.m1, (M1,), (R, gR), dt, tune=False)
+    run_gemm(plan.m2, (M2,), (gQp, Q), dt, tune=False)
+    C = torch.empty_like(M1)
+    L.call('yastn_hip_qr_adjoint_core', ptr(C), ptr(M1), ptr(M2), dt, plan.d_words.data_ptr(), plan.nblocks, st)
+    B = gQp  # B = gQ' + Q C, accumulated in place
+    run_gemm(plan.b, (B,), (Q, C), dt, tune=False)
+    L.call('yastn_hip_qr_adjoint_trsm', ptr(gA), ptr(B), ptr(R), dt, plan.d_words.data_ptr(), plan.nblocks,
+           plan.max_rows, plan.max_k, st)
+    return gA
+
+
+class Qr(torch.autograd.Function):
+    """Batched QR with its adjoint (the reference differentiates torch.linalg.qr through torch's autograd,
+    backend_torch.py:475-484)."""
+
+    @staticmethod
+    def forward(data, meta, sizes, plan):
+        return qr_forward(data, meta, sizes, plan)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        data, meta, sizes, _ = inputs
+        Q, R = output
+        ctx.save_for_backward(data, Q, R)
+        ctx.meta, ctx.sizes = meta, sizes
+        ctx.set_materialize_grads(False)
+
+    @staticmethod
+    def backward(ctx, gQ, gR):
+        data, Q, R = ctx.saved_tensors
+        if gQ is None and gR is None:
+            return None, None, None, None
+        return qr_backward(data, Q, R, gQ, gR, ctx.meta, ctx.sizes), None, None, None

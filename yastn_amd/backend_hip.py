@@ -21,6 +21,7 @@ import torch.multiprocessing as mp
 from torch.utils.checkpoint import checkpoint as _torch_checkpoint
 
 from . import _functions as F
+from . import plans as P
 from ._lib import lib as _load_lib
 
 _load_lib()  # fail loudly at import if the HIP library is missing
@@ -424,6 +425,14 @@ def _magnitude_order(S, meta, u_numel):
 
 
 def qr(data, meta, sizes):
+    """Batched Householder QR of every block, diag(R) >= 0 as in the reference (backend_np.py:511-520),
+    with a hand-written adjoint.  Blocks whose 16-column Householder panel exceeds LDS (more than about
+    1200 rows real, 600 complex) go through torch.linalg.qr on the device, block by block."""
+    plan = P.qr_plan(meta, data.element_size(), data.numel(), sizes, data.device)
+    if plan.native:
+        if not _needs_grad(data):
+            return F.qr_forward(data, meta, sizes, plan)
+        return F.Qr.apply(data, meta, sizes, plan)
     Q = torch.zeros(sizes[0], dtype=data.dtype, device=data.device)
     R = torch.zeros(sizes[1], dtype=data.dtype, device=data.device)
     for sl, D, slQ, DQ, slR, DR in meta:

@@ -32,6 +32,20 @@ def _groups(meta_new, meta_mrg):
         i = j
 
 
+def qr(data, meta, sizes):
+    """The reference torch backend's qr (backend_torch.py:475-484): torch.linalg.qr per block, signs so
+    that diag(R) >= 0, differentiated by torch."""
+    Q = torch.zeros(sizes[0], dtype=data.dtype, device=data.device)
+    R = torch.zeros(sizes[1], dtype=data.dtype, device=data.device)
+    for sl, D, slQ, DQ, slR, DR in meta:
+        q, r = torch.linalg.qr(data[_sl(sl)].view(D))
+        sg = torch.sign(torch.real(torch.diagonal(r)))
+        sg = torch.where(sg == 0, torch.ones_like(sg), sg)
+        Q[_sl(slQ)].view(DQ)[:] = q * sg
+        R[_sl(slR)].view(DR)[:] = sg.reshape(-1, 1) * r
+    return Q, R
+
+
 class _TransposeAndMerge(torch.autograd.Function):
     @staticmethod
     def forward(data, order, meta_new, meta_mrg, Dsize):

@@ -1121,7 +1121,7 @@ int launch_jacobi(const int64_t* d_blocks, const int32_t* d_order, int64_t n_lds
 // column-major, panel after panel), 5 x_off (X, q x q column-major, then V_J, q x q), 6 y_off
 // (two YH_QR_NB x q row-major scratch matrices for the GEMM steps), 7 unused.
 struct QrDesc {
-  int64_t p, q, w_off, vall_off, tm_off, x_off, y_off;
+  int64_t p, q, w_off, vall_off, tm_off, x_off, y_off, k;
 };
 
 __device__ inline QrDesc load_qr(const int64_t* w) {
@@ -1133,6 +1133,7 @@ __device__ inline QrDesc load_qr(const int64_t* w) {
   d.tm_off = w[4];
   d.x_off = w[5];
   d.y_off = w[6];
+  d.k = w[7] > 0 ? w[7] : w[1];  // reflectors: q unless given (min(p, q) for a wide block)
   return d;
 }
 
@@ -1156,7 +1157,7 @@ __global__ void __launch_bounds__(kThreads)
   const QrDesc d = load_qr(qdesc + (int64_t)ids[blockIdx.x] * YH_QR_WORDS);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t k0 = (int64_t)step * NB;
-  const int nb = (int)((d.q - k0) < NB ? (d.q - k0) : NB);
+  const int nb = (int)((d.k - k0) < NB ? (d.k - k0) : NB);
   const int64_t L = d.p - k0;
   const int64_t p = d.p;
   T* P = reinterpret_cast<T*>(smem_raw);  // L x nb, column-major, leading dimension L
@@ -1299,6 +1300,117 @@ __global__ void __launch_bounds__(kThreads)
   const typename Num<CPLX>::T* X = work + qd.x_off;
   finalize_block<CPLX, 2>(d, X, X + qd.q * qd.q, qd.q, qd.q, tr, 0.0, U, S, Vh, qd.p);
   if (threadIdx.x == 0) info[bidx] = sweeps;
+}
+
+// ---- batched QR (the backend's qr): A = Q R per block, economic, diag(R) real and >= 0.
+// Block words (YH_QROP_WORDS): 0 a_off, 1 q_off, 2 r_off (element offsets of A (m x n), Q (m x k) and
+// R (k x n), all row-major); the QR descriptor as for the preconditioning, with x_off holding E (m x k
+// column-major: [I_k; 0], then Q before the sign fix) and word 7 = k = min(m, n).
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    qr_op_init_kernel(const int64_t* __restrict__ qdesc, const int64_t* __restrict__ qblocks,
+                      const typename Num<CPLX>::T* __restrict__ A, typename Num<CPLX>::T* __restrict__ work) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  const QrDesc d = load_qr(qdesc + (int64_t)blockIdx.x * YH_QR_WORDS);
+  const int64_t a_off = qblocks[(int64_t)blockIdx.x * YH_QROP_WORDS];
+  const int64_t m = d.p, n = d.q, k = d.k;
+  T* W0 = work + d.w_off;
+  T* E = work + d.x_off;
+  for (int64_t e = threadIdx.x; e < m * n; e += kThreads) {  // row-major read, column-major write
+    const int64_t r = e / n, c = e - r * n;
+    W0[r + c * m] = A[a_off + e];
+  }
+  for (int64_t e = threadIdx.x; e < m * k; e += kThreads) {
+    const int64_t c = e / m, r = e - c * m;
+    E[e] = N::make(r == c ? 1.0 : 0.0, 0.0);
+  }
+}
+
+// sR_j = sign(Re R_jj) (+1 for 0): Q(:, j) sR_j and sR_j R(j, :), the reference's positive diagonal.
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    qr_op_finalize_kernel(const int64_t* __restrict__ qdesc, const int64_t* __restrict__ qblocks,
+                          const typename Num<CPLX>::T* __restrict__ work, typename Num<CPLX>::T* __restrict__ Q,
+                          typename Num<CPLX>::T* __restrict__ R) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  const QrDesc d = load_qr(qdesc + (int64_t)blockIdx.x * YH_QR_WORDS);
+  const int64_t* bw = qblocks + (int64_t)blockIdx.x * YH_QROP_WORDS;
+  const int64_t q_off = bw[1], r_off = bw[2];
+  const int64_t m = d.p, n = d.q, k = d.k;
+  const T* W0 = work + d.w_off;
+  const T* E = work + d.x_off;
+  auto sgn = [&](int64_t j) { return N::re(W0[j + j * m]) < 0.0 ? -1.0 : 1.0; };
+  for (int64_t e = threadIdx.x; e < k * n; e += kThreads) {
+    const int64_t j = e / n, c = e - j * n;
+    const double sj = sgn(j);
+    const T w = W0[j + c * m];
+    R[r_off + e] = c >= j ? N::make(sj * N::re(w), sj * N::im(w)) : N::make(0.0, 0.0);
+  }
+  for (int64_t e = threadIdx.x; e < m * k; e += kThreads) {
+    const int64_t r = e / k, j = e - r * k;
+    const double sj = sgn(j);
+    const T v = E[r + j * m];
+    Q[q_off + e] = N::make(sj * N::re(v), sj * N::im(v));
+  }
+}
+
+// QR adjoint (per block, m x n, k = min(m, n), R1 = R(:, :k)); with M1 = R1 gR1^H and M2 = gQ'^H Q
+// (gQ' = gQ + A(:, k:) gR(:, k:)^H for m < n), the core is C = copyltu(M1 - M2): M(i, j) below the
+// diagonal, conj(M(j, i)) above it and Re M(i, i) on it (the real part only, because the gauge of the
+// factorisation fixes diag(R) real: the imaginary diagonal of Q^H dQ is not free).  Words (YH_QRADJ_WORDS): 0 m, 1 n, 2 k, 3 m_off (M1, M2 and C,
+// k x k row-major each, at the same offset of their buffers), 4 b_off (B = gQ' + Q C, m x k row-major),
+// 5 ga_off (the block of gA, m x n row-major), 6 r_off (R, k x n row-major), 7 unused.
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    qr_adj_core_kernel(const int64_t* __restrict__ words, const typename Num<CPLX>::T* __restrict__ M1,
+                       const typename Num<CPLX>::T* __restrict__ M2, typename Num<CPLX>::T* __restrict__ C) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  const int64_t* w = words + (int64_t)blockIdx.x * YH_QRADJ_WORDS;
+  const int64_t k = w[2], off = w[3];
+  for (int64_t e = threadIdx.x; e < k * k; e += kThreads) {
+    const int64_t i = e / k, j = e - i * k;
+    const int64_t src = i >= j ? e : j * k + i;
+    const T a = M1[off + src], b = M2[off + src];
+    const T d = N::make(N::re(a) - N::re(b), N::im(a) - N::im(b));
+    C[off + e] = i > j ? d : (i == j ? N::make(N::re(d), 0.0) : N::conj(d));
+  }
+}
+
+// gA(r, :k) = B(r, :) R1^{-H}: back substitution of x R1^H = b for one row per single-wave workgroup
+// (grid: rows x blocks); the row lives in LDS, x_j = (b_j - sum_{l > j} x_l conj(R1(j, l))) / R1(j, j)
+// (the diagonal of R is real), applied right-looking: once x_j is known, x_i -= x_j conj(R1(i, j)), i < j.
+template <bool CPLX>
+__global__ void __launch_bounds__(64)
+    qr_adj_trsm_kernel(const int64_t* __restrict__ words, const typename Num<CPLX>::T* __restrict__ B,
+                       const typename Num<CPLX>::T* __restrict__ R, typename Num<CPLX>::T* __restrict__ gA) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  extern __shared__ double smem_raw[];
+  const int64_t* w = words + (int64_t)blockIdx.y * YH_QRADJ_WORDS;
+  const int64_t m = w[0], n = w[1], k = w[2], b_off = w[4], ga_off = w[5], r_off = w[6];
+  const int64_t r = blockIdx.x;
+  if (r >= m) return;  // uniform: this block has fewer rows
+  const int lane = threadIdx.x;
+  T* x = reinterpret_cast<T*>(smem_raw);
+  for (int64_t i = lane; i < k; i += 64) x[i] = B[b_off + r * k + i];
+  __syncthreads();
+  for (int64_t j = k - 1; j >= 0; --j) {
+    const double rjj = N::re(R[r_off + j * n + j]);
+    const T xj = N::make(N::re(x[j]) / rjj, N::im(x[j]) / rjj);
+    __syncthreads();  // every lane has read x[j]
+    if (lane == 0) x[j] = xj;
+    for (int64_t i = lane; i < j; i += 64) {
+      const T rij = R[r_off + i * n + j];  // x_i -= x_j conj(R1(i, j))
+      const double cr = N::re(xj) * N::re(rij) + N::im(xj) * N::im(rij);
+      const double ci = N::im(xj) * N::re(rij) - N::re(xj) * N::im(rij);
+      x[i] = N::make(N::re(x[i]) - cr, N::im(x[i]) - ci);
+    }
+    __syncthreads();
+  }
+  for (int64_t i = lane; i < k; i += 64) gA[ga_off + r * n + i] = x[i];
 }
 
 }  // namespace
@@ -1588,4 +1700,100 @@ extern "C" int yastn_hip_svd_precond_finalize(void* U, void* S, void* Vh, const 
   if (st) return st;
   return dtype == YH_F64 ? launch_complete<false>(d_blocks, d_ids, n, U, S, Vh, stream, 1)
                          : launch_complete<true>(d_blocks, d_ids, n, U, S, Vh, stream, 1);
+}
+
+extern "C" int yastn_hip_check_qr_op_plan(const int64_t* h_qdesc, const int64_t* h_qblocks, int64_t nblocks,
+                                          int dtype, int64_t a_numel, int64_t q_numel, int64_t r_numel,
+                                          int64_t work_numel, int64_t lds_bytes) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  const int64_t isz = yh::itemsize_of(dtype);
+  for (int64_t b = 0; b < nblocks; ++b) {
+    const int64_t* w = h_qdesc + b * YH_QR_WORDS;
+    const int64_t* bw = h_qblocks + b * YH_QROP_WORDS;
+    const int64_t m = w[0], n = w[1], k = w[7];
+    YH_CHECK_ARG(m >= 1 && n >= 1 && k == (m < n ? m : n), "qr block %lld: %lld x %lld, k %lld", (long long)b,
+                 (long long)m, (long long)n, (long long)k);
+    YH_CHECK_ARG(m * YH_QR_NB * isz <= lds_bytes, "qr block %lld: panel of %lld rows exceeds LDS", (long long)b,
+                 (long long)m);
+    const int64_t np = (k + YH_QR_NB - 1) / YH_QR_NB;
+    const bool ok = w[2] >= 0 && w[2] + m * n <= work_numel && w[3] >= 0 && w[3] + m * k <= work_numel &&
+                    w[4] >= 0 && w[4] + np * YH_QR_NB * YH_QR_NB <= work_numel && w[5] >= 0 &&
+                    w[5] + m * k <= work_numel && w[6] >= 0 && w[6] + 2 * YH_QR_NB * n <= work_numel &&
+                    bw[0] >= 0 && bw[0] + m * n <= a_numel && bw[1] >= 0 && bw[1] + m * k <= q_numel &&
+                    bw[2] >= 0 && bw[2] + k * n <= r_numel;
+    YH_CHECK_ARG(ok, "qr block %lld: offsets out of bounds", (long long)b);
+  }
+  return YH_OK;
+}
+
+extern "C" int yastn_hip_qr_init(const void* A, void* work, int dtype, const int64_t* d_qdesc,
+                                 const int64_t* d_qblocks, int64_t nblocks, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  if (nblocks == 0) return YH_OK;
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(qr_op_init_kernel<false>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, d_qdesc,
+                       d_qblocks, (const double*)A, (double*)work);
+  else
+    hipLaunchKernelGGL(qr_op_init_kernel<true>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, d_qdesc,
+                       d_qblocks, (const double2*)A, (double2*)work);
+  return yh::hip_status(hipGetLastError(), "qr init launch");
+}
+
+extern "C" int yastn_hip_qr_finalize(void* Q, void* R, const void* work, int dtype, const int64_t* d_qdesc,
+                                     const int64_t* d_qblocks, int64_t nblocks, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  if (nblocks == 0) return YH_OK;
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(qr_op_finalize_kernel<false>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, d_qdesc,
+                       d_qblocks, (const double*)work, (double*)Q, (double*)R);
+  else
+    hipLaunchKernelGGL(qr_op_finalize_kernel<true>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, d_qdesc,
+                       d_qblocks, (const double2*)work, (double2*)Q, (double2*)R);
+  return yh::hip_status(hipGetLastError(), "qr finalize launch");
+}
+
+extern "C" int yastn_hip_check_qr_adjoint_plan(const int64_t* h_words, int64_t nblocks, int dtype, int64_t m_numel,
+                                               int64_t b_numel, int64_t ga_numel, int64_t r_numel) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  const int64_t isz = yh::itemsize_of(dtype);
+  for (int64_t b = 0; b < nblocks; ++b) {
+    const int64_t* w = h_words + b * YH_QRADJ_WORDS;
+    const int64_t m = w[0], n = w[1], k = w[2];
+    YH_CHECK_ARG(m >= 1 && n >= 1 && k == (m < n ? m : n) && k * isz <= 65536 && m <= 65535,
+                 "qr adjoint block %lld: %lld x %lld", (long long)b, (long long)m, (long long)n);
+    const bool ok = w[3] >= 0 && w[3] + k * k <= m_numel && w[4] >= 0 && w[4] + m * k <= b_numel && w[5] >= 0 &&
+                    w[5] + m * n <= ga_numel && w[6] >= 0 && w[6] + k * n <= r_numel;
+    YH_CHECK_ARG(ok, "qr adjoint block %lld: offsets out of bounds", (long long)b);
+  }
+  return YH_OK;
+}
+
+extern "C" int yastn_hip_qr_adjoint_core(void* C, const void* M1, const void* M2, int dtype, const int64_t* d_words,
+                                         int64_t nblocks, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  if (nblocks == 0) return YH_OK;
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(qr_adj_core_kernel<false>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, d_words,
+                       (const double*)M1, (const double*)M2, (double*)C);
+  else
+    hipLaunchKernelGGL(qr_adj_core_kernel<true>, dim3((unsigned)nblocks), dim3(kThreads), 0, stream, d_words,
+                       (const double2*)M1, (const double2*)M2, (double2*)C);
+  return yh::hip_status(hipGetLastError(), "qr adjoint core launch");
+}
+
+extern "C" int yastn_hip_qr_adjoint_trsm(void* gA, const void* B, const void* R, int dtype, const int64_t* d_words,
+                                         int64_t nblocks, int64_t max_rows, int64_t max_k, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(max_rows >= 0 && max_rows <= 65535 && nblocks <= 65535, "qr adjoint grid");
+  if (nblocks == 0 || max_rows == 0) return YH_OK;
+  const int64_t lds = max_k * yh::itemsize_of(dtype);
+  YH_CHECK_ARG(lds >= 0 && lds <= 65536, "qr adjoint row of %lld exceeds LDS", (long long)max_k);
+  const dim3 g((unsigned)max_rows, (unsigned)nblocks);
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(qr_adj_trsm_kernel<false>, g, dim3(64), (size_t)lds, stream, d_words, (const double*)B,
+                       (const double*)R, (double*)gA);
+  else
+    hipLaunchKernelGGL(qr_adj_trsm_kernel<true>, g, dim3(64), (size_t)lds, stream, d_words, (const double2*)B,
+                       (const double2*)R, (double2*)gA);
+  return yh::hip_status(hipGetLastError(), "qr adjoint trsm launch");
 }

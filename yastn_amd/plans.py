@@ -828,6 +828,138 @@ class SvdPlan:
         self.q_apply.reverse()
 
 
+class QrPlan:
+    """Batched Householder QR of the blocks of backend.qr (meta rows (sl, (m, n), slQ, (m, k), slR,
+    (k, n))): per block the workspace holds W0 (m x n), the reflectors (m x k), -T per panel, E (m x k)
+    and the GEMM scratch; the panel steps and the GEMM plans follow the preconditioning of the SVD
+    (trailing updates of W0 after each panel, then E <- H_s E for the panels in reverse).  `native` is
+    False when a block's 16-column Householder panel does not fit LDS."""
+
+    def __init__(self, meta, itemsize, a_numel, sizes, device):
+        NB = L.YH_QR_NB
+        self.dt = L.YH_C128 if itemsize == 16 else L.YH_F64
+        qd, qb, work = [], [], 0
+        for sl, D, slQ, DQ, slR, DR in meta:
+            m, n = int(D[0]), int(D[1])
+            k = min(m, n)
+            if tuple(DQ) != (m, k) or tuple(DR) != (k, n):
+                raise ValueError(f"qr meta shapes {D} {DQ} {DR}")
+            npan = -(-k // NB)
+            offs = []
+            for size in (m * n, m * k, npan * NB * NB, m * k, 2 * NB * n):
+                offs.append(work)
+                work += size
+            qd.append((m, n, *offs, k))
+            qb.append((int(sl[0]), int(slQ[0]), int(slR[0])))
+        self.nblocks = len(qd)
+        self.work_numel = max(work, 1)
+        self.h_qdesc = np.ascontiguousarray(np.array(qd, dtype=np.int64).reshape(-1, L.YH_QR_WORDS))
+        self.h_qblocks = np.ascontiguousarray(np.array(qb, dtype=np.int64).reshape(-1, L.YH_QROP_WORDS))
+        pmax = max((r[0] for r in qd), default=1)
+        self.lds = pmax * NB * itemsize
+        self.native = self.nblocks == 0 or self.lds <= L.device_lds_per_block() - 8 * 1024
+        if not self.native:
+            return
+        L.call('yastn_hip_check_qr_op_plan', self.h_qdesc.ctypes.data, self.h_qblocks.ctypes.data, self.nblocks,
+               self.dt, a_numel, sizes[0], sizes[1], self.work_numel, max(self.lds, 1))
+        self.d_qdesc = torch.from_numpy(self.h_qdesc).to(device)
+        self.d_qblocks = torch.from_numpy(self.h_qblocks).to(device)
+        nsteps = max((-(-r[7] // NB) for r in qd), default=0)
+        bufs = (self.work_numel,)
+        self.steps, self.q_apply = [], []
+        for s in range(nsteps):
+            k0 = s * NB
+            act = [b for b in range(self.nblocks) if qd[b][7] > k0]
+            g1, t1, g2, t2, g3, t3 = [], [], [], [], [], []
+            h1, u1, h2, u2, h3, u3 = [], [], [], [], [], []
+            for b in act:
+                m, n, w_off, vall_off, tm_off, e_off, y_off, k = qd[b]
+                nbs = min(NB, k - k0)
+                Lr = m - k0
+                tm = tm_off + s * NB * NB
+                y2 = y_off + NB * n
+                rest = n - k0 - nbs
+                v0 = vall_off + k0 + k0 * m  # reflectors: m x k column-major
+                if rest > 0:  # W0(:, right of the panel) <- (I - V T^H V^H) W0(...), Tm = -T
+                    g1.append((nbs, rest, y_off, n, 1, len(t1), 1, 0))
+                    t1.append((v0, m, 1, w_off + k0 + (k0 + nbs) * m, 1, m, Lr, 1))
+                    g2.append((nbs, rest, y2, n, 1, len(t2), 1, 0))
+                    t2.append((tm, NB, 1, y_off, n, 1, nbs, 1))
+                    g3.append((Lr, rest, w_off + k0 + (k0 + nbs) * m, 1, m, len(t3), 1, 1))
+                    t3.append((v0, 1, m, y2, n, 1, nbs, 0))
+                # E(k0:, :) <- (I - V T V^H) E(k0:, :)
+                h1.append((nbs, k, y_off, n, 1, len(u1), 1, 0))
+                u1.append((v0, m, 1, e_off + k0, 1, m, Lr, 1))
+                h2.append((nbs, k, y2, n, 1, len(u2), 1, 0))
+                u2.append((tm, 1, NB, y_off, n, 1, nbs, 0))
+                h3.append((Lr, k, e_off + k0, 1, m, len(u3), 1, 1))
+                u3.append((v0, 1, m, y2, n, 1, nbs, 0))
+            d_ids = torch.from_numpy(np.array(act, dtype=np.int32)).to(device)
+            trail = None
+            if g1:
+                trail = tuple(GemmPlan(g, tt, bufs, bufs, device) for g, tt in ((g1, t1), (g2, t2), (g3, t3)))
+            self.steps.append((d_ids, len(act), trail))
+            self.q_apply.append(tuple(GemmPlan(g, tt, bufs, bufs, device) for g, tt in ((h1, u1), (h2, u2),
+                                                                                           (h3, u3))))
+        self.q_apply.reverse()
+
+
+class QrAdjointPlan:
+    """Grouped-GEMM plans and words of the QR adjoint (see yastn_hip_qr_adjoint_core), one plan per
+    product, each with its own (outputs; inputs):
+      gq: gQ' += A(:, k:) gR(:, k:)^H   (gQ'; A, gR)     only blocks with m < n
+      gy: gA(:, k:) = Q gR(:, k:)        (gA; Q, gR)      only blocks with m < n
+      m1: M1 = R1 gR1^H                  (M1; R, gR)
+      m2: M2 = gQ'^H Q                   (M2; gQ', Q)
+      b:  B += Q C, B starting as gQ'    (B; Q, C)"""
+
+    def __init__(self, meta, itemsize, a_numel, sizes, device):
+        self.dt = L.YH_C128 if itemsize == 16 else L.YH_F64
+        words, mo = [], 0
+        gq, tq, gy, ty, gm1, tm1, gm2, tm2, gb, tb = ([] for _ in range(10))
+        self.max_rows = self.max_k = 0
+        for sl, D, slQ, DQ, slR, DR in meta:
+            m, n = int(D[0]), int(D[1])
+            k = min(m, n)
+            a, qo, ro = int(sl[0]), int(slQ[0]), int(slR[0])
+            words.append((m, n, k, mo, qo, a, ro, 0))
+            self.max_rows, self.max_k = max(self.max_rows, m), max(self.max_k, k)
+            if n > k:
+                gq.append((m, k, qo, k, 1, len(tq), 1, 1))
+                tq.append((a + k, n, 1, ro + k, 1, n, n - k, 2 | (0 << 8) | (1 << 16)))
+                gy.append((m, n - k, a + k, n, 1, len(ty), 1, 0))
+                ty.append((qo, k, 1, ro + k, n, 1, k, (0 << 8) | (1 << 16)))
+            gm1.append((k, k, mo, k, 1, len(tm1), 1, 0))
+            tm1.append((ro, n, 1, ro, 1, n, k, 2 | (0 << 8) | (1 << 16)))
+            gm2.append((k, k, mo, k, 1, len(tm2), 1, 0))
+            tm2.append((qo, 1, k, qo, k, 1, m, 1 | (0 << 8) | (1 << 16)))
+            gb.append((m, k, qo, k, 1, len(tb), 1, 1))
+            tb.append((qo, k, 1, mo, k, 1, k, (0 << 8) | (1 << 16)))
+            mo += k * k
+        self.nblocks = len(words)
+        self.m_numel = max(mo, 1)
+        self.h_words = np.ascontiguousarray(np.array(words, dtype=np.int64).reshape(-1, L.YH_QRADJ_WORDS))
+        L.call('yastn_hip_check_qr_adjoint_plan', self.h_words.ctypes.data, self.nblocks, self.dt, self.m_numel,
+               sizes[0], a_numel, sizes[1])
+        self.d_words = torch.from_numpy(self.h_words).to(device)
+        nq, nr = max(sizes[0], 1), max(sizes[1], 1)
+        self.gq = GemmPlan(gq, tq, (a_numel, nr), (nq,), device) if gq else None
+        self.gy = GemmPlan(gy, ty, (nq, nr), (a_numel,), device) if gy else None
+        self.m1 = GemmPlan(gm1, tm1, (nr, nr), (self.m_numel,), device)
+        self.m2 = GemmPlan(gm2, tm2, (nq, nq), (self.m_numel,), device)
+        self.b = GemmPlan(gb, tb, (nq, self.m_numel), (nq,), device)
+
+
+def qr_plan(meta, itemsize, a_numel, sizes, device):
+    return _CACHE.get(('qr', id(meta), itemsize, a_numel, tuple(sizes), _device_key(device)), (meta,),
+                      lambda: QrPlan(meta, itemsize, a_numel, sizes, device))
+
+
+def qr_adjoint_plan(meta, itemsize, a_numel, sizes, device):
+    return _CACHE.get(('qradj', id(meta), itemsize, a_numel, tuple(sizes), _device_key(device)), (meta,),
+                      lambda: QrAdjointPlan(meta, itemsize, a_numel, sizes, device))
+
+
 def svd_plan(meta, itemsize, a_numel, sizes, device):
     def build():
         blocks = [(sl[0], D[0], D[1], slU[0], slS[0], slV[0]) for sl, D, slU, DU, slS, slV, DV in meta]
