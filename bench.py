@@ -276,6 +276,51 @@ def bench_qr(args, out):
     out['qr'] = res
 
 
+def bench_tds(args, out):
+    """backend.transpose_dot_sum (no-fusion tensordot) against the reference torch kernel's per-block loop
+    (_backend_torch_backwards.py:141-166) on the same GPU: 64 output blocks, each the sum of three
+    products of permuted rank-3 blocks, sides 8 ... 64, float64."""
+    import numpy as np
+    import torch
+    from yastn_amd import backend_hip
+    sync = torch.cuda.synchronize
+    rng = np.random.default_rng(0)
+    Ar, Br, md, ao, bo, co, flops = [], [], [], 0, 0, 0, 0
+    for _ in range(64):
+        m, n = (int(x) for x in rng.integers(8, 65, 2))
+        tab = []
+        for _ in range(3):
+            c0, c2 = (int(x) for x in rng.integers(2, 9, 2))
+            Ar.append(((ao, ao + c0 * m * c2), (c0, m, c2), m, c0 * c2))
+            Br.append(((bo, bo + c2 * c0 * n), (c2, c0, n), c0 * c2, n))
+            ao, bo = ao + c0 * m * c2, bo + c2 * c0 * n
+            tab.append((len(Ar) - 1, len(Br) - 1))
+            flops += 2 * m * n * c0 * c2
+        md.append(((co, co + m * n), (m, n), tuple(tab)))
+        co += m * n
+    Ar, Br, md = tuple(Ar), tuple(Br), tuple(md)
+    A = torch.as_tensor(rng.standard_normal(ao), device='cuda:0')
+    B = torch.as_tensor(rng.standard_normal(bo), device='cuda:0')
+    order = (1, 0, 2)
+
+    def plain():
+        At = [A[sl[0]:sl[1]].view(Di).permute(order).reshape(Dl, Dr) for sl, Di, Dl, Dr in Ar]
+        Bt = [B[sl[0]:sl[1]].view(Di).permute(order).reshape(Dl, Dr) for sl, Di, Dl, Dr in Br]
+        C = torch.zeros(co, dtype=A.dtype, device=A.device)
+        for sl, Dslc, tab in md:
+            blk = C[sl[0]:sl[1]].view(Dslc)
+            for ta, tb in tab:
+                blk += At[ta] @ Bt[tb]
+        return C
+
+    with torch.no_grad():
+        tb_ = _timeit(lambda: backend_hip.transpose_dot_sum(A, B, md, Ar, Br, order, order, co), 20, 3, sync)
+        tp = _timeit(plain, 5, 1, sync)
+    out['transpose_dot_sum'] = {'workload': '64 output blocks x 3 products of permuted rank-3 float64 blocks, sides '
+                                            '8...64 (no-fusion tensordot)', 'useful_gflop': flops / 1e9,
+                                'ms_batched': tb_ * 1e3, 'ms_plain': tp * 1e3, 'speedup': tp / tb_}
+
+
 def bench_cpu(args, out, flops):
     """The numpy oracle on the host (per-block numpy.dot through OpenBLAS), bounded sample of the same
     Heff2 workload."""
@@ -340,6 +385,8 @@ def main():
         bench_u1xu1(args, out)
         print('bench: qr', flush=True)
         bench_qr(args, out)
+        print('bench: transpose_dot_sum', flush=True)
+        bench_tds(args, out)
         if not args.no_ctmrg:
             from yastn_amd import ctmrg
             print('bench: ctmrg', flush=True)

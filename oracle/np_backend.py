@@ -136,6 +136,21 @@ def qr(data, meta, sizes):
     return Q, R
 
 
+def transpose_dot_sum(Adata, Bdata, meta_dot, Areshape, Breshape, Aorder, Border, Dsize):
+    """backend_np.py:590-601 (no-fusion tensordot): blocks permuted and reshaped to matrices, each output
+    block the sum of its listed products, in list order."""
+    dtype = np.promote_types(Adata.dtype, Bdata.dtype)
+    out = np.zeros(Dsize, dtype=dtype)
+    Ad = [Adata[_sl(sl)].reshape(Di).transpose(Aorder).reshape(Dl, Dr) for sl, Di, Dl, Dr in Areshape]
+    Bd = [Bdata[_sl(sl)].reshape(Di).transpose(Border).reshape(Dl, Dr) for sl, Di, Dl, Dr in Breshape]
+    for sl, Dslc, tab in meta_dot:
+        acc = np.zeros(Dslc, dtype=dtype)
+        for ta, tb in tab:
+            acc = acc + Ad[ta] @ Bd[tb]
+        out[_sl(sl)] = acc.ravel()
+    return out
+
+
 def fix_svd_signs(U, V, meta):
     """backend_np.py:447-458: make the entry of largest (2^40-quantised) modulus in each U column real positive.
 

@@ -1,0 +1,55 @@
+"""GPU parity of the no-fusion tensordot kernel (backend.transpose_dot_sum) and its adjoint.
+
+Tolerances: output vs the oracle <= 1e-12 relative Frobenius; gradients vs torch autograd of the
+reference torch kernel (_backend_torch_backwards.py:141-226, restated below with torch ops on the CPU)
+<= 1e-12 relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+import tds_case
+from oracle import np_backend as O
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device('cuda:0')
+
+
+def _torch_ref(A, B, meta_dot, Ar, Br, Ao, Bo, Dsize):
+    dtype = torch.promote_types(A.dtype, B.dtype)
+    A, B = A.to(dtype), B.to(dtype)
+    At = [A[sl[0]:sl[1]].view(Di).permute(Ao).reshape(Dl, Dr) for sl, Di, Dl, Dr in Ar]
+    Bt = [B[sl[0]:sl[1]].view(Di).permute(Bo).reshape(Dl, Dr) for sl, Di, Dl, Dr in Br]
+    parts = []
+    for sl, Dslc, tab in meta_dot:
+        acc = 0
+        for ta, tb in tab:
+            acc = acc + At[ta] @ Bt[tb]
+        parts.append(acc.reshape(-1))
+    return torch.cat(parts)
+
+
+@pytest.mark.parametrize('ca,cb', [(False, False), (True, False), (True, True)])
+def test_transpose_dot_sum(ca, cb):
+    from yastn_amd import backend_hip as B
+    rng = np.random.default_rng(41)
+    A, Bd, meta_dot, Ar, Br, Ao, Bo, Dsize = tds_case.make(rng, ca, cb)
+    print('transpose_dot_sum complex', ca, cb, flush=True)
+    C = B.transpose_dot_sum(torch.as_tensor(A, device=DEV), torch.as_tensor(Bd, device=DEV), meta_dot, Ar, Br,
+                            Ao, Bo, Dsize)
+    ref = O.transpose_dot_sum(A, Bd, meta_dot, Ar, Br, Ao, Bo, Dsize)
+    got = C.cpu().numpy()
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+    # adjoint
+    w = rng.standard_normal(Dsize) + (1j * rng.standard_normal(Dsize) if (ca or cb) else 0)
+    ta = torch.as_tensor(A, device=DEV).requires_grad_(True)
+    tb = torch.as_tensor(Bd, device=DEV).requires_grad_(True)
+    out = B.transpose_dot_sum(ta, tb, meta_dot, Ar, Br, Ao, Bo, Dsize)
+    (out * torch.as_tensor(w, device=DEV)).real.sum().backward()
+    ca_ = torch.as_tensor(A).requires_grad_(True)
+    cb_ = torch.as_tensor(Bd).requires_grad_(True)
+    (_torch_ref(ca_, cb_, meta_dot, Ar, Br, Ao, Bo, Dsize) * torch.as_tensor(w)).real.sum().backward()
+    for g, gr in ((ta.grad, ca_.grad), (tb.grad, cb_.grad)):
+        g, gr = g.cpu().numpy(), gr.numpy()
+        assert np.linalg.norm(g - gr) <= 1e-12 * np.linalg.norm(gr)

@@ -337,3 +337,19 @@ def test_oracle_qr_invariants_and_gram_schmidt():
                 g[:, j] = v / np.linalg.norm(v)
             assert np.linalg.norm(g - q) <= 1e-12
             assert np.linalg.norm(g.conj().T @ a - r) <= 1e-12 * np.linalg.norm(a)
+
+
+def test_oracle_transpose_dot_sum_closed_form():
+    """Oracle transpose_dot_sum against np.tensordot of each block pair over the contracted legs."""
+    import tds_case
+    rng = np.random.default_rng(23)
+    for ca, cb in ((False, False), (True, False), (True, True)):
+        A, B, meta_dot, Ar, Br, Ao, Bo, Dsize = tds_case.make(rng, ca, cb)
+        C = O.transpose_dot_sum(A, B, meta_dot, Ar, Br, Ao, Bo, Dsize)
+        for sl, Dslc, tab in meta_dot:
+            ref = 0
+            for ta, tb in tab:
+                a = A[Ar[ta][0][0]:Ar[ta][0][1]].reshape(Ar[ta][1])  # (c0, m, c2)
+                b = B[Br[tb][0][0]:Br[tb][0][1]].reshape(Br[tb][1])  # (c2, c0, n)
+                ref = ref + np.tensordot(a, b, axes=((0, 2), (1, 0)))
+            assert np.linalg.norm(C[sl[0]:sl[1]].reshape(Dslc) - ref) <= 1e-13 * max(1.0, np.linalg.norm(ref))

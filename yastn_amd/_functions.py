@@ -715,3 +715,58 @@ class Qr(torch.autograd.Function):
         if gQ is None and gR is None:
             return None, None, None, None
         return qr_backward(data, Q, R, gQ, gR, ctx.meta, ctx.sizes), None, None, None
+
+
+class PermutePack(torch.autograd.Function):
+    """Blocks of a flat buffer permuted into packed matrices (one box gather); the adjoint is the
+    inverse gather into a zeroed buffer of the source's size."""
+
+    @staticmethod
+    def forward(data, fwd, bwd, packed_numel):
+        data = raw(data)
+        out = torch.empty(packed_numel, dtype=data.dtype, device=data.device)
+        return run_gather('yastn_hip_block_gather', fwd, out, data)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        data, _, bwd, _ = inputs
+        ctx.bwd, ctx.numel = bwd, data.numel()
+
+    @staticmethod
+    def backward(ctx, g):
+        g = raw(g).contiguous()
+        out = torch.zeros(ctx.numel, dtype=g.dtype, device=g.device)
+        return run_gather('yastn_hip_block_gather', ctx.bwd, out, g), None, None, None
+
+
+class TdsCore(torch.autograd.Function):
+    """Output blocks as sums of products of packed matrices (one grouped GEMM); adjoint: two grouped
+    GEMMs accumulating into the packed gradients."""
+
+    @staticmethod
+    def forward(At, Bt, plan, Dsize):
+        At, Bt = raw(At), raw(Bt)
+        C = torch.zeros(Dsize, dtype=At.dtype, device=At.device)
+        if plan.dot.ngemms:
+            run_gemm(plan.dot, (C,), (At, Bt), dtype_code(C))
+        return C
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        At, Bt, plan, _ = inputs
+        ctx.save_for_backward(At, Bt)
+        ctx.plan = plan
+
+    @staticmethod
+    def backward(ctx, gC):
+        At, Bt = ctx.saved_tensors
+        gC = raw(gC).contiguous()
+        plan = ctx.plan
+        gA = torch.zeros_like(At)
+        gB = torch.zeros_like(Bt)
+        dt = dtype_code(gA)
+        if plan.dot_a is not None and plan.dot_a.ngemms:
+            run_gemm(plan.dot_a, (gA,), (gC, Bt), dt)
+        if plan.dot_b is not None and plan.dot_b.ngemms:
+            run_gemm(plan.dot_b, (gB,), (At, gC), dt)
+        return gA, gB, None, None

@@ -550,22 +550,20 @@ def dot(Adata, Bdata, meta_dot, Dsize):
 
 
 def transpose_dot_sum(Adata, Bdata, meta_dot, Areshape, Breshape, Aorder, Border, Dsize):
-    """no_fusion policy: permute every participating block (one grouped gather per operand),
-    then accumulate the block products (reference backend_np.py:590-601)."""
+    """no_fusion policy (reference backend_np.py:590-601, _backend_torch_backwards.py:141-226): the
+    participating blocks of each operand are permuted into packed matrices by one box gather, then every
+    output block is one GEMM summing its (ta, tb) products, all outputs in one grouped launch; the
+    adjoint runs two grouped GEMMs and the inverse gathers."""
     dtype = torch.promote_types(Adata.dtype, Bdata.dtype)
     A, B = Adata.to(dtype), Bdata.to(dtype)
-    At = [A[_sl(sl)].view(Di).permute(Aorder).reshape(Dl, Dr) for sl, Di, Dl, Dr in Areshape]
-    Bt = [B[_sl(sl)].view(Di).permute(Border).reshape(Dl, Dr) for sl, Di, Dl, Dr in Breshape]
-    parts = []
-    for sl, Dslc, tab in meta_dot:
-        acc = None
-        for ta, tb in tab:
-            p = At[ta] @ Bt[tb]
-            acc = p if acc is None else acc + p
-        parts.append(acc.reshape(-1))
-    if not parts:
-        return torch.zeros(Dsize, dtype=dtype, device=Adata.device)
-    return torch.cat(parts)
+    plan = P.tds_plan(meta_dot, Areshape, Breshape, Aorder, Border, A.numel(), B.numel(), Dsize, A.device)
+    if not _needs_grad(A, B):
+        At = F.PermutePack.forward(A, plan.a_fwd, plan.a_bwd, plan.at_numel)
+        Bt = F.PermutePack.forward(B, plan.b_fwd, plan.b_bwd, plan.bt_numel)
+        return F.TdsCore.forward(At, Bt, plan, Dsize)
+    At = F.PermutePack.apply(A, plan.a_fwd, plan.a_bwd, plan.at_numel)
+    Bt = F.PermutePack.apply(B, plan.b_fwd, plan.b_bwd, plan.bt_numel)
+    return F.TdsCore.apply(At, Bt, plan, Dsize)
 
 
 def dot_diag(Adata, Bdata, meta, Dsize, axis, a_ndim):

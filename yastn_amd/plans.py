@@ -950,6 +950,71 @@ class QrAdjointPlan:
         self.b = GemmPlan(gb, tb, (nq, self.m_numel), (nq,), device)
 
 
+class TdsPlan:
+    """transpose_dot_sum (no-fusion policy, reference backend_np.py:590-601,
+    _backend_torch_backwards.py:141-226): the participating blocks of A and B are permuted into packed
+    matrices At (Dl x Dr each) and Bt by box gathers (with their inverse maps for the adjoint), then
+    every output block is one GEMM whose terms are its (ta, tb) products, summed in order.  The adjoint
+    is two grouped GEMMs whose outputs are the packed matrices: gAt[ta] = sum gC Bt[tb]^H and
+    gBt[tb] = sum At[ta]^H gC over the outputs that use them (no atomics)."""
+
+    def __init__(self, meta_dot, Areshape, Breshape, Aorder, Border, a_numel, b_numel, Dsize, device):
+        def pack(reshape, order, numel):
+            inv = tuple(int(x) for x in np.argsort(order))
+            fwd, bwd, offs, o = [], [], [], 0
+            for sl, Di, Dl, Dr in reshape:
+                Di = tuple(int(x) for x in Di)
+                Dn = tuple(Di[i] for i in order)
+                st = _c_strides(Di)
+                fwd.append((int(sl[0]), o, Dn, tuple(st[i] for i in order), _c_strides(Dn)))
+                sn = _c_strides(Dn)
+                bwd.append((o, int(sl[0]), Di, tuple(sn[i] for i in inv), st))
+                offs.append(o)
+                o += int(Dl) * int(Dr)
+            packed = max(o, 1)
+            return (GatherPlan(fwd, numel, packed, device), GatherPlan(bwd, packed, numel, device), offs, packed)
+
+        self.a_fwd, self.a_bwd, a_off, self.at_numel = pack(Areshape, Aorder, a_numel)
+        self.b_fwd, self.b_bwd, b_off, self.bt_numel = pack(Breshape, Border, b_numel)
+        g, tm = [], []
+        ua = [[] for _ in Areshape]
+        ub = [[] for _ in Breshape]
+        for sl, Dslc, tab in meta_dot:
+            M, N = int(Dslc[0]), int(Dslc[1])
+            g.append((M, N, int(sl[0]), N, 1, len(tm), len(tab), 0))
+            for ta, tb in tab:
+                Dla, K = int(Areshape[ta][2]), int(Areshape[ta][3])
+                Klb, Nb = int(Breshape[tb][2]), int(Breshape[tb][3])
+                if Dla != M or Klb != K or Nb != N:
+                    raise ValueError(f"transpose_dot_sum shapes ({M}, {N}) from ({Dla}, {K}) x ({Klb}, {Nb})")
+                tm.append((a_off[ta], K, 1, b_off[tb], N, 1, K, (0 << 8) | (1 << 16)))
+                ua[ta].append((int(sl[0]), M, N, tb))
+                ub[tb].append((int(sl[0]), M, N, ta))
+        self.dot = GemmPlan(g, tm, (self.at_numel, self.bt_numel), (max(Dsize, 1),), device)
+        ga, ta_ = [], []
+        for ia, uses in enumerate(ua):
+            M, K = int(Areshape[ia][2]), int(Areshape[ia][3])
+            ga.append((M, K, a_off[ia], K, 1, len(ta_), len(uses), 0))
+            for c0, _, N, tb in uses:  # gC (M x N) Bt[tb]^H (N x K)
+                ta_.append((c0, N, 1, b_off[tb], 1, N, N, 2 | (0 << 8) | (1 << 16)))
+        gb, tb_ = [], []
+        for ib, uses in enumerate(ub):
+            K, N = int(Breshape[ib][2]), int(Breshape[ib][3])
+            gb.append((K, N, b_off[ib], N, 1, len(tb_), len(uses), 0))
+            for c0, M, _, ia in uses:  # At[ia]^H (K x M) gC (M x N)
+                tb_.append((a_off[ia], 1, K, c0, N, 1, M, 1 | (0 << 8) | (1 << 16)))
+        self.dot_a = GemmPlan(ga, ta_, (max(Dsize, 1), self.bt_numel), (self.at_numel,), device) if ga else None
+        self.dot_b = GemmPlan(gb, tb_, (self.at_numel, max(Dsize, 1)), (self.bt_numel,), device) if gb else None
+
+
+def tds_plan(meta_dot, Areshape, Breshape, Aorder, Border, a_numel, b_numel, Dsize, device):
+    refs = (meta_dot, Areshape, Breshape)
+    key = ('tds', id(meta_dot), id(Areshape), id(Breshape), tuple(Aorder), tuple(Border), a_numel, b_numel, Dsize,
+           _device_key(device))
+    return _CACHE.get(key, refs, lambda: TdsPlan(meta_dot, Areshape, Breshape, Aorder, Border, a_numel, b_numel,
+                                                  Dsize, device))
+
+
 def qr_plan(meta, itemsize, a_numel, sizes, device):
     return _CACHE.get(('qr', id(meta), itemsize, a_numel, tuple(sizes), _device_key(device)), (meta,),
                       lambda: QrPlan(meta, itemsize, a_numel, sizes, device))
