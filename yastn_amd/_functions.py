@@ -3,6 +3,7 @@ torch.func transforms and non-reentrant checkpoint recompute work as with the re
 yastn/backend/_backend_torch_backwards.py).  Every forward and backward is one or a few grouped
 HIP launches over all blocks of the call; nothing falls back to per-block torch ops.
 """
+import functools
 import os
 
 import numpy as np
@@ -64,6 +65,59 @@ def _real_dtype(t):
 
 # ------------------------------------------------------------------ gather family
 
+class _Opaque(torch.autograd.Function):
+    """Runs fn on plain tensors: under torch.func transforms, apply() hands forward the unwrapped tensors.
+    Used by the adjoints below, which launch kernels through raw device pointers; an adjoint is not
+    differentiated again."""
+
+    @staticmethod
+    def forward(fn, *args):
+        return fn(*args)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        pass
+
+    @staticmethod
+    def backward(ctx, *grads):
+        raise RuntimeError("yastn_amd: the adjoint kernels are not differentiable again")
+
+
+class _CtxProxy:
+    """The autograd context with its saved tensors replaced by plain ones."""
+
+    def __init__(self, ctx, saved):
+        self._ctx, self.saved_tensors = ctx, saved
+
+    def __getattr__(self, name):
+        return getattr(self._ctx, name)
+
+
+def _is_wrapped(t):
+    return torch.is_tensor(t) and torch._C._functorch.is_functorch_wrapped_tensor(t)
+
+
+def _plain_backward(fn):
+    """Decorator for the backward of a Function: when its saved tensors or incoming gradients are wrapped
+    by a torch.func transform (the reference's fixed-point AD calls torch.func.vjp, fixed_pt.py:853), the
+    backward runs inside _Opaque, which unwraps them; otherwise it runs as it is."""
+    @functools.wraps(fn)
+    def wrapper(ctx, *grads):
+        saved = ctx.saved_tensors  # unpacked once (checkpointing allows no second unpack)
+        if not any(_is_wrapped(t) for t in (*saved, *grads)):
+            return fn(_CtxProxy(ctx, saved), *grads)
+        ns, slots = len(saved), []
+
+        def run(*ts):
+            out = fn(_CtxProxy(ctx, ts[:ns]), *ts[ns:])
+            slots[:] = [o is not None for o in out]
+            return tuple(o for o in out if o is not None)
+        res = _Opaque.apply(run, *saved, *grads)
+        res = iter(res if isinstance(res, tuple) else (res,))
+        return tuple(next(res) if s else None for s in slots)
+    return wrapper
+
+
 def run_gather(entry, plan, dst, src):
     L.call(entry, ptr(dst), dst.numel(), ptr(src), src.numel(), dtype_code(dst), *plan.args(), stream_of(dst))
     return dst
@@ -90,6 +144,7 @@ class TransposeAndMerge(torch.autograd.Function):
         ctx.order, ctx.meta_new, ctx.meta_mrg, ctx.Dsize, ctx.src_numel = tuple(order), meta_new, meta_mrg, Dsize, data.numel()
 
     @staticmethod
+    @_plain_backward
     def backward(ctx, g):
         g = raw(g)
         plan = P.merge_adjoint_plan(ctx.order, ctx.meta_new, ctx.meta_mrg, ctx.Dsize, ctx.src_numel, g.device)
@@ -113,6 +168,7 @@ class Unmerge(torch.autograd.Function):
         ctx.meta, ctx.numel = meta, data.numel()
 
     @staticmethod
+    @_plain_backward
     def backward(ctx, g):
         g = raw(g)
         plan = P.unmerge_adjoint_plan(ctx.meta, ctx.numel, g.device)
@@ -136,6 +192,7 @@ class Transpose(torch.autograd.Function):
         ctx.axes, ctx.meta, ctx.numel = tuple(axes), meta, data.numel()
 
     @staticmethod
+    @_plain_backward
     def backward(ctx, g):
         g = raw(g)
         inv = tuple(int(x) for x in np.argsort(ctx.axes))
@@ -203,6 +260,7 @@ class ApplyMask(torch.autograd.Function):
         ctx.meta, ctx.axis, ctx.size_a = meta, axis, Adata.numel()
 
     @staticmethod
+    @_plain_backward
     def backward(ctx, g):
         g = raw(g)
         out = _mask_gather(g, ctx.mh, _mask_key(ctx.mh), _swapped_mask_meta(ctx.meta), ctx.size_a, ctx.axis, True)
@@ -225,6 +283,7 @@ class EmbedMask(torch.autograd.Function):
         ctx.meta, ctx.axis, ctx.size_a = meta, axis, Adata.numel()
 
     @staticmethod
+    @_plain_backward
     def backward(ctx, g):
         g = raw(g)
         out = _mask_gather(g, ctx.mh, _mask_key(ctx.mh), _swapped_mask_meta(ctx.meta), ctx.size_a, ctx.axis, False)
@@ -286,6 +345,7 @@ class Dot(torch.autograd.Function):
         ctx.meta_dot = meta_dot
 
     @staticmethod
+    @_plain_backward
     def backward(ctx, gC):
         Adata, Bdata = ctx.saved_tensors
         dtype = torch.promote_types(Adata.dtype, Bdata.dtype)
@@ -504,6 +564,7 @@ class Svd(torch.autograd.Function):
         ctx.set_materialize_grads(False)
 
     @staticmethod
+    @_plain_backward
     def backward(ctx, gU, gS, gV):
         U, S, V = ctx.saved_tensors
         gA = svd_backward(U, S, V, gU, gS, gV, ctx.meta, ctx.data_size)
@@ -567,6 +628,7 @@ class Eigh(torch.autograd.Function):
         ctx.set_materialize_grads(False)
 
     @staticmethod
+    @_plain_backward
     def backward(ctx, gS, gU):
         S, U = ctx.saved_tensors
         dims = (U.numel(), S.numel(), ctx.data_size)
@@ -710,6 +772,7 @@ class Qr(torch.autograd.Function):
         ctx.set_materialize_grads(False)
 
     @staticmethod
+    @_plain_backward
     def backward(ctx, gQ, gR):
         data, Q, R = ctx.saved_tensors
         if gQ is None and gR is None:
@@ -733,6 +796,7 @@ class PermutePack(torch.autograd.Function):
         ctx.bwd, ctx.numel = bwd, data.numel()
 
     @staticmethod
+    @_plain_backward
     def backward(ctx, g):
         g = raw(g).contiguous()
         out = torch.zeros(ctx.numel, dtype=g.dtype, device=g.device)
@@ -758,6 +822,7 @@ class TdsCore(torch.autograd.Function):
         ctx.plan = plan
 
     @staticmethod
+    @_plain_backward
     def backward(ctx, gC):
         At, Bt = ctx.saved_tensors
         gC = raw(gC).contiguous()
