@@ -358,6 +358,12 @@ def main():
     ap.add_argument('--no-plain', action='store_true', help='skip the plain per-block baselines (for profiling)')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run one 256 MiB device copy, the reference dispatch for scaling HBM counters')
+    ap.add_argument('--unrolled', action='store_true',
+                    help='add the unrolled CTMRG workload (D=8, chi=128, complex128, 20 steps + energy)')
+    ap.add_argument('--only-unrolled', action='store_true', help='run the unrolled CTMRG workload alone')
+    ap.add_argument('--unrolled-steps', type=int, default=20)
+    ap.add_argument('--unrolled-plain-steps', type=int, default=1,
+                    help='CTMRG steps of the short batched-vs-plain comparison (0 skips the plain path)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU sharding of "
@@ -378,6 +384,13 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.only_unrolled:
+        from yastn_amd import ctmrg
+        print('bench: unrolled ctmrg', flush=True)
+        out['ctmrg_unrolled'] = ctmrg.bench_unrolled(n_iter=args.unrolled_steps,
+                                                     plain_iter=args.unrolled_plain_steps)
+        print(json.dumps(out), flush=True)
+        return
     print('bench: heff2', flush=True)
     AA, FL, FR, Wm, flops = bench_heff2(args, out)
     if not args.only_heff2:
@@ -391,6 +404,10 @@ def main():
             from yastn_amd import ctmrg
             print('bench: ctmrg', flush=True)
             out['ctmrg'] = ctmrg.bench(steps=max(2, args.steps // 5))
+            if args.unrolled:
+                print('bench: unrolled ctmrg', flush=True)
+                out['ctmrg_unrolled'] = ctmrg.bench_unrolled(n_iter=args.unrolled_steps,
+                                                             plain_iter=args.unrolled_plain_steps)
     if not args.no_cpu:
         print('bench: cpu baseline', flush=True)
         bench_cpu(args, out, flops)

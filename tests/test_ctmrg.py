@@ -71,3 +71,88 @@ def _fd_check(B):
         an = float((g * d).sum())
         # relative to the gradient scale |g| |d| (directions with cancelling projections included)
         assert abs(fd - an) <= 1e-7 * float(g.norm() * d.norm()), (fd, an)
+
+
+# ------------------------------------------------------------------ unrolled CTMRG + bond energy
+
+def _trivial_env_plaquette(cfg, dtype, seed):
+    """Site tensor with virtual charges {-1, 0, 1} (dimension 1 each) and an environment of chi = 1, charge 0,
+    all entries 1: the double-layer edges then trace the outer virtual legs of the 2x2 plaquette (ket = bra),
+    so the cluster's reduced density matrices are those of the plaquette state with open outer legs."""
+    L = lambda s, t, D: bs.leg(cfg, s, t, D)
+    vt, vD = (-1, 0, 1), (1, 1, 1)
+    legs = [L(-1, vt, vD), L(1, vt, vD), L(1, vt, vD), L(-1, vt, vD), L(1, (0, 1), (1, 1))]
+    st, _ = bs._fill_struct(cfg, legs, 0)
+    A = bs.from_flat(cfg, legs, 0, bs.seeded_normal(st.size, dtype, seed))
+    a = C.double_layer(A)
+    aL = [a.get_legs(i) for i in range(4)]
+    one = lambda s: L(s, (0,), (1,))
+    DL = lambda i, s: bs.Leg(s, aL[i].t, aL[i].D)
+    specs = {'tl': [one(-1), one(1)], 'tr': [one(-1), one(1)], 'br': [one(-1), one(1)], 'bl': [one(-1), one(1)],
+             't': [one(-1), DL(0, 1), one(1)], 'r': [one(-1), DL(3, 1), one(1)],
+             'b': [one(-1), DL(2, -1), one(1)], 'l': [one(-1), DL(1, -1), one(1)]}
+    fused_of = {'t': (0, True), 'r': (3, True), 'b': (2, False), 'l': (1, False)}
+    env = {}
+    for k in C.Env.NAMES:
+        st2, _ = bs._fill_struct(cfg, specs[k], 0)
+        x = bs.from_flat(cfg, specs[k], 0, np.ones(st2.size))
+        if k in fused_of:
+            ia, flip = fused_of[k]
+            x = x._replace(hfs=(x.hfs[0], a.hfs[ia].conj() if flip else a.hfs[ia], x.hfs[2]))
+        env[k] = x
+    return A, C.Env(**env)
+
+
+def _dense_plaquette_energy(Ad):
+    """<S.S> on the horizontal (s00, s01) plus the vertical (s10, s00) bond of the 2x2 plaquette state
+    psi = s00 s01 s10 s11 (inner bonds contracted, outer legs open), by dense linear algebra."""
+    psi = np.einsum('abcdp,edfgq,chijr,fjkls->pqrsabeghikl', Ad, Ad, Ad, Ad).reshape(2, 2, 2, 2, -1)
+    Sz = np.diag([-0.5, 0.5])
+    Sp = np.array([[0., 0.], [1., 0.]])
+    H = (np.kron(Sz, Sz) + 0.5 * (np.kron(Sp, Sp.T) + np.kron(Sp.T, Sp))).reshape(2, 2, 2, 2)
+    rho_h = np.einsum('abcdo,efcdo->abef', psi, psi.conj())   # (i0, i1, o0, o1)
+    rho_v = np.einsum('abcdo,ebgdo->cage', psi, psi.conj())   # (i2, i0, o2, o0)
+    e = lambda r: np.einsum('abef,efab->', r, H) / np.einsum('abab->', r)
+    return e(rho_h) + e(rho_v)
+
+
+@pytest.mark.parametrize('dtype', ['float64', 'complex128'])
+def test_bond_energy_matches_dense_plaquette(dtype):
+    """The bond energy contraction (open double layer, two-site density matrices of the 2x2 cluster, the
+    U(1) S.S tensor) against an independent dense computation on a plaquette."""
+    for B in (NB, _plain()):
+        cfg = bs.make_config(B, 'U1')
+        A, env = _trivial_env_plaquette(cfg, dtype, seed=5)
+        e = complex(C.bond_energy(env, A, C.heisenberg_bond(cfg)))
+        ref = _dense_plaquette_energy(A.to_dense())
+        assert abs(e - ref) <= 1e-12 * max(1.0, abs(ref)), (e, ref)
+        assert abs(ref.imag) <= 1e-12
+
+
+def test_open_double_layer_traces_to_double_layer():
+    cfg = bs.make_config(NB, 'U1')
+    A, env, sz = C.setup(cfg, D=3, chi=12, seed=1)
+    ao = C.double_layer_open(A)
+    pt = (0, 1)
+    one = bs.from_flat(cfg, [bs.leg(cfg, 1, pt, (1, 1)), bs.leg(cfg, -1, pt, (1, 1))], 0, np.array([1.0, 1.0]))
+    for op, ref in ((one, C.double_layer(A)), (sz, C.double_layer(A, sz))):
+        got = bs.tensordot(ao, op, axes=((4, 5), (1, 0)))
+        assert got.struct.t == ref.struct.t and got.struct.D == ref.struct.D
+        assert np.allclose(got.to_dense(), ref.to_dense(), rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize('D,chi,dtype', [(2, 8, 'float64'), (3, 12, 'complex128')])
+def test_unrolled_energy_oracle_vs_plain_and_checkpoint_cpu(D, chi, dtype):
+    """Three CTMRG steps + bond energy: numpy oracle vs the plain torch path; the gradient with checkpointed
+    segments of two steps equals the gradient without checkpointing."""
+    cn = bs.make_config(NB, 'U1')
+    A, env, _ = C.setup(cn, D=D, chi=chi, dtype=dtype, seed=1)
+    v_np = float(np.real(C.energy_loss_fn(A, env, C.heisenberg_bond(cn), chi, 3)))
+    cp = bs.make_config(_plain(), 'U1', device='cpu')
+    A, env, _ = C.setup(cp, D=D, chi=chi, dtype=dtype, seed=1)
+    H = C.heisenberg_bond(cp)
+    l0, g0 = C.energy_fwd_bwd(A, env, H, chi, 3, segment=0)
+    l2, g2 = C.energy_fwd_bwd(A, env, H, chi, 3, segment=2)
+    assert abs(float(l0) - v_np) <= 1e-10 * max(1.0, abs(v_np))
+    assert float(l0) == float(l2)
+    assert float((g0 - g2).norm()) <= 1e-12 * float(g0.norm())

@@ -46,3 +46,8 @@ def test_vjp_and_checkpoint_match_autograd():
     checkpoint(f, a2, s2, use_reentrant=False).backward()
     assert torch.linalg.norm(a2.grad - ga) <= 1e-12 * torch.linalg.norm(ga)
     assert torch.linalg.norm(s2.grad - gs) <= 1e-12 * torch.linalg.norm(gs)
+    print('reentrant checkpoint', flush=True)
+    a3, s3 = a0.clone().requires_grad_(True), s0.clone().requires_grad_(True)
+    checkpoint(f, a3, s3, use_reentrant=True).backward()
+    assert torch.linalg.norm(a3.grad - ga) <= 1e-12 * torch.linalg.norm(ga)
+    assert torch.linalg.norm(s3.grad - gs) <= 1e-12 * torch.linalg.norm(gs)

@@ -42,28 +42,33 @@ def double_layer(A, op=None):
     return x.fuse_legs(axes=((0, 4), (1, 5), (2, 6), (3, 7)))
 
 
+def _extra(a):
+    """Legs of the double-layer site beyond (t, l, b, r): the open physical legs of an energy contraction."""
+    return tuple(range(4, a.ndim))
+
+
 def corner_tl(env, a):
     c = env.l @ env.tl @ env.t
     c = bs.tensordot(c, a, axes=((2, 1), (0, 1)))
-    return c.fuse_legs(axes=((0, 2), (1, 3)))
+    return c.fuse_legs(axes=((0, 2), (1, 3)) + _extra(a))
 
 
 def corner_tr(env, a):
     c = env.t @ env.tr @ env.r
     c = bs.tensordot(c, a, axes=((1, 2), (0, 3)))
-    return c.fuse_legs(axes=((0, 2), (1, 3)))
+    return c.fuse_legs(axes=((0, 2), (1, 3)) + _extra(a))
 
 
 def corner_br(env, a):
     c = env.r @ env.br @ env.b
     c = bs.tensordot(c, a, axes=((2, 1), (2, 3)))
-    return c.fuse_legs(axes=((0, 2), (1, 3)))
+    return c.fuse_legs(axes=((0, 2), (1, 3)) + _extra(a))
 
 
 def corner_bl(env, a):
     c = env.b @ env.bl @ env.l
     c = bs.tensordot(c, a, axes=((2, 1), (1, 2)))
-    return c.fuse_legs(axes=((0, 3), (1, 2)))
+    return c.fuse_legs(axes=((0, 3), (1, 2)) + _extra(a))
 
 
 def proj_corners(r0, r1, chi, svd_opts=None):
@@ -147,6 +152,114 @@ def cluster_expectation(env, A, op):
         top = corner @ ctr
         return bs.tensordot(top, bottom, axes=((0, 1), (1, 0))).to_number()
     return Z(corner_tl(env, aO)) / Z(corner_tl(env, a))
+
+
+def double_layer_open(A):
+    """a(t, l, b, r, p, p*) = A(t, l, b, r, p) A*(t, l, b, r, p*), legs fused (ket, bra) pairwise and the
+    physical legs left open.  The outer product is a contraction over a trailing dimension-1, charge-0
+    leg, which leaves the block buffer unchanged."""
+    cfg = A.config
+    legs = [A.get_legs(i) for i in range(A.ndim)]
+    one = bs.leg(cfg, 1, ((0,) * cfg.sym.NSYM,), (1,))
+    Ax = bs.from_flat(cfg, legs + [one], A.n, A.data)
+    x = bs.tensordot(Ax, Ax.conj(), axes=(5, 5))  # (t, l, b, r, p, t*, l*, b*, r*, p*)
+    return x.fuse_legs(axes=((0, 5), (1, 6), (2, 7), (3, 8), 4, 9))
+
+
+def heisenberg_bond(config):
+    """S.S on two spin-1/2 sites as a U(1)-symmetric 4-leg tensor H(o1, o2, i1, i2), s = (+1, +1, -1, -1),
+    physical charges 0 (Sz = -1/2) and 1 (Sz = +1/2): <o1 o2|Sz Sz + (S+S- + S-S+)/2|i1 i2>."""
+    pt, pD = (0, 1), (1, 1)
+    legs = [bs.leg(config, s, pt, pD) for s in (1, 1, -1, -1)]
+    st, _ = bs._fill_struct(config, legs, 0)
+    sz = {0: -0.5, 1: 0.5}
+    vals = []
+    for o1, o2, i1, i2 in st.t:
+        if (o1, o2) == (i1, i2):
+            vals.append(sz[o1] * sz[o2])
+        else:  # (o1, o2) = (i2, i1) with o1 != o2: one spin flip on each site
+            vals.append(0.5)
+    return bs.from_flat(config, legs, 0, np.array(vals))
+
+
+def _bond_identity(config):
+    pt, pD = (0, 1), (1, 1)
+    legs = [bs.leg(config, s, pt, pD) for s in (1, 1, -1, -1)]
+    st, _ = bs._fill_struct(config, legs, 0)
+    return bs.from_flat(config, legs, 0, np.array([1.0 if t[:2] == t[2:] else 0.0 for t in st.t]))
+
+
+def bond_energy(env, A, H):
+    """Energy per site of a nearest-neighbour bond operator on a 1x1 unit cell: <H> on the horizontal bond
+    (tl, tr) plus <H> on the vertical bond (bl, tl) of the 2x2 cluster, each as Tr(rho H) / Tr(rho) with
+    rho the two-site reduced density matrix (the bond expectation values of the reference's measure_nn,
+    _env_ctm_measure.py:79-175, in the 2x2-cluster form)."""
+    a, ao = double_layer(A), double_layer_open(A)
+    I = _bond_identity(A.config)
+    ctl, ctr, cbr, cbl = corner_tl(env, ao), corner_tr(env, ao), corner_br(env, a), corner_bl(env, ao)
+
+    def expect(rho):  # rho(p1, p1*, p2, p2*)
+        num = bs.tensordot(rho, H, axes=((0, 1, 2, 3), (2, 0, 3, 1))).to_number()
+        den = bs.tensordot(rho, I, axes=((0, 1, 2, 3), (2, 0, 3, 1))).to_number()
+        return num / den
+    # horizontal: Tr(ctl ctr cbr cbl) with open legs on tl and tr
+    top = bs.tensordot(ctl, ctr, axes=(1, 0))                 # (x, p1, p1*, y, p2, p2*)
+    bottom = cbr @ corner_bl(env, a)                          # (y, x)
+    e_h = expect(bs.tensordot(top, bottom, axes=((0, 3), (1, 0))))
+    # vertical: open legs on bl and tl
+    left = bs.tensordot(cbl, corner_tl(env, ao), axes=(1, 0))  # (x, pb, pb*, y, pt, pt*)
+    right = corner_tr(env, a) @ cbr                           # (y, x)
+    e_v = expect(bs.tensordot(left, right, axes=((0, 3), (1, 0))))
+    return e_h + e_v
+
+
+def ctm_iterations(A, env, chi, n_iter, segment=0, svd_opts=None, on_step=None):
+    """``n_iter`` 'hv' steps from ``env``.  With ``segment`` > 0 and gradients enabled, each run of
+    ``segment`` steps is a non-reentrant checkpoint region (the reference's checkpoint_move,
+    _env_ctm.py:556-589): its environments are recomputed in the backward pass instead of kept."""
+    import torch
+    from torch.utils.checkpoint import checkpoint
+    a = None
+    done = 0
+    while done < n_iter:
+        k = min(segment, n_iter - done) if segment else n_iter - done
+        if segment and torch.is_grad_enabled():
+            held = []
+
+            def run(adata, *edata, k=k, env0=env):
+                e = Env(**{n: getattr(env0, n)._replace(data=d) for n, d in zip(Env.NAMES, edata)})
+                aa = double_layer(A._replace(data=adata))
+                for _ in range(k):
+                    e = ctm_step(e, aa, chi, svd_opts)
+                    if on_step is not None:
+                        on_step()
+                held[:] = [e]
+                return tuple(t.data for t in e.tensors())
+            out = checkpoint(run, A.data, *(t.data for t in env.tensors()), use_reentrant=False)
+            env = Env(**{n: getattr(held[0], n)._replace(data=d) for n, d in zip(Env.NAMES, out)})
+        else:
+            a = double_layer(A) if a is None else a
+            for _ in range(k):
+                env = ctm_step(env, a, chi, svd_opts)
+                if on_step is not None:
+                    on_step()
+        done += k
+    return env
+
+
+def energy_loss_fn(A, env, H, chi, n_iter, segment=0, svd_opts=None, on_step=None):
+    """``n_iter`` CTMRG steps followed by the nearest-neighbour bond energy per site (real part)."""
+    env2 = ctm_iterations(A, env, chi, n_iter, segment, svd_opts, on_step)
+    val = bond_energy(env2, A, H)
+    return val.real if hasattr(val, 'is_complex') and val.is_complex() else val
+
+
+def energy_fwd_bwd(A, env, H, chi, n_iter, segment=0, svd_opts=None, on_step=None):
+    """Unrolled CTMRG + energy, forward and backward w.r.t. the site tensor's data; returns (loss, grad)."""
+    data = A.data.detach().clone().requires_grad_(True)
+    loss = energy_loss_fn(A._replace(data=data), env, H, chi, n_iter, segment, svd_opts, on_step)
+    loss.backward()
+    return loss.detach(), data.grad
 
 
 # ------------------------------------------------------------------ workload setup
@@ -263,4 +376,55 @@ def bench(steps=3, D=6, chi=64, plain=True, cpu=True):
         out['s_per_step_host_torch'] = tc
         out['host_threads'] = torch.get_num_threads()
         out['speedup_vs_host'] = tc / tb
+    return out
+
+
+def bench_unrolled(D=8, chi=128, dtype='complex128', n_iter=20, segment=4, repeats=1, plain_iter=0,
+                   no_checkpoint_memory=False):
+    """Seconds per step of ``n_iter`` CTMRG 'hv' steps + the nearest-neighbour Heisenberg energy, forward and
+    backward with checkpoint segments of ``segment`` steps, on the batched HIP backend, with the peak device
+    memory of the step.  With ``plain_iter`` > 0 the batched and the plain per-block GPU paths also run a
+    ``plain_iter``-step version of the same step (the plain path is too slow for the full unrolling)."""
+    import torch
+    from . import backend_hip, backend_plain
+    out = {'workload': f'{n_iter} CTMRG hv steps + nearest-neighbour Heisenberg energy per site, fwd+bwd, U(1) '
+                       f'iPEPS 1x1, D={D}, chi={chi}, {dtype}; checkpoint segments of {segment} steps'}
+    count = [0]
+
+    def tick():
+        count[0] += 1
+        if count[0] % 4 == 0:
+            print(f'  ctm steps run: {count[0]}', flush=True)
+
+    def run(backend, n, nit, seg):
+        cfg = bs.make_config(backend, 'U1', device='cuda:0')
+        A, env, _ = setup(cfg, D=D, chi=chi, dtype=dtype, seed=0)
+        H = heisenberg_bond(cfg)
+        best, peak = float('inf'), 0
+        for i in range(n + 1):  # the first run builds and tunes the plans
+            torch.cuda.synchronize()
+            torch.cuda.reset_peak_memory_stats()
+            base = torch.cuda.memory_allocated()
+            t0 = time.perf_counter()
+            loss, g = energy_fwd_bwd(A, env, H, chi, nit, seg, on_step=tick)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            print(f'  {backend.__name__.split(".")[-1]} {nit} steps segment {seg}: {dt:.3f} s', flush=True)
+            if i > 0 or n == 0:
+                best = min(best, dt)
+                peak = torch.cuda.max_memory_allocated() - base
+        return best, float(loss), g, peak
+    tb, lb, gb, pk = run(backend_hip, repeats, n_iter, segment)
+    out.update(s_per_step_batched=tb, energy=lb, peak_hbm_gb=pk / 1e9)
+    if no_checkpoint_memory:
+        t0, l0, g0, pk0 = run(backend_hip, 0, n_iter, 0)
+        out['no_checkpoint'] = {'s_per_step': t0, 'peak_hbm_gb': pk0 / 1e9,
+                                'grad_rel_diff_vs_checkpointed': float((g0 - gb).norm() / gb.norm())}
+    if plain_iter:
+        seg = min(segment, plain_iter)
+        ts, ls, gs, _ = run(backend_hip, repeats, plain_iter, seg)
+        tp, lp, gp, _ = run(backend_plain, 0, plain_iter, seg)
+        out['short'] = {'ctm_steps': plain_iter, 'segment': seg, 's_batched': ts, 's_plain_gpu': tp,
+                        'speedup_vs_plain_gpu': tp / ts, 'energy_diff': abs(ls - lp),
+                        'grad_rel_diff_vs_plain': float((gs - gp).norm() / gp.norm())}
     return out
