@@ -176,11 +176,12 @@ def test_eigh(B):
 
 
 def test_svd_random_shapes(B):
-    """Single blocks of assorted shapes (LDS and workspace instances, with and without the QR
-    preconditioning, whose Householder panel must fit LDS: not for 1300 rows), real and complex."""
+    """Single blocks of assorted shapes (LDS and workspace instances; the workspace instance is QR
+    preconditioned, with the Householder panel in LDS, or in place for 1300 rows, and 700 complex
+    rows), real and complex."""
     rng = np.random.default_rng(9)
     for (m, n) in ((1, 1), (1, 7), (7, 1), (2, 2), (17, 5), (5, 17), (64, 64), (90, 40), (33, 130), (150, 150),
-                   (300, 170), (170, 300), (440, 440), (1300, 40), (40, 1300)):
+                   (300, 170), (170, 300), (440, 440), (1300, 40), (40, 1300), (700, 300)):
         for cplx in (False, True):
             a = rng.standard_normal((m, n)) + (1j * rng.standard_normal((m, n)) if cplx else 0)
             k = min(m, n)

@@ -38,12 +38,17 @@ def B():
     return backend_hip
 
 
+# blocks whose 16-column Householder panel exceeds LDS: the whole call factors its panels in place
+LONG = [(1300, 40), (700, 90), (90, 700)]
+
+
 @pytest.mark.parametrize('cplx', [False, True])
-def test_qr_matches_oracle(B, cplx):
+@pytest.mark.parametrize('long_blocks', [False, True])
+def test_qr_matches_oracle(B, cplx, long_blocks):
     rng = np.random.default_rng(31)
-    meta, na, sizes = _meta(SHAPES)
+    meta, na, sizes = _meta(SHAPES + (LONG if long_blocks else []))
     a = _data(rng, na, cplx)
-    print('qr batched blocks', len(meta), 'complex', cplx, flush=True)
+    print('qr batched blocks', len(meta), 'complex', cplx, 'long', long_blocks, flush=True)
     Q, R = B.qr(torch.as_tensor(a, device=DEV), meta, sizes)
     Q, R = Q.cpu().numpy(), R.cpu().numpy()
     Qo, Ro = O.qr(a, meta, sizes)

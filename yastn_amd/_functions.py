@@ -695,7 +695,7 @@ def fix_svd_signs(Udata, Vdata, meta):
     return Udata * ph.conj()[col], Vdata * ph[row]
 
 def qr_forward(data, meta, sizes, plan=None):
-    """Batched Householder QR of every block (plan: plans.QrPlan with native=True)."""
+    """Batched Householder QR of every block (plan: plans.QrPlan)."""
     data = raw(data)
     if plan is None:
         plan = P.qr_plan(meta, data.element_size(), data.numel(), sizes, data.device)

@@ -426,22 +426,12 @@ def _magnitude_order(S, meta, u_numel):
 
 def qr(data, meta, sizes):
     """Batched Householder QR of every block, diag(R) >= 0 as in the reference (backend_np.py:511-520),
-    with a hand-written adjoint.  Blocks whose 16-column Householder panel exceeds LDS (more than about
-    1200 rows real, 600 complex) go through torch.linalg.qr on the device, block by block."""
+    with a hand-written adjoint.  The 16-column Householder panels are staged in LDS when they fit and
+    factored in place in the workspace otherwise."""
     plan = P.qr_plan(meta, data.element_size(), data.numel(), sizes, data.device)
-    if plan.native:
-        if not _needs_grad(data):
-            return F.qr_forward(data, meta, sizes, plan)
-        return F.Qr.apply(data, meta, sizes, plan)
-    Q = torch.zeros(sizes[0], dtype=data.dtype, device=data.device)
-    R = torch.zeros(sizes[1], dtype=data.dtype, device=data.device)
-    for sl, D, slQ, DQ, slR, DR in meta:
-        q, r = torch.linalg.qr(data[_sl(sl)].view(D))
-        sg = torch.sign(torch.real(torch.diagonal(r)))
-        sg = torch.where(sg == 0, torch.ones_like(sg), sg)
-        Q[_sl(slQ)].view(DQ)[:] = q * sg
-        R[_sl(slR)].view(DR)[:] = sg.reshape(-1, 1) * r
-    return Q, R
+    if not _needs_grad(data):
+        return F.qr_forward(data, meta, sizes, plan)
+    return F.Qr.apply(data, meta, sizes, plan)
 
 
 def pinv(A, rcond=None, hermitian=False, out=None, atol=None, rtol=None):

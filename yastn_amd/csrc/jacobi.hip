@@ -1138,11 +1138,14 @@ __device__ inline QrDesc load_qr(const int64_t* w) {
 }
 
 // One panel of the blocked Householder QR (LAPACK geqrf / larfg / larft conventions): the
-// (p - k0) x nb panel at column k0 = step * YH_QR_NB is factored in LDS column by column,
+// (p - k0) x nb panel at column k0 = step * YH_QR_NB is factored column by column,
 // H_j = I - tau_j v_j v_j^H with H_j^H x = beta e_1 (beta real), applying H_j^H to the panel's
 // later columns.  The vectors go to Vall, the R part back to W0, and -T, with H_1 ... H_nb =
 // I - V T V^H, to Tm.  The columns right of the panel are updated by grouped GEMMs on the host's plan.
-template <bool CPLX>
+// The panel is staged in LDS (INPLACE false) or, when it does not fit, factored in place in W0
+// (leading dimension p; the workgroup's barriers order its global reads and writes, and the
+// panel of one block, a few hundred KB, stays in L2).
+template <bool CPLX, bool INPLACE>
 __global__ void __launch_bounds__(kThreads)
     qr_panel_kernel(const int64_t* __restrict__ qdesc, const int32_t* __restrict__ ids,
                     typename Num<CPLX>::T* __restrict__ work, int step) {
@@ -1160,16 +1163,20 @@ __global__ void __launch_bounds__(kThreads)
   const int nb = (int)((d.k - k0) < NB ? (d.k - k0) : NB);
   const int64_t L = d.p - k0;
   const int64_t p = d.p;
-  T* P = reinterpret_cast<T*>(smem_raw);  // L x nb, column-major, leading dimension L
   T* W0 = work + d.w_off;
   T* Vall = work + d.vall_off;
-  for (int64_t e = tid; e < L * nb; e += kThreads) {
-    const int64_t c = e / L, r = e - c * L;
-    P[e] = W0[(k0 + r) + (k0 + c) * p];
+  // the L x nb panel, column-major with leading dimension ld
+  const int64_t ld = INPLACE ? p : L;
+  T* P = INPLACE ? W0 + k0 + k0 * p : reinterpret_cast<T*>(smem_raw);
+  if (!INPLACE) {
+    for (int64_t e = tid; e < L * nb; e += kThreads) {
+      const int64_t c = e / L, r = e - c * L;
+      P[e] = W0[(k0 + r) + (k0 + c) * p];
+    }
+    __syncthreads();
   }
-  __syncthreads();
   for (int j = 0; j < nb; ++j) {
-    T* x = P + (int64_t)j * L;
+    T* x = P + (int64_t)j * ld;
     double loc = 0.0;
     for (int64_t r = j + 1 + tid; r < L; r += kThreads) loc += N::abs2(x[r]);
     const double xn2 = block_sum(loc, s_red, lane, wave);
@@ -1200,7 +1207,7 @@ __global__ void __launch_bounds__(kThreads)
     __syncthreads();
     // H_j^H = I - conj(tau) v v^H on the later panel columns, one wave per column; v = [1; x[j+1:]]
     for (int c = j + 1 + wave; c < nb; c += kWaves) {
-      T* y = P + (int64_t)c * L;
+      T* y = P + (int64_t)c * ld;
       double wr = 0.0, wi = 0.0;
       for (int64_t r = j + 1 + lane; r < L; r += 64) {
         const T v = x[r], u = y[r];
@@ -1221,9 +1228,9 @@ __global__ void __launch_bounds__(kThreads)
   }
   // T by columns: T(j, j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) z, z_i = V(:, i)^H v_j
   for (int j = 0; j < nb; ++j) {
-    const T* vj = P + (int64_t)j * L;
+    const T* vj = P + (int64_t)j * ld;
     for (int i = wave; i < j; i += kWaves) {
-      const T* vi = P + (int64_t)i * L;
+      const T* vi = P + (int64_t)i * ld;
       double zr = 0.0, zi = 0.0;
       for (int64_t r = j + 1 + lane; r < L; r += 64) {
         const T a = vi[r], b = vj[r];
@@ -1261,7 +1268,7 @@ __global__ void __launch_bounds__(kThreads)
   }
   for (int64_t e = tid; e < L * nb; e += kThreads) {
     const int64_t c = e / L, r = e - c * L;
-    const T v = P[e];
+    const T v = P[r + c * ld];  // in place: the same element as the W0 store below
     W0[(k0 + r) + (k0 + c) * p] = r <= c ? v : N::make(0.0, 0.0);
     Vall[(k0 + r) + (k0 + c) * p] = r < c ? N::make(0.0, 0.0) : (r == c ? N::make(1.0, 0.0) : v);
   }
@@ -1639,7 +1646,8 @@ extern "C" int yastn_hip_check_qr_plan(const int64_t* h_qdesc, const int64_t* h_
     YH_CHECK_ARG(b >= 0 && q >= 1 && p >= q && q <= YH_SVD_MAX_Q, "qr block %d: %lld x %lld", b, (long long)p,
                  (long long)q);
     YH_CHECK_ARG((sm >= sn ? sm : sn) == p && (sm < sn ? sm : sn) == q, "qr block %d: shape mismatch", b);
-    YH_CHECK_ARG(p * YH_QR_NB * isz <= lds_bytes, "qr block %d: panel of %lld rows exceeds LDS", b, (long long)p);
+    YH_CHECK_ARG(lds_bytes == 0 || p * YH_QR_NB * isz <= lds_bytes, "qr block %d: panel of %lld rows exceeds LDS",
+                 b, (long long)p);
     const bool ok = w[2] >= 0 && w[2] + p * q <= work_numel && w[3] >= 0 && w[3] + p * q <= work_numel &&
                     w[4] >= 0 && w[4] + np * YH_QR_NB * YH_QR_NB <= work_numel && w[5] >= 0 &&
                     w[5] + 2 * q * q <= work_numel && w[6] >= 0 && w[6] + 2 * YH_QR_NB * q <= work_numel;
@@ -1651,22 +1659,29 @@ extern "C" int yastn_hip_check_qr_plan(const int64_t* h_qdesc, const int64_t* h_
 extern "C" int yastn_hip_svd_qr_panel(void* work, int dtype, const int64_t* d_qdesc, const int32_t* d_ids, int64_t n,
                                       int step, int64_t lds_bytes, hipStream_t stream) {
   YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
-  YH_CHECK_ARG(step >= 0 && lds_bytes > 0 && lds_bytes <= 160 * 1024, "qr panel step %d / lds %lld", step,
+  YH_CHECK_ARG(step >= 0 && lds_bytes >= 0 && lds_bytes <= 160 * 1024, "qr panel step %d / lds %lld", step,
                (long long)lds_bytes);
   if (n == 0) return YH_OK;
   const dim3 g((unsigned)n), t(kThreads);
-  if (dtype == YH_F64) {
-    const int st = yh::hip_status(hipFuncSetAttribute((const void*)qr_panel_kernel<false>,
+  if (lds_bytes == 0) {  // panels factored in place in the workspace
+    if (dtype == YH_F64)
+      hipLaunchKernelGGL((qr_panel_kernel<false, true>), g, t, 0, stream, d_qdesc, d_ids, (double*)work, step);
+    else
+      hipLaunchKernelGGL((qr_panel_kernel<true, true>), g, t, 0, stream, d_qdesc, d_ids, (double2*)work, step);
+  } else if (dtype == YH_F64) {
+    const int st = yh::hip_status(hipFuncSetAttribute((const void*)qr_panel_kernel<false, false>,
                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
                                   "svd qr panel LDS size");
     if (st) return st;
-    hipLaunchKernelGGL(qr_panel_kernel<false>, g, t, (size_t)lds_bytes, stream, d_qdesc, d_ids, (double*)work, step);
+    hipLaunchKernelGGL((qr_panel_kernel<false, false>), g, t, (size_t)lds_bytes, stream, d_qdesc, d_ids,
+                       (double*)work, step);
   } else {
-    const int st = yh::hip_status(hipFuncSetAttribute((const void*)qr_panel_kernel<true>,
+    const int st = yh::hip_status(hipFuncSetAttribute((const void*)qr_panel_kernel<true, false>,
                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
                                   "svd qr panel LDS size");
     if (st) return st;
-    hipLaunchKernelGGL(qr_panel_kernel<true>, g, t, (size_t)lds_bytes, stream, d_qdesc, d_ids, (double2*)work, step);
+    hipLaunchKernelGGL((qr_panel_kernel<true, false>), g, t, (size_t)lds_bytes, stream, d_qdesc, d_ids,
+                       (double2*)work, step);
   }
   return yh::hip_status(hipGetLastError(), "svd qr panel launch");
 }
@@ -1713,8 +1728,8 @@ extern "C" int yastn_hip_check_qr_op_plan(const int64_t* h_qdesc, const int64_t*
     const int64_t m = w[0], n = w[1], k = w[7];
     YH_CHECK_ARG(m >= 1 && n >= 1 && k == (m < n ? m : n), "qr block %lld: %lld x %lld, k %lld", (long long)b,
                  (long long)m, (long long)n, (long long)k);
-    YH_CHECK_ARG(m * YH_QR_NB * isz <= lds_bytes, "qr block %lld: panel of %lld rows exceeds LDS", (long long)b,
-                 (long long)m);
+    YH_CHECK_ARG(lds_bytes == 0 || m * YH_QR_NB * isz <= lds_bytes, "qr block %lld: panel of %lld rows exceeds LDS",
+                 (long long)b, (long long)m);
     const int64_t np = (k + YH_QR_NB - 1) / YH_QR_NB;
     const bool ok = w[2] >= 0 && w[2] + m * n <= work_numel && w[3] >= 0 && w[3] + m * k <= work_numel &&
                     w[4] >= 0 && w[4] + np * YH_QR_NB * YH_QR_NB <= work_numel && w[5] >= 0 &&

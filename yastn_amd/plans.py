@@ -650,9 +650,10 @@ class SvdPlan:
     Blocks whose working matrices (W and V, (p q + q^2) elements) fit the LDS limit run in the
     LDS-resident kernel, one workgroup per block, all in one launch.  Larger blocks use the panel
     path: W and V in a workspace, panels of width b, one launch per round of panel pairs.  For SVD
-    blocks whose Householder panel (p x YH_QR_NB) fits LDS, the panel path first factors W0 = Q R
-    (blocked Householder QR: a panel kernel plus three grouped GEMMs per step) and iterates on
-    X = R^H; the p-side vectors are multiplied by Q at the end (three grouped GEMMs per step).
+    blocks the panel path first factors W0 = Q R (blocked Householder QR: a panel kernel plus three
+    grouped GEMMs per step; the p x YH_QR_NB panel is staged in LDS when every block's fits, else
+    factored in place in the workspace) and iterates on X = R^H; the p-side vectors are multiplied
+    by Q at the end (three grouped GEMMs per step).
     """
 
     def __init__(self, blocks, itemsize, mode, a_numel, u_numel, s_numel, v_numel, device):
@@ -664,7 +665,6 @@ class SvdPlan:
         dev_lds = L.device_lds_per_block()
         # the LDS kernel also holds ~8 KiB of static shared memory
         lds_limit = min(L.YH_SVD_LDS_BYTES, dev_lds - 9 * 1024)
-        qr_limit = dev_lds - 8 * 1024
         qd = np.zeros((len(blocks), L.YH_QR_WORDS), dtype=np.int64)
         jwords = []
         for b, (a_off, m, n, u_off, s_off, v_off) in enumerate(blocks):
@@ -676,7 +676,7 @@ class SvdPlan:
                 small.append(b)
                 lds_bytes = max(lds_bytes, need * itemsize)
                 w_off = 0
-            elif mode == 0 and PRECOND and p * NB * itemsize <= qr_limit:
+            elif mode == 0 and PRECOND:
                 big.append(b)
                 pre.append(b)
                 nst = -(-q // NB)
@@ -773,7 +773,10 @@ class SvdPlan:
         NB = L.YH_QR_NB
         self.h_qdesc = np.ascontiguousarray(qd)
         pmax = max(int(qd[b, 0]) for b in pre)
+        # panel staged in LDS, or 0: factored in place in the workspace
         self.qr_lds = pmax * NB * itemsize
+        if self.qr_lds > L.device_lds_per_block() - 8 * 1024:
+            self.qr_lds = 0
         ids = np.array(pre, dtype=np.int32)
         L.call('yastn_hip_check_qr_plan', self.h_qdesc.ctypes.data, self.h_blocks.ctypes.data, ids.ctypes.data,
                len(ids), self.dt, max(work, 0), self.qr_lds)
@@ -832,8 +835,9 @@ class QrPlan:
     """Batched Householder QR of the blocks of backend.qr (meta rows (sl, (m, n), slQ, (m, k), slR,
     (k, n))): per block the workspace holds W0 (m x n), the reflectors (m x k), -T per panel, E (m x k)
     and the GEMM scratch; the panel steps and the GEMM plans follow the preconditioning of the SVD
-    (trailing updates of W0 after each panel, then E <- H_s E for the panels in reverse).  `native` is
-    False when a block's 16-column Householder panel does not fit LDS."""
+    (trailing updates of W0 after each panel, then E <- H_s E for the panels in reverse).  The
+    16-column Householder panels are staged in LDS when every block's fits (``lds`` > 0), else
+    factored in place in the workspace (``lds`` = 0)."""
 
     def __init__(self, meta, itemsize, a_numel, sizes, device):
         NB = L.YH_QR_NB
@@ -857,11 +861,11 @@ class QrPlan:
         self.h_qblocks = np.ascontiguousarray(np.array(qb, dtype=np.int64).reshape(-1, L.YH_QROP_WORDS))
         pmax = max((r[0] for r in qd), default=1)
         self.lds = pmax * NB * itemsize
-        self.native = self.nblocks == 0 or self.lds <= L.device_lds_per_block() - 8 * 1024
-        if not self.native:
-            return
+        if self.lds > L.device_lds_per_block() - 8 * 1024:
+            self.lds = 0
+        self.native = True
         L.call('yastn_hip_check_qr_op_plan', self.h_qdesc.ctypes.data, self.h_qblocks.ctypes.data, self.nblocks,
-               self.dt, a_numel, sizes[0], sizes[1], self.work_numel, max(self.lds, 1))
+               self.dt, a_numel, sizes[0], sizes[1], self.work_numel, self.lds)
         self.d_qdesc = torch.from_numpy(self.h_qdesc).to(device)
         self.d_qblocks = torch.from_numpy(self.h_qblocks).to(device)
         nsteps = max((-(-r[7] // NB) for r in qd), default=0)
