@@ -122,20 +122,30 @@ def call(name, *args):
     return st
 
 
-_DEVICE_LDS = None
+_DEVICE_INFO = None
+
+
+def _device_info():
+    """(LDS bytes per workgroup, compute units) of the current device; (64 KiB, 256) without a GPU."""
+    global _DEVICE_INFO
+    if _DEVICE_INFO is None:
+        try:
+            import torch
+            if not torch.cuda.is_available():
+                return 65536, 256
+        except ImportError:
+            return 65536, 256
+        lds, cu = ctypes.c_int(0), ctypes.c_int(0)
+        call('yastn_hip_device_info', ctypes.byref(lds), ctypes.byref(cu))
+        _DEVICE_INFO = (int(lds.value), int(cu.value))
+    return _DEVICE_INFO
 
 
 def device_lds_per_block():
     """LDS bytes one workgroup may allocate on the current device (64 KiB when no GPU is visible)."""
-    global _DEVICE_LDS
-    if _DEVICE_LDS is None:
-        try:
-            import torch
-            if not torch.cuda.is_available():
-                return 65536
-        except ImportError:
-            return 65536
-        lds, cu = ctypes.c_int(0), ctypes.c_int(0)
-        call('yastn_hip_device_info', ctypes.byref(lds), ctypes.byref(cu))
-        _DEVICE_LDS = int(lds.value)
-    return _DEVICE_LDS
+    return _device_info()[0]
+
+
+def device_cu_count():
+    """Compute units of the current device (256 when no GPU is visible)."""
+    return _device_info()[1]

@@ -728,9 +728,6 @@ class SvdPlan:
             pmax = max(max(jwords[b][1], jwords[b][2]) for b in big)
             qmax = max(min(jwords[b][1], jwords[b][2]) for b in big)
             self.maxch_p = -(-pmax // PANEL_CHUNK_ROWS)
-            # workgroups per panel pair in the inner kernel: each updates a slice of the 16-row stripes
-            # of W and V, about one stripe per wave (16 waves)
-            self.panel_slices = PANEL_SLICES or max(1, min(64, -(-(-(-pmax // 16) - (-qmax // 16)) // 16)))
             # a sweep: every panel against itself (round -1), then the circle order of panel pairs
             per_round = {}
             for b in big:
@@ -750,11 +747,20 @@ class SvdPlan:
                 offs.append((len(jobs), len(per_round[r])))
                 jobs.extend(per_round[r])
             self.rounds = offs
+            # workgroups per panel pair in the inner kernel: each sums the Gram partials and repeats the
+            # inner sweep, then updates its slice of the 16-row stripes of W and V.  At most about one
+            # stripe per wave (16 waves); and about one workgroup per CU over the largest round of cross
+            # pairs, since every further slice repeats the sequential inner sweep (measured: rounds of
+            # ~260 pairs of up to 1134-row complex blocks run 2x faster with 1 slice than with 9, rounds
+            # of ~75 pairs of up to 440-row blocks fastest with 3).
+            per_wave = -(-(-(-pmax // 16) - (-qmax // 16)) // 16)
+            maxn = max(n for _, n in offs)
+            cross = max((n for _, n in offs[1:]), default=maxn)
+            self.panel_slices = PANEL_SLICES or max(1, min(64, per_wave, L.device_cu_count() // cross))
             self.h_rounds = np.ascontiguousarray(np.array(offs, dtype=np.int64).reshape(-1, 2))
             self.h_jobs = np.ascontiguousarray(np.array(jobs, dtype=np.int32).reshape(-1, 3))
             L.call('yastn_hip_check_panel_jobs', self.h_jblocks.ctypes.data, self.nblocks, self.h_jobs.ctypes.data,
                    len(jobs), width, mode, self.maxch_p)
-            maxn = max(n for _, n in offs)
             t16 = -(-2 * width // 16)
             self.part_numel = maxn * self.maxch_p * (t16 * (t16 + 1) // 2) * 256 * (itemsize // 8)
             self.d_jobs = torch.from_numpy(self.h_jobs).to(device)
