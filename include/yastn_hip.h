@@ -207,7 +207,8 @@ int yastn_hip_splitk_reduce(void* const* outs, const void* partials, int dtype, 
 #define YH_QR_NB 16
 int yastn_hip_check_qr_plan(const int64_t* h_qdesc, const int64_t* h_blocks, const int32_t* h_ids, int64_t n,
                             int dtype, int64_t work_numel, int64_t lds_bytes);
-/* Factor panel `step` (columns step*YH_QR_NB ...) of every listed block (lds_bytes >= p*YH_QR_NB*itemsize). */
+/* Factor panel `step` (columns step*YH_QR_NB ...) of every listed block: staged in LDS when
+ * lds_bytes >= p*YH_QR_NB*itemsize for every block, in place in the workspace when lds_bytes == 0. */
 int yastn_hip_svd_qr_panel(void* work, int dtype, const int64_t* d_qdesc, const int32_t* d_ids, int64_t n, int step,
                            int64_t lds_bytes, hipStream_t stream);
 /* X = R^H, V_J = I. */
