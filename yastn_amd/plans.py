@@ -510,8 +510,11 @@ def _xcd_order(tl):
             if q:
                 out.append(q.pop()[1:])
     return out
-SPLITK_MIN_K = 8192       # split the K of a single-term GEMM with few output tiles beyond this length
-SPLITK_MAX_MN = 64 * 64   # ... when its output has at most this many elements
+# split the K of a single-term GEMM beyond this length, when its output has at most SPLITK_MAX_MN
+# elements, into chunks of at least SPLITK_MIN_KC (tuning knobs)
+SPLITK_MIN_K = int(os.environ.get('YASTN_HIP_SPLITK_MIN_K', '8192'))
+SPLITK_MAX_MN = int(os.environ.get('YASTN_HIP_SPLITK_MAX_MN', str(64 * 64)))
+SPLITK_MIN_KC = int(os.environ.get('YASTN_HIP_SPLITK_MIN_KC', '2048'))
 SPLITK_MAX_S = 512
 
 
@@ -524,7 +527,7 @@ def _split_k(g0, t0, out_buf):
         M, N, c_off, c_rs, c_cs, tb, tc, fl = (int(x) for x in row)
         if tc == 1 and out_buf < L.YH_MAX_BUFS and M * N <= SPLITK_MAX_MN and int(t0[tb, 6]) >= SPLITK_MIN_K:
             a_off, a_rs, a_cs, b_off, b_rs, b_cs, K, tf = (int(x) for x in t0[tb])
-            kc = max(2048, -(-K // SPLITK_MAX_S))
+            kc = max(SPLITK_MIN_KC, -(-K // SPLITK_MAX_S))
             kc = -(-kc // 16) * 16
             S = -(-K // kc)
             for sidx in range(S):
