@@ -463,19 +463,24 @@ def eig(data, meta=None, sizes=(1, 1), **kwargs):
 
 
 def eigh_lowrank(data, meta, sizes, thresh=None, **kwargs):
-    """Partial eigh policy; computed from the full Jacobi eigh, keeping k per block in ``which`` order."""
+    """Partial eigh policy (the reference runs scipy's eigsh per block on the host, backend_torch.py:424-449):
+    one batched Jacobi eigh of all blocks, then per block the k eigenpairs first in ``which`` order."""
     which = kwargs.get('which', 'LM')
-    S = torch.zeros(sizes[0], dtype=data.real.dtype if data.is_complex() else data.dtype, device=data.device)
-    U = torch.zeros(sizes[1], dtype=data.dtype, device=data.device)
+    full, u0, s0 = [], 0, 0
     for sl, D, slU, DU, slS in meta:
+        n = D[0]
+        full.append((sl, (n, n), (u0, u0 + n * n), (n, n), (s0, s0 + n)))
+        u0, s0 = u0 + n * n, s0 + n
+    Sf, Uf = F.Eigh.apply(data, tuple(full), (s0, u0))
+    S = torch.zeros(sizes[0], dtype=Sf.dtype, device=data.device)
+    U = torch.zeros(sizes[1], dtype=data.dtype, device=data.device)
+    for (sl, D, slU, DU, slS), (_, _, fU, _, fS) in zip(meta, full):
         k = slS[1] - slS[0]
         n = D[0]
-        m1 = (((0, n * n), (n, n), (0, n * n), (n, n), (0, n)),)
-        s, u = F.Eigh.apply(data[_sl(sl)].contiguous(), m1, (n, n * n))
-        u = u.reshape(n, n)
+        s = Sf[_sl(fS)]
         o = eigs_which(s, which)[:k]
         S[_sl(slS)] = s[o]
-        U[_sl(slU)].reshape(DU)[:] = u[:, o]
+        U[_sl(slU)].reshape(DU)[:] = Uf[_sl(fU)].reshape(n, n)[:, o]
     return S, U
 
 
