@@ -204,7 +204,7 @@ int yastn_hip_splitk_reduce(void* const* outs, const void* partials, int dtype, 
  * The trailing-column updates and the final multiplication by Q run through yastn_hip_grouped_gemm
  * with plans built by the host. */
 #define YH_QR_WORDS 8
-#define YH_QR_NB 16
+#define YH_QR_NB 32
 int yastn_hip_check_qr_plan(const int64_t* h_qdesc, const int64_t* h_blocks, const int32_t* h_ids, int64_t n,
                             int dtype, int64_t work_numel, int64_t lds_bytes);
 /* Factor panel `step` (columns step*YH_QR_NB ...) of every listed block: staged in LDS when

@@ -91,7 +91,7 @@ def svd_adjoint_core(descs, J, T, X, Y, S, gS, gU, gVh, eps, has_gS, has_gU, has
             T[tt:tt + k * n] = t.ravel()
 
 
-QR_NB = 16
+QR_NB = 32
 
 
 def _colmajor(buf, off, rows, cols):

@@ -29,7 +29,7 @@ YH_SVD_WORDS = 8
 YH_SVD_MAX_Q = 2048
 YH_QR_WORDS = 8
 YH_SPLITK_WORDS = 8
-YH_QR_NB = 16
+YH_QR_NB = 32
 YH_QROP_WORDS = 3
 YH_QRADJ_WORDS = 8
 YH_SVD_LDS_BYTES = 131072

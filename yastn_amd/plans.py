@@ -782,9 +782,10 @@ class SvdPlan:
         NB = L.YH_QR_NB
         self.h_qdesc = np.ascontiguousarray(qd)
         pmax = max(int(qd[b, 0]) for b in pre)
-        # panel staged in LDS, or 0: factored in place in the workspace
+        # panel staged in LDS, or 0: factored in place in the workspace (the panel kernel's static LDS,
+        # the T matrices, takes ~17.5 KiB at 32 columns)
         self.qr_lds = pmax * NB * itemsize
-        if self.qr_lds > L.device_lds_per_block() - 8 * 1024:
+        if self.qr_lds > L.device_lds_per_block() - 20 * 1024:
             self.qr_lds = 0
         ids = np.array(pre, dtype=np.int32)
         L.call('yastn_hip_check_qr_plan', self.h_qdesc.ctypes.data, self.h_blocks.ctypes.data, ids.ctypes.data,
@@ -870,7 +871,7 @@ class QrPlan:
         self.h_qblocks = np.ascontiguousarray(np.array(qb, dtype=np.int64).reshape(-1, L.YH_QROP_WORDS))
         pmax = max((r[0] for r in qd), default=1)
         self.lds = pmax * NB * itemsize
-        if self.lds > L.device_lds_per_block() - 8 * 1024:
+        if self.lds > L.device_lds_per_block() - 20 * 1024:  # static LDS of the panel kernel: ~17.5 KiB
             self.lds = 0
         self.native = True
         L.call('yastn_hip_check_qr_op_plan', self.h_qdesc.ctypes.data, self.h_qblocks.ctypes.data, self.nblocks,
