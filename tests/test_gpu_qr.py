@@ -38,7 +38,7 @@ def B():
     return backend_hip
 
 
-# blocks whose 16-column Householder panel exceeds LDS: the whole call factors its panels in place
+# blocks whose Householder panel exceeds LDS: the whole call factors its panels in place
 LONG = [(1300, 40), (700, 90), (90, 700)]
 
 

@@ -426,7 +426,7 @@ def _magnitude_order(S, meta, u_numel):
 
 def qr(data, meta, sizes):
     """Batched Householder QR of every block, diag(R) >= 0 as in the reference (backend_np.py:511-520),
-    with a hand-written adjoint.  The 16-column Householder panels are staged in LDS when they fit and
+    with a hand-written adjoint.  The 32-column Householder panels are staged in LDS when they fit and
     factored in place in the workspace otherwise."""
     plan = P.qr_plan(meta, data.element_size(), data.numel(), sizes, data.device)
     if not _needs_grad(data):

@@ -846,7 +846,7 @@ class QrPlan:
     (k, n))): per block the workspace holds W0 (m x n), the reflectors (m x k), -T per panel, E (m x k)
     and the GEMM scratch; the panel steps and the GEMM plans follow the preconditioning of the SVD
     (trailing updates of W0 after each panel, then E <- H_s E for the panels in reverse).  The
-    16-column Householder panels are staged in LDS when every block's fits (``lds`` > 0), else
+    32-column Householder panels are staged in LDS when every block's fits (``lds`` > 0), else
     factored in place in the workspace (``lds`` = 0)."""
 
     def __init__(self, meta, itemsize, a_numel, sizes, device):
