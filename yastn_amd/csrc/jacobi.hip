@@ -947,6 +947,140 @@ __device__ inline double block_sum(double v, double* s_red, int lane, int wave) 
 // Gram-Schmidt, two passes): first the scaled column itself, then unit vectors e_r if that leaves
 // less than 1e-6 of the start.  An all-zero block thus gets the leading unit vectors, as LAPACK's
 // SVD returns for a zero matrix.  The accumulated side (V) is orthonormal by construction.
+//
+// Two kernels share the work.  complete_project_kernel removes, for every noise column j >= c0 at
+// once (grid.y workgroups per block), its components along the c0 kept vectors (two passes, after
+// scaling the column to unit norm).  complete_kernel then runs the sequence above, whose first
+// attempt for column c only has to project against the noise columns c0 ... c-1 (its acceptance
+// test, against the start's norm, is unchanged: the start was scaled to norm 1); the unit-vector
+// attempts project against every earlier vector as before.
+
+// The kept count c0 of a block (S descending) and the vector layout of its normalised-column side.
+struct CompleteSide {
+  int64_t k, p, cs, xs, c0;
+  bool side_vh;
+};
+
+template <bool CPLX>
+__device__ inline CompleteSide complete_side(const BlockDesc& d, const double* __restrict__ S, int swap,
+                                             double* s_red, int tid, int lane, int wave) {
+  CompleteSide c;
+  c.k = d.m < d.n ? d.m : d.n;
+  // columns of U (length m) or rows of Vh (length n); without preconditioning it is U when m >= n,
+  // with it (swap) the other one
+  c.side_vh = (d.m < d.n) != (swap != 0);
+  c.p = c.side_vh ? d.n : d.m;
+  c.cs = c.side_vh ? d.n : 1;  // vector v, element x, at base[v * cs + x * xs]
+  c.xs = c.side_vh ? 1 : c.k;
+  double f = 0.0;
+  for (int64_t j = tid; j < c.k; j += kThreads) f += S[d.s_off + j] * S[d.s_off + j];
+  const double thr2 = 4.0 * DBL_EPSILON * DBL_EPSILON * block_sum(f, s_red, lane, wave);
+  double cnt = 0.0;
+  for (int64_t j = tid; j < c.k; j += kThreads) cnt += (S[d.s_off + j] * S[d.s_off + j] > thr2) ? 1.0 : 0.0;
+  c.c0 = (int64_t)block_sum(cnt, s_red, lane, wave);
+  return c;
+}
+
+// v_c -= sum_{l in [lb, c)} <v_l, v_c> v_l, twice (classical Gram-Schmidt, two passes), by the whole
+// workgroup.  s_cr / s_ci hold the coefficients (indexed by l), s_pr / s_pim the partial sums.
+template <bool CPLX>
+__device__ inline void cgs2(typename Num<CPLX>::T* __restrict__ base, typename Num<CPLX>::T* __restrict__ vc,
+                            int64_t lb, int64_t c, int64_t p, int64_t cs, int64_t xs, double* s_cr, double* s_ci,
+                            double* s_pr, double* s_pim, int tid, int lane, int wave) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  for (int pass = 0; pass < 2; ++pass) {
+    // h_l = <v_l, v_c>: a wave takes four l at a time, so their loads are in flight together
+    for (int64_t l0 = lb + 4 * wave; l0 < c; l0 += 4 * kWaves) {
+      double hr[4] = {0.0, 0.0, 0.0, 0.0}, hi[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int64_t x = lane; x < p; x += 64) {
+        const T b = vc[x * xs];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (l0 + u < c) {
+            const T a = base[(l0 + u) * cs + x * xs];
+            hr[u] += N::re(a) * N::re(b) + N::im(a) * N::im(b);  // conj(a) b
+            hi[u] += N::re(a) * N::im(b) - N::im(a) * N::re(b);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double r = wave_sum(hr[u]), i = wave_sum(hi[u]);
+        if (lane == 0 && l0 + u < c) {
+          s_cr[l0 + u] = r;
+          s_ci[l0 + u] = i;
+        }
+      }
+    }
+    __syncthreads();
+    // v_c -= sum_l h_l v_l.  With ng = kThreads / p >= 2 groups of p threads, thread (g, x) sums the
+    // terms l = lb + g mod ng and the partials meet in LDS; otherwise one thread per x does all l.
+    const int64_t ng = p * 2 <= kThreads ? kThreads / p : 1;
+    for (int64_t e = tid; e < ng * p; e += kThreads) {
+      const int64_t g = e / p, x = e - g * p;
+      double ar = 0.0, ai = 0.0;
+#pragma unroll 8
+      for (int64_t l = lb + g; l < c; l += ng) {
+        const T a = base[l * cs + x * xs];
+        ar += s_cr[l] * N::re(a) - s_ci[l] * N::im(a);
+        ai += s_cr[l] * N::im(a) + s_ci[l] * N::re(a);
+      }
+      if (ng > 1) {
+        s_pr[e] = ar;
+        if constexpr (CPLX) s_pim[e] = ai;
+      } else {
+        const T b = vc[x * xs];
+        vc[x * xs] = N::make(N::re(b) - ar, N::im(b) - ai);
+      }
+    }
+    if (ng > 1) {
+      __syncthreads();
+      for (int64_t x = tid; x < p; x += kThreads) {
+        double ar = 0.0, ai = 0.0;
+        for (int64_t g = 0; g < ng; ++g) {
+          ar += s_pr[g * p + x];
+          if constexpr (CPLX) ai += s_pim[g * p + x];
+        }
+        const T b = vc[x * xs];
+        vc[x * xs] = N::make(N::re(b) - ar, N::im(b) - ai);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    complete_project_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ ids,
+                            typename Num<CPLX>::T* __restrict__ U, const double* __restrict__ S,
+                            typename Num<CPLX>::T* __restrict__ Vh, int swap) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  __shared__ double s_red[kWaves];
+  __shared__ double s_cr[YH_SVD_MAX_Q], s_ci[YH_SVD_MAX_Q];
+  __shared__ double s_pr[kThreads], s_pim[CPLX ? kThreads : 1];
+  const BlockDesc d = load_desc(blocks + (int64_t)ids[blockIdx.x] * YH_SVD_WORDS);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const CompleteSide sd = complete_side<CPLX>(d, S, swap, s_red, tid, lane, wave);
+  if (sd.c0 == 0 || sd.c0 == sd.k) return;  // uniform per workgroup
+  T* base = sd.side_vh ? Vh + d.v_off : U + d.u_off;
+  for (int64_t c = sd.c0 + blockIdx.y; c < sd.k; c += gridDim.y) {
+    T* vc = base + c * sd.cs;
+    double n0 = 0.0;
+    for (int64_t x = tid; x < sd.p; x += kThreads) n0 += N::abs2(vc[x * sd.xs]);
+    n0 = block_sum(n0, s_red, lane, wave);
+    if (!(n0 > 0.0)) continue;  // uniform: a zero start stays zero and falls back to unit vectors
+    const double inv = 1.0 / sqrt(n0);
+    for (int64_t x = tid; x < sd.p; x += kThreads) {
+      const T b = vc[x * sd.xs];
+      vc[x * sd.xs] = N::make(N::re(b) * inv, N::im(b) * inv);
+    }
+    __syncthreads();
+    cgs2<CPLX>(base, vc, 0, sd.c0, sd.p, sd.cs, sd.xs, s_cr, s_ci, s_pr, s_pim, tid, lane, wave);
+  }
+}
+
 template <bool CPLX>
 __global__ void __launch_bounds__(kThreads)
     complete_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ ids,
@@ -959,21 +1093,10 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ double s_pr[kThreads], s_pim[CPLX ? kThreads : 1];  // partial sums of the update
   const int bidx = ids[blockIdx.x];
   const BlockDesc d = load_desc(blocks + (int64_t)bidx * YH_SVD_WORDS);
-  const int64_t k = d.m < d.n ? d.m : d.n;
-  // the normalised-column side: columns of U (length m) or rows of Vh (length n); without
-  // preconditioning it is U when m >= n, with it (swap) the other one
-  const bool side_vh = (d.m < d.n) != (swap != 0);
-  const int64_t p = side_vh ? d.n : d.m;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // vector c, element x, at base[c * cs + x * xs]
-  T* base = side_vh ? Vh + d.v_off : U + d.u_off;
-  const int64_t cs = side_vh ? d.n : 1, xs = side_vh ? 1 : k;
-  double f = 0.0;
-  for (int64_t j = tid; j < k; j += kThreads) f += S[d.s_off + j] * S[d.s_off + j];
-  const double thr2 = 4.0 * DBL_EPSILON * DBL_EPSILON * block_sum(f, s_red, lane, wave);
-  double cnt = 0.0;
-  for (int64_t j = tid; j < k; j += kThreads) cnt += (S[d.s_off + j] * S[d.s_off + j] > thr2) ? 1.0 : 0.0;
-  const int64_t c0 = (int64_t)block_sum(cnt, s_red, lane, wave);  // S is descending
+  const CompleteSide sd = complete_side<CPLX>(d, S, swap, s_red, tid, lane, wave);
+  const int64_t k = sd.k, p = sd.p, cs = sd.cs, xs = sd.xs, c0 = sd.c0;
+  T* base = sd.side_vh ? Vh + d.v_off : U + d.u_off;
   if (c0 == 0) {
     // an all-noise (e.g. all-zero) block: the procedure below would accept e_c for every c, so write
     // the leading unit vectors directly instead of k sequential orthogonalisations
@@ -994,64 +1117,12 @@ __global__ void __launch_bounds__(kThreads)
       for (int64_t x = tid; x < p; x += kThreads) n0 += N::abs2(vc[x * xs]);
       n0 = block_sum(n0, s_red, lane, wave);
       if (!(n0 > 0.0)) continue;
-      for (int pass = 0; pass < 2; ++pass) {
-        // h_l = <v_l, v_c>, l < c: a wave takes four l at a time, so their loads are in flight together
-        for (int64_t l0 = 4 * wave; l0 < c; l0 += 4 * kWaves) {
-          double hr[4] = {0.0, 0.0, 0.0, 0.0}, hi[4] = {0.0, 0.0, 0.0, 0.0};
-          for (int64_t x = lane; x < p; x += 64) {
-            const T b = vc[x * xs];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              if (l0 + u < c) {
-                const T a = base[(l0 + u) * cs + x * xs];
-                hr[u] += N::re(a) * N::re(b) + N::im(a) * N::im(b);  // conj(a) b
-                hi[u] += N::re(a) * N::im(b) - N::im(a) * N::re(b);
-              }
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const double r = wave_sum(hr[u]), i = wave_sum(hi[u]);
-            if (lane == 0 && l0 + u < c) {
-              s_cr[l0 + u] = r;
-              s_ci[l0 + u] = i;
-            }
-          }
-        }
-        __syncthreads();
-        // v_c -= sum_l h_l v_l.  With ng = kThreads / p >= 2 groups of p threads, thread (g, x) sums the
-        // terms l = g mod ng and the partials meet in LDS; otherwise one thread per x does all l.
-        const int64_t ng = p * 2 <= kThreads ? kThreads / p : 1;
-        for (int64_t e = tid; e < ng * p; e += kThreads) {
-          const int64_t g = e / p, x = e - g * p;
-          double ar = 0.0, ai = 0.0;
-#pragma unroll 8
-          for (int64_t l = g; l < c; l += ng) {
-            const T a = base[l * cs + x * xs];
-            ar += s_cr[l] * N::re(a) - s_ci[l] * N::im(a);
-            ai += s_cr[l] * N::im(a) + s_ci[l] * N::re(a);
-          }
-          if (ng > 1) {
-            s_pr[e] = ar;
-            if constexpr (CPLX) s_pim[e] = ai;
-          } else {
-            const T b = vc[x * xs];
-            vc[x * xs] = N::make(N::re(b) - ar, N::im(b) - ai);
-          }
-        }
-        if (ng > 1) {
-          __syncthreads();
-          for (int64_t x = tid; x < p; x += kThreads) {
-            double ar = 0.0, ai = 0.0;
-            for (int64_t g = 0; g < ng; ++g) {
-              ar += s_pr[g * p + x];
-              if constexpr (CPLX) ai += s_pim[g * p + x];
-            }
-            const T b = vc[x * xs];
-            vc[x * xs] = N::make(N::re(b) - ar, N::im(b) - ai);
-          }
-        }
-        __syncthreads();
+      if (t == 0) {
+        // the start: scaled to norm 1 and free of the kept vectors' components (complete_project_kernel)
+        n0 = 1.0;
+        cgs2<CPLX>(base, vc, c0, c, p, cs, xs, s_cr, s_ci, s_pr, s_pim, tid, lane, wave);
+      } else {
+        cgs2<CPLX>(base, vc, 0, c, p, cs, xs, s_cr, s_ci, s_pr, s_pim, tid, lane, wave);
       }
       double n1 = 0.0;
       for (int64_t x = tid; x < p; x += kThreads) n1 += N::abs2(vc[x * xs]);
@@ -1074,6 +1145,10 @@ int launch_complete(const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, v
                     hipStream_t stream, int swap = 0) {
   typedef typename Num<CPLX>::T T;
   if (nb == 0) return YH_OK;
+  // workgroups per block of the projection stage, each taking every kCompleteSlices-th noise column
+  constexpr unsigned kCompleteSlices = 32;
+  hipLaunchKernelGGL(complete_project_kernel<CPLX>, dim3((unsigned)nb, kCompleteSlices), dim3(kThreads), 0, stream,
+                     d_blocks, d_ids, (T*)U, (const double*)S, (T*)Vh, swap);
   hipLaunchKernelGGL(complete_kernel<CPLX>, dim3((unsigned)nb), dim3(kThreads), 0, stream, d_blocks, d_ids, (T*)U,
                      (const double*)S, (T*)Vh, swap);
   return yh::hip_status(hipGetLastError(), "svd completion launch");
