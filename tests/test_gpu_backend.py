@@ -192,6 +192,41 @@ def test_svd_random_shapes(B):
             _check_svd(a.ravel(), meta, _n(U), _n(S), _n(V), s_ref)
 
 
+def _sector_sizes(chi, d2):
+    """Sector sizes of the fused chi x D^2 leg: charges q1 + q2 with dims chi(q1) D^2(q2)."""
+    dim = lambda d, q: d.get(abs(q), 0)
+    out = [sum(dim(chi, q1) * dim(d2, q - q1) for q1 in range(-8, 9)) for q in range(-16, 17)]
+    return [n for n in out if n > 0]
+
+
+@pytest.mark.parametrize('case', ['d6_chi64_real', 'd8_chi128_complex'])
+def test_svd_ctmrg_projector_sizes(B, case):
+    """One batched call over all projector blocks of a CTMRG workload (13 real blocks up to 440, or 21
+    complex blocks up to 1134): many panel pairs per round with several workgroups each, next to the
+    LDS-kernel launch on the side stream, the setting in which the workgroups of a pair start at
+    different times."""
+    if case == 'd6_chi64_real':
+        sizes, cplx = _sector_sizes({0: 14, 1: 13, 2: 8, 3: 3, 4: 1}, {0: 12, 1: 8, 2: 4}), False
+    else:
+        sizes, cplx = _sector_sizes({0: 22, 1: 21, 2: 15, 3: 9, 4: 5, 5: 2, 6: 1}, {0: 14, 1: 12, 2: 8, 3: 4, 4: 1}), True
+    rng = np.random.default_rng(12)
+    meta, off = [], 0
+    for n in sizes:
+        meta.append(((off, off + n * n), (n, n), (off, off + n * n), (n, n), (0, 0), (off, off + n * n), (n, n)))
+        off += n * n
+    so = 0
+    for i, n in enumerate(sizes):
+        sl, D, slU, DU, _, slV, DV = meta[i]
+        meta[i] = (sl, D, slU, DU, (so, so + n), slV, DV)
+        so += n
+    meta = tuple(meta)
+    a = rng.standard_normal(off) + (1j * rng.standard_normal(off) if cplx else 0)
+    print('svd projector sizes', case, sizes, flush=True)
+    U, S, V = B.svd(_t(a), meta, (off, so, off))
+    s_ref = np.concatenate([np.linalg.svd(a[m[0][0]:m[0][1]].reshape(m[1]), compute_uv=False) for m in meta])
+    _check_svd(a, meta, _n(U), _n(S), _n(V), s_ref)
+
+
 def test_svd_graded_spectrum(B):
     """Singular values spanning 12 decades keep their relative accuracy (one-sided Jacobi)."""
     rng = np.random.default_rng(10)
