@@ -364,6 +364,8 @@ def main():
     ap.add_argument('--unrolled-steps', type=int, default=20)
     ap.add_argument('--unrolled-plain-steps', type=int, default=1,
                     help='CTMRG steps of the short batched-vs-plain comparison (0 skips the plain path)')
+    ap.add_argument('--unrolled-no-checkpoint', action='store_true',
+                    help='also run the unrolled workload once without checkpointing (peak memory comparison)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU sharding of "
@@ -388,7 +390,8 @@ def main():
         from yastn_amd import ctmrg
         print('bench: unrolled ctmrg', flush=True)
         out['ctmrg_unrolled'] = ctmrg.bench_unrolled(n_iter=args.unrolled_steps,
-                                                     plain_iter=args.unrolled_plain_steps)
+                                                     plain_iter=args.unrolled_plain_steps,
+                                                     no_checkpoint_memory=args.unrolled_no_checkpoint)
         print(json.dumps(out), flush=True)
         return
     print('bench: heff2', flush=True)
