@@ -156,3 +156,20 @@ def test_unrolled_energy_oracle_vs_plain_and_checkpoint_cpu(D, chi, dtype):
     assert abs(float(l0) - v_np) <= 1e-10 * max(1.0, abs(v_np))
     assert float(l0) == float(l2)
     assert float((g0 - g2).norm()) <= 1e-12 * float(g0.norm())
+
+
+@pytest.mark.parametrize('dtype', ['float64', 'complex128'])
+def test_batched_projector_svds_equal_sequential_cpu(dtype):
+    """The two projector SVDs of a move in one batched call (blocksparse.svd_many) give the same loss and
+    gradient as the reference's one-after-the-other calls."""
+    B = _plain()
+    cp = bs.make_config(B, 'U1', device='cpu')
+    A, env, sz = C.setup(cp, D=3, chi=12, dtype=dtype, seed=4)
+    l0, g0 = C.fwd_bwd(A, env, sz, 12)
+    l1, g1 = C.fwd_bwd(A, env, sz, 12, batch=True)
+    assert abs(float(l0) - float(l1)) <= 1e-13 * max(1.0, abs(float(l0)))
+    assert float((g0 - g1).norm()) <= 1e-12 * float(g0.norm())
+    cn = bs.make_config(NB, 'U1')
+    A, env, sz = C.setup(cn, D=3, chi=12, dtype=dtype, seed=4)
+    v0, v1 = C.loss_fn(A, env, sz, 12), C.loss_fn(A, env, sz, 12, batch=True)
+    assert abs(complex(v0) - complex(v1)) <= 1e-13 * max(1.0, abs(complex(v0)))

@@ -68,3 +68,19 @@ def test_hip_unrolled_energy_checkpointed_vs_plain():
     assert abs(float(lh) - float(lp)) <= 1e-10 * max(1.0, abs(float(lp)))
     rel = float((gh - gp).norm() / gp.norm())
     assert rel <= 1e-8, rel
+
+
+def test_hip_move_batched_projector_svds():
+    """The two projector SVDs of each move in one batched HIP call (blocksparse.svd_many) against the
+    plain path's one-after-the-other SVDs: loss and gradient."""
+    BH, BP = _backends()
+    print('ctmrg move-batched projector svds D=3 chi=24', flush=True)
+    res = []
+    for B, batch in ((BH, True), (BP, False)):
+        cfg = bs.make_config(B, 'U1', device='cuda:0')
+        A, env, sz = C.setup(cfg, D=3, chi=24, seed=3)
+        res.append(C.fwd_bwd(A, env, sz, 24, batch=batch))
+    (lh, gh), (lp, gp) = res
+    assert abs(float(lh) - float(lp)) <= 1e-10 * max(1.0, abs(float(lp)))
+    rel = float((gh - gp).norm() / gp.norm())
+    assert rel <= 1e-8, rel

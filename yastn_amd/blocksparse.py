@@ -1014,6 +1014,57 @@ def svd(a, axes=(0, 1), sU=1, nU=True, Uaxis=-1, Vaxis=0, fix_signs=False, **kwa
     return U.moveaxis(-1, Uaxis), S, V.moveaxis(0, Vaxis)
 
 
+def _cat(datas):
+    """Concatenate flat data buffers (torch tensors or numpy arrays)."""
+    if isinstance(datas[0], np.ndarray):
+        return np.concatenate(datas)
+    import torch
+    return torch.cat(datas)
+
+
+def svd_many(items, fix_signs=False, **kwargs):
+    """Blockwise thin SVDs of several tensors in one backend call.  ``items`` is a list of
+    (a, axes, sU, nU).  Each tensor is merged to its sector matrices as in :func:`svd`; the blocks of all
+    of them are concatenated into one buffer with one meta (offsets shifted), so the backend batches every
+    block of every tensor into the same launches; the factors are split back per tensor.  Returns a list of
+    (U, S, V) as :func:`svd` would for each item (Uaxis = -1, Vaxis = 0)."""
+    parts, datas, metas = [], [], []
+    off = [0, 0, 0, 0]  # data, U, S, V
+    for a, axes, sU, nU in items:
+        out_l, out_r = ((x,) if isinstance(x, int) else tuple(x) for x in axes)
+        out_hl = tuple(a.trans[ax] for ax in out_l)
+        out_hr = tuple(a.trans[ax] for ax in out_r)
+        data, struct, slices, ls_l, ls_r = _merge_to_matrix(a, (out_hl, out_hr))
+        minD = tuple(min(ds) for ds in struct.D)
+        meta, Ust, Usl, Sst, Ssl, Vst, Vsl = _meta_svd(a.config, struct, slices, minD, sU, nU)
+        sh = lambda x, o: (x[0] + o, x[1] + o)
+        metas.extend((sh(sl, off[0]), d, sh(su, off[1]), du, sh(ss, off[2]), sh(sv, off[3]), dv)
+                     for sl, d, su, du, ss, sv, dv in meta)
+        parts.append((a, tuple(off), (Ust, Usl, Sst, Ssl, Vst, Vsl), ls_l, ls_r, out_hl, out_hr, sU))
+        datas.append(data)
+        off = [off[0] + len(data), off[1] + Ust.size, off[2] + Sst.size, off[3] + Vst.size]
+    cfg = items[0][0].config
+    be = cfg.backend
+    meta_all = tuple(metas)
+    Ud, Sd, Vd = be.svd(_cat(datas), meta_all, tuple(off[1:]), diagnostics=kwargs.get('diagnostics', None))
+    if fix_signs:
+        Ud, Vd = be.fix_svd_signs(Ud, Vd, meta_all)
+    out = []
+    for a, o, (Ust, Usl, Sst, Ssl, Vst, Vsl), ls_l, ls_r, out_hl, out_hr, sU in parts:
+        ls_s = _leg_struct_trivial(Sst, axis=0)
+        S = Tensor(a.config, Sst, Ssl, Sd[o[2]:o[2] + Sst.size], hfs=(Fusion(s=(-sU,)), Fusion(s=(sU,))))
+        Us = tuple(a.struct.s[i] for i in out_hl) + (sU,)
+        mu, Ust2, Usl2 = _meta_unmerge_matrix(a.config, Ust, Usl, ls_l, ls_s, Us)
+        U = Tensor(a.config, Ust2, Usl2, _unmerge(a.config, Ud[o[1]:o[1] + Ust.size], mu),
+                   hfs=tuple(a.hfs[i] for i in out_hl) + (Fusion(s=(sU,)),))
+        Vs = (-sU,) + tuple(a.struct.s[i] for i in out_hr)
+        mv, Vst2, Vsl2 = _meta_unmerge_matrix(a.config, Vst, Vsl, ls_s, ls_r, Vs)
+        V = Tensor(a.config, Vst2, Vsl2, _unmerge(a.config, Vd[o[3]:o[3] + Vst.size], mv),
+                   hfs=(Fusion(s=(-sU,)),) + tuple(a.hfs[i] for i in out_hr))
+        out.append((U, S, V))
+    return out
+
+
 def truncation_mask(S, tol=0, tol_block=0, D_block=float('inf'), D_total=float('inf'), truncate_multiplets=False,
                     mask_f=None, **kwargs):
     """Boolean diagonal mask of the kept singular values (reference linalg.py:637-735)."""

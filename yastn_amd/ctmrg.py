@@ -83,16 +83,38 @@ def proj_corners(r0, r1, chi, svd_opts=None):
     return p0, p1
 
 
+def proj_corners_many(pairs, chi, svd_opts=None):
+    """proj_corners for several (r0, r1) pairs, with all their projector SVDs in one batched backend call
+    (blocksparse.svd_many); truncation and the projectors per pair as in proj_corners."""
+    opts = dict(D_total=chi)
+    opts.update(svd_opts or {})
+    rrs = [bs.tensordot(r0, r1, axes=(1, 1)) for r0, r1 in pairs]
+    usv = bs.svd_many([(rr, (0, 1), r0.s[1], True) for rr, (r0, _) in zip(rrs, pairs)], fix_signs=True)
+    out = []
+    for (r0, r1), (u, s, v) in zip(pairs, usv):
+        mask = bs.truncation_mask(s, **opts)
+        u, s, v = bs.apply_mask(mask, u, s, v, axes=(-1, 0, 0))
+        rs = s.rsqrt()
+        p0 = bs.tensordot(r1, (rs @ v).conj(), axes=(0, 1)).unfuse_legs(axes=0)
+        p1 = bs.tensordot(r0, (u @ rs).conj(), axes=(0, 0)).unfuse_legs(axes=0)
+        out.append((p0, p1))
+    return out
+
+
 def _normalize(t):
     return t / t.config.backend.norm(t.data, 'inf')
 
 
-def move_h(env, a, chi, svd_opts=None):
+def move_h(env, a, chi, svd_opts=None, batch=False):
     ctl, ctr, cbr, cbl = corner_tl(env, a), corner_tr(env, a), corner_br(env, a), corner_bl(env, a)
     cor_tt = ctl @ ctr
     cor_bb = cbr @ cbl
-    hrb, hrt = proj_corners(cor_tt, cor_bb.transpose((1, 0)), chi, svd_opts)   # 'rh'
-    hlb, hlt = proj_corners(cor_tt.transpose((1, 0)), cor_bb, chi, svd_opts)   # 'lh'
+    pairs = [(cor_tt, cor_bb.transpose((1, 0))), (cor_tt.transpose((1, 0)), cor_bb)]   # 'rh', 'lh'
+    if batch:
+        (hrb, hrt), (hlb, hlt) = proj_corners_many(pairs, chi, svd_opts)
+    else:
+        hrb, hrt = proj_corners(*pairs[0], chi, svd_opts)
+        hlb, hlt = proj_corners(*pairs[1], chi, svd_opts)
     new = {}
     tmp = env.l @ hlt
     tmp = bs.tensordot(a, tmp, axes=((0, 1), (2, 1)))
@@ -111,12 +133,16 @@ def move_h(env, a, chi, svd_opts=None):
     return Env(**kw)
 
 
-def move_v(env, a, chi, svd_opts=None):
+def move_v(env, a, chi, svd_opts=None, batch=False):
     ctl, ctr, cbr, cbl = corner_tl(env, a), corner_tr(env, a), corner_br(env, a), corner_bl(env, a)
     cor_ll = cbl @ ctl
     cor_rr = ctr @ cbr
-    vtr, vtl = proj_corners(cor_ll, cor_rr.transpose((1, 0)), chi, svd_opts)   # 'tv'
-    vbr, vbl = proj_corners(cor_ll.transpose((1, 0)), cor_rr, chi, svd_opts)   # 'bv'
+    pairs = [(cor_ll, cor_rr.transpose((1, 0))), (cor_ll.transpose((1, 0)), cor_rr)]   # 'tv', 'bv'
+    if batch:
+        (vtr, vtl), (vbr, vbl) = proj_corners_many(pairs, chi, svd_opts)
+    else:
+        vtr, vtl = proj_corners(*pairs[0], chi, svd_opts)
+        vbr, vbl = proj_corners(*pairs[1], chi, svd_opts)
     new = {}
     tmp = bs.tensordot(vtl, env.t, axes=(0, 0))
     tmp = bs.tensordot(tmp, a, axes=((2, 0), (0, 1)))
@@ -135,10 +161,11 @@ def move_v(env, a, chi, svd_opts=None):
     return Env(**kw)
 
 
-def ctm_step(env, a, chi, svd_opts=None):
-    """One 'hv' CTMRG step: 4 projector SVDs (two per move)."""
-    env = move_h(env, a, chi, svd_opts)
-    return move_v(env, a, chi, svd_opts)
+def ctm_step(env, a, chi, svd_opts=None, batch=False):
+    """One 'hv' CTMRG step: 4 projector SVDs (two per move; with ``batch`` the two of a move go out as one
+    batched SVD call, the reference computes them one after the other)."""
+    env = move_h(env, a, chi, svd_opts, batch)
+    return move_v(env, a, chi, svd_opts, batch)
 
 
 def cluster_expectation(env, A, op):
@@ -326,20 +353,20 @@ def setup(config, D=6, chi=64, dtype='float64', seed=0):
     return A, Env(**env), sz
 
 
-def loss_fn(A, env, sz, chi, svd_opts=None):
+def loss_fn(A, env, sz, chi, svd_opts=None, batch=False):
     """Forward of one CTMRG 'hv' step followed by the 2x2-cluster <Sz>."""
     a = double_layer(A)
-    env2 = ctm_step(env, a, chi, svd_opts)
+    env2 = ctm_step(env, a, chi, svd_opts, batch)
     val = cluster_expectation(env2, A, sz)
     return val.real if hasattr(val, 'is_complex') and val.is_complex() else val
 
 
-def fwd_bwd(A, env, sz, chi, svd_opts=None):
+def fwd_bwd(A, env, sz, chi, svd_opts=None, batch=False):
     """One CTMRG step forward + backward w.r.t. the site tensor's data; returns (loss, grad)."""
     import torch
     data = A.data.detach().clone().requires_grad_(True)
     A2 = A._replace(data=data)
-    loss = loss_fn(A2, env, sz, chi, svd_opts)
+    loss = loss_fn(A2, env, sz, chi, svd_opts, batch)
     loss.backward()
     return loss.detach(), data.grad
 
@@ -351,24 +378,29 @@ def bench(steps=3, D=6, chi=64, plain=True, cpu=True):
     from . import backend_hip, backend_plain
     out = {'workload': f'CTMRG hv step fwd+bwd, U(1) iPEPS 1x1, D={D}, chi={chi}, float64; loss = 2x2 cluster <Sz>'}
 
-    def run(backend, device, n):
+    def run(backend, device, n, batch=False):
         cfg = bs.make_config(backend, 'U1', device=device)
         A, env, sz = setup(cfg, D=D, chi=chi, seed=0)
-        fwd_bwd(A, env, sz, chi)  # warm-up (plans)
+        fwd_bwd(A, env, sz, chi, batch=batch)  # warm-up (plans)
         sync = torch.cuda.synchronize if device != 'cpu' else (lambda: None)
         sync()
         t0 = time.perf_counter()
         for _ in range(n):
-            loss, g = fwd_bwd(A, env, sz, chi)
+            loss, g = fwd_bwd(A, env, sz, chi, batch=batch)
         sync()
         return (time.perf_counter() - t0) / n, float(loss), g
     tb, lb, gb = run(backend_hip, 'cuda:0', steps)
     out['s_per_step_batched'] = tb
     out['loss'] = lb
+    # the two projector SVDs of each move in one batched call (the reference calls them one by one)
+    tm, lm, gm = run(backend_hip, 'cuda:0', steps, batch=True)
+    out['move_batched_svds'] = {'s_per_step': tm, 'loss_diff': abs(lm - lb),
+                                'grad_rel_diff': float((gm - gb).norm() / gb.norm())}
     if plain:
         tp, lp, gp = run(backend_plain, 'cuda:0', max(1, steps // 2))
         out['s_per_step_plain_gpu'] = tp
         out['speedup_vs_plain_gpu'] = tp / tb
+        out['move_batched_svds']['speedup_vs_plain_gpu'] = tp / tm
         out['grad_rel_diff_vs_plain'] = float((gb - gp).norm() / gp.norm())
     if cpu:
         import os
@@ -376,6 +408,7 @@ def bench(steps=3, D=6, chi=64, plain=True, cpu=True):
         out['s_per_step_host_torch'] = tc
         out['host_threads'] = torch.get_num_threads()
         out['speedup_vs_host'] = tc / tb
+        out['move_batched_svds']['speedup_vs_host'] = tc / tm
     return out
 
 
