@@ -240,7 +240,7 @@ def bond_energy(env, A, H):
     return e_h + e_v
 
 
-def ctm_iterations(A, env, chi, n_iter, segment=0, svd_opts=None, on_step=None):
+def ctm_iterations(A, env, chi, n_iter, segment=0, svd_opts=None, on_step=None, batch=False):
     """``n_iter`` 'hv' steps from ``env``.  With ``segment`` > 0 and gradients enabled, each run of
     ``segment`` steps is a non-reentrant checkpoint region (the reference's checkpoint_move,
     _env_ctm.py:556-589): its environments are recomputed in the backward pass instead of kept."""
@@ -257,7 +257,7 @@ def ctm_iterations(A, env, chi, n_iter, segment=0, svd_opts=None, on_step=None):
                 e = Env(**{n: getattr(env0, n)._replace(data=d) for n, d in zip(Env.NAMES, edata)})
                 aa = double_layer(A._replace(data=adata))
                 for _ in range(k):
-                    e = ctm_step(e, aa, chi, svd_opts)
+                    e = ctm_step(e, aa, chi, svd_opts, batch)
                     if on_step is not None:
                         on_step()
                 held[:] = [e]
@@ -267,24 +267,24 @@ def ctm_iterations(A, env, chi, n_iter, segment=0, svd_opts=None, on_step=None):
         else:
             a = double_layer(A) if a is None else a
             for _ in range(k):
-                env = ctm_step(env, a, chi, svd_opts)
+                env = ctm_step(env, a, chi, svd_opts, batch)
                 if on_step is not None:
                     on_step()
         done += k
     return env
 
 
-def energy_loss_fn(A, env, H, chi, n_iter, segment=0, svd_opts=None, on_step=None):
+def energy_loss_fn(A, env, H, chi, n_iter, segment=0, svd_opts=None, on_step=None, batch=False):
     """``n_iter`` CTMRG steps followed by the nearest-neighbour bond energy per site (real part)."""
-    env2 = ctm_iterations(A, env, chi, n_iter, segment, svd_opts, on_step)
+    env2 = ctm_iterations(A, env, chi, n_iter, segment, svd_opts, on_step, batch)
     val = bond_energy(env2, A, H)
     return val.real if hasattr(val, 'is_complex') and val.is_complex() else val
 
 
-def energy_fwd_bwd(A, env, H, chi, n_iter, segment=0, svd_opts=None, on_step=None):
+def energy_fwd_bwd(A, env, H, chi, n_iter, segment=0, svd_opts=None, on_step=None, batch=False):
     """Unrolled CTMRG + energy, forward and backward w.r.t. the site tensor's data; returns (loss, grad)."""
     data = A.data.detach().clone().requires_grad_(True)
-    loss = energy_loss_fn(A._replace(data=data), env, H, chi, n_iter, segment, svd_opts, on_step)
+    loss = energy_loss_fn(A._replace(data=data), env, H, chi, n_iter, segment, svd_opts, on_step, batch)
     loss.backward()
     return loss.detach(), data.grad
 
@@ -429,7 +429,7 @@ def bench_unrolled(D=8, chi=128, dtype='complex128', n_iter=20, segment=4, repea
         if count[0] % 4 == 0:
             print(f'  ctm steps run: {count[0]}', flush=True)
 
-    def run(backend, n, nit, seg):
+    def run(backend, n, nit, seg, batch=False):
         cfg = bs.make_config(backend, 'U1', device='cuda:0')
         A, env, _ = setup(cfg, D=D, chi=chi, dtype=dtype, seed=0)
         H = heisenberg_bond(cfg)
@@ -439,7 +439,7 @@ def bench_unrolled(D=8, chi=128, dtype='complex128', n_iter=20, segment=4, repea
             torch.cuda.reset_peak_memory_stats()
             base = torch.cuda.memory_allocated()
             t0 = time.perf_counter()
-            loss, g = energy_fwd_bwd(A, env, H, chi, nit, seg, on_step=tick)
+            loss, g = energy_fwd_bwd(A, env, H, chi, nit, seg, on_step=tick, batch=batch)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             print(f'  {backend.__name__.split(".")[-1]} {nit} steps segment {seg}: {dt:.3f} s', flush=True)
@@ -457,7 +457,11 @@ def bench_unrolled(D=8, chi=128, dtype='complex128', n_iter=20, segment=4, repea
         seg = min(segment, plain_iter)
         ts, ls, gs, _ = run(backend_hip, repeats, plain_iter, seg)
         tp, lp, gp, _ = run(backend_plain, 0, plain_iter, seg)
+        tm, lm, gm, _ = run(backend_hip, repeats, plain_iter, seg, batch=True)
         out['short'] = {'ctm_steps': plain_iter, 'segment': seg, 's_batched': ts, 's_plain_gpu': tp,
                         'speedup_vs_plain_gpu': tp / ts, 'energy_diff': abs(ls - lp),
-                        'grad_rel_diff_vs_plain': float((gs - gp).norm() / gp.norm())}
+                        'grad_rel_diff_vs_plain': float((gs - gp).norm() / gp.norm()),
+                        'move_batched_svds': {'s': tm, 'speedup_vs_plain_gpu': tp / tm,
+                                              'energy_diff': abs(lm - lp),
+                                              'grad_rel_diff_vs_plain': float((gm - gp).norm() / gp.norm())}}
     return out
