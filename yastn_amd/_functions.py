@@ -8,6 +8,7 @@ import os
 
 import numpy as np
 import torch
+from torch._functorch.utils import unwrap_dead_wrappers as _unwrap_dead_wrappers
 
 from . import _lib as L
 from . import plans as P
@@ -128,7 +129,20 @@ def run_gather_same(entry, plan, dst, src):
     return dst
 
 
-class TransposeAndMerge(torch.autograd.Function):
+class _Fn(torch.autograd.Function):
+    """autograd.Function whose ``apply`` skips torch's per-call ``inspect.signature`` binding of default
+    arguments (~30 us a call, ~200 calls per CTMRG step): every call in this package passes all forward
+    arguments positionally, so the binding is the identity.  Under a torch.func transform, or with keyword
+    arguments, torch's own path runs."""
+
+    @classmethod
+    def apply(cls, *args, **kwargs):
+        if kwargs or torch._C._are_functorch_transforms_active():
+            return super().apply(*args, **kwargs)
+        return super(torch.autograd.Function, cls).apply(*_unwrap_dead_wrappers(args))
+
+
+class TransposeAndMerge(_Fn):
     """A1 / A6 (reference _backend_torch_backwards.py:338-392)."""
 
     @staticmethod
@@ -152,7 +166,7 @@ class TransposeAndMerge(torch.autograd.Function):
         return run_gather('yastn_hip_transpose_and_merge_adjoint', plan, out, g), None, None, None, None
 
 
-class Unmerge(torch.autograd.Function):
+class Unmerge(_Fn):
     """A2 / A8 (reference _backend_torch_backwards.py:395-426)."""
 
     @staticmethod
@@ -176,7 +190,7 @@ class Unmerge(torch.autograd.Function):
         return run_gather_same('yastn_hip_unmerge_adjoint', plan, out, g), None
 
 
-class Transpose(torch.autograd.Function):
+class Transpose(_Fn):
     """backend.transpose (reference _backend_torch_backwards.py:313-335)."""
 
     @staticmethod
@@ -244,7 +258,7 @@ def _mask_gather(Adata, mh, key, meta, Dsize, axis, embed):
     return run_gather('yastn_hip_embed_mask' if embed else 'yastn_hip_apply_mask', plan, out, Adata)
 
 
-class ApplyMask(torch.autograd.Function):
+class ApplyMask(_Fn):
     """A12 apply_mask, adjoint = embed (reference _backend_torch_backwards.py:251-279)."""
 
     @staticmethod
@@ -267,7 +281,7 @@ class ApplyMask(torch.autograd.Function):
         return out, None, None, None, None, None
 
 
-class EmbedMask(torch.autograd.Function):
+class EmbedMask(_Fn):
     """A12 embed_mask, adjoint = apply (reference _backend_torch_backwards.py:282-310)."""
 
     @staticmethod
@@ -323,7 +337,7 @@ def run_gemm(plan, outs, ins, dtype, tune=True):
     run()
 
 
-class Dot(torch.autograd.Function):
+class Dot(_Fn):
     """A3 / A7 (reference _backend_torch_backwards.py:98-138)."""
 
     @staticmethod
@@ -546,7 +560,7 @@ def svd_backward(U, S, V, gU, gS, gV, meta, data_size):
     return gA
 
 
-class Svd(torch.autograd.Function):
+class Svd(_Fn):
     """A4 / A9 (reference _backend_torch_backwards.py:26-66, linalg/torch_svd_gesdd.py)."""
 
     @staticmethod
@@ -611,7 +625,7 @@ class _EighAdjointPlans:
 _EIGH_ADJ = P._Cache()
 
 
-class Eigh(torch.autograd.Function):
+class Eigh(_Fn):
     """A5 / A10: Hermitian eigendecomposition with the (unbroadened) torch.linalg.eigh adjoint
     that the reference backend uses (backend_torch.py:378)."""
 
@@ -755,7 +769,7 @@ def qr_backward(data, Q, R, gQ, gR, meta, sizes):
     return gA
 
 
-class Qr(torch.autograd.Function):
+class Qr(_Fn):
     """Batched QR with its adjoint (the reference differentiates torch.linalg.qr through torch's autograd,
     backend_torch.py:475-484)."""
 
@@ -780,7 +794,7 @@ class Qr(torch.autograd.Function):
         return qr_backward(data, Q, R, gQ, gR, ctx.meta, ctx.sizes), None, None, None
 
 
-class PermutePack(torch.autograd.Function):
+class PermutePack(_Fn):
     """Blocks of a flat buffer permuted into packed matrices (one box gather); the adjoint is the
     inverse gather into a zeroed buffer of the source's size."""
 
@@ -803,7 +817,7 @@ class PermutePack(torch.autograd.Function):
         return run_gather('yastn_hip_block_gather', ctx.bwd, out, g), None, None, None
 
 
-class TdsCore(torch.autograd.Function):
+class TdsCore(_Fn):
     """Output blocks as sums of products of packed matrices (one grouped GEMM); adjoint: two grouped
     GEMMs accumulating into the packed gradients."""
 
