@@ -439,3 +439,25 @@ def test_dot_split_k_long_contractions(B):
         got = _n(B.dot(_t(A), _t(Bm), meta, co))
         ref = O.dot(A, Bm, meta, co)
         assert _rel(got, ref) <= 1e-12
+
+
+@pytest.mark.parametrize('n', [40, 300])
+def test_svd_nonconvergence_raises(B, n):
+    """A block that needs more sweeps than the cap raises (the reference raises LAPACK's error,
+    backend_np.py:397-416).  The cap is lowered from the host to one sweep, so every kernel still exits
+    normally: the LDS kernel (n = 40) stops at its sweep bound, the panel path (n = 300) after one host
+    sweep call; the finalize kernels mark the blocks and the host check raises."""
+    from yastn_amd import _functions as F, _lib as L
+    rng = np.random.default_rng(13)
+    a = rng.standard_normal(n * n)
+    meta = (((0, n * n), (n, n), (0, n * n), (n, n), (0, n), (0, n * n), (n, n)),)
+    print('svd non-convergence', n, flush=True)
+    cap = F.MAX_SWEEPS
+    F.MAX_SWEEPS = 1
+    try:
+        with pytest.raises(L.HipBackendError, match='did not converge'):
+            B.svd(_t(a), meta, (n * n, n, n * n))
+    finally:
+        F.MAX_SWEEPS = cap
+    U, S, V = B.svd(_t(a), meta, (n * n, n, n * n))  # and the normal cap converges
+    _check_svd(a, meta, _n(U), _n(S), _n(V), np.linalg.svd(a.reshape(n, n), compute_uv=False))
