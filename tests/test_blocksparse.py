@@ -191,3 +191,21 @@ def test_fix_signs_scenario():
     U, S, V = bs.svd(USV, axes=(0, 1), fix_signs=True)
     same_tensor(U, ts['U'], atol=1e-12)
     compare_calls(rec, 'fix_signs_kat')
+
+
+def test_svd_many_equals_separate_svds():
+    """svd_many (the blocks of several tensors in one backend svd call, offsets shifted) returns, per
+    tensor, the factors of a separate svd call: same structures, same values (numpy oracle backend)."""
+    cfg = bs.make_config(NB, 'U1')
+    L = lambda s, t, D: bs.leg(cfg, s, t, D)
+    legs_a = [L(1, (-1, 0, 1), (2, 3, 2)), L(1, (0, 1), (1, 1)), L(-1, (-1, 0, 1, 2), (3, 4, 3, 1))]
+    legs_b = [L(-1, (0, 1, 2), (4, 2, 3)), L(1, (0, 1, 2), (5, 1, 2))]
+    a = bs.rand(cfg, legs_a, n=0, seed=3)
+    b = bs.rand(cfg, legs_b, n=0, dtype='complex128', seed=4)
+    items = [(a, ((0, 1), 2), 1, True), (b, (0, 1), -1, True), (a.transpose((2, 0, 1)), (0, (1, 2)), 1, False)]
+    many = bs.svd_many(items, fix_signs=True)
+    for (t, axes, sU, nU), (U, S, V) in zip(items, many):
+        U0, S0, V0 = bs.svd(t, axes=axes, sU=sU, nU=nU, fix_signs=True)
+        for x, y in ((U, U0), (S, S0), (V, V0)):
+            assert x.struct == y.struct and x.slices == y.slices and x.hfs == y.hfs
+            assert np.array_equal(np.asarray(x.data), np.asarray(y.data))

@@ -1023,11 +1023,22 @@ def _cat(datas):
 
 
 def svd_many(items, fix_signs=False, **kwargs):
-    """Blockwise thin SVDs of several tensors in one backend call.  ``items`` is a list of
+    """Blockwise thin SVDs of several tensors in one backend call per data type.  ``items`` is a list of
     (a, axes, sU, nU).  Each tensor is merged to its sector matrices as in :func:`svd`; the blocks of all
-    of them are concatenated into one buffer with one meta (offsets shifted), so the backend batches every
-    block of every tensor into the same launches; the factors are split back per tensor.  Returns a list of
-    (U, S, V) as :func:`svd` would for each item (Uaxis = -1, Vaxis = 0)."""
+    tensors of one data type are concatenated into one buffer with one meta (offsets shifted), so the
+    backend batches every block of them into the same launches; the factors are split back per tensor.
+    Returns a list of (U, S, V) as :func:`svd` would for each item (Uaxis = -1, Vaxis = 0)."""
+    groups = {}
+    for i, it in enumerate(items):
+        groups.setdefault(str(it[0].data.dtype), []).append(i)
+    out = [None] * len(items)
+    for idx in groups.values():
+        for i, r in zip(idx, _svd_many_same_dtype([items[i] for i in idx], fix_signs, **kwargs)):
+            out[i] = r
+    return out
+
+
+def _svd_many_same_dtype(items, fix_signs=False, **kwargs):
     parts, datas, metas = [], [], []
     off = [0, 0, 0, 0]  # data, U, S, V
     for a, axes, sU, nU in items:
