@@ -234,3 +234,76 @@ def test_split_k_plan_reproduces_long_contractions():
     for (slc, Dc, sla, Da, slb, Db) in meta:
         ref = A[sla[0]:sla[1]].reshape(Da) @ B[slb[0]:slb[1]].reshape(Db)
         np.testing.assert_allclose(out[slc[0]:slc[1]].reshape(Dc), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_plan_cache_bounded_for_fresh_equal_metas():
+    """Metas rebuilt on every call (equal content, new objects) hit the content map and keep both maps
+    bounded; the LRU evicts one entry at a time instead of clearing the cache."""
+    cache = P._Cache(maxsize=8)
+    builds = []
+
+    def meta():
+        return tuple((i, (i, i + 1)) for i in range(5))
+
+    first = None
+    for _ in range(100):
+        m = meta()
+        v = cache.get(('k', id(m), 3), (m,), lambda: builds.append(1) or object())
+        first = first or v
+        assert v is first
+    assert len(builds) == 1
+    assert len(cache.d) <= 8 and len(cache.by_content) <= 8
+    # distinct structures beyond the bound: the oldest go, the recent ones stay
+    keep = []
+    for i in range(20):
+        m = (i,)
+        keep.append(m)
+        cache.get(('k', id(m)), (m,), lambda i=i: ('plan', i))
+    assert len(cache.d) == 8 and len(cache.by_content) <= 8
+    assert cache.get(('k', id(keep[-1])), (keep[-1],), lambda: 'rebuilt') == ('plan', 19)
+
+
+def test_partial_svd_meta_cache_is_content_keyed():
+    from yastn_amd import backend_hip as B
+    m1 = (((0, 12), (3, 4), (0, 6), (3, 2), (0, 2), (0, 8), (2, 4)),)
+    m2 = tuple(tuple(x) for x in m1)  # equal content, new object
+    a = B._full_meta_of_partial(m1, CPU)
+    n0 = len(B._PARTIAL.by_content)
+    b = B._full_meta_of_partial(m2, CPU)
+    assert a is b and len(B._PARTIAL.by_content) == n0
+    full, sizes, (iu, is_, iv) = a
+    assert sizes == (9, 3, 12) and iu.device == CPU
+    np.testing.assert_array_equal(is_.numpy(), [0, 1])
+    np.testing.assert_array_equal(iu.numpy(), [0, 1, 3, 4, 6, 7])
+
+
+class _Twice(F._Fn):
+    @staticmethod
+    def forward(x, scale=2.0):
+        return x * scale
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        ctx.scale = inputs[1] if len(inputs) > 1 else 2.0
+
+    @staticmethod
+    def backward(ctx, g):
+        return g * ctx.scale, None
+
+
+@pytest.mark.parametrize('fast', [True, False])
+def test_fast_apply_matches_public_apply(fast, monkeypatch):
+    """_Fn.apply's fast path (torch internals) and torch's public Function.apply give the same outputs
+    and gradients, with and without defaulted arguments."""
+    monkeypatch.setattr(F, 'FAST_APPLY', fast and F._UNWRAP_DEAD is not None)
+    x = torch.arange(5.0, dtype=torch.float64, requires_grad=True)
+    y = _Twice.apply(x, 3.0)
+    y.sum().backward()
+    np.testing.assert_array_equal(y.detach().numpy(), 3.0 * np.arange(5.0))
+    np.testing.assert_array_equal(x.grad.numpy(), np.full(5, 3.0))
+    x.grad = None
+    z = _Twice.apply(x)
+    (z * z).sum().backward()
+    np.testing.assert_array_equal(x.grad.numpy(), 8.0 * np.arange(5.0))
+    gx, = torch.func.vjp(lambda t: _Twice.apply(t, 3.0), x.detach())[1](torch.ones(5, dtype=torch.float64))
+    np.testing.assert_array_equal(gx.numpy(), np.full(5, 3.0))

@@ -8,7 +8,6 @@ import os
 
 import numpy as np
 import torch
-from torch._functorch.utils import unwrap_dead_wrappers as _unwrap_dead_wrappers
 
 from . import _lib as L
 from . import plans as P
@@ -129,17 +128,41 @@ def run_gather_same(entry, plan, dst, src):
     return dst
 
 
+def _fast_apply_available():
+    """The fast path of _Fn.apply relies on torch internals (torch 2.10): the C base class of
+    autograd.Function, torch._C._are_functorch_transforms_active and
+    torch._functorch.utils.unwrap_dead_wrappers.  Returns that helper, or None (then torch's public
+    Function.apply runs on every call) when any of them is missing or has another shape."""
+    try:
+        from torch._functorch.utils import unwrap_dead_wrappers as unwrap
+        import inspect
+        if len(inspect.signature(unwrap).parameters) != 1:
+            return None
+        if not callable(getattr(torch._C, '_are_functorch_transforms_active', None)):
+            return None
+        base = getattr(torch._C, '_FunctionBase', None)
+        if base is None or base not in torch.autograd.Function.__mro__ or not hasattr(base, 'apply'):
+            return None
+        return unwrap
+    except Exception:
+        return None
+
+
+_UNWRAP_DEAD = _fast_apply_available()
+FAST_APPLY = _UNWRAP_DEAD is not None and os.environ.get('YASTN_HIP_FAST_APPLY', '1') != '0'
+
+
 class _Fn(torch.autograd.Function):
     """autograd.Function whose ``apply`` skips torch's per-call ``inspect.signature`` binding of default
     arguments (~30 us a call, ~200 calls per CTMRG step): every call in this package passes all forward
-    arguments positionally, so the binding is the identity.  Under a torch.func transform, or with keyword
-    arguments, torch's own path runs."""
+    arguments positionally, so the binding is the identity.  Under a torch.func transform, with keyword
+    arguments, or when the torch internals it needs are absent (FAST_APPLY false), torch's own path runs."""
 
     @classmethod
     def apply(cls, *args, **kwargs):
-        if kwargs or torch._C._are_functorch_transforms_active():
+        if not FAST_APPLY or kwargs or torch._C._are_functorch_transforms_active():
             return super().apply(*args, **kwargs)
-        return super(torch.autograd.Function, cls).apply(*_unwrap_dead_wrappers(args))
+        return super(torch.autograd.Function, cls).apply(*_UNWRAP_DEAD(args))
 
 
 class TransposeAndMerge(_Fn):
@@ -216,16 +239,12 @@ class Transpose(_Fn):
         return run_gather_same('yastn_hip_transpose', plan, out, g), None, None
 
 
-_INV_T = {}
+_INV_T = P._Cache(maxsize=1024)
 
 
 def _inverse_transpose_meta(meta):
-    hit = _INV_T.get(id(meta))
-    if hit is not None and hit[0] is meta:
-        return hit[1]
-    inv = tuple((slo, Do, sln, Dn) for sln, Dn, slo, Do in meta)
-    _INV_T[id(meta)] = (meta, inv)
-    return inv
+    return _INV_T.get(('inv_t', id(meta)), (meta,),
+                      lambda: tuple((slo, Do, sln, Dn) for sln, Dn, slo, Do in meta))
 
 
 def _mask_host(mask):
@@ -240,16 +259,12 @@ def _mask_key(mh):
     return tuple((k, v.tobytes()) for k, v in sorted(mh.items()))
 
 
-_SWAP = {}
+_SWAP = P._Cache(maxsize=1024)
 
 
 def _swapped_mask_meta(meta):
-    hit = _SWAP.get(id(meta))
-    if hit is not None and hit[0] is meta:
-        return hit[1]
-    sw = tuple((sla, Da, sln, Dn, tm) for sln, Dn, sla, Da, tm in meta)
-    _SWAP[id(meta)] = (meta, sw)
-    return sw
+    return _SWAP.get(('swap', id(meta)), (meta,),
+                     lambda: tuple((sla, Da, sln, Dn, tm) for sln, Dn, sla, Da, tm in meta))
 
 
 def _mask_gather(Adata, mh, key, meta, Dsize, axis, embed):
@@ -424,52 +439,55 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
         else:
             L.call('yastn_hip_eigh', ptr(out_s), ptr(out_a), ptr(data), dt, plan.d_blocks.data_ptr(), plan.n_lds,
                    plan.d_order.data_ptr(), plan.n_lds, plan.lds_bytes, ptr(work), ptr(info), MAX_SWEEPS, lds_st)
-    sweeps = 0
-    if plan.n_big:
-        mu = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
-        floors = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
-        flags = torch.zeros(max(plan.njobs, 1), dtype=torch.int32, device=data.device)
-        L.call('yastn_hip_svd_panel_init', ptr(data), ptr(work), ptr(mu), ptr(floors), dt, mode,
-               plan.d_blocks.data_ptr(), plan.d_big.data_ptr(), plan.n_big, st)
-        if plan.n_pre:
-            # W0 = Q R (Householder, one panel kernel and three grouped GEMMs per step), then X = R^H
-            for step, (d_ids, n, trail) in enumerate(plan.qr_steps):
-                L.call('yastn_hip_svd_qr_panel', ptr(work), dt, plan.d_qdesc.data_ptr(), d_ids.data_ptr(), n, step,
-                       plan.qr_lds, st)
-                if trail is not None:
-                    for gp in trail:
-                        run_gemm(gp, (work,), (work,), dt, tune=False)
-            L.call('yastn_hip_svd_precond_init', ptr(work), dt, plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(),
-                   plan.n_pre, st)
-        part = torch.empty(max(plan.part_numel, 1), dtype=torch.float64, device=data.device)
-        converged = False
-        while sweeps < MAX_SWEEPS:
-            L.call('yastn_hip_svd_panel_sweep', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
-                   plan.d_jobs.data_ptr(), plan.h_rounds.ctypes.data, len(plan.rounds), plan.panel_width,
-                   plan.maxch_p, plan.panel_slices, ptr(part), part.numel(), ptr(flags), st)
-            sweeps += 1
-            fmin, fmax = torch.stack([flags.min(), flags.max()]).tolist()
-            if fmin < 0:
-                raise RuntimeError("yastn_amd: svd panel round: LDS plan smaller than a panel pair")
-            if fmax == 0:
-                converged = True
-                break
-        done = sweeps if converged else -1
-        if plan.n_plain:
-            L.call('yastn_hip_svd_panel_finalize', ptr(out_a), ptr(out_s), ptr(out_b) if mode == 0 else None,
-                   ptr(work), ptr(mu), dt, mode, plan.d_blocks.data_ptr(), plan.d_plain.data_ptr(), plan.n_plain,
-                   ptr(info), done, st)
-        if plan.n_pre:
-            L.call('yastn_hip_svd_precond_finalize', ptr(out_a), ptr(out_s), ptr(out_b), ptr(work), dt,
-                   plan.d_blocks.data_ptr(), plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(), plan.n_pre, ptr(info),
-                   done, st)
-            # p-side vectors: [V_J; 0] -> Q [V_J; 0], panels in reverse order
-            for h1, h2, h3 in plan.q_apply:
-                run_gemm(h1, (work,), (work, out_a, out_b), dt, tune=False)
-                run_gemm(h2, (work,), (work,), dt, tune=False)
-                run_gemm(h3, (work, out_a, out_b), (work,), dt, tune=False)
-    if joined is not None:
-        joined[0].wait_stream(joined[1])
+    try:
+        sweeps = 0
+        if plan.n_big:
+            mu = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
+            floors = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
+            flags = torch.zeros(max(plan.njobs, 1), dtype=torch.int32, device=data.device)
+            L.call('yastn_hip_svd_panel_init', ptr(data), ptr(work), ptr(mu), ptr(floors), dt, mode,
+                   plan.d_blocks.data_ptr(), plan.d_big.data_ptr(), plan.n_big, st)
+            if plan.n_pre:
+                # W0 = Q R (Householder, one panel kernel and three grouped GEMMs per step), then X = R^H
+                for step, (d_ids, n, trail) in enumerate(plan.qr_steps):
+                    L.call('yastn_hip_svd_qr_panel', ptr(work), dt, plan.d_qdesc.data_ptr(), d_ids.data_ptr(), n, step,
+                           plan.qr_lds, st)
+                    if trail is not None:
+                        for gp in trail:
+                            run_gemm(gp, (work,), (work,), dt, tune=False)
+                L.call('yastn_hip_svd_precond_init', ptr(work), dt, plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(),
+                       plan.n_pre, st)
+            part = torch.empty(max(plan.part_numel, 1), dtype=torch.float64, device=data.device)
+            converged = False
+            while sweeps < MAX_SWEEPS:
+                L.call('yastn_hip_svd_panel_sweep', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
+                       plan.d_jobs.data_ptr(), plan.h_rounds.ctypes.data, len(plan.rounds), plan.panel_width,
+                       plan.maxch_p, plan.panel_slices, ptr(part), part.numel(), ptr(flags), st)
+                sweeps += 1
+                fmin, fmax = torch.stack([flags.min(), flags.max()]).tolist()
+                if fmin < 0:
+                    raise RuntimeError("yastn_amd: svd panel round: LDS plan smaller than a panel pair")
+                if fmax == 0:
+                    converged = True
+                    break
+            done = sweeps if converged else -1
+            if plan.n_plain:
+                L.call('yastn_hip_svd_panel_finalize', ptr(out_a), ptr(out_s), ptr(out_b) if mode == 0 else None,
+                       ptr(work), ptr(mu), dt, mode, plan.d_blocks.data_ptr(), plan.d_plain.data_ptr(), plan.n_plain,
+                       ptr(info), done, st)
+            if plan.n_pre:
+                L.call('yastn_hip_svd_precond_finalize', ptr(out_a), ptr(out_s), ptr(out_b), ptr(work), dt,
+                       plan.d_blocks.data_ptr(), plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(), plan.n_pre, ptr(info),
+                       done, st)
+                # p-side vectors: [V_J; 0] -> Q [V_J; 0], panels in reverse order
+                for h1, h2, h3 in plan.q_apply:
+                    run_gemm(h1, (work,), (work, out_a, out_b), dt, tune=False)
+                    run_gemm(h2, (work,), (work,), dt, tune=False)
+                    run_gemm(h3, (work, out_a, out_b), (work,), dt, tune=False)
+    finally:
+        # the side-stream launch reads data and writes the outputs: joined even when the panel path raises
+        if joined is not None:
+            joined[0].wait_stream(joined[1])
     _check_info(info[:plan.nblocks], 'svd' if mode == 0 else 'eigh')
 
 

@@ -362,35 +362,34 @@ def svds_scipy(data, meta, sizes, thresh=0.1, solver='arpack', **kwargs):
 
 
 def _partial_from_full(data, meta, sizes):
-    full_meta, full_sizes, picks = _full_meta_of_partial(meta)
+    full_meta, full_sizes, picks = _full_meta_of_partial(meta, data.device)
     U, S, V = F.Svd.apply(data, full_meta, full_sizes, False, 1e-12, None)
     iu, is_, iv = picks
-    dev = data.device
-    return U[iu.to(dev)], S[is_.to(dev)], V[iv.to(dev)]
+    return U[iu], S[is_], V[iv]
 
 
-_PARTIAL = {}
+_PARTIAL = P._Cache(maxsize=256)
 
 
-def _full_meta_of_partial(meta):
-    hit = _PARTIAL.get(id(meta))
-    if hit is not None and hit[0] is meta:
-        return hit[1]
-    full, iu, is_, iv = [], [], [], []
-    u0 = s0 = v0 = 0
-    for sl, D, slU, DU, slS, slV, DV in meta:
-        m, n = D
-        kf = min(m, n)
-        k = slS[1] - slS[0]
-        full.append((sl, D, (u0, u0 + m * kf), (m, kf), (s0, s0 + kf), (v0, v0 + kf * n), (kf, n)))
-        iu.append((u0 + np.arange(m)[:, None] * kf + np.arange(k)[None, :]).ravel())
-        is_.append(s0 + np.arange(k))
-        iv.append((v0 + np.arange(k)[:, None] * n + np.arange(n)[None, :]).ravel())
-        u0, s0, v0 = u0 + m * kf, s0 + kf, v0 + kf * n
-    cat = lambda x: torch.from_numpy(np.concatenate(x).astype(np.int64)) if x else torch.zeros(0, dtype=torch.int64)
-    out = (tuple(full), (u0, s0, v0), (cat(iu), cat(is_), cat(iv)))
-    _PARTIAL[id(meta)] = (meta, out)
-    return out
+def _full_meta_of_partial(meta, device):
+    """The full-rank SVD meta of a partial-SVD meta, and device index tensors that pick the k leading
+    triplets of every block out of the full factors.  Cached by meta identity, then content (bounded)."""
+    def build():
+        full, iu, is_, iv = [], [], [], []
+        u0 = s0 = v0 = 0
+        for sl, D, slU, DU, slS, slV, DV in meta:
+            m, n = D
+            kf = min(m, n)
+            k = slS[1] - slS[0]
+            full.append((sl, D, (u0, u0 + m * kf), (m, kf), (s0, s0 + kf), (v0, v0 + kf * n), (kf, n)))
+            iu.append((u0 + np.arange(m)[:, None] * kf + np.arange(k)[None, :]).ravel())
+            is_.append(s0 + np.arange(k))
+            iv.append((v0 + np.arange(k)[:, None] * n + np.arange(n)[None, :]).ravel())
+            u0, s0, v0 = u0 + m * kf, s0 + kf, v0 + kf * n
+        cat = lambda x: (torch.from_numpy(np.concatenate(x).astype(np.int64)) if x
+                         else torch.zeros(0, dtype=torch.int64)).to(device)
+        return (tuple(full), (u0, s0, v0), (cat(iu), cat(is_), cat(iv)))
+    return _PARTIAL.get(('partial', id(meta), device.type, device.index), (meta,), build)
 
 
 def eigh(data, meta=None, sizes=(1, 1), order_by_magnitude=False, ad_decomp_reg=1.0e-12):

@@ -9,6 +9,7 @@ scalar arguments.  Each plan is validated once on the host by the library's
 with an out-of-bounds work list.
 """
 import ctypes
+from collections import OrderedDict
 import os
 import math
 from itertools import product
@@ -22,18 +23,34 @@ from . import _lib as L
 # --------------------------------------------------------------------------- cache
 
 class _Cache:
-    """Plan cache.  Lookup by the identity of the meta objects first (the reference planner returns
-    lru-cached tuples); on a miss, by their content (some metas, e.g. the SVD's, are rebuilt on every
-    call by the reference), so a plan is compiled once per distinct structure."""
+    """Bounded LRU plan cache.  Lookup by the identity of the meta objects first (the reference planner
+    returns lru-cached tuples); on a miss, by their content (some metas, e.g. the SVD's, are rebuilt on
+    every call by the reference), so a plan is compiled once per distinct structure.
+
+    Both maps are LRU-ordered and bounded: an identity entry holds its meta objects strongly (so that
+    their ids cannot be reused while the entry lives), and metas rebuilt on every call would otherwise
+    accumulate one entry per call.  The least recently used entries are evicted one at a time; a plan
+    evicted and rebuilt later gets the same tile choice, which the autotuner keeps per block-size
+    histogram."""
 
     def __init__(self, maxsize=4096):
-        self.d = {}
-        self.by_content = {}
+        self.d = OrderedDict()
+        self.by_content = OrderedDict()
         self.maxsize = maxsize
+
+    def _put(self, od, key, val):
+        od[key] = val
+        od.move_to_end(key)
+        while len(od) > self.maxsize:
+            od.popitem(last=False)
+
+    def __len__(self):
+        return len(self.d)
 
     def get(self, key, refs, build):
         hit = self.d.get(key)
         if hit is not None and all(a is b for a, b in zip(hit[0], refs)):
+            self.d.move_to_end(key)
             return hit[1]
         ckey = (key[0],) + tuple(x for x in key[1:] if not isinstance(x, int) or x not in {id(r) for r in refs})
         try:
@@ -43,15 +60,13 @@ class _Cache:
         if chash is not None:
             hit2 = self.by_content.get(chash)
             if hit2 is not None and hit2[0] == (ckey, refs):
-                self.d[key] = (refs, hit2[1])
+                self.by_content.move_to_end(chash)
+                self._put(self.d, key, (refs, hit2[1]))
                 return hit2[1]
         val = build()
-        if len(self.d) >= self.maxsize:
-            self.d.clear()
-            self.by_content.clear()
-        self.d[key] = (refs, val)
+        self._put(self.d, key, (refs, val))
         if chash is not None:
-            self.by_content[chash] = ((ckey, refs), val)
+            self._put(self.by_content, chash, ((ckey, refs), val))
         return val
 
 
