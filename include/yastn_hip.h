@@ -35,7 +35,7 @@ extern "C" {
 #define YH_F64 0
 #define YH_C128 1
 
-#define YH_ABI_VERSION 8
+#define YH_ABI_VERSION 9
 
 /* ---- box-gather plan (one job per copied or zero-filled box) -------------
  * int64 words per job (YH_JOB_WORDS):
@@ -297,6 +297,27 @@ int yastn_hip_apply_mask(void* newdata, int64_t Dsize, const void* data, int64_t
 int yastn_hip_embed_mask(void* newdata, int64_t Dsize, const void* data, int64_t data_numel, int dtype,
                          const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job, int64_t ntiles,
                          hipStream_t stream);
+
+/* Backend dot_diag(Adata, Bdata, meta, Dsize, axis, a_ndim) (backend_np.py:604-611,
+ *      backend_torch.py:557-564; caller _contractions.py:496): every output block is a dense block of B
+ *      scaled along `axis` by a diagonal block of A, all blocks in one launch.  Job words
+ *      (YH_DIAG_WORDS int64): 0 out_off, 1 a_off, 2 b_off, 3 n (block elements), 4 inner (product of the
+ *      block dimensions after `axis`), 5 D (dimension of `axis`); max_n = largest n.  conj_a = 1 uses
+ *      conj(A), which is the adjoint with respect to B (gB = g * conj(A), same jobs with B := g). */
+#define YH_DIAG_WORDS 8
+int yastn_hip_check_dot_diag_plan(const int64_t* h_jobs, int64_t njobs, const int64_t* h_groups,
+                                  int64_t ngroups, const int64_t* h_job_ids, int64_t nids, int64_t out_numel,
+                                  int64_t a_numel, int64_t b_numel);
+int yastn_hip_dot_diag(void* out, const void* A, const void* B, int dtype, const int64_t* d_jobs, int64_t njobs,
+                       int64_t max_n, int conj_a, hipStream_t stream);
+
+/* Adjoint of dot_diag with respect to the diagonal (the reference differentiates the torch ops of
+ *      backend_torch.py:557-564): gA[a_off + i] = sum of g conj(B) over the elements of index i along
+ *      `axis`, summed over every job that reads the same A block (group words x4: 0 a_off, 1 D,
+ *      2 first index into job_ids, 3 count), in a fixed order (no atomics). */
+int yastn_hip_dot_diag_adjoint_a(void* gA, const void* g, const void* B, int dtype, const int64_t* d_jobs,
+                                 const int64_t* d_groups, int64_t ngroups, const int64_t* d_job_ids, int64_t max_D,
+                                 hipStream_t stream);
 
 /* Backend transpose(data, axes, meta_transpose) (backend_np.py:646-650): per-block permutation. */
 int yastn_hip_transpose(void* newdata, const void* data, int64_t numel, int dtype, const int64_t* d_jobs,

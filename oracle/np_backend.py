@@ -182,11 +182,17 @@ def eigh(data, meta, sizes):
     return S, U
 
 
+def _idx(m):
+    """A mask value: an index list, or a slice (flip_charges embeds with {0: slice(None)},
+    tensor/_single.py:222-225)."""
+    return m if isinstance(m, slice) else np.asarray(m)
+
+
 def apply_mask(A, mask, meta, Dsize, axis, ndim):
     """backend_np.py:626-633: select index lists ``mask[tm]`` along ``axis`` of every block."""
     out = np.empty(Dsize, dtype=A.dtype)
     for sln, Dn, sla, Da, tm in meta:
-        idx = (slice(None),) * axis + (np.asarray(mask[tm]),)
+        idx = (slice(None),) * axis + (_idx(mask[tm]),)
         out[_sl(sln)] = A[_sl(sla)].reshape(Da)[idx].ravel()
     return out
 
@@ -195,7 +201,7 @@ def apply_mask_adjoint(grad, mask, meta, size_a, axis, ndim):
     """_backend_torch_backwards.py:270-279: zero-filled scatter of the selected rows."""
     out = np.zeros(size_a, dtype=grad.dtype)
     for sln, Dn, sla, Da, tm in meta:
-        idx = (slice(None),) * axis + (np.asarray(mask[tm]),)
+        idx = (slice(None),) * axis + (_idx(mask[tm]),)
         out[_sl(sla)].reshape(Da)[idx] = grad[_sl(sln)].reshape(Dn)
     return out
 
@@ -204,9 +210,40 @@ def embed_mask(A, mask, meta, Dsize, axis, ndim):
     """backend_np.py:636-643: zero-filled scatter along ``axis``."""
     out = np.zeros(Dsize, dtype=A.dtype)
     for sln, Dn, sla, Da, tm in meta:
-        idx = (slice(None),) * axis + (np.asarray(mask[tm]),)
+        idx = (slice(None),) * axis + (_idx(mask[tm]),)
         out[_sl(sln)].reshape(Dn)[idx] = A[_sl(sla)].reshape(Da)
     return out
+
+
+def dot_diag(Adata, Bdata, meta, Dsize, axis, a_ndim):
+    """backend_np.py:604-611: every output block is B's block scaled along ``axis`` by a diagonal block."""
+    dim = [1] * a_ndim
+    dim[axis] = -1
+    out = np.empty(Dsize, dtype=np.promote_types(Adata.dtype, Bdata.dtype))
+    for sln, slb, Db, sla in meta:
+        out[_sl(sln)] = (Adata[_sl(sla)].reshape(dim) * Bdata[_sl(slb)].reshape(Db)).ravel()
+    return out
+
+
+def dot_diag_adjoint(g, Adata, Bdata, meta, axis, a_ndim):
+    """Gradients of sum(conj(g) * dot_diag(A, B)) in torch's convention (the reference differentiates the
+    torch ops of backend_torch.py:557-564): gB = g conj(A) broadcast, gA = the sum of g conj(B) over the
+    other axes, accumulated over the blocks that read the same diagonal block."""
+    dim = [1] * a_ndim
+    dim[axis] = -1
+    dt = np.promote_types(Adata.dtype, Bdata.dtype)
+    gA = np.zeros(Adata.size, dtype=dt)
+    gB = np.zeros(Bdata.size, dtype=dt)
+    other = tuple(d for d in range(a_ndim) if d != axis)
+    for sln, slb, Db, sla in meta:
+        gb = g[_sl(sln)].reshape(Db)
+        gB[_sl(slb)] = (gb * np.conj(Adata[_sl(sla)]).reshape(dim)).ravel()
+        gA[_sl(sla)] += (gb * np.conj(Bdata[_sl(slb)].reshape(Db))).sum(axis=other)
+    if not np.iscomplexobj(Adata):
+        gA = gA.real
+    if not np.iscomplexobj(Bdata):
+        gB = gB.real
+    return gA, gB
 
 
 def _lorentz(x, eps=1e-12):

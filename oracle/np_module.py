@@ -18,7 +18,10 @@ svd = lambda data, meta, sizes, **kw: O.svd(data, meta, sizes)
 fix_svd_signs = O.fix_svd_signs
 eigh = lambda data, meta, sizes: O.eigh(data, meta, sizes)
 apply_mask = O.apply_mask
+embed_mask = O.embed_mask
 transpose = O.transpose
+transpose_dot_sum = O.transpose_dot_sum
+dot_diag = O.dot_diag
 
 
 def zeros(D, dtype='float64', device='cpu'):
@@ -78,17 +81,19 @@ def sqrt(x):
 
 
 def norm(x, p):
-    return np.linalg.norm(x)
+    """backend_np.py:154-158: 'fro' is the Frobenius norm, 'inf' the largest modulus."""
+    if p == 'fro':
+        return np.linalg.norm(x)
+    return np.max(np.abs(x)) if x.size > 0 else np.float64(0.)
 
 
 def vdot(a, b, meta):
     return sum(np.dot(a[x[0]:x[1]], b[y[0]:y[1]]) for x, y in meta)
 
 
-def dot_diag(Adata, Bdata, meta, Dsize, axis, a_ndim):
-    dim = [1] * a_ndim
-    dim[axis] = -1
-    out = np.empty(Dsize, dtype=np.promote_types(Adata.dtype, Bdata.dtype))
-    for sln, slb, Db, sla in meta:
-        out[sln[0]:sln[1]] = (Adata[sla[0]:sla[1]].reshape(dim) * Bdata[slb[0]:slb[1]].reshape(Db)).ravel()
-    return out
+def maximum(x, y):
+    return np.maximum(x, y)
+
+
+def absolute(x):
+    return np.abs(x)

@@ -29,7 +29,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 # hot-path backend functions whose calls are recorded
 RECORDED = ('transpose_and_merge', 'unmerge', 'dot', 'svd', 'eigh', 'fix_svd_signs',
-            'apply_mask', 'transpose', 'embed_mask')
+            'apply_mask', 'transpose', 'embed_mask', 'transpose_dot_sum', 'dot_diag')
 
 
 def _stub_opt_einsum():
@@ -87,7 +87,8 @@ def make_proxy(base, rec):
                     if isinstance(a, np.ndarray):
                         entry['args'].append({'array': rec.add_array(f'c{i}_a{j}', a)})
                     elif isinstance(a, dict):  # mask dict: charge -> index array
-                        entry['args'].append({'mask': [[_jsonable(k), _jsonable(np.asarray(v).tolist())]
+                        entry['args'].append({'mask': [[_jsonable(k), {'slice': [v.start, v.stop, v.step]}
+                                                        if isinstance(v, slice) else _jsonable(np.asarray(v).tolist())]
                                                        for k, v in a.items()]})
                     else:
                         entry['args'].append({'value': _jsonable(a)})
@@ -392,5 +393,171 @@ def main(ref_root):
     print('backend surface', len(names), 'names')
 
 
+def more(ref_root, only=None):
+    """Scenarios added later (each writes its own files; the ones above are left as they are):
+    the no-fusion tensordot policy (transpose_dot_sum with the planner's list-typed metas), dot_diag and
+    embed_mask calls, truncation_mask_multiplets decisions, and the Kagome RVB state of the reference's
+    tests/peps/test_ad.py with CTMRG losses and gradients of the reference torch backend on the CPU."""
+    _stub_opt_einsum()
+    if ref_root not in sys.path:
+        sys.path.insert(0, ref_root)
+    import yastn
+    from yastn.backend import backend_np
+
+    def want(name):
+        return only is None or name in only
+
+    def cfg(sym, rec, policy='fuse_contracted'):
+        return yastn.make_config(sym=sym, backend=make_proxy(backend_np, rec), tensordot_policy=policy)
+
+    def tensor(config, legs, n, dtype, seed):
+        a = yastn.zeros(config=config, legs=legs, n=n, dtype=dtype)
+        a._data = seeded(a.size, dtype, seed).astype(dtype)
+        return a
+
+    tq = tuple(range(-4, 5))
+    Dq = (2, 5, 10, 14, 16, 14, 10, 5, 2)
+
+    def rank4_legs(c):
+        la = [yastn.Leg(c, s=1, t=(-1, 0, 1), D=(2, 3, 2)), yastn.Leg(c, s=1, t=(0, 1), D=(2, 3)),
+              yastn.Leg(c, s=-1, t=(-1, 0, 1, 2), D=(1, 2, 3, 1)), yastn.Leg(c, s=-1, t=(0, 1), D=(3, 1))]
+        lb = [la[2].conj(), la[0].conj(), yastn.Leg(c, s=1, t=(0, 1, 2), D=(2, 1, 2))]
+        return la, lb
+
+    # ---------------- no-fusion tensordot: transpose_dot_sum with list-typed metas ---------------
+    if want('nofusion_u1'):
+        rec = Recorder()
+        sc = Scenario('nofusion_u1', rec)
+        c = cfg('U1', rec, 'no_fusion')
+        legs = [yastn.Leg(c, s=1, t=tq, D=Dq), yastn.Leg(c, s=1, t=(0, 1), D=(1, 1)),
+                yastn.Leg(c, s=-1, t=tq, D=Dq)]
+        A = tensor(c, legs, 0, 'float64', 0)
+        sc.keep('A', A)
+        sc.keep('T', yastn.tensordot(A, A.conj(), axes=((0, 1), (0, 1))))
+        la, lb = rank4_legs(c)
+        a = tensor(c, la, 1, 'float64', 5)
+        b = tensor(c, lb, 0, 'float64', 6)
+        ac = tensor(c, la, 1, 'complex128', 13)
+        sc.keep('a', a); sc.keep('b', b); sc.keep('ac', ac)
+        sc.keep('r1', yastn.tensordot(a.transpose(axes=(3, 2, 1, 0)), b, axes=((1, 3), (0, 1))))
+        sc.keep('r2', yastn.tensordot(b, a, axes=((0, 1), (2, 0))))
+        sc.keep('r4', yastn.tensordot(b, ac, axes=((0, 1), (2, 0))))
+        sc.keep('r5', yastn.tensordot(ac, ac.conj(), axes=((0, 1), (0, 1))))
+        sc.dump()
+
+    # ---------------- dot_diag and embed_mask calls ----------------------------------------------
+    if want('diag_embed_u1'):
+        rec = Recorder()
+        sc = Scenario('diag_embed_u1', rec)
+        c = cfg('U1', rec)
+        la, _ = rank4_legs(c)
+        rec.on = False
+        a = tensor(c, la, 1, 'float64', 7)
+        U, S, V = yastn.linalg.svd(a, axes=((0, 1), (2, 3)), sU=-1)
+        ac = tensor(c, la, 1, 'complex128', 8)
+        Uc, Sc, Vc = yastn.linalg.svd(ac, axes=((0, 1), (2, 3)), sU=1)
+        rec.on = True
+        for lab, t in (('U', U), ('S', S), ('V', V), ('Uc', Uc), ('Sc', Sc), ('Vc', Vc)):
+            sc.keep(lab, t)
+        sc.keep('US', U @ S)
+        sc.keep('SV', S @ V)
+        sc.keep('rsV', S.rsqrt() @ V)
+        sc.keep('UcSc', Uc @ Sc)
+        sc.keep('ScVc', Sc @ Vc)
+        # embed_mask: flip_charges (mask {0: slice(None)}) and addition of tensors whose hard-fused legs
+        # differ (_embed_tensor)
+        sc.keep('a', a)
+        sc.keep('flip', a.flip_charges(axes=1))
+        l0 = [yastn.Leg(c, s=1, t=(-1, 0, 1), D=(2, 3, 2)), yastn.Leg(c, s=1, t=(0, 1), D=(2, 3)),
+              yastn.Leg(c, s=-1, t=(0, 1), D=(2, 2))]
+        l1 = [yastn.Leg(c, s=1, t=(0, 1), D=(3, 2)), yastn.Leg(c, s=1, t=(0, 1), D=(2, 3)),
+              yastn.Leg(c, s=-1, t=(0, 1), D=(2, 2))]
+        x = tensor(c, l0, 0, 'float64', 14).fuse_legs(axes=((0, 1), 2), mode='hard')
+        y = tensor(c, l1, 0, 'float64', 15).fuse_legs(axes=((0, 1), 2), mode='hard')
+        sc.keep('x', x); sc.keep('y', y)
+        sc.keep('xy', x + y)
+        sc.dump()
+
+    # ---------------- truncation_mask_multiplets (yastn/tensor/linalg.py:539-634) ------------------
+    if want('multiplets_u1'):
+        rec = Recorder()
+        sc = Scenario('multiplets_u1', rec)
+        c = cfg('U1', rec)
+        rec.on = False
+        legs = [yastn.Leg(c, s=1, t=(-1, 0, 1), D=(2, 3, 2)), yastn.Leg(c, s=1, t=(-1, 0, 1), D=(3, 4, 3)),
+                yastn.Leg(c, s=-1, t=(-1, 0, 1), D=(4, 5, 4)), yastn.Leg(c, s=-1, t=(-1, 0, 1), D=(5, 6, 5))]
+        a = tensor(c, legs, 0, 'float64', 16)
+        _, S, _ = yastn.linalg.svd(a, axes=((0, 1), (2, 3)))
+        # the spectrum of the reference's test_svd_multiplets (tests/tensor/test_svd.py:344-388)
+        v00 = [1, 1, 0.1001, 0.1000, 0.1000, 0.0999, 0.001001, 0.001000] + [0] * 16
+        v11 = [1, 1, 0.1001, 0.1000, 0.0999, 0.001000, 0.000999] + [0] * 10
+        v22 = [1, 1, 0.1001, 0.1000, 0.001000, 0]
+        S.set_block(ts=(0, 0), Ds=24, val=v00)
+        S.set_block(ts=(1, 1), Ds=17, val=v11)
+        S.set_block(ts=(-1, -1), Ds=17, val=v11)
+        S.set_block(ts=(2, 2), Ds=6, val=v22)
+        S.set_block(ts=(-2, -2), Ds=6, val=v22)
+        # a second spectrum: exact multiplets of sizes 1-3 spread over the blocks, and a random tail
+        rng = np.random.default_rng(17)
+        S2 = S.copy()
+        vals = {}
+        levels = np.sort(rng.uniform(0.01, 1, 12))[::-1]
+        for t in S2.struct.t:
+            D = S2[t].shape[0] if hasattr(S2[t], 'shape') else len(S2[t])
+            v = list(levels[rng.integers(0, 12, size=D)])
+            v = sorted(v, reverse=True)
+            vals[t] = v
+            S2.set_block(ts=t, Ds=D, val=v)
+        cases = [(0, {'tol': 1e-4, 'D_total': 30, 'eps_multiplet': 1e-3}),
+                 (0, {'tol': 1e-4, 'D_total': 20, 'eps_multiplet': 1e-3}),
+                 (0, {'tol': 1e-4, 'D_total': 12, 'eps_multiplet': 1e-3}),
+                 (0, {'tol': 0.05, 'D_total': 64, 'eps_multiplet': 1e-3}),
+                 (0, {'tol': 1e-4, 'D_total': 30, 'eps_multiplet': 1e-2}),
+                 (0, {'tol': 0.0, 'D_total': 1000, 'eps_multiplet': 1e-8}),
+                 (0, {'tol': 1e-8, 'D_total': 25, 'eps_multiplet': 1e-8, 'hermitian': True}),
+                 (1, {'tol': 1e-8, 'D_total': 17, 'eps_multiplet': 1e-8}),
+                 (1, {'tol': 1e-8, 'D_total': 40, 'eps_multiplet': 1e-8}),
+                 (1, {'tol': 0.3, 'D_total': 40, 'eps_multiplet': 1e-8}),
+                 (1, {'tol': 1e-8, 'D_total': 9, 'eps_multiplet': 1e-8, 'hermitian': True})]
+        sc.keep('S0', S)
+        sc.keep('S1', S2)
+        res = []
+        for k, (which, opts) in enumerate(cases):
+            m = yastn.linalg.truncation_mask_multiplets((S, S2)[which], **opts)
+            sc.keep(f'mask_{k}', m)
+            res.append({'S': which, 'opts': _jsonable(opts)})
+        sc.dump({'cases': res})
+
+    # ---------------- Kagome RVB D=3 U(1) state and CTMRG gradients (tests/peps/test_ad.py) -------
+    if want('rvb_kagome'):
+        import torch
+        sys.path.insert(0, os.path.join(ref_root, 'tests', 'peps'))
+        import test_ad as T
+        torch.set_default_dtype(torch.float64)
+        kw = {'backend': 'torch', 'default_device': 'cpu'}
+        A, Ag, cost = T.prepare_RVB((dict(kw), torch, None))
+        rec = Recorder()
+        sc = Scenario('rvb_kagome', rec)
+        sc.keep('A', A.detach())
+        rec.add_array('grad_expected', Ag.data.detach().numpy())
+        runs = []
+        for sweeps, init in ((1, 'eye'), (1, 'dl'), (2, 'dl'), (20, 'dl')):
+            x = A.data.detach().clone().requires_grad_(True)
+            loss = cost(x, slice(0, x.numel()), sweeps, ctm_init=init, fix_signs=False,
+                        truncate_multiplets_mode='truncate', checkpoint_move=False)
+            loss.backward()
+            tag = f's{sweeps}_{init}'
+            rec.add_array(f'grad_{tag}', x.grad.detach().numpy())
+            runs.append({'sweeps': sweeps, 'init': init, 'loss': float(loss.detach()), 'tag': tag})
+            print('rvb', tag, float(loss.detach()), flush=True)
+        sc.dump({'runs': runs, 'leg_order': 't,l,b,r,p (after transpose (1, 2, 3, 4, 0))'})
+
+
 if __name__ == '__main__':
-    main(sys.argv[1] if len(sys.argv) > 1 else '/root/reference')
+    args = [a for a in sys.argv[1:] if not a.startswith('--')]
+    root = args[0] if args else '/root/reference'
+    if '--more' in sys.argv:
+        more(root, set(args[1:]) or None)
+    else:
+        main(root)
+        more(root)

@@ -37,9 +37,14 @@ def calls(name, fn=None):
             if 'array' in a:
                 args.append(arrays[a['array']])
             elif 'mask' in a:
-                args.append({tup(k): np.asarray(v, dtype=np.int64) for k, v in a['mask']})
+                args.append({tup(k): slice(*v['slice']) if isinstance(v, dict) else np.asarray(v, dtype=np.int64)
+                             for k, v in a['mask']})
             else:
                 args.append(tup(a['value']))
+        if c['fn'] == 'transpose_dot_sum':
+            # the no-fusion planner's meta_dot is a list of (slice, (Dl, Dr), [[ia, ib], ...])
+            # (yastn/tensor/_contractions.py:444-446); restore that typing
+            args[2] = [(sl, D, [list(p) for p in grp]) for sl, D, grp in args[2]]
         outs = [arrays[k] for k in c['out']]
         out.append((c['fn'], args, c['kwargs'], outs))
     return out

@@ -209,3 +209,71 @@ def test_svd_many_equals_separate_svds():
         for x, y in ((U, U0), (S, S0), (V, V0)):
             assert x.struct == y.struct and x.slices == y.slices and x.hfs == y.hfs
             assert np.array_equal(np.asarray(x.data), np.asarray(y.data))
+
+
+def exact(cfg, t):
+    """A golden tensor with exactly its recorded blocks (also diagonal tensors)."""
+    x = bs.from_blocks(cfg, t['s'], t['n'], t['t'], t['D'], np.array(t['data']), isdiag=t['diag'])
+    assert [list(s.slcs[0]) for s in x.slices] == t['slices']
+    return x
+
+
+def test_no_fusion_policy_metas():
+    """The restated no-fusion planner (reference _contractions.py:349-450) issues the reference's
+    transpose_dot_sum calls: the same list-typed meta_dot (output slices, matrix shapes, the ordered
+    lists of [a block, b block] products), reshape tuples and permutations; real, complex and mixed."""
+    be, rec = recording_backend()
+    be.transpose_dot_sum = lambda *a: (rec.append(('transpose_dot_sum', a)), NB.transpose_dot_sum(*a))[1]
+    cfg = bs.make_config(be, 'U1', tensordot_policy='no_fusion')
+    ts = G.tensors('nofusion_u1')
+    A, a, b, ac = (rebuild(cfg, ts[k]) for k in ('A', 'a', 'b', 'ac'))
+    same_tensor(bs.tensordot(A, A.conj(), axes=((0, 1), (0, 1))), ts['T'], atol=1e-12)
+    same_tensor(bs.tensordot(a.transpose((3, 2, 1, 0)), b, axes=((1, 3), (0, 1))), ts['r1'], atol=1e-12)
+    same_tensor(bs.tensordot(b, a, axes=((0, 1), (2, 0))), ts['r2'], atol=1e-12)
+    same_tensor(bs.tensordot(b, ac, axes=((0, 1), (2, 0))), ts['r4'], atol=1e-12)
+    same_tensor(bs.tensordot(ac, ac.conj(), axes=((0, 1), (0, 1))), ts['r5'], atol=1e-12)
+    compare_calls(rec, 'nofusion_u1')
+    for (_, mine), (_, gold, _, _) in zip(rec, G.calls('nofusion_u1')):
+        for md in (mine[2], gold[2]):
+            assert isinstance(md, list) and all(isinstance(p, list) for _, _, grp in md for p in grp)
+
+
+def test_dot_diag_metas_and_values():
+    """Diagonal times block tensor (U S, S V, S^-1/2 V, complex U with real S) through the restated
+    broadcast planner: the reference's dot_diag calls and bit-identical products."""
+    HOT_D = HOT + ('dot_diag',)
+    rec = []
+    be = types.ModuleType('rec_backend')
+    for k in dir(NB):
+        if not k.startswith('__'):
+            setattr(be, k, getattr(NB, k))
+    be.dot_diag = lambda *a: (rec.append(('dot_diag', a)), NB.dot_diag(*a))[1]
+    cfg = bs.make_config(be, 'U1')
+    ts = G.tensors('diag_embed_u1')
+    U, S, V, Uc, Sc, Vc = (exact(cfg, ts[k]) for k in ('U', 'S', 'V', 'Uc', 'Sc', 'Vc'))
+    same_tensor(U @ S, ts['US'])
+    same_tensor(S @ V, ts['SV'])
+    same_tensor(S.rsqrt() @ V, ts['rsV'], atol=1e-15)
+    same_tensor(Uc @ Sc, ts['UcSc'])
+    same_tensor(Sc @ Vc, ts['ScVc'])
+    gold = [c for c in G.calls('diag_embed_u1') if c[0] in HOT_D]
+    assert [c[0] for c in rec] == [c[0] for c in gold]
+    for (fn, a1), (_, a2, _, _) in zip(rec, gold):
+        assert tuple(_norm_meta(x) for x in a1) == tuple(_norm_meta(x) for x in a2)
+
+
+def test_truncation_mask_multiplets_known_answers():
+    """Masks of the reference's truncation_mask_multiplets (linalg.py:539-634) on the spectrum of its
+    tests/tensor/test_svd.py:344-388 and on a spectrum of exact multiplets, with and without the
+    hermitian pairing: identical kept sets."""
+    cfg = bs.make_config(NB, 'U1')
+    js, _ = G.load('multiplets_u1')
+    ts = G.tensors('multiplets_u1')
+    Ss = [exact(cfg, ts['S0']), exact(cfg, ts['S1'])]
+    for k, case in enumerate(js['cases']):
+        m = bs.truncation_mask_multiplets(Ss[case['S']], **case['opts'])
+        want = ts[f'mask_{k}']['data']
+        np.testing.assert_array_equal(np.asarray(m.data), want.astype(bool), err_msg=str(case))
+    # the decisions are not all trivial
+    kept = {int(ts[f'mask_{k}']['data'].sum()) for k in range(len(js['cases']))}
+    assert len(kept) >= 5, kept

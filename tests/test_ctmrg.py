@@ -173,3 +173,41 @@ def test_batched_projector_svds_equal_sequential_cpu(dtype):
     A, env, sz = C.setup(cn, D=3, chi=12, dtype=dtype, seed=4)
     v0, v1 = C.loss_fn(A, env, sz, 12), C.loss_fn(A, env, sz, 12, batch=True)
     assert abs(complex(v0) - complex(v1)) <= 1e-13 * max(1.0, abs(complex(v0)))
+
+
+@pytest.mark.parametrize('k', [0, 1, 2])
+def test_rvb_kagome_matches_reference_ctmrg(k):
+    """The reference's own CTMRG (yastn.tn.fpeps EnvCTM.ctmrg_, torch backend on the CPU; golden vectors) on
+    the Kagome RVB D=3 U(1) complex state of tests/peps/test_ad.py, chi=64, multiplet truncation, use_qr=False:
+    1 sweep from the 'eye' and from the 'dl' environment, 2 sweeps from 'dl'.  The restated harness (2x2
+    corners, proj_corners, _update_env_, expand_outward_, rdm1x1) on the plain torch path gives the same
+    loss (<= 1e-12) and gradient (<= 1e-9 relative): a wrong projector leg, a wrong environment update or a
+    wrong truncation would not."""
+    import rvb_case as R
+    sweeps, init, loss_ref, g_ref = R.runs()[k]
+    cfg = bs.make_config(_plain(), 'U1', device='cpu')
+    A = R.state(cfg)
+    x = A.data.detach().clone().requires_grad_(True)
+    loss = C.rdm_loss(A._replace(data=x), R.CHI, sweeps, init, R.svd_opts(), block=R.BLOCK)
+    loss.backward()
+    assert abs(float(loss) - loss_ref) <= 1e-12
+    g = x.grad.numpy()
+    assert np.abs(g - g_ref).max() <= 1e-9 * np.abs(g_ref).max()
+
+
+def test_rvb_kagome_20_sweeps_known_answer():
+    """tests/peps/test_ad.py:264-280 (test_Kagome_RVB_D3_U1_sym_vs_pepstorch): 20 sweeps from 'dl', the
+    gradient equals grad_expected (rtol 1e-3, atol 1e-5, as the reference asserts); it also equals the
+    reference CTMRG's own gradient to 1e-8 (relative)."""
+    import rvb_case as R
+    sweeps, init, loss_ref, g_ref = R.runs()[3]
+    assert (sweeps, init) == (20, 'dl')
+    cfg = bs.make_config(_plain(), 'U1', device='cpu')
+    A = R.state(cfg)
+    x = A.data.detach().clone().requires_grad_(True)
+    loss = C.rdm_loss(A._replace(data=x), R.CHI, sweeps, init, R.svd_opts(), block=R.BLOCK)
+    loss.backward()
+    g = x.grad.numpy()
+    assert np.allclose(g, R.grad_expected(), rtol=1e-3, atol=1e-5)
+    assert abs(float(loss) - loss_ref) <= 1e-10
+    assert np.abs(g - g_ref).max() <= 1e-8 * np.abs(g_ref).max()

@@ -461,3 +461,46 @@ def test_svd_nonconvergence_raises(B, n):
         F.MAX_SWEEPS = cap
     U, S, V = B.svd(_t(a), meta, (n * n, n, n * n))  # and the normal cap converges
     _check_svd(a, meta, _n(U), _n(S), _n(V), np.linalg.svd(a.reshape(n, n), compute_uv=False))
+
+
+def test_dot_diag_embed_mask_transpose_golden(B):
+    """dot_diag, embed_mask (index-list and slice masks) and transpose against the reference numpy
+    backend's recorded calls: bit-exact (the diagonal is real in every recorded dot_diag, so each
+    element is one rounded product, as in numpy)."""
+    n = {'dot_diag': 0, 'embed_mask': 0, 'transpose': 0}
+    for scenario in G.SCENARIOS + ('diag_embed_u1',):
+        for fn, args, kw, outs in G.calls(scenario):
+            if fn == 'dot_diag':
+                A, Bd, meta, Dsize, axis, nd = args
+                assert not np.iscomplexobj(A)
+                got = B.dot_diag(_t(A), _t(Bd), meta, Dsize, axis, nd)
+            elif fn == 'embed_mask':
+                A, mask, meta, Dsize, axis, nd = args
+                m = {k: v if isinstance(v, slice) else torch.as_tensor(v) for k, v in mask.items()}
+                got = B.embed_mask(_t(A), m, meta, Dsize, axis, nd)
+            elif fn == 'transpose':
+                got = B.transpose(_t(args[0]), args[1], args[2])
+            else:
+                continue
+            print(scenario, fn, flush=True)
+            torch.cuda.synchronize()
+            res = _n(got)
+            assert res.dtype == outs[0].dtype
+            np.testing.assert_array_equal(res, outs[0])
+            n[fn] += 1
+    assert n == {'dot_diag': 5, 'embed_mask': 2, 'transpose': 2}, n
+
+
+def test_dot_diag_adjoint_vs_oracle(B):
+    """Gradients of dot_diag with respect to both operands (one launch each) against the oracle's adjoint,
+    which is pinned to torch's autograd of the reference torch backend's dot_diag: <= 1e-14 relative."""
+    rng = np.random.default_rng(32)
+    for fn, args, kw, outs in G.calls('diag_embed_u1', 'dot_diag'):
+        A, Bd, meta, Dsize, axis, nd = args
+        cplx = np.iscomplexobj(A) or np.iscomplexobj(Bd)
+        g = rng.standard_normal(Dsize) + (1j * rng.standard_normal(Dsize) if cplx else 0)
+        gA, gB = O.dot_diag_adjoint(g, A, Bd, meta, axis, nd)
+        ta, tb = _t(A).requires_grad_(True), _t(Bd).requires_grad_(True)
+        out = B.dot_diag(ta, tb, meta, Dsize, axis, nd)
+        (out * _t(np.conj(g))).real.sum().backward()
+        assert _rel(_n(ta.grad), gA) <= 1e-14 and _rel(_n(tb.grad), gB) <= 1e-14

@@ -84,3 +84,29 @@ def test_hip_move_batched_projector_svds():
     assert abs(float(lh) - float(lp)) <= 1e-10 * max(1.0, abs(float(lp)))
     rel = float((gh - gp).norm() / gp.norm())
     assert rel <= 1e-8, rel
+
+
+@pytest.mark.parametrize('k', [0, 2, 3])
+def test_rvb_kagome_reference_ctmrg_on_gpu(k):
+    """The Kagome RVB D=3 U(1) complex state of the reference's tests/peps/test_ad.py through the HIP backend
+    (chi=64, multiplet truncation, use_qr=False, fix_signs=False): 1 sweep from 'eye' and 2 sweeps from 'dl'
+    against the reference CTMRG's loss (<= 1e-12) and gradient (<= 1e-9 relative); 20 sweeps from 'dl'
+    against the reference test's grad_expected (rtol 1e-3, atol 1e-5) and the reference CTMRG's gradient
+    (<= 1e-8 relative)."""
+    import rvb_case as R
+    from yastn_amd import backend_hip
+    sweeps, init, loss_ref, g_ref = R.runs()[k]
+    print('rvb kagome', sweeps, 'sweeps from', init, flush=True)
+    cfg = bs.make_config(backend_hip, 'U1', device='cuda:0')
+    A = R.state(cfg)
+    x = A.data.detach().clone().requires_grad_(True)
+    loss = C.rdm_loss(A._replace(data=x), R.CHI, sweeps, init, R.svd_opts(), block=R.BLOCK)
+    loss.backward()
+    g = x.grad.cpu().numpy()
+    if sweeps == 20:
+        assert np.allclose(g, R.grad_expected(), rtol=1e-3, atol=1e-5)
+        assert abs(float(loss) - loss_ref) <= 1e-10
+        assert np.abs(g - g_ref).max() <= 1e-8 * np.abs(g_ref).max()
+    else:
+        assert abs(float(loss) - loss_ref) <= 1e-12
+        assert np.abs(g - g_ref).max() <= 1e-9 * np.abs(g_ref).max()

@@ -53,3 +53,33 @@ def test_transpose_dot_sum(ca, cb):
     for g, gr in ((ta.grad, ca_.grad), (tb.grad, cb_.grad)):
         g, gr = g.cpu().numpy(), gr.numpy()
         assert np.linalg.norm(g - gr) <= 1e-12 * np.linalg.norm(gr)
+
+
+def test_transpose_dot_sum_reference_metas():
+    """The reference planner's own no-fusion calls (tests/golden nofusion_u1: list-typed meta_dot with
+    [a block, b block] product lists, real, mixed real x complex and complex): output vs the reference numpy
+    backend's recorded result <= 1e-12, gradients vs torch autograd of the reference kernel <= 1e-12."""
+    import golden_io as G
+    from yastn_amd import backend_hip as B
+    rng = np.random.default_rng(43)
+    calls = G.calls('nofusion_u1', 'transpose_dot_sum')
+    assert len(calls) == 5
+    for fn, args, kw, outs in calls:
+        A, Bd, meta_dot, Ar, Br, Ao, Bo, Dsize = args
+        assert isinstance(meta_dot, list)
+        print('transpose_dot_sum reference meta', len(meta_dot), 'blocks', flush=True)
+        ta = torch.as_tensor(A, device=DEV).requires_grad_(True)
+        tb = torch.as_tensor(Bd, device=DEV).requires_grad_(True)
+        out = B.transpose_dot_sum(ta, tb, meta_dot, Ar, Br, Ao, Bo, Dsize)
+        got = out.detach().cpu().numpy()
+        assert got.dtype == outs[0].dtype
+        assert np.linalg.norm(got - outs[0]) <= 1e-12 * np.linalg.norm(outs[0])
+        cplx = np.iscomplexobj(got)
+        w = rng.standard_normal(Dsize) + (1j * rng.standard_normal(Dsize) if cplx else 0)
+        (out * torch.as_tensor(w, device=DEV)).real.sum().backward()
+        ca_ = torch.as_tensor(A).requires_grad_(True)
+        cb_ = torch.as_tensor(Bd).requires_grad_(True)
+        (_torch_ref(ca_, cb_, meta_dot, Ar, Br, Ao, Bo, Dsize) * torch.as_tensor(w)).real.sum().backward()
+        for g, gr in ((ta.grad, ca_.grad), (tb.grad, cb_.grad)):
+            g, gr = g.cpu().numpy(), gr.numpy()
+            assert np.linalg.norm(g - gr) <= 1e-12 * np.linalg.norm(gr)

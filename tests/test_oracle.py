@@ -353,3 +353,44 @@ def test_oracle_transpose_dot_sum_closed_form():
                 b = B[Br[tb][0][0]:Br[tb][0][1]].reshape(Br[tb][1])  # (c2, c0, n)
                 ref = ref + np.tensordot(a, b, axes=((0, 2), (1, 0)))
             assert np.linalg.norm(C[sl[0]:sl[1]].reshape(Dslc) - ref) <= 1e-13 * max(1.0, np.linalg.norm(ref))
+
+
+def test_oracle_dot_diag_embed_and_nofusion_match_golden():
+    """dot_diag and embed_mask (index and slice masks) bit-exact against the reference numpy backend's
+    recorded calls; transpose_dot_sum against its no-fusion calls (list-typed metas) to 1e-13."""
+    n = {'dot_diag': 0, 'embed_mask': 0, 'transpose_dot_sum': 0}
+    for scenario in ('diag_embed_u1', 'nofusion_u1'):
+        for fn, args, kw, outs in G.calls(scenario):
+            if fn == 'dot_diag':
+                _cmp_exact(O.dot_diag(*args), outs[0])
+            elif fn == 'embed_mask':
+                _cmp_exact(O.embed_mask(*args), outs[0])
+            elif fn == 'transpose_dot_sum':
+                got = O.transpose_dot_sum(*args)
+                assert got.dtype == outs[0].dtype
+                assert np.linalg.norm(got - outs[0]) <= 1e-13 * np.linalg.norm(outs[0])
+            else:
+                continue
+            n[fn] += 1
+    assert n == {'dot_diag': 5, 'embed_mask': 2, 'transpose_dot_sum': 5}, n
+
+
+def test_oracle_dot_diag_adjoint_vs_torch_autograd():
+    """The oracle's dot_diag adjoint equals torch's autograd of the reference torch backend's dot_diag
+    (backend_torch.py:557-564, restated with torch ops on the CPU) on the recorded calls, real and complex."""
+    import torch
+    rng = np.random.default_rng(31)
+    for fn, args, kw, outs in G.calls('diag_embed_u1', 'dot_diag'):
+        A, B, meta, Dsize, axis, nd = args
+        cplx = np.iscomplexobj(A) or np.iscomplexobj(B)
+        g = rng.standard_normal(Dsize) + (1j * rng.standard_normal(Dsize) if cplx else 0)
+        gA, gB = O.dot_diag_adjoint(g, A, B, meta, axis, nd)
+        ta = torch.as_tensor(A).requires_grad_(True)
+        tb = torch.as_tensor(B).requires_grad_(True)
+        dim = [1] * nd
+        dim[axis] = -1
+        out = torch.cat([(ta[sla[0]:sla[1]].reshape(dim) * tb[slb[0]:slb[1]].reshape(Db)).reshape(-1)
+                         for sln, slb, Db, sla in meta])
+        (out * torch.as_tensor(np.conj(g))).real.sum().backward()
+        np.testing.assert_allclose(gA, ta.grad.numpy(), rtol=1e-13, atol=1e-14)
+        np.testing.assert_allclose(gB, tb.grad.numpy(), rtol=1e-13, atol=1e-14)

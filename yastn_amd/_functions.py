@@ -248,15 +248,25 @@ def _inverse_transpose_meta(meta):
 
 
 def _mask_host(mask):
+    """Host copy of a mask dict (charge -> kept indices).  A value may also be a ``slice`` (the
+    reference's flip_charges embeds with {0: slice(None)}, tensor/_single.py:222-225); it is resolved
+    against the masked dimension of each block when the plan is built."""
     out = {}
     for k, v in mask.items():
+        key = tuple(k) if isinstance(k, (list, tuple)) else k
+        if isinstance(v, slice):
+            out[key] = v
+            continue
         arr = v.detach().cpu().numpy() if torch.is_tensor(v) else np.asarray(v)
-        out[tuple(k) if isinstance(k, (list, tuple)) else k] = arr.astype(np.int64).ravel()
+        if arr.dtype == np.bool_:
+            arr = np.nonzero(arr.ravel())[0]
+        out[key] = arr.astype(np.int64).ravel()
     return out
 
 
 def _mask_key(mh):
-    return tuple((k, v.tobytes()) for k, v in sorted(mh.items()))
+    return tuple((k, ('slice', v.start, v.stop, v.step) if isinstance(v, slice) else v.tobytes())
+                 for k, v in sorted(mh.items(), key=lambda kv: repr(kv[0])))
 
 
 _SWAP = P._Cache(maxsize=1024)
@@ -392,6 +402,54 @@ class Dot(_Fn):
         if not Bdata.is_complex() and gB.is_complex():
             gB = gB.real.contiguous()
         return gA, gB, None, None
+
+
+class DotDiag(_Fn):
+    """backend.dot_diag (reference backend_np.py:604-611, backend_torch.py:557-564): every output block
+    is a dense block of B scaled along ``axis`` by a diagonal block of A, one launch.  The reference
+    differentiates its torch ops; here gB = g conj(A) (the same kernel) and gA = the sum of g conj(B)
+    along the other axes (one deterministic reduction launch)."""
+
+    @staticmethod
+    def forward(Adata, Bdata, meta, Dsize, axis, a_ndim):
+        dtype = torch.promote_types(Adata.dtype, Bdata.dtype)
+        A, B = raw(Adata.to(dtype)), raw(Bdata.to(dtype))
+        plan = P.dot_diag_plan(meta, axis, a_ndim, A.numel(), B.numel(), Dsize, A.device)
+        out = (torch.empty if plan.covered == Dsize else torch.zeros)(Dsize, dtype=dtype, device=A.device)
+        L.call('yastn_hip_dot_diag', ptr(out), ptr(A), ptr(B), dtype_code(out), plan.d_jobs.data_ptr(), plan.njobs,
+               plan.max_n, 0, stream_of(out))
+        return out
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        Adata, Bdata, meta, Dsize, axis, a_ndim = inputs
+        ctx.save_for_backward(Adata, Bdata)
+        ctx.meta, ctx.Dsize, ctx.axis, ctx.a_ndim = meta, Dsize, axis, a_ndim
+
+    @staticmethod
+    @_plain_backward
+    def backward(ctx, g):
+        Adata, Bdata = ctx.saved_tensors
+        dtype = torch.promote_types(Adata.dtype, Bdata.dtype)
+        A, B, g = raw(Adata.to(dtype)), raw(Bdata.to(dtype)), raw(g.to(dtype))
+        plan = P.dot_diag_plan(ctx.meta, ctx.axis, ctx.a_ndim, A.numel(), B.numel(), ctx.Dsize, A.device)
+        st, dt = stream_of(g), dtype_code(g)
+        gA = gB = None
+        if ctx.needs_input_grad[0]:
+            gA = (torch.empty if plan.a_covered == A.numel() else torch.zeros)(A.numel(), dtype=dtype,
+                                                                              device=A.device)
+            L.call('yastn_hip_dot_diag_adjoint_a', ptr(gA), ptr(g), ptr(B), dt, plan.d_jobs.data_ptr(),
+                   plan.d_groups.data_ptr(), plan.ngroups, plan.d_ids.data_ptr(), plan.max_D, st)
+            if not Adata.is_complex() and gA.is_complex():
+                gA = gA.real.contiguous()
+        if ctx.needs_input_grad[1]:
+            gB = (torch.empty if plan.b_covered == B.numel() else torch.zeros)(B.numel(), dtype=dtype,
+                                                                              device=B.device)
+            L.call('yastn_hip_dot_diag', ptr(gB), ptr(A), ptr(g), dt, plan.d_jobs_t.data_ptr(), plan.njobs,
+                   plan.max_n, 1, st)
+            if not Bdata.is_complex() and gB.is_complex():
+                gB = gB.real.contiguous()
+        return gA, gB, None, None, None, None
 
 
 # ------------------------------------------------------------------ SVD / eigh

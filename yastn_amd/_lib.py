@@ -12,7 +12,7 @@ LIB_PATH = os.environ.get('YASTN_HIP_LIB', os.path.join(HERE, 'libyastn_hip.so')
 # constants mirrored from include/yastn_hip.h
 YH_OK, YH_ERR_ARG, YH_ERR_HIP, YH_ERR_NUMERIC = 0, -1, -2, -3
 YH_F64, YH_C128 = 0, 1
-YH_ABI_VERSION = 8
+YH_ABI_VERSION = 9
 YH_MAXR = 8
 YH_JOB_WORDS = 32
 YH_GATHER_TILE = 2048
@@ -33,6 +33,7 @@ YH_QR_NB = 32
 YH_QROP_WORDS = 3
 YH_QRADJ_WORDS = 8
 YH_SVD_LDS_BYTES = 131072
+YH_DIAG_WORDS = 8
 
 _p = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -79,6 +80,9 @@ _SIGS = {
     'yastn_hip_svd_sign_pivots': [_p, _p, _i32, _p, _i64, _p],
     'yastn_hip_svd_adjoint_core': [_p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _i64, _dbl, _i32, _i32, _i32, _p],
     'yastn_hip_eigh_adjoint_core': [_p, _p, _p, _p, _i32, _p, _i64, _i32, _i32, _p],
+    'yastn_hip_check_dot_diag_plan': [_p, _i64, _p, _i64, _p, _i64, _i64, _i64, _i64],
+    'yastn_hip_dot_diag': [_p, _p, _p, _i32, _p, _i64, _i64, _i32, _p],
+    'yastn_hip_dot_diag_adjoint_a': [_p, _p, _p, _i32, _p, _p, _i64, _p, _i64, _p],
 }
 
 _lib = None

@@ -561,13 +561,10 @@ def transpose_dot_sum(Adata, Bdata, meta_dot, Areshape, Breshape, Aorder, Border
 
 
 def dot_diag(Adata, Bdata, meta, Dsize, axis, a_ndim):
-    dim = [1] * a_ndim
-    dim[axis] = -1
-    dtype = torch.promote_types(Adata.dtype, Bdata.dtype)
-    parts = [(Adata[_sl(sla)].reshape(dim) * Bdata[_sl(slb)].reshape(Db)).reshape(-1) for sln, slb, Db, sla in meta]
-    if not parts:
-        return torch.empty(Dsize, dtype=dtype, device=Adata.device)
-    return torch.cat(parts).to(dtype)
+    """Diagonal x block tensor: every output block scaled along ``axis`` (one grouped launch)."""
+    if not _needs_grad(Adata, Bdata):
+        return F.DotDiag.forward(Adata, Bdata, meta, Dsize, axis, a_ndim)
+    return F.DotDiag.apply(Adata, Bdata, meta, Dsize, axis, a_ndim)
 
 
 def negate_blocks(Adata, slices):

@@ -264,3 +264,15 @@ def transpose(data, axes, meta_transpose):
     for sln, Dn, slo, Do in meta_transpose:
         out[_sl(sln)].view(Dn)[:] = data[_sl(slo)].view(Do).permute(axes)
     return out
+
+
+def dot_diag(Adata, Bdata, meta, Dsize, axis, a_ndim):
+    """The reference torch backend's dot_diag (backend_torch.py:557-564): one broadcast product per block,
+    differentiated by torch."""
+    dim = [1] * a_ndim
+    dim[axis] = -1
+    dtype = torch.promote_types(Adata.dtype, Bdata.dtype)
+    newdata = torch.empty(Dsize, dtype=dtype, device=Adata.device)
+    for sln, slb, Db, sla in meta:
+        newdata[_sl(sln)].reshape(Db)[:] = Adata[_sl(sla)].reshape(dim) * Bdata[_sl(slb)].reshape(Db)
+    return newdata

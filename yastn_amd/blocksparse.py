@@ -11,9 +11,9 @@ Restated from (reference file: lines):
   block creation order  yastn/tensor/_initialize.py:158-253
   merge to matrix       yastn/tensor/_merging.py:76-187 (+ no-change shortcuts 84-134)
   hard fusion           yastn/tensor/_merging.py:281-377, 488-549, 640-697
-  tensordot             yastn/tensor/_contractions.py:57-346, 453-528 (broadcast)
+  tensordot             yastn/tensor/_contractions.py:57-346, 349-450 (no fusion), 453-528 (broadcast)
   masks                 yastn/tensor/_merging.py:554-583, _contractions.py:529-574
-  svd / truncation      yastn/tensor/linalg.py:49-132, 135-380, 637-735
+  svd / truncation      yastn/tensor/linalg.py:49-132, 135-380, 539-634 (multiplets), 637-735
   eigh                  yastn/tensor/linalg.py:831-1010
 Meta fusion (mfs), direct-sum fusion, leg-intersection masks and fermionic swap gates are not
 restated: the workloads here do not use them.
@@ -330,6 +330,23 @@ def from_flat(config, legs, n, flat):
         dtype = 'complex128' if np.iscomplexobj(flat) else 'float64'
         flat = config.backend.to_tensor(flat, dtype=dtype, device=config.device)
     return Tensor(config, struct, slices, flat)
+
+
+def from_blocks(config, s, n, t, D, flat, isdiag=False):
+    """Tensor with the listed blocks (charges t, shapes D, in ascending charge order) and the block buffer
+    ``flat`` (the layout of the reference's Tensor.from_dict, tensor/_output.py)."""
+    t = tuple(tuple(x) for x in t)
+    D = tuple(tuple(x) for x in D)
+    if list(t) != sorted(set(t)):
+        raise BlockSparseError('block charges must be unique and in ascending order')
+    Dps = tuple(d[0] if isdiag else int(np.prod(d)) for d in D)
+    struct = Struct(s=tuple(s), n=tuple(n), diag=isdiag, t=t, D=D, size=sum(Dps))
+    if len(flat) != struct.size:
+        raise BlockSparseError(f"data has {len(flat)} elements, structure needs {struct.size}")
+    if isinstance(flat, np.ndarray):
+        dtype = 'complex128' if np.iscomplexobj(flat) else ('bool' if flat.dtype == np.bool_ else 'float64')
+        flat = config.backend.to_tensor(flat, dtype=dtype, device=config.device)
+    return Tensor(config, struct, _slices_of(Dps, D), flat)
 
 
 def seeded_normal(n, dtype, seed):
@@ -790,6 +807,64 @@ def _meta_tensordot_fc(struct_a, slices_a, struct_b, slices_b):
     return meta, Struct(s=struct_a.s[:-1] + struct_b.s[1:], t=t_c, D=D_c, size=sum(Dp)), slices_c
 
 
+@lru_cache(maxsize=1024)
+def _meta_tensordot_nf(struct_a, slices_a, struct_b, slices_b, ind_a, ind_b, nout_a, nin_a, nin_b, nout_b):
+    """No-fusion policy (reference _contractions.py:349-450).  Returns (meta_dot, reshape_a, reshape_b,
+    struct_c, slices_c): reshape_* = (slice, block shape, rows, cols) of each participating block as the
+    matrix its permutation gives; meta_dot lists per output block (slice, (rows, cols), [[ia, ib], ...]),
+    the block products to sum.  The output blocks come in ascending charge order; the products of a
+    block keep the order that the reference's (unstable) argsorts give them, which is reproduced by
+    running the same numpy sorts on the same arrays, so summation order and metas are identical."""
+    nsym = len(struct_a.n)
+
+    def side(struct, slices, ind, nout, nin):
+        sel = range(len(struct.t)) if ind is None else ind
+        t = [struct.t[i] for i in sel]
+        D = [struct.D[i] for i in sel]
+        sl = [slices[i].slcs[0] for i in sel]
+        nb, nd = len(t), len(struct.s)
+        at = np.array(t, dtype=np.int64).reshape((nb, nd, nsym))
+        aD = np.array(D, dtype=np.int64).reshape((nb, nd))
+        t_out = at[:, nout, :].reshape(nb, len(nout) * nsym)
+        t_in = at[:, nin, :].reshape(nb, len(nin) * nsym)
+        D_out, D_in = aD[:, nout], aD[:, nin]
+        sectors, inv, cnt = np.unique(np.hstack([t_in, D_in]), return_inverse=True, return_counts=True, axis=0)
+        return dict(t=t, D=D, sl=sl, t_out=t_out, D_out=D_out,
+                    Dp_out=np.prod(D_out, axis=1, dtype=np.int64), Dp_in=np.prod(D_in, axis=1, dtype=np.int64),
+                    sectors=sectors, by_sector=np.argsort(inv), cnt=cnt)
+
+    A = side(struct_a, slices_a, ind_a, nout_a, nin_a)
+    B = side(struct_b, slices_b, ind_b, nout_b, nin_b)
+    if not np.array_equal(A['sectors'], B['sectors']):
+        raise BlockSparseError('Bond dimensions do not match.')
+    reshape_a = tuple((sl, D, int(r), int(c)) for sl, D, r, c in zip(A['sl'], A['D'], A['Dp_out'], A['Dp_in']))
+    reshape_b = tuple((sl, D, int(r), int(c)) for sl, D, r, c in zip(B['sl'], B['D'], B['Dp_in'], B['Dp_out']))
+    # every (a block, b block) pair of a common contracted sector, a-major within the sector
+    pa, pb, oa, ob = [], [], 0, 0
+    for na, nb in zip(A['cnt'].tolist(), B['cnt'].tolist()):
+        ia, ib = A['by_sector'][oa:oa + na], B['by_sector'][ob:ob + nb]
+        pa.append(np.repeat(ia, nb))
+        pb.append(np.tile(ib, na))
+        oa, ob = oa + na, ob + nb
+    pa = np.concatenate(pa) if pa else np.zeros(0, dtype=np.int64)
+    pb = np.concatenate(pb) if pb else np.zeros(0, dtype=np.int64)
+    tc = np.hstack([A['t_out'][pa], B['t_out'][pb]])
+    t_c, first, inv, cnt = np.unique(tc, return_index=True, return_inverse=True, return_counts=True, axis=0)
+    fa, fb = pa[first], pb[first]
+    D_c = tuple(map(tuple, np.hstack([A['D_out'][fa], B['D_out'][fb]]).tolist()))
+    Dp2 = tuple(map(tuple, np.column_stack([A['Dp_out'][fa], B['Dp_out'][fb]]).tolist()))
+    Dp = [r * c for r, c in Dp2]
+    offs = tuple(accumulate(Dp, initial=0))
+    slc_c = tuple(zip(offs, offs[1:]))
+    slices_c = tuple(Slc((sl,), D, dp) for sl, D, dp in zip(slc_c, D_c, Dp))
+    pairs = np.column_stack([pa, pb])[np.argsort(inv)].tolist()
+    bounds = tuple(accumulate(cnt.tolist(), initial=0))
+    meta_dot = [(sl, d2, pairs[i:j]) for sl, d2, i, j in zip(slc_c, Dp2, bounds, bounds[1:])]
+    s_c = tuple(struct_a.s[i] for i in nout_a) + tuple(struct_b.s[i] for i in nout_b)
+    struct_c = Struct(s=s_c, t=tuple(map(tuple, t_c.tolist())), D=D_c, size=offs[-1])
+    return meta_dot, reshape_a, reshape_b, struct_c, slices_c
+
+
 def tensordot(a, b, axes, conj=(0, 0)):
     """Contract legs axes[0] of a with axes[1] of b (reference _contractions.py:57-125)."""
     if conj[0]:
@@ -823,6 +898,8 @@ def tensordot(a, b, axes, conj=(0, 0)):
         data, struct_c, slices_c = _tensordot_f2m(a, b, nout_a, nin_a, nin_b, nout_b, s_c)
     elif a.config.tensordot_policy == 'fuse_contracted':
         data, struct_c, slices_c = _tensordot_fc(a, b, nout_a, nin_a, nin_b, nout_b)
+    elif a.config.tensordot_policy == 'no_fusion':
+        data, struct_c, slices_c = _tensordot_nf(a, b, nout_a, nin_a, nin_b, nout_b)
     else:
         raise BlockSparseError(f"tensordot policy {a.config.tensordot_policy} not restated")
     return Tensor(a.config, struct_c._replace(n=n_c), slices_c, data, hfs=hfs_c, trans=None)
@@ -856,6 +933,16 @@ def _tensordot_fc(a, b, nout_a, nin_a, nin_b, nout_b):
         raise BlockSparseError('Bond dimensions do not match.')
     meta_dot, struct_c, slices_c = _meta_tensordot_fc(struct_a, slices_a, struct_b, slices_b)
     data = a.config.backend.dot(data_a, data_b, meta_dot, struct_c.size)
+    return data, struct_c, slices_c
+
+
+def _tensordot_nf(a, b, nout_a, nin_a, nin_b, nout_b):
+    """reference _contractions.py:186-246 (the transpose_dot_sum branch)."""
+    ind_a, ind_b = _common_inds(a.struct.t, b.struct.t, nin_a, nin_b, a.ndim_n, b.ndim_n, a.config.sym.NSYM)
+    meta_dot, reshape_a, reshape_b, struct_c, slices_c = _meta_tensordot_nf(
+        a.struct, a.slices, b.struct, b.slices, ind_a, ind_b, nout_a, nin_a, nin_b, nout_b)
+    data = a.config.backend.transpose_dot_sum(a._data, b._data, meta_dot, reshape_a, reshape_b, nout_a + nin_a,
+                                              nin_b + nout_b, struct_c.size)
     return data, struct_c, slices_c
 
 
@@ -1118,6 +1205,50 @@ def truncation_mask(S, tol=0, tol_block=0, D_block=float('inf'), D_total=float('
             if gap > abs(Sdata[inds[-p]]):
                 break
     keep[inds[:-D_total]] = False
+    return S._replace(data=keep)
+
+
+def truncation_mask_multiplets(S, tol=0, D_total=float('inf'), eps_multiplet=1e-13, hermitian=False, **kwargs):
+    """Boolean mask keeping at most D_total of the largest values of the diagonal S above tol * max(S)
+    without splitting a multiplet (reference linalg.py:539-634): values are ranked over all blocks, and
+    the cut moves down to the nearest position whose relative gap |s_i - s_i+1| / (max(|s_i|, |s_i+1|) +
+    1e-16) exceeds eps_multiplet (it stays where it is when there is none).  With ``hermitian``, blocks of
+    opposite charges keep only the positions kept in both."""
+    if not S.isdiag:
+        raise BlockSparseError('truncation_mask_multiplets requires a diagonal tensor')
+    be = S.config.backend
+    vals = be.detach(S._data)
+    keep = vals > float('inf')
+    order = be.argsort(-vals)
+    ranked = np.asarray(be.to_numpy(vals[order]))  # one host copy; the cut is decided on the host
+    n_keep = min(int(np.sum(ranked > ranked[0] * tol)), D_total)
+    if n_keep >= len(ranked):
+        return S._replace(data=vals > -float('inf'))
+    big = np.maximum(np.abs(ranked[:-1]), np.abs(ranked[1:])) + 1.0e-16
+    rel_gap = np.abs(ranked[:-1] - ranked[1:]) / big
+    for i in range(n_keep - 1, -1, -1):
+        if rel_gap[i] > eps_multiplet:
+            n_keep = i + 1
+            break
+    keep[order[:n_keep]] = True
+    if hermitian:
+        nsym = S.config.sym.NSYM
+        where = {t: sl.slcs[0] for t, sl in zip(S.struct.t, S.slices)}
+        for t in S.struct.t:
+            lo, hi = where[t]
+            if not bool(keep[lo:hi].any()):
+                continue
+            tc = np.array(t, dtype=np.int64).reshape((1, 1, -1))
+            tn = tuple(S.config.sym.fuse(tc, (1,), -1).ravel().tolist())
+            if tn == t:
+                continue
+            if tn not in where:
+                raise BlockSparseError(f'no block {tn} opposite to {t}')
+            lo2, hi2 = where[tn]
+            k = min(hi - lo, hi2 - lo2)
+            both = keep[lo:lo + k] & keep[lo2:lo2 + k]
+            keep[lo:lo + k] = both
+            keep[lo2:lo2 + k] = both
     return S._replace(data=keep)
 
 

@@ -3,6 +3,8 @@
 //  - the k x k core of the SVD adjoint (Lorentzian-broadened F, G terms)
 //  - the n x n core of the Hermitian eigen adjoint
 // The surrounding products (U^H gU, Vh gVh^H, U J Vh, ...) run through the grouped GEMM.
+#include <algorithm>
+
 #include "common.h"
 
 namespace {
@@ -220,7 +222,121 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// dot_diag: every output block is a dense block B scaled along one axis by a diagonal block A
+// (a vector).  Job words (YH_DIAG_WORDS int64): 0 out_off, 1 a_off, 2 b_off, 3 n (block elements),
+// 4 inner (product of the dimensions after the axis), 5 D (dimension of the axis).  Element e of the
+// block takes A[a_off + (e / inner) % D].  grid = (chunks, jobs); a workgroup strides over its block.
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    dot_diag_kernel(typename Z<CPLX>::T* __restrict__ out, const typename Z<CPLX>::T* __restrict__ A,
+                    const typename Z<CPLX>::T* __restrict__ B, const int64_t* __restrict__ jobs, int conj_a) {
+  typedef Z<CPLX> N;
+  const int64_t* w = jobs + (int64_t)blockIdx.y * YH_DIAG_WORDS;
+  const int64_t out_off = w[0], a_off = w[1], b_off = w[2], n = w[3], inner = w[4], D = w[5];
+  const double sa = conj_a ? -1.0 : 1.0;
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < n; e += (int64_t)gridDim.x * kThreads) {
+    const typename N::T a = A[a_off + (e / inner) % D], b = B[b_off + e];
+    const double ar = N::re(a), ai = sa * N::im(a), br = N::re(b), bi = N::im(b);
+    out[out_off + e] = N::mk(ar * br - ai * bi, ar * bi + ai * br);
+  }
+}
+
+__device__ inline double wave_sum64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Adjoint of dot_diag with respect to the diagonal: gA[a_off + i] = sum over the jobs of one group
+// (all jobs reading the same A block, in list order) of sum_e g[out_off + e] conj(B[b_off + e]) over the
+// elements e with (e / inner) % D == i.  Group words: 0 a_off, 1 D, 2 first index into job_ids,
+// 3 count.  One wave per (group, i): 4 waves per workgroup, grid = (ceil(max D / 4), groups).  Lanes
+// take the block's elements of index i in order and the wave sum is a fixed butterfly, so the result is
+// deterministic.
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads)
+    dot_diag_adj_a_kernel(typename Z<CPLX>::T* __restrict__ gA, const typename Z<CPLX>::T* __restrict__ g,
+                          const typename Z<CPLX>::T* __restrict__ B, const int64_t* __restrict__ jobs,
+                          const int64_t* __restrict__ groups, const int64_t* __restrict__ job_ids) {
+  typedef Z<CPLX> N;
+  const int64_t* gw = groups + (int64_t)blockIdx.y * 4;
+  const int64_t a_off = gw[0], D = gw[1], first = gw[2], count = gw[3];
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (i >= D) return;  // wave-uniform
+  double accr = 0.0, acci = 0.0;
+  for (int64_t q = 0; q < count; ++q) {
+    const int64_t* w = jobs + job_ids[first + q] * YH_DIAG_WORDS;
+    const int64_t out_off = w[0], b_off = w[2], n = w[3], inner = w[4];
+    const int64_t per = n / D;  // elements of index i in this block
+    for (int64_t k = lane; k < per; k += 64) {
+      const int64_t e = (k / inner) * D * inner + i * inner + k % inner;
+      const typename N::T x = g[out_off + e], b = B[b_off + e];
+      accr += N::re(x) * N::re(b) + N::im(x) * N::im(b);
+      acci += N::im(x) * N::re(b) - N::re(x) * N::im(b);
+    }
+  }
+  accr = wave_sum64(accr);
+  acci = wave_sum64(acci);
+  if (lane == 0) gA[a_off + i] = N::mk(accr, acci);
+}
+
 }  // namespace
+
+extern "C" int yastn_hip_check_dot_diag_plan(const int64_t* h_jobs, int64_t njobs, const int64_t* h_groups,
+                                             int64_t ngroups, const int64_t* h_job_ids, int64_t nids,
+                                             int64_t out_numel, int64_t a_numel, int64_t b_numel) {
+  for (int64_t j = 0; j < njobs; ++j) {
+    const int64_t* w = h_jobs + j * YH_DIAG_WORDS;
+    const int64_t out_off = w[0], a_off = w[1], b_off = w[2], n = w[3], inner = w[4], D = w[5];
+    YH_CHECK_ARG(n >= 1 && inner >= 1 && D >= 1 && n % (inner * D) == 0, "dot_diag job %lld: shape", (long long)j);
+    YH_CHECK_ARG(out_off >= 0 && out_off + n <= out_numel, "dot_diag job %lld: output range", (long long)j);
+    YH_CHECK_ARG(b_off >= 0 && b_off + n <= b_numel, "dot_diag job %lld: dense operand range", (long long)j);
+    YH_CHECK_ARG(a_off >= 0 && a_off + D <= a_numel, "dot_diag job %lld: diagonal range", (long long)j);
+  }
+  for (int64_t gi = 0; gi < ngroups; ++gi) {
+    const int64_t* gw = h_groups + gi * 4;
+    YH_CHECK_ARG(gw[2] >= 0 && gw[3] >= 1 && gw[2] + gw[3] <= nids, "dot_diag group %lld: job list", (long long)gi);
+    YH_CHECK_ARG(gw[0] >= 0 && gw[1] >= 1 && gw[0] + gw[1] <= a_numel, "dot_diag group %lld: range", (long long)gi);
+    for (int64_t q = gw[2]; q < gw[2] + gw[3]; ++q) {
+      YH_CHECK_ARG(h_job_ids[q] >= 0 && h_job_ids[q] < njobs, "dot_diag group %lld: job id", (long long)gi);
+      const int64_t* w = h_jobs + h_job_ids[q] * YH_DIAG_WORDS;
+      YH_CHECK_ARG(w[1] == gw[0] && w[5] == gw[1], "dot_diag group %lld: job of another block", (long long)gi);
+    }
+  }
+  return YH_OK;
+}
+
+extern "C" int yastn_hip_dot_diag(void* out, const void* A, const void* B, int dtype, const int64_t* d_jobs,
+                                  int64_t njobs, int64_t max_n, int conj_a, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(njobs <= 65535, "dot_diag: %lld jobs", (long long)njobs);
+  if (njobs == 0 || max_n <= 0) return YH_OK;
+  const dim3 g((unsigned)std::min<int64_t>(yh::cdiv(max_n, kThreads), 1024), (unsigned)njobs);
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(dot_diag_kernel<false>, g, dim3(kThreads), 0, stream, (double*)out, (const double*)A,
+                       (const double*)B, d_jobs, conj_a);
+  else
+    hipLaunchKernelGGL(dot_diag_kernel<true>, g, dim3(kThreads), 0, stream, (double2*)out, (const double2*)A,
+                       (const double2*)B, d_jobs, conj_a);
+  return yh::hip_status(hipGetLastError(), "dot_diag launch");
+}
+
+extern "C" int yastn_hip_dot_diag_adjoint_a(void* gA, const void* g, const void* B, int dtype, const int64_t* d_jobs,
+                                            const int64_t* d_groups, int64_t ngroups, const int64_t* d_job_ids,
+                                            int64_t max_D, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(ngroups <= 65535, "dot_diag adjoint: %lld groups", (long long)ngroups);
+  if (ngroups == 0 || max_D <= 0) return YH_OK;
+  const dim3 grid((unsigned)yh::cdiv(max_D, kThreads / 64), (unsigned)ngroups);
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(dot_diag_adj_a_kernel<false>, grid, dim3(kThreads), 0, stream, (double*)gA, (const double*)g,
+                       (const double*)B, d_jobs, d_groups, d_job_ids);
+  else
+    hipLaunchKernelGGL(dot_diag_adj_a_kernel<true>, grid, dim3(kThreads), 0, stream, (double2*)gA,
+                       (const double2*)g, (const double2*)B, d_jobs, d_groups, d_job_ids);
+  return yh::hip_status(hipGetLastError(), "dot_diag adjoint launch");
+}
 
 extern "C" int yastn_hip_fix_svd_signs(void* U, void* Vh, int dtype, const int64_t* d_blocks, int64_t nblocks,
                                        hipStream_t stream) {

@@ -186,9 +186,10 @@ def double_layer_open(A):
     physical legs left open.  The outer product is a contraction over a trailing dimension-1, charge-0
     leg, which leaves the block buffer unchanged."""
     cfg = A.config
-    legs = [A.get_legs(i) for i in range(A.ndim)]
-    one = bs.leg(cfg, 1, ((0,) * cfg.sym.NSYM,), (1,))
-    Ax = bs.from_flat(cfg, legs + [one], A.n, A.data)
+    A = A.consume_transpose()
+    z = cfg.sym.zero()
+    Ax = bs.from_blocks(cfg, A.struct.s + (1,), A.n, tuple(t + z for t in A.struct.t),
+                        tuple(D + (1,) for D in A.struct.D), A.data)._replace(hfs=A.hfs + (bs.Fusion(s=(1,)),))
     x = bs.tensordot(Ax, Ax.conj(), axes=(5, 5))  # (t, l, b, r, p, t*, l*, b*, r*, p*)
     return x.fuse_legs(axes=((0, 5), (1, 6), (2, 7), (3, 8), 4, 9))
 
@@ -240,13 +241,12 @@ def bond_energy(env, A, H):
     return e_h + e_v
 
 
-def ctm_iterations(A, env, chi, n_iter, segment=0, svd_opts=None, on_step=None, batch=False):
+def ctm_iterations(A, env, chi, n_iter, segment=0, svd_opts=None, on_step=None, batch=False, a=None):
     """``n_iter`` 'hv' steps from ``env``.  With ``segment`` > 0 and gradients enabled, each run of
     ``segment`` steps is a non-reentrant checkpoint region (the reference's checkpoint_move,
     _env_ctm.py:556-589): its environments are recomputed in the backward pass instead of kept."""
     import torch
     from torch.utils.checkpoint import checkpoint
-    a = None
     done = 0
     while done < n_iter:
         k = min(segment, n_iter - done) if segment else n_iter - done
@@ -287,6 +287,116 @@ def energy_fwd_bwd(A, env, H, chi, n_iter, segment=0, svd_opts=None, on_step=Non
     loss = energy_loss_fn(A._replace(data=data), env, H, chi, n_iter, segment, svd_opts, on_step, batch)
     loss.backward()
     return loss.detach(), data.grad
+
+
+# ------------------------------------------------------------------ initial environments, rdm1x1
+
+def _pad_legs(x, s_left, s_right):
+    """A rank-1 tensor x(k) as the rank-3 tensor (1, k, 1) with charge-zero legs of dimension 1 and
+    signatures s_left, s_right (the buffer is unchanged)."""
+    cfg = x.config
+    z = cfg.sym.zero()
+    t = tuple(z + tk + z for tk in x.struct.t)
+    D = tuple((1, Dk[0], 1) for Dk in x.struct.D)
+    y = bs.from_blocks(cfg, (s_left, x.struct.s[0], s_right), x.struct.n, t, D, x.data)
+    return y._replace(hfs=(bs.Fusion(s=(s_left,)), x.hfs[0], bs.Fusion(s=(s_right,))))
+
+
+def _identity(config, leg):
+    """The identity matrix on leg.conj() x leg (blocks (q, q), unit diagonal)."""
+    blocks = [((q, q), np.eye(d).ravel()) for q, d in zip(leg.t, leg.D)]
+    t = tuple(tuple(q1) + tuple(q2) for (q1, q2), _ in blocks)
+    D = tuple((d, d) for d in leg.D)
+    data = np.concatenate([b for _, b in blocks]) if blocks else np.zeros(0)
+    return bs.from_blocks(config, (-leg.s, leg.s), config.sym.zero(), t, D, data)
+
+
+def env_eye(A, corner_s=(1, -1)):
+    """The 'eye' initial environment of the reference (EnvCTM.reset_(init='eye'), _env_ctm.py:284-320):
+    corners are the 1 x 1 identity of charge zero with signatures corner_s; each edge is
+    identity_boundary (_env_contractions.py:46-50) of the double-layer leg it faces, i.e. the ket leg traced
+    against the bra leg, between two dimension-1 legs of signatures (-corner_s[1], -corner_s[0])."""
+    cfg = A.config
+    z = cfg.sym.zero()
+    one = lambda s: bs.leg(cfg, s, (z,), (1,))
+    corner = bs.from_flat(cfg, [one(corner_s[0]), one(corner_s[1])], z, np.ones(1))
+    edges = {}
+    for k, i in (('t', 0), ('l', 1), ('b', 2), ('r', 3)):
+        # the double layer fuses (ket leg i, bra leg i); the edge meets it with the conjugate fused leg
+        leg = A.get_legs(i)
+        bnd = _identity(cfg, leg).fuse_legs(axes=((0, 1),))
+        edges[k] = _pad_legs(bnd, -corner_s[1], -corner_s[0])
+    return Env(tl=corner, tr=corner, br=corner, bl=corner, **edges)
+
+
+def env_expand(env, a):
+    """One layer of double-layer sites added to every environment tensor without truncation (the reference's
+    expand_outward_, _env_ctm.py:322-467, whose identity projectors amount to fusing each edge's outer leg with
+    the site leg beside it); every tensor is rescaled by its largest modulus."""
+    new = {'tl': corner_tl(env, a), 'tr': corner_tr(env, a), 'br': corner_br(env, a), 'bl': corner_bl(env, a)}
+    x = bs.tensordot(env.l, a, axes=(1, 1))                       # (b, t, T, B, R)
+    new['l'] = x.transpose((0, 3, 4, 1, 2)).fuse_legs(axes=((0, 1), 2, (3, 4)))
+    x = bs.tensordot(env.r, a, axes=(1, 3))                       # (t, b, T, L, B)
+    new['r'] = x.transpose((0, 2, 3, 1, 4)).fuse_legs(axes=((0, 1), 2, (3, 4)))
+    x = bs.tensordot(env.t, a, axes=(1, 0))                       # (l, r, L, B, R)
+    new['t'] = x.transpose((0, 2, 3, 1, 4)).fuse_legs(axes=((0, 1), 2, (3, 4)))
+    x = bs.tensordot(env.b, a, axes=(1, 2))                       # (r, l, T, L, R)
+    new['b'] = x.transpose((0, 4, 2, 1, 3)).fuse_legs(axes=((0, 1), 2, (3, 4)))
+    return Env(**{k: _normalize(v) for k, v in new.items()})
+
+
+def env_dl(A, a=None, corner_s=(1, -1)):
+    """The 'dl' initial environment (EnvCTM.reset_(init='dl'), _env_ctm.py:278-285): 'eye', expanded by one
+    layer of the double-layer site, each tensor rescaled by its largest modulus."""
+    a = double_layer(A) if a is None else a
+    return env_expand(env_eye(A, corner_s), a)
+
+
+def rdm1x1(env, A):
+    """One-site reduced density matrix rho(p, p*) / Tr(rho) of a 1x1 cell (the reference's rdm1x1,
+    tn/fpeps/envs/rdm.py:211-243): the eight environment tensors around the double-layer site with open
+    physical legs (ket p, bra p*).  Returns (rho normalised, Tr(rho))."""
+    ao = double_layer_open(A)
+    vect = (env.l @ env.tl) @ (env.t @ env.tr)                    # (b_l, L, T, b_tr)
+    vecb = (env.r @ env.br) @ (env.b @ env.bl)                    # (t_r, R, B, t_bl)
+    tmp = bs.tensordot(vect, ao, axes=((1, 2), (1, 0)))           # (b_l, b_tr, B, R, p, p*)
+    rho = bs.tensordot(vecb, tmp, axes=((0, 1, 2, 3), (1, 3, 2, 0)))  # (p, p*)
+    norm = trace(rho)
+    return rho / norm, norm
+
+
+def trace(x, block=None):
+    """Trace of a rank-2 tensor (of one diagonal block when ``block`` names its charges (q, q))."""
+    cfg = x.config
+    be = cfg.backend
+    x = x.consume_transpose()
+    if block is not None:  # ((q,), (q,)) or (q, q)
+        block = tuple(c for q in block for c in (q if isinstance(q, tuple) else (q,)))
+    parts = []
+    for t, D, sl in zip(x.struct.t, x.struct.D, x.slices):
+        if block is not None and tuple(t) != block:
+            continue
+        if D[0] != D[1]:
+            raise bs.BlockSparseError('trace of a non-square block')
+        lo, hi = sl.slcs[0]
+        parts.append(x._data[lo:hi].reshape(D).diagonal().sum())
+    if not parts:
+        return be.zeros((), dtype=x.yastn_dtype, device=cfg.device)
+    out = parts[0]
+    for p in parts[1:]:
+        out = out + p
+    return out
+
+
+def rdm_loss(A, chi, sweeps, init='dl', svd_opts=None, segment=0, block=None):
+    """Re Tr_block rho after ``sweeps`` 'hv' CTMRG steps from the ``init`` environment (the cost function of the
+    reference's tests/peps/test_ad.py:147-165: use_qr=False, projectors truncated by ``svd_opts``)."""
+    a = double_layer(A)
+    env = env_dl(A, a) if init == 'dl' else env_eye(A)
+    env = ctm_iterations(A, env, chi, sweeps, segment=segment, svd_opts=svd_opts, a=a)
+    rho, _ = rdm1x1(env, A)
+    val = trace(rho, block)
+    return val.real if hasattr(val, 'is_complex') and val.is_complex() else val
 
 
 # ------------------------------------------------------------------ workload setup

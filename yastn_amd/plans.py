@@ -362,6 +362,8 @@ def mask_plan(mask_host, meta, Dsize, src_numel, axis, device, embed=False, mask
         for sln, Dn, sla, Da, tm in meta:
             Dn, Da = _as_shape(Dn), _as_shape(Da)
             idx = mask_host[tuple(tm) if isinstance(tm, (list, tuple)) else tm]
+            if isinstance(idx, slice):  # resolved against the masked dimension of this block
+                idx = range(*idx.indices((Dn if embed else Da)[axis]))
             runs = _runs(idx)
             if not embed:
                 for pos, start, ln in runs:
@@ -1042,6 +1044,55 @@ def tds_plan(meta_dot, Areshape, Breshape, Aorder, Border, a_numel, b_numel, Dsi
            _device_key(device))
     return _CACHE.get(key, refs, lambda: TdsPlan(meta_dot, Areshape, Breshape, Aorder, Border, a_numel, b_numel,
                                                   Dsize, device))
+
+
+class DotDiagPlan:
+    """dot_diag work list: one job per output block (out_off, a_off, b_off, n, inner, D), and the jobs
+    grouped by the diagonal block they read, for the adjoint with respect to the diagonal."""
+
+    def __init__(self, meta, axis, a_ndim, a_numel, b_numel, Dsize, device):
+        jobs, groups = [], {}
+        self.covered = 0
+        for sln, slb, Db, sla in meta:
+            Db = _as_shape(Db)
+            if len(Db) != a_ndim:
+                raise ValueError(f"dot_diag block {Db} of rank != {a_ndim}")
+            n, D = math.prod(Db), Db[axis]
+            inner = math.prod(Db[axis + 1:])
+            if sln[1] - sln[0] != n or slb[1] - slb[0] != n or sla[1] - sla[0] != D:
+                raise ValueError(f"dot_diag slices {sln} {slb} {sla} for block {Db}")
+            if n == 0:
+                continue
+            groups.setdefault((sla[0], D), []).append(len(jobs))
+            jobs.append((sln[0], sla[0], slb[0], n, inner, D, 0, 0))
+            self.covered += n
+        ids, grp = [], []
+        for (a_off, D), js in groups.items():
+            grp.append((a_off, D, len(ids), len(js)))
+            ids.extend(js)
+        self.h_jobs = np.array(jobs, dtype=np.int64).reshape(-1, L.YH_DIAG_WORDS)
+        self.h_groups = np.array(grp, dtype=np.int64).reshape(-1, 4)
+        self.h_ids = np.array(ids, dtype=np.int64)
+        L.call('yastn_hip_check_dot_diag_plan', self.h_jobs.ctypes.data, len(self.h_jobs), self.h_groups.ctypes.data,
+               len(self.h_groups), self.h_ids.ctypes.data, len(self.h_ids), Dsize, a_numel, b_numel)
+        self.njobs, self.ngroups = len(self.h_jobs), len(self.h_groups)
+        self.max_n = int(self.h_jobs[:, 3].max()) if self.njobs else 0
+        self.max_D = int(self.h_groups[:, 1].max()) if self.ngroups else 0
+        self.a_covered = sum(D for _, D, _, _ in grp)
+        # the adjoint with respect to B runs the same kernel with the output and dense slices swapped
+        self.h_jobs_t = self.h_jobs[:, [2, 1, 0, 3, 4, 5, 6, 7]].copy()
+        L.call('yastn_hip_check_dot_diag_plan', self.h_jobs_t.ctypes.data, len(self.h_jobs_t), None, 0, None, 0,
+               b_numel, a_numel, Dsize)
+        self.b_covered = int(self.h_jobs[:, 3].sum()) if self.njobs else 0
+        self.d_jobs = torch.from_numpy(self.h_jobs).to(device)
+        self.d_jobs_t = torch.from_numpy(self.h_jobs_t).to(device)
+        self.d_groups = torch.from_numpy(self.h_groups).to(device)
+        self.d_ids = torch.from_numpy(self.h_ids).to(device)
+
+
+def dot_diag_plan(meta, axis, a_ndim, a_numel, b_numel, Dsize, device):
+    return _CACHE.get(('dot_diag', id(meta), axis, a_ndim, a_numel, b_numel, Dsize, _device_key(device)), (meta,),
+                      lambda: DotDiagPlan(meta, axis, a_ndim, a_numel, b_numel, Dsize, device))
 
 
 def qr_plan(meta, itemsize, a_numel, sizes, device):
