@@ -35,7 +35,7 @@ extern "C" {
 #define YH_F64 0
 #define YH_C128 1
 
-#define YH_ABI_VERSION 9
+#define YH_ABI_VERSION 10
 
 /* ---- box-gather plan (one job per copied or zero-filled box) -------------
  * int64 words per job (YH_JOB_WORDS):
@@ -179,9 +179,24 @@ int yastn_hip_svd_panel_sweep(void* work, const double* floors, int dtype, const
                               int slices, void* part, int64_t part_numel, int32_t* d_flags, hipStream_t stream);
 int yastn_hip_check_panel_jobs(const int64_t* h_blocks, int64_t nblocks, const int32_t* h_jobs, int64_t njobs, int b,
                                int mode, int maxch_p);
+/* All sweeps of the panel path in one launch (no host round trip per sweep; capturable in a HIP graph):
+ * a persistent grid of workgroups, all resident at once, meets at a grid barrier after every round;
+ * per round every (job, slice) forms the job's 2b x 2b Gram matrix on the matrix cores, runs the inner
+ * sweep and updates its slice of the stripes.  The iteration stops after the first sweep that rotates
+ * nothing, or after max_sweeps.  d_rounds: nrounds (offset, count) int64 pairs into d_jobs (device);
+ * max_round_jobs: the largest count.  d_state: state_numel >= 4 + max_sweeps + njobs int32 words,
+ * ZERO on entry (the caller clears them); word 3 receives the sweeps used, -1 if not converged within
+ * max_sweeps, -2 if a wait timed out (the iteration was abandoned).  b <= 16.  Every wait of the grid
+ * polls a bounded number of times, so the launch always ends. */
+int yastn_hip_svd_panel_sweeps(void* work, const double* floors, int dtype, const int64_t* d_blocks,
+                               const int32_t* d_jobs, const int64_t* d_rounds, int64_t nrounds, int64_t max_round_jobs,
+                               int b, int slices, int max_sweeps, int32_t* d_state, int64_t state_numel,
+                               int64_t njobs, hipStream_t stream);
+/* d_sweeps (nullable): a device word holding the sweep count written to info (then `sweeps` is
+ * ignored), e.g. word 3 of yastn_hip_svd_panel_sweeps' state. */
 int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const void* work, const double* mu, int dtype, int mode,
                                  const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, int32_t* d_info,
-                                 int sweeps, hipStream_t stream);
+                                 int sweeps, const int32_t* d_sweeps, hipStream_t stream);
 
 /* Split-K for GEMMs with few output tiles and a long K (e.g. a full contraction to a scalar): the
  * host plan cuts K into S chunks whose GEMMs write partial results (row-major M x N each) to a
@@ -219,7 +234,7 @@ int yastn_hip_svd_precond_init(void* work, int dtype, const int64_t* d_qdesc, co
  * by Q.  d_blocks are the block's SVD words (original m, n and output offsets). */
 int yastn_hip_svd_precond_finalize(void* U, void* S, void* Vh, const void* work, int dtype, const int64_t* d_blocks,
                                    const int64_t* d_qdesc, const int32_t* d_ids, int64_t n, int32_t* d_info,
-                                   int sweeps, hipStream_t stream);
+                                   int sweeps, const int32_t* d_sweeps, hipStream_t stream);
 
 /* Batched QR: replaces backend.qr(data, meta, sizes) (backend_np.py:511-520, backend_torch.py:475-484):
  * per block A (m x n) = Q (m x k) R (k x n), k = min(m, n), economic, with the sign of each R row and Q

@@ -153,3 +153,127 @@ def splitk_reduce(h_red, outs, part):
             C[ic] += acc
         else:
             C[ic] = acc
+
+
+class SlotOrder:
+    """Mirror of the kernel's slot order (jacobi.hip SlotOrder): slot k of sub-round r -> (i, j), j == nc
+    when the column has no partner."""
+
+    def __init__(self, intra, ci, cj, nc, k):
+        self.intra, self.ci, self.cj, self.nc, self.k, self.r = intra, ci, cj, nc, k, 0
+        if intra:
+            ne = nc + (nc & 1)
+            self.n1 = ne - 1
+            m = self.n1 if self.n1 > 0 else 1
+            self.ph1, self.ph2 = k % m, (m - k % m) % m
+            self.kind = 0
+        elif ci <= cj:
+            self.kind, self.ph1 = 1, k % cj
+        else:
+            self.kind, self.ph1 = 2, k % ci
+
+    def get(self):
+        k, ci, cj, nc = self.k, self.ci, self.cj, self.nc
+        if self.kind == 0:
+            return (self.r, self.n1) if k == 0 else (min(self.ph1, self.ph2), max(self.ph1, self.ph2))
+        if self.kind == 1:
+            return (k, ci + self.ph1) if k < ci else (ci + self.ph1, nc)
+        return (self.ph1, ci + k if k < cj else nc)
+
+    def advance(self):
+        self.r += 1
+        if self.kind == 0:
+            self.ph1 = 0 if self.ph1 + 1 == self.n1 else self.ph1 + 1
+            self.ph2 = 0 if self.ph2 + 1 == self.n1 else self.ph2 + 1
+        elif self.kind == 1:
+            self.ph1 = 0 if self.ph1 + 1 == self.cj else self.ph1 + 1
+        else:
+            self.ph1 = 0 if self.ph1 + 1 == self.ci else self.ph1 + 1
+
+
+def _rotation(a, b, g, tol, floor2):
+    g2 = abs(g) ** 2
+    if not (g2 > tol * tol * a * b) or a <= floor2 or b <= floor2:
+        return None
+    e = g / abs(g)
+    zeta = 0.5 * (b - a) / abs(g)
+    t = 1.0 / (abs(zeta) + np.sqrt(1 + zeta * zeta))
+    if zeta < 0:
+        t = -t
+    c = 1 / np.sqrt(1 + t * t)
+    return c, c * t, e
+
+
+def inner_sweep(G, intra, ci, cj, tol, floor2, check_cover=True):
+    """The kernel's inner sweep on the pair's Gram matrix G (nc x nc): per sub-round every G block
+    (slot pa rows, slot pb columns) and Q piece (row, slot pb) is recomputed from the previous G, into a
+    fresh buffer.  Returns (Q, rotated); asserts that each sub-round writes every entry of G."""
+    nc = G.shape[0]
+    ne = nc + (nc & 1)
+    npr = ne // 2 if intra else max(ci, cj)
+    nsub = ne - 1 if intra else npr
+    Q = np.eye(nc, dtype=G.dtype)
+    orders = [SlotOrder(intra, ci, cj, nc, k) for k in range(npr)]
+    rotated = False
+    for _ in range(nsub):
+        slots = [o.get() for o in orders]
+        rots = []
+        for i, j in slots:
+            r = None if j >= nc else _rotation(G[i, i].real, G[j, j].real, G[i, j], tol, floor2)
+            rots.append(r)
+            rotated |= r is not None
+        Gn = np.full_like(G, np.nan)
+        J = np.eye(nc, dtype=G.dtype)
+        for (i, j), r in zip(slots, rots):
+            if r is not None:
+                c, s, e = r
+                # columns (w_i, w_j) <- (c w_i - s conj(e) w_j, s e w_i + c w_j)
+                J[i, i], J[j, i], J[i, j], J[j, j] = c, -s * np.conj(e), s * e, c
+        blk = J.conj().T @ G @ J
+        if not check_cover:
+            Gn = blk
+        for pa, (i, j) in enumerate(slots if check_cover else ()):
+            rows = [i] + ([j] if j < nc else [])
+            for pb, (i2, j2) in enumerate(slots):
+                cols = [i2] + ([j2] if j2 < nc else [])
+                Gn[np.ix_(rows, cols)] = blk[np.ix_(rows, cols)]
+        if check_cover:
+            assert not np.isnan(Gn).any(), 'a G entry was not written in a sub-round'
+        G = Gn
+        Q = Q @ J
+        for o in orders:
+            o.advance()
+    # every column pair rotated exactly once in the sweep's order: each (i, j) appears once
+    return Q, rotated
+
+
+def panel_sweeps(plan_jobs, plan_rounds, blocks_words, work, floors, b, max_sweeps, mode_tr):
+    """The persistent kernel's iteration on W (p x q, column-major at work_off) and V (q x q after it):
+    per sweep the rounds in order, per job the Gram matrix of its columns, the inner sweep and W <- W Q,
+    V <- V Q.  Returns the sweeps used (or -1)."""
+    for sweep in range(max_sweeps):
+        rotated = False
+        for off, cnt in plan_rounds:
+            for job in plan_jobs[off:off + cnt]:
+                blk, pi, pj = (int(x) for x in job)
+                w = blocks_words[blk]
+                m, n, w_off = int(w[1]), int(w[2]), int(w[6])
+                tr = m < n
+                p, q = (n, m) if tr else (m, n)
+                intra = pi == pj
+                c0i, c0j = pi * b, pj * b
+                ci = min(b, q - c0i)
+                cj = 0 if intra else min(b, q - c0j)
+                cols = list(range(c0i, c0i + ci)) + ([] if intra else list(range(c0j, c0j + cj)))
+                W = work[w_off:w_off + p * q].reshape(q, p).T
+                V = work[w_off + p * q:w_off + p * q + q * q].reshape(q, q).T
+                G = W[:, cols].conj().T @ W[:, cols]
+                tol = np.finfo(float).eps * np.sqrt(max(p, 1))
+                Qm, rot = inner_sweep(G, intra, ci, cj, tol, floors[blk], check_cover=sweep == 0)
+                if rot:
+                    W[:, cols] = W[:, cols] @ Qm
+                    V[:, cols] = V[:, cols] @ Qm
+                    rotated = True
+        if not rotated:
+            return sweep + 1
+    return -1

@@ -211,3 +211,88 @@ def test_rvb_kagome_20_sweeps_known_answer():
     assert np.allclose(g, R.grad_expected(), rtol=1e-3, atol=1e-5)
     assert abs(float(loss) - loss_ref) <= 1e-10
     assert np.abs(g - g_ref).max() <= 1e-8 * np.abs(g_ref).max()
+
+
+def test_static_truncation_mask_equals_eager_rule():
+    """truncation_mask_static(S, D_block) keeps the same entries as the reference rule
+    truncation_mask(S, tol=0, D_block=D_block, D_total=sum(D_block)) when the kept values are positive, and
+    its device counters flag a zero kept value and a dropped value above the smallest kept one."""
+    cfg = bs.make_config(_plain(), 'U1', device='cpu')
+    L = lambda s, t, D: bs.leg(cfg, s, t, D)
+    a = bs.rand(cfg, [L(1, (-1, 0, 1), (4, 6, 4)), L(-1, (-1, 0, 1), (5, 7, 5))], n=0, seed=2)
+    U, S, V = bs.svd(a, axes=(0, 1))
+    D_block = {(-1,): 2, (0,): 3, (1,): 1}
+    want = bs.truncation_mask(S, tol=0, D_block=D_block, D_total=6)
+    check = torch.zeros(2, dtype=torch.int64)
+    got = bs.truncation_mask_static(S, D_block, check)
+    assert torch.equal(got.data, want.data)
+    assert {k: v.tolist() for k, v in got.host_index.items()} == {(-1,): [0, 1], (0,): [0, 1, 2], (1,): [0]}
+    _, s1, _ = bs.apply_mask(got, U, S, V, axes=(-1, 0, 0))
+    _, s2, _ = bs.apply_mask(want, U, S, V, axes=(-1, 0, 0))
+    assert torch.equal(s1.data, s2.data) and s1.struct == s2.struct
+    # global top-6 here differs from the fixed 2/3/1 split unless the spectrum happens to agree
+    top = np.sort(S.data.numpy())[::-1][:6]
+    kept = np.sort(S.data.numpy()[want.data.numpy()])[::-1]
+    assert check.tolist() == [0, int(not np.array_equal(top, kept))]
+    S0 = S._replace(data=torch.where(torch.arange(S.size) == 0, torch.zeros(()), S.data))
+    check.zero_()
+    bs.truncation_mask_static(S0, D_block, check)
+    assert check[0] == 1
+
+
+def test_dict_serialization_and_split_combine():
+    """Tensor.to_dict / from_dict at levels 0 and 2, and split_data_and_meta / combine_data_and_meta (the
+    reference's _split_combine_dict.py semantics: data arrays in sorted-key order, replaced by positions)."""
+    cfg = bs.make_config(_plain(), 'U1', device='cpu')
+    A, env, sz = C.setup(cfg, D=2, chi=8, seed=1)
+    a = C.double_layer(A)
+    for level in (0, 2):
+        d = a.to_dict(level)
+        b = bs.Tensor.from_dict(d, cfg)
+        assert b.struct == a.struct and b.slices == a.slices and b.hfs == a.hfs and b.trans == a.trans
+        assert torch.equal(torch.as_tensor(b.data), a.data)
+    d = env.to_dict(level=0)
+    data, meta = bs.split_data_and_meta(d)
+    assert len(data) == 8 and meta['env']['b']['data'] == 0 and meta['env']['tl']['data'] == 6  # sorted names
+    back = C.Env.from_dict(bs.combine_data_and_meta(data, meta), cfg)
+    for x, y in zip(back.tensors(), env.tensors()):
+        assert x.struct == y.struct and torch.equal(x.data, y.data)
+    one, m1 = bs.split_data_and_meta(A.to_dict(0), squeeze=True)
+    assert one is A.data and bs.combine_data_and_meta(one, m1)['data'] is A.data
+
+
+@pytest.mark.parametrize('mode', ['nonreentrant', 'reentrant'])
+def test_checkpoint_move_equals_plain(mode):
+    """Per-move checkpointing through split_data_and_meta (the reference's checkpoint_move) gives the same
+    loss and gradient as the unrolled step (<= 1e-12)."""
+    cfg = bs.make_config(_plain(), 'U1', device='cpu')
+    A, env, sz = C.setup(cfg, D=2, chi=8, seed=1)
+    out = []
+    for cm in (False, mode):
+        x = A.data.detach().clone().requires_grad_(True)
+        Ax = A._replace(data=x)
+        e2 = C.ctm_step(env, C.double_layer(Ax), 8, checkpoint_move=cm, A=Ax)
+        loss = C.cluster_expectation(e2, Ax, sz)
+        loss.backward()
+        out.append((float(loss), x.grad))
+    assert abs(out[0][0] - out[1][0]) <= 1e-12
+    assert float((out[0][1] - out[1][1]).norm()) <= 1e-12 * float(out[0][1].norm())
+
+
+def test_functorch_vjp_of_ctm_step_plain_cpu():
+    """torch.func.vjp of one whole CTM step f(C, a) (environment and site buffers in, environment buffer
+    out, static-structure truncation), as the reference's fixed-point AD takes it (fixed_pt.py:853), equals
+    torch.autograd.grad of the same step (plain path on the CPU; the HIP path is checked on the GPU)."""
+    cfg = bs.make_config(_plain(), 'U1', device='cpu')
+    A, env, sz = C.setup(cfg, D=2, chi=8, seed=1)
+    opts = C.static_svd_opts(8)
+    tmpl = C.ctm_step(env, C.double_layer(A), 8, opts)
+    f = C.ctm_step_flat(A, tmpl, 8, opts)
+    C0 = C.env_flat(tmpl).detach()
+    out, vjp = torch.func.vjp(f, C0, A.data)
+    v = torch.as_tensor(np.random.default_rng(3).standard_normal(out.numel()))
+    gC, gA = vjp(v)
+    c = C0.clone().requires_grad_(True)
+    x = A.data.clone().requires_grad_(True)
+    rC, rA = torch.autograd.grad(f(c, x), (c, x), grad_outputs=v)
+    assert torch.allclose(gC, rC, rtol=1e-12, atol=1e-13) and torch.allclose(gA, rA, rtol=1e-12, atol=1e-13)

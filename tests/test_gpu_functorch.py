@@ -51,3 +51,30 @@ def test_vjp_and_checkpoint_match_autograd():
     checkpoint(f, a3, s3, use_reentrant=True).backward()
     assert torch.linalg.norm(a3.grad - ga) <= 1e-12 * torch.linalg.norm(ga)
     assert torch.linalg.norm(s3.grad - gs) <= 1e-12 * torch.linalg.norm(gs)
+
+
+@pytest.mark.parametrize('D,chi,dtype', [(3, 24, 'float64'), (2, 16, 'complex128')])
+def test_vjp_of_whole_ctm_step(D, chi, dtype):
+    """torch.func.vjp of one whole CTM step f(C, a) through the HIP backend (merges, grouped GEMMs, projector
+    SVDs with fix_signs, dot_diag, masks, unmerges), as the reference's fixed-point AD takes it
+    (fixed_pt.py:853), equals torch.autograd.grad of the same step: <= 1e-11 relative."""
+    from yastn_amd import backend_hip
+    from yastn_amd import blocksparse as bs
+    from yastn_amd import ctmrg as C
+    cfg = bs.make_config(backend_hip, 'U1', device='cuda:0')
+    A, env, _ = C.setup(cfg, D=D, chi=chi, dtype=dtype, seed=1)
+    opts = C.static_svd_opts(chi)
+    tmpl = C.ctm_step(env, C.double_layer(A), chi, opts)
+    f = C.ctm_step_flat(A, tmpl, chi, opts)
+    C0 = C.env_flat(tmpl).detach()
+    print('vjp of a CTM step', D, chi, dtype, flush=True)
+    out, vjp = torch.func.vjp(f, C0, A.data)
+    rng = np.random.default_rng(4)
+    v = torch.as_tensor(rng.standard_normal(out.numel()) + (1j * rng.standard_normal(out.numel())
+                        if out.is_complex() else 0), device=DEV)
+    gC, gA = vjp(v)
+    c = C0.clone().requires_grad_(True)
+    x = A.data.clone().requires_grad_(True)
+    rC, rA = torch.autograd.grad(f(c, x), (c, x), grad_outputs=v)
+    for g, r in ((gC, rC), (gA, rA)):
+        assert float((g - r).norm()) <= 1e-11 * float(r.norm())

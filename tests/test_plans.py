@@ -307,3 +307,46 @@ def test_fast_apply_matches_public_apply(fast, monkeypatch):
     np.testing.assert_array_equal(x.grad.numpy(), 8.0 * np.arange(5.0))
     gx, = torch.func.vjp(lambda t: _Twice.apply(t, 3.0), x.detach())[1](torch.ones(5, dtype=torch.float64))
     np.testing.assert_array_equal(gx.numpy(), np.full(5, 3.0))
+
+
+def test_one_launch_panel_sweeps_schedule_emulated():
+    """The schedule of the one-launch panel Jacobi (yastn_hip_svd_panel_sweeps), run in numpy on a real
+    SvdPlan's job and round lists: on the QR-preconditioned blocks (X = R^H) every sub-round of every
+    pair writes every entry of its Gram matrix (double-buffered sweep), each column pair of a block is
+    rotated once per sweep, the iteration converges, and the column norms are the singular values."""
+    import plan_emulator as E
+    rng = np.random.default_rng(5)
+    shapes = [(200, 150), (130, 130), (300, 100), (90, 140), (17, 12)]
+    blocks, a_off, u_off, s_off, v_off, mats = [], 0, 0, 0, 0, []
+    for m, n in shapes:
+        k = min(m, n)
+        blocks.append((a_off, m, n, u_off, s_off, v_off))
+        mats.append(rng.standard_normal((m, n)) * np.logspace(0, -8, n)[None, :])
+        a_off, u_off, s_off, v_off = a_off + m * n, u_off + m * k, s_off + k, v_off + k * n
+    plan = P.SvdPlan(blocks, 8, 0, a_off, u_off, s_off, v_off, torch.device('cpu'))
+    assert plan.n_big >= 3 and plan.n_pre == plan.n_big and plan.panel_width <= 16
+    work = np.zeros(plan.work_numel)
+    floors = np.zeros(len(blocks))
+    big = set(int(x) for x in plan.h_big)
+    for bi, ((m, n), A) in enumerate(zip(shapes, mats)):
+        if bi not in big:
+            continue
+        W0 = A.conj().T if m < n else A
+        R = np.linalg.qr(W0, mode='r')
+        X = R.conj().T
+        q = X.shape[1]
+        x_off = int(plan.h_jblocks[bi][6])
+        work[x_off:x_off + q * q] = X.T.ravel()          # column-major X
+        work[x_off + q * q:x_off + 2 * q * q] = np.eye(q).ravel()
+        floors[bi] = np.finfo(float).eps ** 2 * np.sum(X ** 2)
+    sweeps = E.panel_sweeps(plan.h_jobs, plan.h_rounds, plan.h_jblocks, work, floors, plan.panel_width, 60, None)
+    assert 0 < sweeps < 30, sweeps
+    for bi, ((m, n), A) in enumerate(zip(shapes, mats)):
+        if bi not in big:
+            continue
+        q = min(m, n)
+        x_off = int(plan.h_jblocks[bi][6])
+        W = work[x_off:x_off + q * q].reshape(q, q).T
+        s = np.sort(np.linalg.norm(W, axis=0))[::-1]
+        s_ref = np.linalg.svd(A, compute_uv=False)
+        assert np.max(np.abs(s - s_ref)) <= 1e-12 * s_ref[0]

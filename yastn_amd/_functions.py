@@ -454,11 +454,50 @@ class DotDiag(_Fn):
 
 # ------------------------------------------------------------------ SVD / eigh
 
+# One-launch panel sweeps (all sweeps of the panel path in one persistent kernel, no host read per sweep);
+# off: one launch per round with a host check after every sweep
+ONE_LAUNCH_SWEEPS = os.environ.get('YASTN_HIP_ONE_LAUNCH_SWEEPS', '1') != '0'
+_DEFERRED = []  # stack of lists collecting convergence flags to check later (deferred_checks)
+
+
+class deferred_checks:
+    """Within this context (and always while a HIP graph is being captured into it) the convergence flags
+    of the Jacobi kernels are not read on the host after each call: they are collected, and ``check()``
+    reads them all at once (one synchronisation), e.g. after each replay of a captured graph."""
+
+    def __init__(self):
+        self.flags = []
+
+    def __enter__(self):
+        _DEFERRED.append(self.flags)
+        return self
+
+    def __exit__(self, *exc):
+        _DEFERRED.remove(self.flags)
+        return False
+
+    def check(self):
+        for what, info in self.flags:
+            _raise_on_info(info, what)
+
+
+def _raise_on_info(info, what):
+    if info.numel() == 0:
+        return
+    lo, bad = torch.stack([info.min(), (info < 0).sum()]).tolist()
+    if lo == -2:
+        raise L.HipBackendError(f"{what}: the one-launch Jacobi sweeps were abandoned (a grid wait timed out)")
+    if bad:
+        raise L.HipBackendError(f"{what}: Jacobi did not converge in {MAX_SWEEPS} sweeps for {bad} block(s)")
+
+
 def _check_info(info, what):
-    if SYNC_CHECKS and info.numel() > 0:
-        bad = int((info < 0).sum().item())
-        if bad:
-            raise L.HipBackendError(f"{what}: Jacobi did not converge in {MAX_SWEEPS} sweeps for {bad} block(s)")
+    if _DEFERRED:
+        _DEFERRED[-1].append((what, info))
+    elif torch.cuda.is_current_stream_capturing():
+        raise RuntimeError("yastn_amd: SVD/eigh captured in a HIP graph needs deferred_checks() around the capture")
+    elif SYNC_CHECKS:
+        _raise_on_info(info, what)
 
 
 _SIDE = {}
@@ -502,7 +541,6 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
         if plan.n_big:
             mu = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
             floors = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
-            flags = torch.zeros(max(plan.njobs, 1), dtype=torch.int32, device=data.device)
             L.call('yastn_hip_svd_panel_init', ptr(data), ptr(work), ptr(mu), ptr(floors), dt, mode,
                    plan.d_blocks.data_ptr(), plan.d_big.data_ptr(), plan.n_big, st)
             if plan.n_pre:
@@ -515,9 +553,19 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
                             run_gemm(gp, (work,), (work,), dt, tune=False)
                 L.call('yastn_hip_svd_precond_init', ptr(work), dt, plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(),
                        plan.n_pre, st)
-            part = torch.empty(max(plan.part_numel, 1), dtype=torch.float64, device=data.device)
-            converged = False
-            while sweeps < MAX_SWEEPS:
+            d_sweeps = None
+            if ONE_LAUNCH_SWEEPS and plan.panel_width <= 16:
+                # all sweeps in one persistent launch; the sweep count stays on the device (state word 3)
+                state = torch.zeros(4 + MAX_SWEEPS + plan.njobs, dtype=torch.int32, device=data.device)
+                L.call('yastn_hip_svd_panel_sweeps', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
+                       plan.d_jobs.data_ptr(), plan.d_rounds.data_ptr(), len(plan.rounds), plan.max_round_jobs,
+                       plan.panel_width, plan.sweep_slices, MAX_SWEEPS, ptr(state), state.numel(), plan.njobs, st)
+                d_sweeps = state.data_ptr() + 3 * state.element_size()
+            converged = d_sweeps is not None
+            if not converged:
+                part = torch.empty(max(plan.part_numel, 1), dtype=torch.float64, device=data.device)
+                flags = torch.zeros(max(plan.njobs, 1), dtype=torch.int32, device=data.device)
+            while not converged and sweeps < MAX_SWEEPS:
                 L.call('yastn_hip_svd_panel_sweep', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
                        plan.d_jobs.data_ptr(), plan.h_rounds.ctypes.data, len(plan.rounds), plan.panel_width,
                        plan.maxch_p, plan.panel_slices, ptr(part), part.numel(), ptr(flags), st)
@@ -532,11 +580,11 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
             if plan.n_plain:
                 L.call('yastn_hip_svd_panel_finalize', ptr(out_a), ptr(out_s), ptr(out_b) if mode == 0 else None,
                        ptr(work), ptr(mu), dt, mode, plan.d_blocks.data_ptr(), plan.d_plain.data_ptr(), plan.n_plain,
-                       ptr(info), done, st)
+                       ptr(info), done, d_sweeps, st)
             if plan.n_pre:
                 L.call('yastn_hip_svd_precond_finalize', ptr(out_a), ptr(out_s), ptr(out_b), ptr(work), dt,
                        plan.d_blocks.data_ptr(), plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(), plan.n_pre, ptr(info),
-                       done, st)
+                       done, d_sweeps, st)
                 # p-side vectors: [V_J; 0] -> Q [V_J; 0], panels in reverse order
                 for h1, h2, h3 in plan.q_apply:
                     run_gemm(h1, (work,), (work, out_a, out_b), dt, tune=False)
@@ -777,9 +825,13 @@ def fix_svd_signs(Udata, Vdata, meta):
         V2 = raw(Vdata).clone()
         L.call('yastn_hip_fix_svd_signs', ptr(U2), ptr(V2), dt, blocks.data_ptr(), nb, stream_of(U2))
         return U2, V2
-    Ur = raw(Udata.detach())
-    piv = torch.empty(max(s_numel, 1), dtype=torch.int64, device=Udata.device)
-    L.call('yastn_hip_svd_sign_pivots', ptr(Ur), ptr(piv), dt, blocks.data_ptr(), nb, stream_of(Ur))
+    def pivots(u):  # on plain tensors: under a torch.func transform _Opaque hands it the unwrapped U
+        u = raw(u)
+        piv = torch.empty(max(s_numel, 1), dtype=torch.int64, device=u.device)
+        L.call('yastn_hip_svd_sign_pivots', ptr(u), ptr(piv), dt, blocks.data_ptr(), nb, stream_of(u))
+        return piv
+    U0 = Udata.detach()
+    piv = _Opaque.apply(pivots, U0) if _is_wrapped(U0) else pivots(U0)
     ph = Udata[piv[:s_numel]]
     ph = ph / ph.abs()
     return Udata * ph.conj()[col], Vdata * ph[row]

@@ -295,17 +295,19 @@ def trace(data, order, meta, Dsize):
 
 
 def rsqrt(data, cutoff=0):
-    out = torch.zeros_like(data)
+    """1/sqrt(x) where |x| > cutoff, else 0.  Selected with torch.where rather than boolean indexing, so
+    there is no host synchronisation (capturable in a HIP graph); values and gradients equal the
+    indexed form, and the unselected entries see a harmless 1 inside rsqrt."""
     keep = data.abs() > cutoff
-    out[keep] = data[keep].rsqrt()
-    return out
+    safe = torch.where(keep, data, torch.ones_like(data))
+    return torch.where(keep, safe.rsqrt(), torch.zeros_like(data))
 
 
 def reciprocal(data, cutoff=0):
-    out = torch.zeros_like(data)
+    """1/x where |x| > cutoff, else 0 (no host synchronisation, as rsqrt)."""
     keep = data.abs() > cutoff
-    out[keep] = 1. / data[keep]
-    return out
+    safe = torch.where(keep, data, torch.ones_like(data))
+    return torch.where(keep, 1. / safe, torch.zeros_like(data))
 
 
 def exp(data, step):

@@ -272,6 +272,45 @@ class Tensor:
     def svd(self, axes=(0, 1), **kw):
         return svd(self, axes=axes, **kw)
 
+    def to_dict(self, level=2):
+        """Everything needed to rebuild the tensor (the fields of the reference's Tensor.to_dict,
+        tensor/_output.py:35-110): at level 0 the structure objects as they are, at level >= 1 as basic
+        Python data, at level 2 also the data as a numpy array."""
+        nd = len(self.struct.s)
+        if level >= 1:
+            config = self.config._asdict()
+            config['sym'] = self.config.sym.SYM_ID
+            config['backend'] = getattr(self.config.backend, 'BACKEND_ID', None)
+            struct = self.struct._asdict()
+            slices = [tuple(x) for x in self.slices]
+            hfs = tuple(h._asdict() for h in self.hfs)
+        else:
+            config, struct, slices, hfs = self.config, self.struct, self.slices, self.hfs
+        data = self._data if level < 2 else self.config.backend.to_numpy(self._data)
+        return {'type': 'Tensor', 'dict_ver': 2, 'level': level, 'config': config, 'data': data,
+                'struct': struct, 'slices': slices, 'trans': self.trans, 'isdiag': self.isdiag, 'hfs': hfs,
+                'mfs': ((1,),) * nd}
+
+    @staticmethod
+    def from_dict(d, config=None):
+        """Inverse of to_dict (reference tensor/_initialize.py from_dict); a level >= 1 dictionary needs the
+        config (its backend and symmetry) or takes the configuration's names from a passed config."""
+        if d.get('level', 0) >= 1 or isinstance(d['struct'], dict):
+            if config is None:
+                raise BlockSparseError('from_dict of a level >= 1 dictionary needs a config')
+            tup = lambda x: tuple(tup(y) for y in x) if isinstance(x, (list, tuple)) else x
+            sd = d['struct']
+            struct = Struct(s=tup(sd['s']), n=tup(sd['n']), diag=bool(sd['diag']), t=tup(sd['t']), D=tup(sd['D']),
+                            size=int(sd['size']))
+            slices = tuple(Slc(tup(x[0]), tup(x[1]), int(x[2])) for x in d['slices'])
+            hfs = tuple(Fusion(**{k: (tup(v) if k != 'op' else v) for k, v in h.items()}) for h in d['hfs'])
+            data = d['data']
+            if isinstance(data, np.ndarray):
+                dtype = 'complex128' if np.iscomplexobj(data) else ('bool' if data.dtype == np.bool_ else 'float64')
+                data = config.backend.to_tensor(data, dtype=dtype, device=config.device)
+            return Tensor(config, struct, slices, data, hfs=hfs, trans=tup(d['trans']))
+        return Tensor(config or d['config'], d['struct'], d['slices'], d['data'], hfs=d['hfs'], trans=d['trans'])
+
     def to_dense(self):
         """Dense numpy array (legs in logical order, sectors in ascending charge)."""
         a = consume_transpose(self)
@@ -347,6 +386,42 @@ def from_blocks(config, s, n, t, D, flat, isdiag=False):
         dtype = 'complex128' if np.iscomplexobj(flat) else ('bool' if flat.dtype == np.bool_ else 'float64')
         flat = config.backend.to_tensor(flat, dtype=dtype, device=config.device)
     return Tensor(config, struct, _slices_of(Dps, D), flat)
+
+
+def split_data_and_meta(d, squeeze=False):
+    """(data, meta) of a to_dict dictionary (the reference's yastn/_split_combine_dict.py:21-45): every
+    'data' entry, visited in sorted key order through nested dictionaries, goes to the data tuple and is
+    replaced in the meta copy by its position.  With ``squeeze``, a single data array is not wrapped."""
+    data = []
+
+    def walk(x):
+        out = {}
+        for k in sorted(x):
+            if k == 'data':
+                data.append(x[k])
+                out[k] = len(data) - 1
+            elif isinstance(x[k], dict):
+                out[k] = walk(x[k])
+            else:
+                out[k] = x[k]
+        return out
+    meta = walk(d)
+    return (data[0] if squeeze and len(data) == 1 else tuple(data)), meta
+
+
+def combine_data_and_meta(data, meta):
+    """Inverse of split_data_and_meta (reference _split_combine_dict.py:57-70)."""
+    if not isinstance(data, (list, tuple)):
+        data = [data]
+    out = {}
+    for k in sorted(meta):
+        if k == 'data':
+            out[k] = data[meta[k]]
+        elif isinstance(meta[k], dict):
+            out[k] = combine_data_and_meta(data, meta[k])
+        else:
+            out[k] = meta[k]
+    return out
 
 
 def seeded_normal(n, dtype, seed):
@@ -1018,7 +1093,10 @@ def apply_mask(mask, *args, axes=0):
     axes = tuple(axes) if hasattr(axes, '__iter__') else (axes,)
     nsym = mask.config.sym.NSYM
     be = mask.config.backend
-    m = {t[:nsym]: be.to_mask(mask._data[sl.slcs[0][0]:sl.slcs[0][1]]) for t, sl in zip(mask.struct.t, mask.slices)}
+    m = getattr(mask, 'host_index', None)  # a static mask knows its indices on the host
+    if m is None:
+        m = {t[:nsym]: be.to_mask(mask._data[sl.slcs[0][0]:sl.slcs[0][1]])
+             for t, sl in zip(mask.struct.t, mask.slices)}
     mask_t = tuple(m.keys())
     mask_D = tuple(len(v) for v in m.values())
     out = []
@@ -1250,6 +1328,49 @@ def truncation_mask_multiplets(S, tol=0, D_total=float('inf'), eps_multiplet=1e-
             keep[lo:lo + k] = both
             keep[lo2:lo2 + k] = both
     return S._replace(data=keep)
+
+
+_STATIC_MASKS = {}
+
+
+def truncation_mask_static(S, D_block, check=None):
+    """Mask keeping the first D_block[t] values of every block t of the diagonal S (the values of a block
+    are in descending order), decided from the structure alone: S is not read on the host, so the mask
+    (and the step around it) can be captured in a HIP graph and replayed.  Blocks absent from D_block
+    keep nothing.  It equals truncation_mask(S, tol=0, D_block=D_block, D_total=sum(D_block)) whenever
+    every kept value is positive.  With ``check`` (a device int64 tensor of two counters) the step adds, on
+    the device, 1 to check[0] when a kept value is 0 (the eager rule would then keep fewer: the fixed
+    structure is invalid) and 1 to check[1] when a dropped value exceeds the smallest kept one (the
+    reference's global rule truncation_mask(S, D_total=sum) would pick another set)."""
+    nsym = S.config.sym.NSYM
+    be = S.config.backend
+    norm_key = lambda k: (k,) if isinstance(k, (int, np.integer)) else tuple(k)
+    want = {norm_key(k): int(v) for k, v in D_block.items()}
+    key = (S.struct, S.slices, tuple(sorted(want.items())), str(S.config.device))
+    hit = _STATIC_MASKS.get(key)
+    if hit is None:
+        keep = np.zeros(S.struct.size, dtype=bool)
+        index = {}
+        for t, sl in zip(S.struct.t, S.slices):
+            lo, hi = sl.slcs[0]
+            k = min(want.get(t[:nsym], 0), hi - lo)
+            keep[lo:lo + k] = True
+            index[t[:nsym]] = be.to_mask(keep[lo:hi])  # host index list, in the backend's mask type
+        hit = (index, be.to_tensor(keep, dtype='bool', device=S.config.device))
+        if len(_STATIC_MASKS) > 256:
+            _STATIC_MASKS.clear()
+        _STATIC_MASKS[key] = hit
+    index, kept = hit
+    if check is not None:
+        import torch
+        s = be.detach(S._data)
+        inf = torch.full_like(s, float('inf'))
+        kmin = torch.where(kept, s, inf).amin()
+        dmax = torch.where(kept, -inf, s).amax()
+        check.add_(torch.stack([(kmin <= 0), (dmax > kmin)]).to(check.dtype))
+    m = S._replace(data=kept)
+    m.host_index = index
+    return m
 
 
 def svd_with_truncation(a, axes=(0, 1), sU=1, nU=True, Uaxis=-1, Vaxis=0, fix_signs=False, tol=0, tol_block=0,

@@ -34,6 +34,14 @@ class Env:
     def tensors(self):
         return [getattr(self, k) for k in self.NAMES]
 
+    def to_dict(self, level=2):
+        """{'type', 'env': {name: Tensor.to_dict}} (the environment part of the reference's EnvCTM.to_dict)."""
+        return {'type': 'EnvCTM1x1', 'dict_ver': 1, 'env': {k: getattr(self, k).to_dict(level) for k in self.NAMES}}
+
+    @classmethod
+    def from_dict(cls, d, config=None):
+        return cls(**{k: bs.Tensor.from_dict(v, config) for k, v in d['env'].items()})
+
 
 def double_layer(A, op=None):
     """a(t, l, b, r) = sum_p A(t, l, b, r, p) [op] A*(t, l, b, r, p), legs fused (ket, bra) pairwise."""
@@ -161,11 +169,35 @@ def move_v(env, a, chi, svd_opts=None, batch=False):
     return Env(**kw)
 
 
-def ctm_step(env, a, chi, svd_opts=None, batch=False):
+def ctm_step(env, a, chi, svd_opts=None, batch=False, checkpoint_move=False, A=None):
     """One 'hv' CTMRG step: 4 projector SVDs (two per move; with ``batch`` the two of a move go out as one
-    batched SVD call, the reference computes them one after the other)."""
-    env = move_h(env, a, chi, svd_opts, batch)
-    return move_v(env, a, chi, svd_opts, batch)
+    batched SVD call, the reference computes them one after the other).
+
+    checkpoint_move ('reentrant' / 'nonreentrant', needs the site tensor A): each move is a checkpoint region
+    whose inputs and outputs cross it as split_data_and_meta of the environment and site dictionaries, as the
+    reference's EnvCTM.update_ does (_env_ctm.py:566-589); the move is recomputed in the backward pass."""
+    if not checkpoint_move:
+        env = move_h(env, a, chi, svd_opts, batch)
+        return move_v(env, a, chi, svd_opts, batch)
+    import torch.utils.checkpoint as tc
+    cfg = A.config
+
+    def f_update(move, meta, *inputs):
+        d = bs.combine_data_and_meta(inputs, meta)
+        e = Env.from_dict(d, cfg)
+        aa = double_layer(bs.Tensor.from_dict(d['psi'], cfg))
+        e = (move_h if move == 'h' else move_v)(e, aa, chi, svd_opts, batch)
+        data, meta_out = bs.split_data_and_meta(e.to_dict(level=0))
+        return meta_out, *data
+
+    for move in 'hv':
+        d = env.to_dict(level=0)
+        d['psi'] = A.to_dict(level=0)
+        inputs, meta = bs.split_data_and_meta(d)
+        meta_out, *data = tc.checkpoint(f_update, move, meta, *inputs,
+                                        use_reentrant=checkpoint_move == 'reentrant')
+        env = Env.from_dict(bs.combine_data_and_meta(data, meta_out), cfg)
+    return env
 
 
 def cluster_expectation(env, A, op):
@@ -399,6 +431,35 @@ def rdm_loss(A, chi, sweeps, init='dl', svd_opts=None, segment=0, block=None):
     return val.real if hasattr(val, 'is_complex') and val.is_complex() else val
 
 
+# ------------------------------------------------------------------ CTM step as a function of flat buffers
+
+def env_flat(env):
+    """The eight environment buffers concatenated (the 1-D form the reference's FixedPoint differentiates,
+    fixed_pt.py _concat_data)."""
+    import torch
+    return torch.cat([t.data.reshape(-1) for t in env.tensors()])
+
+
+def env_from_flat(template, flat):
+    """An environment with template's structures and the buffers cut from ``flat``."""
+    out, off = {}, 0
+    for k in Env.NAMES:
+        t = getattr(template, k)
+        out[k] = t._replace(data=flat[off:off + t.size])
+        off += t.size
+    return Env(**out)
+
+
+def ctm_step_flat(A, template, chi, svd_opts=None):
+    """f(C, a) = one CTM step of the environment buffer C for the site buffer a, flattened: the map whose
+    vector-Jacobian products the reference's fixed-point AD takes with torch.func.vjp (fixed_pt.py:853).
+    The output structure must stay that of ``template`` (static-structure truncation)."""
+    def f(C, adata):
+        e = ctm_step(env_from_flat(template, C), double_layer(A._replace(data=adata)), chi, svd_opts)
+        return env_flat(e)
+    return f
+
+
 # ------------------------------------------------------------------ workload setup
 
 def site_legs(config, D):
@@ -479,6 +540,80 @@ def fwd_bwd(A, env, sz, chi, svd_opts=None, batch=False):
     loss = loss_fn(A2, env, sz, chi, svd_opts, batch)
     loss.backward()
     return loss.detach(), data.grad
+
+
+def env_split(chi):
+    """Per-charge dimensions of the environment leg at bond dimension chi (the setup's split)."""
+    et, eD = env_charges(chi)
+    return {(t,): D for t, D in zip(et, eD)}
+
+
+def static_svd_opts(chi, check=None):
+    """Projector truncation with a fixed structure: every projector keeps env_split(chi)[t] values of sector
+    t (blocksparse.truncation_mask_static, passed as the reference's custom mask function mask_f), so the
+    environment keeps its sectors from step to step and a step has no host synchronisation."""
+    D_block = env_split(chi)
+    return {'mask_f': lambda S: bs.truncation_mask_static(S, D_block, check)}
+
+
+class CapturedStep:
+    """A CTMRG step and its loss, forward and backward, captured as one HIP graph (torch.cuda.CUDAGraph):
+    static-structure truncation, the Jacobi convergence flags and the structure check read after a replay
+    (one synchronisation), all plans built and tuned by ``warmup`` eager runs first.
+
+    ``run(data)`` is the step (it must call backward on the loss and return it); the graph's inputs are the
+    tensors it closes over, ``data`` (requires grad) among them.  ``replay()`` returns the loss and gradient
+    tensors of the graph (overwritten by the next replay)."""
+
+    def __init__(self, run, data, check, warmup=2):
+        import torch
+        from . import _functions as F
+        self.run, self.data, self.check = run, data, check
+        for _ in range(warmup):
+            self.data.grad = None
+            self.run(self.data)
+        torch.cuda.synchronize()
+        self.data.grad = None
+        self.graph = torch.cuda.CUDAGraph()
+        self.checks = F.deferred_checks()
+        with self.checks:
+            with torch.cuda.graph(self.graph):
+                self.loss = self.run(self.data)
+        self.grad = self.data.grad
+
+    def replay(self):
+        self.check.zero_()
+        self.graph.replay()
+        return self.loss, self.grad
+
+    def validate(self):
+        """Raise if a Jacobi call of the last replay did not converge or the fixed structure was invalid;
+        return the number of projector truncations where the global rule would have kept another set."""
+        import torch
+        torch.cuda.synchronize()
+        self.checks.check()
+        zero_kept, other_set = self.check.tolist()
+        if zero_kept:
+            raise RuntimeError(f"static truncation kept a zero singular value in {zero_kept} projector(s)")
+        return other_set
+
+
+def captured_fwd_bwd(A, env, sz, chi, warmup=2):
+    """One CTMRG step + <Sz> with static-structure projectors (fwd + bwd) as a CapturedStep; also returns the
+    eager (loss, grad) of the same step for comparison."""
+    import torch
+    check = torch.zeros(2, dtype=torch.int64, device=A.data.device)
+    opts = static_svd_opts(chi, check)
+
+    def run(data):
+        loss = loss_fn(A._replace(data=data), env, sz, chi, opts)
+        loss.backward()
+        return loss
+    data = A.data.detach().clone().requires_grad_(True)
+    cs = CapturedStep(run, data, check, warmup)
+    d2 = A.data.detach().clone().requires_grad_(True)
+    l2 = run(d2)
+    return cs, (l2.detach(), d2.grad)
 
 
 def bench(steps=3, D=6, chi=64, plain=True, cpu=True):
