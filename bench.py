@@ -361,6 +361,7 @@ def main():
     ap.add_argument('--unrolled', action='store_true',
                     help='add the unrolled CTMRG workload (D=8, chi=128, complex128, 20 steps + energy)')
     ap.add_argument('--only-unrolled', action='store_true', help='run the unrolled CTMRG workload alone')
+    ap.add_argument('--only-ctmrg', action='store_true', help='run the D=6, chi=64 CTMRG step comparison alone')
     ap.add_argument('--unrolled-steps', type=int, default=20)
     ap.add_argument('--unrolled-plain-steps', type=int, default=1,
                     help='CTMRG steps of the short batched-vs-plain comparison (0 skips the plain path)')
@@ -392,6 +393,12 @@ def main():
         out['ctmrg_unrolled'] = ctmrg.bench_unrolled(n_iter=args.unrolled_steps,
                                                      plain_iter=args.unrolled_plain_steps,
                                                      no_checkpoint_memory=args.unrolled_no_checkpoint)
+        print(json.dumps(out), flush=True)
+        return
+    if args.only_ctmrg:
+        from yastn_amd import ctmrg
+        print('bench: ctmrg', flush=True)
+        out['ctmrg'] = ctmrg.bench(steps=max(2, args.steps // 5))
         print(json.dumps(out), flush=True)
         return
     print('bench: heff2', flush=True)

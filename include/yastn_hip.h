@@ -35,7 +35,7 @@ extern "C" {
 #define YH_F64 0
 #define YH_C128 1
 
-#define YH_ABI_VERSION 10
+#define YH_ABI_VERSION 11
 
 /* ---- box-gather plan (one job per copied or zero-filled box) -------------
  * int64 words per job (YH_JOB_WORDS):
@@ -179,21 +179,23 @@ int yastn_hip_svd_panel_sweep(void* work, const double* floors, int dtype, const
                               int slices, void* part, int64_t part_numel, int32_t* d_flags, hipStream_t stream);
 int yastn_hip_check_panel_jobs(const int64_t* h_blocks, int64_t nblocks, const int32_t* h_jobs, int64_t njobs, int b,
                                int mode, int maxch_p);
-/* All sweeps of the panel path in one launch (no host round trip per sweep; capturable in a HIP graph):
- * a persistent grid of workgroups, all resident at once, meets at a grid barrier after every round;
- * per round every (job, slice) forms the job's 2b x 2b Gram matrix on the matrix cores, runs the inner
- * sweep and updates its slice of the stripes.  The iteration stops after the first sweep that rotates
- * nothing, or after max_sweeps.  d_rounds: nrounds (offset, count) int64 pairs into d_jobs (device);
- * max_round_jobs: the largest count.  d_state: state_numel >= 4 + max_sweeps + njobs int32 words,
- * ZERO on entry (the caller clears them); word 3 receives the sweeps used, -1 if not converged within
- * max_sweeps, -2 if a wait timed out (the iteration was abandoned).  b <= 16.  Every wait of the grid
- * polls a bounded number of times, so the launch always ends. */
-int yastn_hip_svd_panel_sweeps(void* work, const double* floors, int dtype, const int64_t* d_blocks,
-                               const int32_t* d_jobs, const int64_t* d_rounds, int64_t nrounds, int64_t max_round_jobs,
-                               int b, int slices, int max_sweeps, int32_t* d_state, int64_t state_numel,
-                               int64_t njobs, hipStream_t stream);
+/* One sweep of the panel path with one fused launch per round: per (job, slice) a workgroup forms the
+ * job's 2b x 2b Gram matrix on the matrix cores, waits for the job's other slices (a counter per job),
+ * runs the inner sweep and updates its slice of the stripes of W and V in place.  `sweep` indexes the
+ * per-sweep rotation flags: a round of sweep s > 0 whose previous sweep rotated nothing exits at once,
+ * so a fixed number of sweeps can be launched with no host read in between (HIP graph capture).
+ * h_rounds: nrounds (offset, count) pairs into d_jobs (host); d_state: state_numel >= 4 + max_sweeps +
+ * njobs int32 words, ZERO before sweep 0 (the caller clears them).  b <= 16.  The wait for sibling slices
+ * polls a bounded number of times (it then abandons the iteration: result -2).
+ * yastn_hip_svd_panel_result writes word 3 of the state: the sweeps used (the first of the `launched`
+ * sweeps that rotated nothing, + 1), -1 if every launched sweep rotated, -2 if abandoned. */
+int yastn_hip_svd_panel_rounds(void* work, const double* floors, int dtype, const int64_t* d_blocks,
+                               const int32_t* d_jobs, const int64_t* h_rounds, int64_t nrounds, int b, int slices,
+                               int sweep, int max_sweeps, int32_t* d_state, int64_t state_numel, int64_t njobs,
+                               hipStream_t stream);
+int yastn_hip_svd_panel_result(int32_t* d_state, int64_t state_numel, int launched, hipStream_t stream);
 /* d_sweeps (nullable): a device word holding the sweep count written to info (then `sweeps` is
- * ignored), e.g. word 3 of yastn_hip_svd_panel_sweeps' state. */
+ * ignored), e.g. word 3 of the panel rounds' state. */
 int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const void* work, const double* mu, int dtype, int mode,
                                  const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, int32_t* d_info,
                                  int sweeps, const int32_t* d_sweeps, hipStream_t stream);

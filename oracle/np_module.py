@@ -22,6 +22,7 @@ embed_mask = O.embed_mask
 transpose = O.transpose
 transpose_dot_sum = O.transpose_dot_sum
 dot_diag = O.dot_diag
+qr = O.qr
 
 
 def zeros(D, dtype='float64', device='cpu'):

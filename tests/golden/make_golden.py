@@ -29,7 +29,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 # hot-path backend functions whose calls are recorded
 RECORDED = ('transpose_and_merge', 'unmerge', 'dot', 'svd', 'eigh', 'fix_svd_signs',
-            'apply_mask', 'transpose', 'embed_mask', 'transpose_dot_sum', 'dot_diag')
+            'apply_mask', 'transpose', 'embed_mask', 'transpose_dot_sum', 'dot_diag', 'qr')
 
 
 def _stub_opt_einsum():
@@ -528,6 +528,26 @@ def more(ref_root, only=None):
             res.append({'S': which, 'opts': _jsonable(opts)})
         sc.dump({'cases': res})
 
+    # ---------------- qr (yastn/tensor/linalg.py:738-830): metas for both signatures and charges -------
+    if want('qr_u1'):
+        rec = Recorder()
+        sc = Scenario('qr_u1', rec)
+        c = cfg('U1', rec)
+        la, _ = rank4_legs(c)
+        a = tensor(c, la, 1, 'float64', 18)
+        ac = tensor(c, la, 0, 'complex128', 19)
+        sc.keep('a', a); sc.keep('ac', ac)
+        for k, (t, axes, sQ, Qaxis, Raxis) in enumerate(((a, ((0, 1), (2, 3)), 1, -1, 0),
+                                                         (a, ((2, 0), (3, 1)), -1, 0, -1),
+                                                         (ac, ((0, 3), (1, 2)), 1, 1, 1),
+                                                         (ac, ((1, 2, 3), 0), -1, -1, 0))):
+            Q, R = yastn.linalg.qr(t, axes=axes, sQ=sQ, Qaxis=Qaxis, Raxis=Raxis)
+            sc.keep(f'Q{k}', Q); sc.keep(f'R{k}', R)
+        sc.dump({'cases': [{'a': 'a', 'axes': [[0, 1], [2, 3]], 'sQ': 1, 'Qaxis': -1, 'Raxis': 0},
+                           {'a': 'a', 'axes': [[2, 0], [3, 1]], 'sQ': -1, 'Qaxis': 0, 'Raxis': -1},
+                           {'a': 'ac', 'axes': [[0, 3], [1, 2]], 'sQ': 1, 'Qaxis': 1, 'Raxis': 1},
+                           {'a': 'ac', 'axes': [[1, 2, 3], [0]], 'sQ': -1, 'Qaxis': -1, 'Raxis': 0}]})
+
     # ---------------- Kagome RVB D=3 U(1) state and CTMRG gradients (tests/peps/test_ad.py) -------
     if want('rvb_kagome'):
         import torch
@@ -551,6 +571,46 @@ def more(ref_root, only=None):
             runs.append({'sweeps': sweeps, 'init': init, 'loss': float(loss.detach()), 'tag': tag})
             print('rvb', tag, float(loss.detach()), flush=True)
         sc.dump({'runs': runs, 'leg_order': 't,l,b,r,p (after transpose (1, 2, 3, 4, 0))'})
+
+    # ---------------- the same state with QR-reduced projectors (use_qr=True, the reference default) ---
+    if want('rvb_kagome_qr'):
+        import torch
+        import yastn.tn.fpeps as fpeps
+        from yastn.tn.fpeps.envs.rdm import rdm1x1
+        sys.path.insert(0, os.path.join(ref_root, 'tests', 'peps'))
+        import test_ad as T
+        torch.set_default_dtype(torch.float64)
+        kw = {'backend': 'torch', 'default_device': 'cpu'}
+        A, _, _ = T.prepare_RVB((dict(kw), torch, None))
+
+        def truncation_f(S):
+            return yastn.linalg.truncation_mask_multiplets(S, keep_multiplets=True, D_total=64, tol=1.0e-8,
+                                                           tol_block=0.0, eps_multiplet=1.0e-8)
+
+        def cost(x, sweeps, init):
+            A0 = A.clone()
+            A0._data = x
+            g = fpeps.SquareLattice(dims=(1, 1), boundary='infinite')
+            psi = fpeps.Peps(g, tensors=dict(zip(g.sites(), [A0])))
+            env = fpeps.EnvCTM(psi, init=init)
+            env.ctmrg_(opts_svd={'D_total': 64, 'fix_signs': False}, max_sweeps=sweeps, truncation_f=truncation_f,
+                       use_qr=True, checkpoint_move=False)
+            r1x1, _ = rdm1x1((0, 0), psi, env)
+            return r1x1[(-1, -1)].trace().real
+
+        rec = Recorder()
+        sc = Scenario('rvb_kagome_qr', rec)
+        runs = []
+        for sweeps, init in ((1, 'dl'), (2, 'dl')):
+            x = A.data.detach().clone().requires_grad_(True)
+            loss = cost(x, sweeps, init)
+            loss.backward()
+            tag = f's{sweeps}_{init}'
+            rec.add_array(f'grad_{tag}', x.grad.detach().numpy())
+            runs.append({'sweeps': sweeps, 'init': init, 'loss': float(loss.detach()), 'tag': tag,
+                         'finite': bool(np.isfinite(x.grad.detach().numpy()).all())})
+            print('rvb qr', tag, runs[-1], flush=True)
+        sc.dump({'runs': runs})
 
 
 if __name__ == '__main__':

@@ -248,7 +248,7 @@ def inner_sweep(G, intra, ci, cj, tol, floor2, check_cover=True):
 
 
 def panel_sweeps(plan_jobs, plan_rounds, blocks_words, work, floors, b, max_sweeps, mode_tr):
-    """The persistent kernel's iteration on W (p x q, column-major at work_off) and V (q x q after it):
+    """The fused rounds' iteration on W (p x q, column-major at work_off) and V (q x q after it):
     per sweep the rounds in order, per job the Gram matrix of its columns, the inner sweep and W <- W Q,
     V <- V Q.  Returns the sweeps used (or -1)."""
     for sweep in range(max_sweeps):

@@ -33,3 +33,11 @@ def runs():
 
 def grad_expected():
     return G.load('rvb_kagome')[1]['grad_expected']
+
+
+def runs_qr():
+    """The reference CTMRG with use_qr=True (its default) on the same state: (sweeps, init, loss, whether the
+    reference torch gradient is finite).  The half-systems of this state are rank-deficient, so the QR
+    adjoint divides by zero there and the reference's gradient is not finite."""
+    js, _ = G.load('rvb_kagome_qr')
+    return [(r['sweeps'], r['init'], r['loss'], r['finite']) for r in js['runs']]

@@ -277,3 +277,32 @@ def test_truncation_mask_multiplets_known_answers():
     # the decisions are not all trivial
     kept = {int(ts[f'mask_{k}']['data'].sum()) for k in range(len(js['cases']))}
     assert len(kept) >= 5, kept
+
+
+def test_qr_metas_and_values():
+    """Block-sparse QR (reference linalg.py:738-830) for both signatures of the connecting leg, a charged
+    and a complex tensor, and Q/R legs moved: the reference's merge / qr / unmerge calls with identical
+    metas, and Q, R equal to the reference's."""
+    HOT_Q = HOT + ('qr',)
+    rec = []
+    be = types.ModuleType('rec_backend')
+    for k in dir(NB):
+        if not k.startswith('__'):
+            setattr(be, k, getattr(NB, k))
+    for name in HOT_Q:
+        be.__dict__[name] = (lambda f, n: lambda *a, **kw: (rec.append((n, a)), f(*a, **kw))[1])(getattr(NB, name), name)
+    cfg = bs.make_config(be, 'U1')
+    ts = G.tensors('qr_u1')
+    src = {k: rebuild(cfg, ts[k]) for k in ('a', 'ac')}
+    rec.clear()
+    for k, case in enumerate(G.load('qr_u1')[0]['cases']):
+        axes = tuple(tuple(x) for x in case['axes'])
+        Q, R = bs.qr(src[case['a']], axes=axes, sQ=case['sQ'], Qaxis=case['Qaxis'], Raxis=case['Raxis'])
+        for x, t in ((Q, ts[f'Q{k}']), (R, ts[f'R{k}'])):
+            assert x.trans == tuple(t['trans'])  # the moved leg is a pending transpose in both
+            same_tensor(x._replace(trans=None), t, atol=1e-13)
+        assert R.struct.n == (0,) and Q.struct.n == src[case['a']].struct.n
+    gold = [c for c in G.calls('qr_u1') if c[0] in HOT_Q]
+    assert [c[0] for c in rec] == [c[0] for c in gold]
+    for (fn, a1), (_, a2, _, _) in zip(rec, gold):
+        assert tuple(_norm_meta(x) for x in a1) == tuple(_norm_meta(x) for x in a2), fn

@@ -195,6 +195,54 @@ def test_rvb_kagome_matches_reference_ctmrg(k):
     assert np.abs(g - g_ref).max() <= 1e-9 * np.abs(g_ref).max()
 
 
+@pytest.mark.parametrize('k', [0, 1])
+def test_rvb_kagome_qr_projectors_match_reference(k):
+    """use_qr=True (the reference's default, _env_ctm.py:1095-1096 ...): the projectors come from the R
+    factors of the half-systems.  The loss equals the reference CTMRG's with use_qr=True (golden) and with
+    use_qr=False to 1e-12, and the gradient is non-finite exactly when the reference's is (this state's
+    half-systems are rank-deficient)."""
+    import rvb_case as R
+    sweeps, init, loss_qr, finite = R.runs_qr()[k]
+    _, _, loss_ref, _ = R.runs()[k + 1]
+    cfg = bs.make_config(_plain(), 'U1', device='cpu')
+    A = R.state(cfg)
+    x = A.data.detach().clone().requires_grad_(True)
+    opts = dict(R.svd_opts(), use_qr=True)
+    loss = C.rdm_loss(A._replace(data=x), R.CHI, sweeps, init, opts, block=R.BLOCK)
+    assert abs(float(loss) - loss_qr) <= 1e-12 and abs(float(loss) - loss_ref) <= 1e-12
+    loss.backward()
+    assert bool(np.isfinite(x.grad.numpy()).all()) == finite
+
+
+def test_qr_projectors_same_gradient_and_fd_cpu():
+    """On a random state (full-rank half-systems) use_qr=True and use_qr=False are the same function: equal
+    loss and gradient, and the use_qr gradient matches central finite differences (unbroadened adjoint)."""
+    B = _plain()
+    B.set_ad_broadening(1e-30)
+    try:
+        cfg = bs.make_config(B, 'U1', device='cpu')
+        A, env, sz = C.setup(cfg, D=2, chi=8, seed=2)
+        out = []
+        for q in (False, True):
+            x = A.data.detach().clone().requires_grad_(True)
+            loss = C.loss_fn(A._replace(data=x), env, sz, 8, {'use_qr': q})
+            loss.backward()
+            out.append((float(loss), x.grad))
+        (l0, g0), (l1, g1) = out
+        assert abs(l0 - l1) <= 1e-12 * max(1.0, abs(l0))
+        assert float((g0 - g1).norm()) <= 1e-8 * float(g0.norm())
+        rng = np.random.default_rng(5)
+        for _ in range(3):
+            d = torch.as_tensor(rng.standard_normal(A.data.numel()))
+            f = lambda h: float(C.loss_fn(A._replace(data=A.data + h * d), env, sz, 8, {'use_qr': True}))
+            h = 1e-6
+            fd = (-f(2 * h) + 8 * f(h) - 8 * f(-h) + f(-2 * h)) / (12 * h)
+            an = float((g1 * d).sum())
+            assert abs(fd - an) <= 1e-6 * abs(an)
+    finally:
+        B.set_ad_broadening(1e-12)
+
+
 def test_rvb_kagome_20_sweeps_known_answer():
     """tests/peps/test_ad.py:264-280 (test_Kagome_RVB_D3_U1_sym_vs_pepstorch): 20 sweeps from 'dl', the
     gradient equals grad_expected (rtol 1e-3, atol 1e-5, as the reference asserts); it also equals the

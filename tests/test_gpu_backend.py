@@ -504,3 +504,102 @@ def test_dot_diag_adjoint_vs_oracle(B):
         out = B.dot_diag(ta, tb, meta, Dsize, axis, nd)
         (out * _t(np.conj(g))).real.sum().backward()
         assert _rel(_n(ta.grad), gA) <= 1e-14 and _rel(_n(tb.grad), gB) <= 1e-14
+
+
+@pytest.mark.parametrize('n,cplx', [(9, True), (120, True), (150, False)])
+def test_eigh_adjoint_vs_oracle_complex_and_panel(B, n, cplx):
+    """eigh adjoint (HIP grouped GEMMs + core kernel) against the oracle's formula, which is pinned by central
+    finite differences, real and complex: an LDS-resident block (n=9) and panel-path blocks (n=120, 150)."""
+    rng = np.random.default_rng(20 + n)
+    x = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)
+    a = (x + x.conj().T) / 2
+    meta = (((0, n * n), (n, n), (0, n * n), (n, n), (0, n)),)
+    cw = rng.standard_normal(n)
+    Y = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)
+    ta = _t(a.ravel()).requires_grad_(True)
+    print('eigh adjoint', n, cplx, flush=True)
+    S, U = B.eigh(ta, meta, (n, n * n))
+    u = U.reshape(n, n)
+    loss = (S * _t(cw)).sum() + (u[:, 0].conj() @ _t(Y) @ u[:, 0]).real
+    loss.backward()
+    w, v = np.linalg.eigh(a)
+    gu = np.zeros((n, n), dtype=a.dtype)
+    gu[:, 0] = (Y + Y.conj().T) @ v[:, 0]
+    ref = O.eigh_block_adjoint(w, v, cw, gu)
+    got = _n(ta.grad).reshape(n, n)
+    assert _rel(got, ref) <= 1e-9, _rel(got, ref)
+
+
+@pytest.mark.parametrize('case', ['d6_chi64_real'])
+def test_svd_adjoint_vs_oracle_panel_sizes(B, case):
+    """SVD adjoint against the oracle's per-block formula (the reference's SVDGESDD.backward restated, pinned by
+    the reference torch gradients and finite differences) on the CTMRG projector block sizes, where the panel
+    path with QR preconditioning runs: <= 1e-9 relative."""
+    rng = np.random.default_rng(33)
+    shapes = [(440, 440), (300, 260), (180, 210), (64, 64), (12, 7)]
+    meta, off, uo, so, vo = [], 0, 0, 0, 0
+    for m, n in shapes:
+        k = min(m, n)
+        meta.append(((off, off + m * n), (m, n), (uo, uo + m * k), (m, k), (so, so + k), (vo, vo + k * n), (k, n)))
+        off, uo, so, vo = off + m * n, uo + m * k, so + k, vo + k * n
+    meta, sizes = tuple(meta), (uo, so, vo)
+    a = rng.standard_normal(off)
+    ta = _t(a).requires_grad_(True)
+    U, S, V = B.svd(ta, meta, sizes)
+    gU, gS, gV = (rng.standard_normal(x) for x in sizes)
+    print('svd adjoint panel sizes', shapes, flush=True)
+    torch.autograd.backward([U, S, V], [_t(gU), _t(gS), _t(gV)])
+    ref = O.svd_adjoint(gU, gS, gV, _n(U), _n(S), _n(V), meta, off)
+    assert _rel(_n(ta.grad), ref) <= 1e-9, _rel(_n(ta.grad), ref)
+
+
+def test_decompositions_reject_nonfinite_input(B):
+    """As the reference (scipy's check_finite, backend_np.py:281-286), an SVD or eigh input with a NaN or an
+    infinity raises ValueError."""
+    n = 5
+    meta = (((0, n * n), (n, n), (0, n * n), (n, n), (0, n), (0, n * n), (n, n)),)
+    emeta = (((0, n * n), (n, n), (0, n * n), (n, n), (0, n)),)
+    for bad in (np.nan, np.inf):
+        a = np.random.default_rng(1).standard_normal(n * n)
+        a[7] = bad
+        with pytest.raises(ValueError, match='infs or NaNs'):
+            B.svd(_t(a), meta, (n * n, n, n * n))
+        with pytest.raises(ValueError, match='infs or NaNs'):
+            B.eigh(_t(a), emeta, (n, n * n))
+
+
+def test_svd_and_ctm_step_bitwise_deterministic(B):
+    """The Jacobi SVD (LDS and panel paths, one-launch sweeps) and a whole CTMRG fwd+bwd step run twice give
+    bitwise identical results (checkpoint recomputation relies on it)."""
+    from yastn_amd import blocksparse as bs
+    from yastn_amd import ctmrg as C
+    rng = np.random.default_rng(9)
+    shapes = [(300, 280), (40, 30), (150, 150)]
+    meta, off, uo, so, vo = [], 0, 0, 0, 0
+    for m, n in shapes:
+        k = min(m, n)
+        meta.append(((off, off + m * n), (m, n), (uo, uo + m * k), (m, k), (so, so + k), (vo, vo + k * n), (k, n)))
+        off, uo, so, vo = off + m * n, uo + m * k, so + k, vo + k * n
+    a = _t(rng.standard_normal(off))
+    r1 = B.svd(a, tuple(meta), (uo, so, vo))
+    r2 = B.svd(a, tuple(meta), (uo, so, vo))
+    assert all(torch.equal(x, y) for x, y in zip(r1, r2))
+    cfg = bs.make_config(B, 'U1', device='cuda:0')
+    A, env, sz = C.setup(cfg, D=6, chi=64, seed=0)
+    print('ctm step twice', flush=True)
+    l1, g1 = C.fwd_bwd(A, env, sz, 64)
+    l2, g2 = C.fwd_bwd(A, env, sz, 64)
+    assert torch.equal(l1, l2) and torch.equal(g1, g2)
+
+
+def test_eigh_dense_meta_none(B):
+    """eigh(data, meta=None) on a dense Hermitian matrix, the form the reference's Krylov solvers call
+    (krylov/_krylov.py:230): ascending eigenvalues vs numpy, U diag(w) U^H = A."""
+    rng = np.random.default_rng(11)
+    for n, cplx in ((12, False), (30, True)):
+        x = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)
+        a = (x + x.conj().T) / 2
+        S, U = B.eigh(_t(a))
+        w, u = _n(S), _n(U)
+        np.testing.assert_allclose(w, np.linalg.eigvalsh(a), rtol=0, atol=1e-12 * np.abs(w).max())
+        np.testing.assert_allclose(u @ np.diag(w) @ u.conj().T, a, atol=1e-12 * np.abs(a).max())

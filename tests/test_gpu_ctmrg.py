@@ -86,6 +86,36 @@ def test_hip_move_batched_projector_svds():
     assert rel <= 1e-8, rel
 
 
+@pytest.mark.parametrize('D,chi,dtype', [(3, 24, 'float64'), (2, 16, 'complex128')])
+def test_hip_qr_projectors_vs_plain(D, chi, dtype):
+    """use_qr=True (R factors of the half-systems through the batched HIP QR and its adjoint) against the
+    plain path's per-block torch QR, autograd-differentiated: loss and gradient."""
+    BH, BP = _backends()
+    print('ctmrg use_qr', D, chi, dtype, flush=True)
+    res = []
+    for B in (BH, BP):
+        cfg = bs.make_config(B, 'U1', device='cuda:0')
+        A, env, sz = C.setup(cfg, D=D, chi=chi, dtype=dtype, seed=3)
+        res.append(C.fwd_bwd(A, env, sz, chi, {'use_qr': True}))
+    (lh, gh), (lp, gp) = res
+    assert abs(float(lh) - float(lp)) <= 1e-10 * max(1.0, abs(float(lp)))
+    rel = float((gh - gp).norm() / gp.norm())
+    assert rel <= 1e-8, rel
+
+
+def test_rvb_kagome_qr_projectors_on_gpu():
+    """The RVB state, 2 sweeps from 'dl', use_qr=True through the HIP backend: the loss of the reference
+    CTMRG with use_qr=True (<= 1e-12)."""
+    import rvb_case as R
+    from yastn_amd import backend_hip
+    sweeps, init, loss_qr, _ = R.runs_qr()[1]
+    print('rvb kagome use_qr', sweeps, 'sweeps from', init, flush=True)
+    cfg = bs.make_config(backend_hip, 'U1', device='cuda:0')
+    with torch.no_grad():
+        loss = C.rdm_loss(R.state(cfg), R.CHI, sweeps, init, dict(R.svd_opts(), use_qr=True), block=R.BLOCK)
+    assert abs(float(loss) - loss_qr) <= 1e-12
+
+
 @pytest.mark.parametrize('k', [0, 2, 3])
 def test_rvb_kagome_reference_ctmrg_on_gpu(k):
     """The Kagome RVB D=3 U(1) complex state of the reference's tests/peps/test_ad.py through the HIP backend

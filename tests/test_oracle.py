@@ -394,3 +394,30 @@ def test_oracle_dot_diag_adjoint_vs_torch_autograd():
         (out * torch.as_tensor(np.conj(g))).real.sum().backward()
         np.testing.assert_allclose(gA, ta.grad.numpy(), rtol=1e-13, atol=1e-14)
         np.testing.assert_allclose(gB, tb.grad.numpy(), rtol=1e-13, atol=1e-14)
+
+
+def test_oracle_eigh_adjoint_fd_complex():
+    """The oracle's eigh adjoint on a complex Hermitian block (torch's gradient convention, gA = dL/dRe A +
+    i dL/dIm A) against central finite differences along random Hermitian directions, for a loss of the
+    eigenvalues and of a phase-invariant projector u0 u0^H."""
+    rng = np.random.default_rng(6)
+    n = 6
+    x = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    a = x + x.conj().T
+    cw = rng.standard_normal(n)
+    Y = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+
+    def loss(y):
+        w, u = np.linalg.eigh(y)
+        return w @ cw + np.real(u[:, 0].conj() @ Y @ u[:, 0])
+    w, u = np.linalg.eigh(a)
+    gu = np.zeros((n, n), dtype=complex)
+    gu[:, 0] = (Y + Y.conj().T) @ u[:, 0]
+    g = O.eigh_block_adjoint(w, u, cw, gu)
+    for _ in range(4):
+        d = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        d = d + d.conj().T
+        h = 1e-6
+        fd = (loss(a + h * d) - loss(a - h * d)) / (2 * h)
+        an = np.real(np.sum(np.conj(g) * d))
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (fd, an)

@@ -309,8 +309,8 @@ def test_fast_apply_matches_public_apply(fast, monkeypatch):
     np.testing.assert_array_equal(gx.numpy(), np.full(5, 3.0))
 
 
-def test_one_launch_panel_sweeps_schedule_emulated():
-    """The schedule of the one-launch panel Jacobi (yastn_hip_svd_panel_sweeps), run in numpy on a real
+def test_fused_round_schedule_emulated():
+    """The schedule of the fused panel rounds (yastn_hip_svd_panel_rounds), run in numpy on a real
     SvdPlan's job and round lists: on the QR-preconditioned blocks (X = R^H) every sub-round of every
     pair writes every entry of its Gram matrix (double-buffered sweep), each column pair of a block is
     rotated once per sweep, the iteration converges, and the column norms are the singular values."""

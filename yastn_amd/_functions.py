@@ -83,6 +83,22 @@ class _Opaque(torch.autograd.Function):
         raise RuntimeError("yastn_amd: the adjoint kernels are not differentiable again")
 
 
+def host_array(t):
+    """Host numpy copy of a tensor that carries no gradient (mask values, indices).  Under a torch.func
+    transform such a tensor is wrapped and has no storage of its own; _Opaque hands its forward the
+    plain tensor, and the copy is taken there."""
+    if not _is_wrapped(t):
+        return t.detach().cpu().numpy()
+    box = []
+
+    def grab(v):
+        box.append(v.detach().cpu().numpy())
+        return v.new_empty(0)
+
+    _Opaque.apply(grab, t)
+    return box[0]
+
+
 class _CtxProxy:
     """The autograd context with its saved tensors replaced by plain ones."""
 
@@ -257,7 +273,7 @@ def _mask_host(mask):
         if isinstance(v, slice):
             out[key] = v
             continue
-        arr = v.detach().cpu().numpy() if torch.is_tensor(v) else np.asarray(v)
+        arr = host_array(v) if torch.is_tensor(v) else np.asarray(v)
         if arr.dtype == np.bool_:
             arr = np.nonzero(arr.ravel())[0]
         out[key] = arr.astype(np.int64).ravel()
@@ -454,9 +470,13 @@ class DotDiag(_Fn):
 
 # ------------------------------------------------------------------ SVD / eigh
 
-# One-launch panel sweeps (all sweeps of the panel path in one persistent kernel, no host read per sweep);
-# off: one launch per round with a host check after every sweep
-ONE_LAUNCH_SWEEPS = os.environ.get('YASTN_HIP_ONE_LAUNCH_SWEEPS', '1') != '0'
+# Fused panel rounds (Gram, inner sweep and update in one launch per round, per-sweep flags on the device);
+# off: the previous driver with two launches per round
+FUSED_ROUNDS = os.environ.get('YASTN_HIP_FUSED_ROUNDS', '1') != '0'
+# while a HIP graph is captured the sweeps cannot stop on a host read: the captured SVD launches the most
+# sweeps an eager call of the same plan used, plus this margin (the extra ones exit at once on the device;
+# a replay that needed more reports non-convergence through the deferred checks)
+CAPTURE_SWEEP_MARGIN = 3
 _DEFERRED = []  # stack of lists collecting convergence flags to check later (deferred_checks)
 
 
@@ -477,27 +497,31 @@ class deferred_checks:
         return False
 
     def check(self):
-        for what, info in self.flags:
-            _raise_on_info(info, what)
+        for what, info, nonfinite in self.flags:
+            _raise_on_info(info, what, nonfinite)
 
 
-def _raise_on_info(info, what):
+def _raise_on_info(info, what, nonfinite=None):
     if info.numel() == 0:
         return
-    lo, bad = torch.stack([info.min(), (info < 0).sum()]).tolist()
+    flags = [info.min(), (info < 0).sum()] + ([nonfinite.to(info.dtype)] if nonfinite is not None else [])
+    lo, bad, *nf = torch.stack(flags).tolist()
+    if nf and nf[0]:
+        # as the reference, whose scipy SVD / eigh check the input (backend_np.py:281-286)
+        raise ValueError(f"{what}: the input contains infs or NaNs")
     if lo == -2:
-        raise L.HipBackendError(f"{what}: the one-launch Jacobi sweeps were abandoned (a grid wait timed out)")
+        raise L.HipBackendError(f"{what}: the panel Jacobi was abandoned (a wait for sibling workgroups timed out)")
     if bad:
         raise L.HipBackendError(f"{what}: Jacobi did not converge in {MAX_SWEEPS} sweeps for {bad} block(s)")
 
 
-def _check_info(info, what):
+def _check_info(info, what, nonfinite=None):
     if _DEFERRED:
-        _DEFERRED[-1].append((what, info))
+        _DEFERRED[-1].append((what, info, nonfinite))
     elif torch.cuda.is_current_stream_capturing():
         raise RuntimeError("yastn_amd: SVD/eigh captured in a HIP graph needs deferred_checks() around the capture")
     elif SYNC_CHECKS:
-        _raise_on_info(info, what)
+        _raise_on_info(info, what, nonfinite)
 
 
 _SIDE = {}
@@ -554,12 +578,22 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
                 L.call('yastn_hip_svd_precond_init', ptr(work), dt, plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(),
                        plan.n_pre, st)
             d_sweeps = None
-            if ONE_LAUNCH_SWEEPS and plan.panel_width <= 16:
-                # all sweeps in one persistent launch; the sweep count stays on the device (state word 3)
+            if FUSED_ROUNDS and plan.panel_width <= 16:
                 state = torch.zeros(4 + MAX_SWEEPS + plan.njobs, dtype=torch.int32, device=data.device)
-                L.call('yastn_hip_svd_panel_sweeps', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
-                       plan.d_jobs.data_ptr(), plan.d_rounds.data_ptr(), len(plan.rounds), plan.max_round_jobs,
-                       plan.panel_width, plan.sweep_slices, MAX_SWEEPS, ptr(state), state.numel(), plan.njobs, st)
+                capturing = torch.cuda.is_current_stream_capturing()
+                budget = MAX_SWEEPS
+                if capturing and getattr(plan, 'sweeps_seen', None):
+                    budget = min(MAX_SWEEPS, plan.sweeps_seen + CAPTURE_SWEEP_MARGIN)
+                launched = 0
+                for sweep in range(budget):
+                    L.call('yastn_hip_svd_panel_rounds', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
+                           plan.d_jobs.data_ptr(), plan.h_rounds.ctypes.data, len(plan.rounds), plan.panel_width,
+                           plan.sweep_slices, sweep, MAX_SWEEPS, ptr(state), state.numel(), plan.njobs, st)
+                    launched += 1
+                    if not capturing and int(state[4 + sweep].item()) == 0:
+                        plan.sweeps_seen = max(getattr(plan, 'sweeps_seen', 0) or 0, sweep + 1)
+                        break
+                L.call('yastn_hip_svd_panel_result', ptr(state), state.numel(), launched, st)
                 d_sweeps = state.data_ptr() + 3 * state.element_size()
             converged = d_sweeps is not None
             if not converged:
@@ -594,7 +628,8 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
         # the side-stream launch reads data and writes the outputs: joined even when the panel path raises
         if joined is not None:
             joined[0].wait_stream(joined[1])
-    _check_info(info[:plan.nblocks], 'svd' if mode == 0 else 'eigh')
+    nonfinite = torch.isfinite(data).logical_not().any() if data.numel() else None
+    _check_info(info[:plan.nblocks], 'svd' if mode == 0 else 'eigh', nonfinite)
 
 
 def svd_forward(data, meta, sizes):

@@ -267,7 +267,7 @@ def to_tensor(val, Ds=None, dtype='float64', device='cpu'):
 @torch.no_grad()
 def to_mask(val):
     """Index list of a boolean mask; kept on the host (the mask plans are built there)."""
-    return torch.as_tensor(val).nonzero().reshape(-1).cpu()
+    return torch.from_numpy(np.nonzero(F.host_array(torch.as_tensor(val)).reshape(-1))[0].astype(np.int64))
 
 
 def square_matrix_from_dict(H, D=None, **kwargs):
@@ -354,8 +354,105 @@ def svd_lowrank(data, meta, sizes, **kwargs):
 
 
 def svd_randomized(data, meta, sizes, q=None, niter=3, **kwargs):
-    """Partial SVD policy; computed from the full Jacobi SVD restricted to the requested ranks."""
-    return _partial_from_full(data, meta, sizes)
+    """Randomized partial SVD of every block at once (the reference runs torch.svd_lowrank per block,
+    backend_torch.py:325-343, with q = max(k, min(2k, m, n)) and niter power iterations): the range finder
+    of Halko, Martinsson & Tropp (SIAM Rev. 53, 2011, algorithms 4.4 and 5.1) on the block batch:
+      Y = A Omega (Omega n x q Gaussian, fixed seed, so a recomputation is identical), Q = qr(Y),
+      niter times: Q = qr(A^H Q), Q = qr(A Q), then B = Q^H A (q x n), B = Ub S Vh, U = Q Ub,
+    keeping the k leading triplets.  Each product is one grouped-GEMM launch over all blocks, each QR one
+    batched Householder QR, the small SVD one batched Jacobi call; all are differentiable backend
+    functions, so the gradient is that of the algorithm (as torch.svd_lowrank's)."""
+    plan = _randomized_plan(meta, data.dtype, data.device, niter)
+    A = data
+    Y = dot(A, plan.omega, plan.m_AO, plan.sz_mq)
+    Q, _ = qr(Y, plan.m_qr_mq, plan.sz_qr_mq)
+    for _ in range(niter):
+        At = transpose(A, (1, 0), plan.m_At)
+        At = At.conj() if At.is_complex() else At
+        Z = dot(At, Q, plan.m_AtQ, plan.sz_nq)
+        Q2, _ = qr(Z, plan.m_qr_nq, plan.sz_qr_nq)
+        Y = dot(A, Q2, plan.m_AQ2, plan.sz_mq)
+        Q, _ = qr(Y, plan.m_qr_mq, plan.sz_qr_mq)
+    Qt = transpose(Q, (1, 0), plan.m_Qt)
+    Qt = Qt.conj() if Qt.is_complex() else Qt
+    Bm = dot(Qt, A, plan.m_QtA, plan.sz_qn)
+    Ub, Sb, Vb = svd(Bm, plan.m_svdB, plan.sz_svdB)
+    Uf = dot(Q, Ub, plan.m_QUb, plan.sz_mkb)
+    iu, is_, iv = plan.picks
+    return Uf[iu], Sb[is_], Vb[iv]
+
+
+class _RandomizedPlan:
+    """Metas of the randomized partial SVD of one svd meta: per block m x n with k requested, the range
+    dimension q = max(k, min(2k, m, n)); buffers for Omega (n x q), Y (m x q), Z (n x q), B (q x n), their
+    QR / SVD factors, and the index picks of the k leading triplets in the output layout."""
+
+    def __init__(self, meta, dtype, device, niter):
+        rows = []
+        for sl, D, slU, DU, slS, slV, DV in meta:
+            m, n = D
+            k = slS[1] - slS[0]
+            q = max(k, min(2 * k, m, n))
+            rows.append((sl, m, n, k, q))
+        off = lambda sizes: [0] + list(np.cumsum(sizes))
+        o_om = off([n * q for _, m, n, k, q in rows])
+        o_mq = off([m * q for _, m, n, k, q in rows])
+        o_nq = off([n * q for _, m, n, k, q in rows])
+        o_qn = off([q * n for _, m, n, k, q in rows])
+        o_qq = off([q * q for _, m, n, k, q in rows])
+        o_a = off([m * n for _, m, n, k, q in rows])
+        o_kb = off([min(q, n) for _, m, n, k, q in rows])
+        o_ub = off([q * min(q, n) for _, m, n, k, q in rows])
+        o_vb = off([min(q, n) * n for _, m, n, k, q in rows])
+        o_mk = off([m * min(q, n) for _, m, n, k, q in rows])
+        self.sz_mq, self.sz_nq, self.sz_qn = int(o_mq[-1]), int(o_nq[-1]), int(o_qn[-1])
+        self.sz_mkb = int(o_mk[-1])
+        rng = np.random.default_rng(20110000 + len(rows))
+        om = rng.standard_normal(int(o_om[-1]))
+        self.omega = torch.as_tensor(om, device=device).to(dtype)
+        R = lambda a, b: (int(a), int(b))
+        self.m_AO = tuple((R(o_mq[i], o_mq[i + 1]), (m, q), sl, (m, n), R(o_om[i], o_om[i + 1]), (n, q))
+                          for i, (sl, m, n, k, q) in enumerate(rows))
+        self.m_qr_mq = tuple((R(o_mq[i], o_mq[i + 1]), (m, q), R(o_mq[i], o_mq[i + 1]), (m, q),
+                              R(o_qq[i], o_qq[i + 1]), (q, q)) for i, (sl, m, n, k, q) in enumerate(rows))
+        self.sz_qr_mq = (self.sz_mq, int(o_qq[-1]))
+        self.m_qr_nq = tuple((R(o_nq[i], o_nq[i + 1]), (n, q), R(o_nq[i], o_nq[i + 1]), (n, q),
+                              R(o_qq[i], o_qq[i + 1]), (q, q)) for i, (sl, m, n, k, q) in enumerate(rows))
+        self.sz_qr_nq = (self.sz_nq, int(o_qq[-1]))
+        self.m_At = tuple((R(o_a[i], o_a[i + 1]), (n, m), sl, (m, n)) for i, (sl, m, n, k, q) in enumerate(rows))
+        self.m_AtQ = tuple((R(o_nq[i], o_nq[i + 1]), (n, q), R(o_a[i], o_a[i + 1]), (n, m), R(o_mq[i], o_mq[i + 1]),
+                            (m, q)) for i, (sl, m, n, k, q) in enumerate(rows))
+        self.m_AQ2 = tuple((R(o_mq[i], o_mq[i + 1]), (m, q), sl, (m, n), R(o_nq[i], o_nq[i + 1]), (n, q))
+                           for i, (sl, m, n, k, q) in enumerate(rows))
+        self.m_Qt = tuple((R(o_mq[i], o_mq[i + 1]), (q, m), R(o_mq[i], o_mq[i + 1]), (m, q))
+                          for i, (sl, m, n, k, q) in enumerate(rows))
+        self.m_QtA = tuple((R(o_qn[i], o_qn[i + 1]), (q, n), R(o_mq[i], o_mq[i + 1]), (q, m), sl, (m, n))
+                           for i, (sl, m, n, k, q) in enumerate(rows))
+        self.m_svdB = tuple((R(o_qn[i], o_qn[i + 1]), (q, n), R(o_ub[i], o_ub[i + 1]), (q, min(q, n)),
+                             R(o_kb[i], o_kb[i + 1]), R(o_vb[i], o_vb[i + 1]), (min(q, n), n))
+                            for i, (sl, m, n, k, q) in enumerate(rows))
+        self.sz_svdB = (int(o_ub[-1]), int(o_kb[-1]), int(o_vb[-1]))
+        self.m_QUb = tuple((R(o_mk[i], o_mk[i + 1]), (m, min(q, n)), R(o_mq[i], o_mq[i + 1]), (m, q),
+                            R(o_ub[i], o_ub[i + 1]), (q, min(q, n))) for i, (sl, m, n, k, q) in enumerate(rows))
+        iu, is_, iv = [], [], []
+        for i, ((sl, m, n, k, q), (_, _, slU, DU, slS, slV, DV)) in enumerate(zip(rows, meta)):
+            kb = min(q, n)
+            iu.append(o_mk[i] + np.arange(m)[:, None] * kb + np.arange(k)[None, :])
+            is_.append(o_kb[i] + np.arange(k))
+            iv.append(o_vb[i] + np.arange(k)[:, None] * n + np.arange(n)[None, :])
+        order_u = np.argsort([x[2][0] for x in meta], kind='stable')
+        cat = lambda xs, order: torch.from_numpy(np.concatenate([xs[j].ravel() for j in order]).astype(np.int64)
+                                                 if xs else np.zeros(0, dtype=np.int64)).to(device)
+        self.picks = (cat(iu, order_u), cat(is_, np.argsort([x[4][0] for x in meta], kind='stable')),
+                      cat(iv, np.argsort([x[5][0] for x in meta], kind='stable')))
+
+
+_RANDOMIZED = P._Cache(maxsize=64)
+
+
+def _randomized_plan(meta, dtype, device, niter):
+    return _RANDOMIZED.get(('rsvd', id(meta), dtype, device.type, device.index), (meta,),
+                           lambda: _RandomizedPlan(meta, dtype, device, niter))
 
 
 def svds_scipy(data, meta, sizes, thresh=0.1, solver='arpack', **kwargs):

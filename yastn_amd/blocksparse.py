@@ -1179,6 +1179,54 @@ def svd(a, axes=(0, 1), sU=1, nU=True, Uaxis=-1, Vaxis=0, fix_signs=False, **kwa
     return U.moveaxis(-1, Uaxis), S, V.moveaxis(0, Vaxis)
 
 
+def _meta_qr(config, struct, slices, sQ):
+    """Sector layout of Q and R for a = Q R (reference linalg.py:795-830): Q(s0, sQ) keeps the charge of a,
+    R(-sQ, s1) has zero charge; the connecting leg of a sector carries the charge that lets R be charge-free.
+    Meta rows: (slice of a, D of a, slice of Q, D of Q, slice of R, D of R); R's blocks are sorted by charge."""
+    sym, nsym = config.sym, config.sym.NSYM
+    if sQ == struct.s[1]:
+        t_con = tuple(x[nsym:] for x in struct.t)
+    else:
+        tc = np.array(struct.t, dtype=np.int64).reshape((len(struct.t), 2, nsym))
+        t_con = tuple(map(tuple, sym.fuse(tc[:, 1:, :], (1,), -1).tolist()))
+    k = tuple(min(ds) for ds in struct.D)
+    Qt = tuple(x[:nsym] + y for x, y in zip(struct.t, t_con))
+    QD = tuple((ds[0], m) for ds, m in zip(struct.D, k))
+    QDp = tuple(a * b for a, b in QD)
+    Qsl = _slices_of(QDp, QD)
+    Rrows = sorted((y + x[nsym:], (m, ds[1])) for y, x, m, ds in zip(t_con, struct.t, k, struct.D))
+    Rt = tuple(r[0] for r in Rrows)
+    RD = tuple(r[1] for r in Rrows)
+    RDp = tuple(a * b for a, b in RD)
+    Rsl = _slices_of(RDp, RD)
+    where_R = {t: sl.slcs[0] for t, sl in zip(Rt, Rsl)}
+    meta = tuple((sl.slcs[0], d, sq.slcs[0], dq, where_R[y + x[nsym:]], (dq[1], d[1]))
+                 for sl, d, sq, dq, y, x in zip(slices, struct.D, Qsl, QD, t_con, struct.t))
+    Qs = Struct(s=(struct.s[0], sQ), n=struct.n, diag=False, t=Qt, D=QD, size=sum(QDp))
+    Rs = Struct(s=(-sQ, struct.s[1]), n=sym.zero(), diag=False, t=Rt, D=RD, size=sum(RDp))
+    return meta, Qs, Qsl, Rs, Rsl
+
+
+def qr(a, axes=(0, 1), sQ=1, Qaxis=-1, Raxis=0):
+    """Blockwise reduced QR a = Q R with diag(R) >= 0 (reference linalg.py:738-792)."""
+    out_l, out_r = ((x,) if isinstance(x, int) else tuple(x) for x in axes)
+    out_hl = tuple(a.trans[ax] for ax in out_l)
+    out_hr = tuple(a.trans[ax] for ax in out_r)
+    data, struct, slices, ls_l, ls_r = _merge_to_matrix(a, (out_hl, out_hr))
+    meta, Qst, Qsl, Rst, Rsl = _meta_qr(a.config, struct, slices, sQ)
+    Qd, Rd = a.config.backend.qr(data, meta, (Qst.size, Rst.size))
+    ls = _leg_struct_trivial(Rst, axis=0)
+    mq, Qst, Qsl = _meta_unmerge_matrix(a.config, Qst, Qsl, ls_l, ls,
+                                        tuple(a.struct.s[i] for i in out_hl) + (sQ,))
+    Q = Tensor(a.config, Qst, Qsl, _unmerge(a.config, Qd, mq),
+               hfs=tuple(a.hfs[i] for i in out_hl) + (Fusion(s=(sQ,)),))
+    mr, Rst, Rsl = _meta_unmerge_matrix(a.config, Rst, Rsl, ls, ls_r,
+                                        (-sQ,) + tuple(a.struct.s[i] for i in out_hr))
+    R = Tensor(a.config, Rst, Rsl, _unmerge(a.config, Rd, mr),
+               hfs=(Fusion(s=(-sQ,)),) + tuple(a.hfs[i] for i in out_hr))
+    return Q.moveaxis(-1, Qaxis), R.moveaxis(0, Raxis)
+
+
 def _cat(datas):
     """Concatenate flat data buffers (torch tensors or numpy arrays)."""
     if isinstance(datas[0], np.ndarray):

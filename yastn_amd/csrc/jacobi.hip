@@ -913,69 +913,41 @@ __global__ void __launch_bounds__(kInnerThreads)
   }
 }
 
-// ---- all sweeps of the panel path in one launch ------------------------------------------------
-// A persistent grid of nwg workgroups, all resident at once (the host launches at most the device's
-// occupancy for this kernel, a multiple of `slices`).  Per sweep and round, the work items (job,
-// slice) of the round are dealt to the workgroups (item = blockIdx.x + k nwg, so the `slices`
-// workgroups of a job are consecutive and meet at every k).  An item
+// ---- fused panel rounds ----------------------------------------------------------------------------
+// One launch per round of a sweep; a workgroup per (job, slice) of the round (slices consecutive):
 //   1. forms the Gram matrix G = Wp^H Wp of its pair's nc <= 32 columns over all p rows on the matrix
 //      cores (15 waves: 3 upper 16 x 16 tiles x 5 row chunks, summed in a fixed order),
-//   2. waits until the job's other slices have formed theirs (a counter per job; after this the pair's
+//   2. waits until the job's other slices have formed theirs (a counter per job: after this the pair's
 //      columns are written, so no slice may still be reading them),
 //   3. runs the inner sweep of rotations on G (below) accumulating Q, and
 //   4. updates its slice of the 16-row stripes of W and V by Q in place (stripes are disjoint).
-// All workgroups then meet at a grid barrier.  A sweep in which no pair rotated ends the iteration.
+// Rounds are ordered by the kernel boundaries.  A round of sweep s > 0 whose previous sweep rotated
+// nothing exits at once, so a fixed number of sweeps can be launched without a host read (graph
+// capture); yastn_hip_svd_panel_result turns the per-sweep flags into the sweep count.
 //
 // Inner sweep: per sub-round every thread owns one item, a 2 x 2 block (slot pa rows, slot pb columns)
 // of G or a (row, slot pb) piece of Q.  It computes the rotations of its slots itself from the
 // sub-round's G (identical arithmetic in every thread, hence identical rotations), applies them and
 // writes the block to the other G buffer, so a sub-round needs one barrier.
 //
-// state (int32, zero on entry): 0 barrier arrivals, 1 barrier generation, 2 abort, 3 result (sweeps
-// used, -1 not converged within max_sweeps, -2 aborted), kStateHead + s: a pair rotated in sweep s,
-// kStateHead + max_sweeps + job: arrivals of the job's slices (the job's g-th visit completes at
-// g slices).  Every wait polls a bounded number of times: on expiry the abort flag is set, every
-// workgroup leaves at its next wait, and the result is -2.
+// state (int32, zero before the first sweep): 2 abort (a sibling wait expired), 3 result (sweeps used,
+// -1 not converged, -2 abandoned), kStateHead + s: a pair rotated in sweep s, kStateHead + max_sweeps +
+// job: arrivals of the job's slices (its visit in sweep s completes at (s + 1) slices).  The sibling
+// wait polls a bounded number of times.
 constexpr int kStateHead = 4;
 constexpr int kSpinLimit = 1 << 22;
 constexpr int kSweepGramWaves = 15;
 
-__device__ __attribute__((always_inline)) inline int32_t load_acquire(int32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Thread 0: poll until *p >= target.  False when another workgroup aborted or after kSpinLimit polls
-// (then the abort flag is set).
+// Thread 0: poll (relaxed loads, no cache invalidation per poll) until *p >= target.  False when a wait
+// expired somewhere (the abort flag), or after kSpinLimit polls (then the flag is set).
 __device__ bool wait_at_least(int32_t* p, int32_t target, int32_t* abort_flag) {
   for (int n = 0; n < kSpinLimit; ++n) {
-    if (load_acquire(p) >= target) return true;
+    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
     if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     __builtin_amdgcn_s_sleep(1);
   }
-  __hip_atomic_store(abort_flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return false;
-}
-
-// All nwg workgroups; nbar counts this workgroup's barriers (the k-th completes when the generation
-// reaches k).  Writes before the barrier are visible to every workgroup after it.
-__device__ bool grid_barrier(int32_t* state, int nwg, int32_t& nbar, int* s_ok) {
-  __threadfence();
-  __syncthreads();
-  ++nbar;
-  if (threadIdx.x == 0) {
-    bool ok = true;
-    const int32_t old = __hip_atomic_fetch_add(&state[0], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == nwg - 1) {
-      __hip_atomic_store(&state[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&state[1], nbar, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      ok = wait_at_least(&state[1], nbar, &state[2]);
-    }
-    *s_ok = ok ? 1 : 0;
-  }
-  __syncthreads();
-  __threadfence();
-  return *s_ok != 0;
 }
 
 // Slot k of the sub-round order of a pair: intra (one panel of nc columns, circle method over
@@ -1056,204 +1028,197 @@ __device__ __attribute__((always_inline)) inline bool slot_rotation(const typena
 
 template <bool CPLX>
 __global__ void __launch_bounds__(kInnerThreads)
-    panel_sweeps_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
-                        const int64_t* __restrict__ rounds, int nrounds, typename Num<CPLX>::T* __restrict__ work,
-                        const double* __restrict__ floors, int b, int slices, int max_sweeps,
-                        int32_t* __restrict__ state) {
+    panel_round_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs, int64_t off,
+                       typename Num<CPLX>::T* __restrict__ work, const double* __restrict__ floors, int b,
+                       int slices, int sweep, int max_sweeps, int32_t* __restrict__ state) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
   constexpr int KS = 8;  // stripe registers: nc <= 32
   extern __shared__ double smem_raw[];
-  __shared__ int s_ok, s_flag;
+  __shared__ int s_ok;
+  int32_t* const rotated = state + kStateHead;
+  int32_t* const pair_cnt = state + kStateHead + max_sweeps;
+  // the previous sweep converged (or the iteration was abandoned): nothing to do
+  if (sweep > 0 && __hip_atomic_load(&rotated[sweep - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+  if (__hip_atomic_load(&state[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nwg = (int)gridDim.x;
   const int nq = 2 * b + 1;
   T* const Gb0 = reinterpret_cast<T*>(smem_raw);
   T* const Gb1 = Gb0 + (int64_t)(2 * b) * nq;
   T* const Q = Gb1 + (int64_t)(2 * b) * nq;
   double* const part = reinterpret_cast<double*>(Gb1);  // Gram partials, before G1 and Q are used
-  int32_t* const rotated = state + kStateHead;
-  int32_t* const pair_cnt = state + kStateHead + max_sweeps;
-  int32_t nbar = 0;
-  int result = -1;
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    for (int rd = 0; rd < nrounds; ++rd) {
-      const int64_t off = rounds[2 * rd], cnt = rounds[2 * rd + 1];
-      for (int64_t it = blockIdx.x; it < cnt * slices; it += nwg) {
-        const int64_t job = off + it / slices;
-        const int slice = (int)(it % slices);
-        const int32_t* jb = jobs + 3 * job;
-        const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
-        const PanelPair pp = panel_pair(d, jb, b);
-        const int64_t p = pp.p;
-        const int nc = pp.nc, ci = (int)pp.ci, cj = (int)pp.cj;
-        T* const Wd = work + d.work_off;
-        // ---- this slice's stripes of W (p rows) then V (q rows); each wave's first stripe is read now
-        const int sw = (int)((p + 15) / 16), sv = (int)((pp.q + 15) / 16);
-        const int per = (sw + sv + slices - 1) / slices;
-        const int s0 = slice * per, s1 = min(sw + sv, s0 + per);
-        auto stripe_col = [&](int st) {
-          const bool isv = st >= sw;
-          T* X = isv ? Wd + p * pp.q : Wd;
-          const int64_t ldx = isv ? pp.q : p;
-          return [=](int c) { return X + pp.col(c) * ldx; };
-        };
-        auto stripe_rows = [&](int st) { return st >= sw ? (int)pp.q : (int)p; };
-        auto stripe_idx = [&](int st) { return st >= sw ? st - sw : st; };
-        T a[KS];
-        const int st0 = s0 + wave;
-        if (st0 < s1) stripe_load<CPLX, KS>(a, stripe_col(st0), stripe_rows(st0), stripe_idx(st0), nc, lane);
-        // ---- 1. Gram matrix: wave w < 15 takes tile w / 5 (of the upper triangle) over row chunk w % 5
-        const int T16 = (nc + 15) / 16, ntile = T16 * (T16 + 1) / 2;
-        const int nchunk = kSweepGramWaves / ntile;  // 5 (three tiles) or 15 (one tile)
-        if (wave < ntile * nchunk) {
-          const int tl = wave / nchunk, chunk = wave % nchunk;
-          int ti, tj;
-          tile_of(tl, T16, ti, tj);
-          const int64_t rc = ((p + 4 * nchunk - 1) / (4 * nchunk)) * 4;
-          const int64_t r0 = chunk * rc, r1 = min(p, r0 + rc);
-          const int ca = ti * 16 + (lane & 15), cb = tj * 16 + (lane & 15);
-          const T* colA = Wd + pp.col(ca < nc ? ca : 0) * p;
-          const T* colB = Wd + pp.col(cb < nc ? cb : 0) * p;
-          d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
-          for (int64_t k0 = r0; k0 < r1; k0 += 4) {
-            const int64_t k = k0 + (lane >> 4);
-            const T av = (k < r1 && ca < nc) ? colA[k] : N::make(0.0, 0.0);
-            const T bv = (k < r1 && cb < nc) ? colB[k] : N::make(0.0, 0.0);
-            accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(av), N::re(bv), accR, 0, 0, 0);
-            if constexpr (CPLX) {  // conj(a) b
-              accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(av), N::im(bv), accR, 0, 0, 0);
-              accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(av), N::im(bv), accI, 0, 0, 0);
-              accI = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(av), N::re(bv), accI, 0, 0, 0);
-            }
-          }
-          double* dst = part + (int64_t)wave * 256 * (CPLX ? 2 : 1);
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            dst[v * 64 + lane] = accR[v];
-            if constexpr (CPLX) dst[256 + v * 64 + lane] = accI[v];
-          }
-        }
-        __syncthreads();
-        for (int e = tid; e < ntile * 256; e += kInnerThreads) {
-          const int tl = e >> 8, idx = e & 255;
-          const int v = idx >> 6, ln = idx & 63;
-          int ti, tj;
-          tile_of(tl, T16, ti, tj);
-          const int i = ti * 16 + (ln >> 4) + 4 * v, j = tj * 16 + (ln & 15);  // D(l / 16 + 4 v, l % 16)
-          double gr = 0.0, gi = 0.0;
-          for (int ch = 0; ch < nchunk; ++ch) {
-            const double* src = part + (int64_t)(tl * nchunk + ch) * 256 * (CPLX ? 2 : 1);
-            gr += src[idx];
-            if constexpr (CPLX) gi += src[256 + idx];
-          }
-          if (i < nc && j < nc) {
-            Gb0[j * nq + i] = N::make(gr, gi);                // G(i, j)
-            if (ti != tj) Gb0[i * nq + j] = N::make(gr, -gi);  // G(j, i) of the mirrored tile
-          }
-        }
-        __syncthreads();  // the partials (aliasing G1 and Q) are consumed
-        for (int e = tid; e < nc * nq; e += kInnerThreads) {
-          const int l = e / nq, k = e - l * nq;
-          Q[e] = N::make(l == k ? 1.0 : 0.0, 0.0);
-        }
-        // ---- 2. the job's slices have all read the pair's columns before any of them writes
-        if (slices > 1) {
-          __syncthreads();
-          if (tid == 0) {
-            __hip_atomic_fetch_add(&pair_cnt[job], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            s_ok = wait_at_least(&pair_cnt[job], slices * (sweep + 1), &state[2]) ? 1 : 0;
-          }
-        }
-        __syncthreads();
-        if (slices > 1 && !s_ok) return;  // uniform: aborted
-        // ---- 3. inner sweep
-        const double tol = DBL_EPSILON * sqrt((double)(p > 1 ? p : 1));
-        const double floor2 = floors[jb[0]];
-        const int ne = nc + (nc & 1);
-        const int npr = pp.intra ? ne / 2 : max(ci, cj);
-        const int nsub = pp.intra ? ne - 1 : npr;
-        const int ng = npr * npr;
-        // the item of this thread: G block (pa, pb), or Q piece (row, pb), or none
-        int pa = -1, pb = -1, row = -1;
-        if (tid < ng) {
-          pa = tid / npr;
-          pb = tid - pa * npr;
-        } else if (tid < ng + npr * nc) {
-          pb = (tid - ng) / nc;
-          row = (tid - ng) - pb * nc;
-        }
-        SlotOrder oa, ob;
-        oa.init(pp.intra, ci, cj, nc, pa < 0 ? 0 : pa);
-        ob.init(pp.intra, ci, cj, nc, pb < 0 ? 0 : pb);
-        int any = 0;
-        for (int r = 0; r < nsub; ++r) {
-          const T* Gc = (r & 1) ? Gb1 : Gb0;
-          T* Gn = (r & 1) ? Gb0 : Gb1;
-          if (pb >= 0) {
-            int i2, j2;
-            ob.get(i2, j2);
-            double cb_, seb, sib;
-            const bool gob = slot_rotation<CPLX>(Gc, nq, i2, j2, nc, tol, floor2, cb_, seb, sib);
-            if (pa >= 0) {
-              int i, j;
-              oa.get(i, j);
-              double ca_, sea, sia;
-              const bool goa = slot_rotation<CPLX>(Gc, nq, i, j, nc, tol, floor2, ca_, sea, sia);
-              const bool hj = j < nc, hj2 = j2 < nc;
-              T b00 = Gc[i2 * nq + i];
-              T b01 = hj2 ? Gc[j2 * nq + i] : N::make(0.0, 0.0);
-              T b10 = hj ? Gc[i2 * nq + j] : N::make(0.0, 0.0);
-              T b11 = (hj && hj2) ? Gc[j2 * nq + j] : N::make(0.0, 0.0);
-              if (goa) {  // rows: J^H
-                apply_rot<CPLX>(b00, b10, ca_, 1.0, sea, -sia);
-                apply_rot<CPLX>(b01, b11, ca_, 1.0, sea, -sia);
-              }
-              if (gob) {  // columns: J
-                apply_rot<CPLX>(b00, b01, cb_, 1.0, seb, sib);
-                apply_rot<CPLX>(b10, b11, cb_, 1.0, seb, sib);
-              }
-              Gn[i2 * nq + i] = b00;
-              if (hj2) Gn[j2 * nq + i] = b01;
-              if (hj) Gn[i2 * nq + j] = b10;
-              if (hj && hj2) Gn[j2 * nq + j] = b11;
-              any |= (goa || gob) ? 1 : 0;
-              oa.advance();
-            } else if (gob) {
-              T u = Q[i2 * nq + row], v = Q[j2 * nq + row];
-              apply_rot<CPLX>(u, v, cb_, 1.0, seb, sib);
-              Q[i2 * nq + row] = u;
-              Q[j2 * nq + row] = v;
-            }
-            ob.advance();
-          }
-          __syncthreads();
-        }
-        const int rot = __syncthreads_or(any);
-        if (rot && slice == 0 && tid == 0)
-          __hip_atomic_store(&rotated[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // ---- 4. update of this slice's stripes (nothing to do when no rotation happened)
-        if (rot) {
-          for (int st = st0; st < s1; st += kInnerThreads / 64) {
-            if (st != st0) stripe_load<CPLX, KS>(a, stripe_col(st), stripe_rows(st), stripe_idx(st), nc, lane);
-            stripe_mul_store<CPLX, KS>(a, stripe_col(st), stripe_rows(st), stripe_idx(st), Q, nq, nc, lane);
-          }
-        }
-        __syncthreads();  // LDS (G, Q) is reused by the next item
+  const int64_t job = off + blockIdx.x / slices;
+  const int slice = (int)(blockIdx.x % slices);
+  const int32_t* jb = jobs + 3 * job;
+  const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
+  const PanelPair pp = panel_pair(d, jb, b);
+  const int64_t p = pp.p;
+  const int nc = pp.nc, ci = (int)pp.ci, cj = (int)pp.cj;
+  T* const Wd = work + d.work_off;
+  // ---- this slice's stripes of W (p rows) then V (q rows); each wave's first stripe is read now
+  const int sw = (int)((p + 15) / 16), sv = (int)((pp.q + 15) / 16);
+  const int per = (sw + sv + slices - 1) / slices;
+  const int s0 = slice * per, s1 = min(sw + sv, s0 + per);
+  auto stripe_col = [&](int st) {
+    const bool isv = st >= sw;
+    T* X = isv ? Wd + p * pp.q : Wd;
+    const int64_t ldx = isv ? pp.q : p;
+    return [=](int c) { return X + pp.col(c) * ldx; };
+  };
+  auto stripe_rows = [&](int st) { return st >= sw ? (int)pp.q : (int)p; };
+  auto stripe_idx = [&](int st) { return st >= sw ? st - sw : st; };
+  T a[KS];
+  const int st0 = s0 + wave;
+  if (st0 < s1) stripe_load<CPLX, KS>(a, stripe_col(st0), stripe_rows(st0), stripe_idx(st0), nc, lane);
+  // ---- 1. Gram matrix: wave w < 15 takes tile w / 5 (of the upper triangle) over row chunk w % 5
+  const int T16 = (nc + 15) / 16, ntile = T16 * (T16 + 1) / 2;
+  const int nchunk = kSweepGramWaves / ntile;  // 5 (three tiles) or 15 (one tile)
+  if (wave < ntile * nchunk) {
+    const int tl = wave / nchunk, chunk = wave % nchunk;
+    int ti, tj;
+    tile_of(tl, T16, ti, tj);
+    const int64_t rc = ((p + 4 * nchunk - 1) / (4 * nchunk)) * 4;
+    const int64_t r0 = chunk * rc, r1 = min(p, r0 + rc);
+    const int ca = ti * 16 + (lane & 15), cb = tj * 16 + (lane & 15);
+    const T* colA = Wd + pp.col(ca < nc ? ca : 0) * p;
+    const T* colB = Wd + pp.col(cb < nc ? cb : 0) * p;
+    d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t k0 = r0; k0 < r1; k0 += 4) {
+      const int64_t k = k0 + (lane >> 4);
+      const T av = (k < r1 && ca < nc) ? colA[k] : N::make(0.0, 0.0);
+      const T bv = (k < r1 && cb < nc) ? colB[k] : N::make(0.0, 0.0);
+      accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(av), N::re(bv), accR, 0, 0, 0);
+      if constexpr (CPLX) {  // conj(a) b
+        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(av), N::im(bv), accR, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(av), N::im(bv), accI, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(av), N::re(bv), accI, 0, 0, 0);
       }
-      if (!grid_barrier(state, nwg, nbar, &s_ok)) return;
     }
-    if (tid == 0) s_flag = load_acquire(&rotated[sweep]);
+    double* dst = part + (int64_t)wave * 256 * (CPLX ? 2 : 1);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      dst[v * 64 + lane] = accR[v];
+      if constexpr (CPLX) dst[256 + v * 64 + lane] = accI[v];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < ntile * 256; e += kInnerThreads) {
+    const int tl = e >> 8, idx = e & 255;
+    const int v = idx >> 6, ln = idx & 63;
+    int ti, tj;
+    tile_of(tl, T16, ti, tj);
+    const int i = ti * 16 + (ln >> 4) + 4 * v, j = tj * 16 + (ln & 15);  // D(l / 16 + 4 v, l % 16)
+    double gr = 0.0, gi = 0.0;
+    for (int ch = 0; ch < nchunk; ++ch) {
+      const double* src = part + (int64_t)(tl * nchunk + ch) * 256 * (CPLX ? 2 : 1);
+      gr += src[idx];
+      if constexpr (CPLX) gi += src[256 + idx];
+    }
+    if (i < nc && j < nc) {
+      Gb0[j * nq + i] = N::make(gr, gi);                // G(i, j)
+      if (ti != tj) Gb0[i * nq + j] = N::make(gr, -gi);  // G(j, i) of the mirrored tile
+    }
+  }
+  __syncthreads();  // the partials (aliasing G1 and Q) are consumed
+  for (int e = tid; e < nc * nq; e += kInnerThreads) {
+    const int l = e / nq, k = e - l * nq;
+    Q[e] = N::make(l == k ? 1.0 : 0.0, 0.0);
+  }
+  // ---- 2. the job's slices have all read the pair's columns before any of them writes
+  if (slices > 1) {
     __syncthreads();
-    if (!s_flag) {
-      result = sweep + 1;
+    if (tid == 0) {
+      __hip_atomic_fetch_add(&pair_cnt[job], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_ok = wait_at_least(&pair_cnt[job], slices * (sweep + 1), &state[2]) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  if (slices > 1 && !s_ok) return;  // uniform: a sibling slice never arrived
+  // ---- 3. inner sweep
+  const double tol = DBL_EPSILON * sqrt((double)(p > 1 ? p : 1));
+  const double floor2 = floors[jb[0]];
+  const int ne = nc + (nc & 1);
+  const int npr = pp.intra ? ne / 2 : max(ci, cj);
+  const int nsub = pp.intra ? ne - 1 : npr;
+  const int ng = npr * npr;
+  // the item of this thread: G block (pa, pb), or Q piece (row, pb), or none
+  int pa = -1, pb = -1, row = -1;
+  if (tid < ng) {
+    pa = tid / npr;
+    pb = tid - pa * npr;
+  } else if (tid < ng + npr * nc) {
+    pb = (tid - ng) / nc;
+    row = (tid - ng) - pb * nc;
+  }
+  SlotOrder oa, ob;
+  oa.init(pp.intra, ci, cj, nc, pa < 0 ? 0 : pa);
+  ob.init(pp.intra, ci, cj, nc, pb < 0 ? 0 : pb);
+  int any = 0;
+  for (int r = 0; r < nsub; ++r) {
+    const T* Gc = (r & 1) ? Gb1 : Gb0;
+    T* Gn = (r & 1) ? Gb0 : Gb1;
+    if (pb >= 0) {
+      int i2, j2;
+      ob.get(i2, j2);
+      double cb_, seb, sib;
+      const bool gob = slot_rotation<CPLX>(Gc, nq, i2, j2, nc, tol, floor2, cb_, seb, sib);
+      if (pa >= 0) {
+        int i, j;
+        oa.get(i, j);
+        double ca_, sea, sia;
+        const bool goa = slot_rotation<CPLX>(Gc, nq, i, j, nc, tol, floor2, ca_, sea, sia);
+        const bool hj = j < nc, hj2 = j2 < nc;
+        T b00 = Gc[i2 * nq + i];
+        T b01 = hj2 ? Gc[j2 * nq + i] : N::make(0.0, 0.0);
+        T b10 = hj ? Gc[i2 * nq + j] : N::make(0.0, 0.0);
+        T b11 = (hj && hj2) ? Gc[j2 * nq + j] : N::make(0.0, 0.0);
+        if (goa) {  // rows: J^H
+          apply_rot<CPLX>(b00, b10, ca_, 1.0, sea, -sia);
+          apply_rot<CPLX>(b01, b11, ca_, 1.0, sea, -sia);
+        }
+        if (gob) {  // columns: J
+          apply_rot<CPLX>(b00, b01, cb_, 1.0, seb, sib);
+          apply_rot<CPLX>(b10, b11, cb_, 1.0, seb, sib);
+        }
+        Gn[i2 * nq + i] = b00;
+        if (hj2) Gn[j2 * nq + i] = b01;
+        if (hj) Gn[i2 * nq + j] = b10;
+        if (hj && hj2) Gn[j2 * nq + j] = b11;
+        any |= (goa || gob) ? 1 : 0;
+        oa.advance();
+      } else if (gob) {
+        T u = Q[i2 * nq + row], v = Q[j2 * nq + row];
+        apply_rot<CPLX>(u, v, cb_, 1.0, seb, sib);
+        Q[i2 * nq + row] = u;
+        Q[j2 * nq + row] = v;
+      }
+      ob.advance();
+    }
+    __syncthreads();
+  }
+  const int rot = __syncthreads_or(any);
+  if (rot && slice == 0 && tid == 0)
+    __hip_atomic_store(&rotated[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // ---- 4. update of this slice's stripes (nothing to do when no rotation happened)
+  if (rot) {
+    for (int st = st0; st < s1; st += kInnerThreads / 64) {
+      if (st != st0) stripe_load<CPLX, KS>(a, stripe_col(st), stripe_rows(st), stripe_idx(st), nc, lane);
+      stripe_mul_store<CPLX, KS>(a, stripe_col(st), stripe_rows(st), stripe_idx(st), Q, nq, nc, lane);
+    }
+  }
+}
+
+// Sweep count from the per-sweep rotation flags: the first sweep that rotated nothing ends the
+// iteration (result = its number + 1); none among the `launched` sweeps: -1; abandoned: -2.
+__global__ void panel_result_kernel(int32_t* __restrict__ state, int launched) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int res = -1;
+  for (int s = 0; s < launched; ++s)
+    if (state[kStateHead + s] == 0) {
+      res = s + 1;
       break;
     }
-  }
-  if (blockIdx.x == 0 && tid == 0) {
-    const int ab = __hip_atomic_load(&state[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&state[3], ab ? -2 : result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  state[3] = state[2] ? -2 : res;
 }
 
 template <bool CPLX, int MODE>
@@ -2006,48 +1971,50 @@ extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int d
   return YH_OK;
 }
 
-extern "C" int yastn_hip_svd_panel_sweeps(void* work, const double* floors, int dtype, const int64_t* d_blocks,
-                                          const int32_t* d_jobs, const int64_t* d_rounds, int64_t nrounds,
-                                          int64_t max_round_jobs, int b, int slices, int max_sweeps,
-                                          int32_t* d_state, int64_t state_numel, int64_t njobs,
-                                          hipStream_t stream) {
+extern "C" int yastn_hip_svd_panel_rounds(void* work, const double* floors, int dtype, const int64_t* d_blocks,
+                                          const int32_t* d_jobs, const int64_t* h_rounds, int64_t nrounds, int b,
+                                          int slices, int sweep, int max_sweeps, int32_t* d_state,
+                                          int64_t state_numel, int64_t njobs, hipStream_t stream) {
   YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
-  YH_CHECK_ARG(b >= 1 && b <= 16, "panel width %d (the one-launch sweeps take b <= 16)", b);
+  YH_CHECK_ARG(b >= 1 && b <= 16, "panel width %d (the fused rounds take b <= 16)", b);
   YH_CHECK_ARG(slices >= 1 && slices <= 64, "update slices %d", slices);
-  YH_CHECK_ARG(max_sweeps >= 1 && max_sweeps <= 1000, "max sweeps %d", max_sweeps);
-  YH_CHECK_ARG(nrounds >= 1 && max_round_jobs >= 1, "rounds %lld, jobs per round %lld", (long long)nrounds,
-               (long long)max_round_jobs);
+  YH_CHECK_ARG(max_sweeps >= 1 && max_sweeps <= 1000 && sweep >= 0 && sweep < max_sweeps, "sweep %d of %d", sweep,
+               max_sweeps);
   YH_CHECK_ARG(state_numel >= kStateHead + max_sweeps + njobs, "state buffer of %lld words", (long long)state_numel);
   const bool cplx = dtype == YH_C128;
   const int64_t isz = cplx ? 16 : 8;
   const int64_t gsz = (int64_t)(2 * b) * (2 * b + 1) * isz;
   const int64_t psz = (int64_t)kSweepGramWaves * 256 * isz;
   const int64_t lds = gsz + std::max(2 * gsz, psz);
-  const void* fn = cplx ? (const void*)panel_sweeps_kernel<true> : (const void*)panel_sweeps_kernel<false>;
+  const void* fn = cplx ? (const void*)panel_round_kernel<true> : (const void*)panel_round_kernel<false>;
   int st = yh::hip_status(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                          "svd panel sweeps LDS size");
+                          "svd panel round LDS size");
   if (st) return st;
-  int dev = 0, per_cu = 0, ncu = 0;
-  st = yh::hip_status(hipGetDevice(&dev), "device");
-  if (!st) st = yh::hip_status(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
-  if (!st)
-    st = yh::hip_status(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kInnerThreads, (size_t)lds),
-                        "svd panel sweeps occupancy");
-  if (st) return st;
-  // every workgroup must be resident at once (they meet at grid barriers): at most the occupancy
-  const int64_t cap = (int64_t)per_cu * ncu;
-  int64_t nwg = std::min(cap, max_round_jobs * slices);
-  nwg -= nwg % slices;
-  YH_CHECK_ARG(nwg >= slices, "svd panel sweeps: %d slices exceed the %lld resident workgroups", slices, (long long)cap);
-  if (cplx)
-    hipLaunchKernelGGL(panel_sweeps_kernel<true>, dim3((unsigned)nwg), dim3(kInnerThreads), (size_t)lds, stream,
-                       d_blocks, d_jobs, d_rounds, (int)nrounds, (double2*)work, floors, b, slices, max_sweeps,
-                       d_state);
-  else
-    hipLaunchKernelGGL(panel_sweeps_kernel<false>, dim3((unsigned)nwg), dim3(kInnerThreads), (size_t)lds, stream,
-                       d_blocks, d_jobs, d_rounds, (int)nrounds, (double*)work, floors, b, slices, max_sweeps,
-                       d_state);
-  return yh::hip_status(hipGetLastError(), "svd panel sweeps launch");
+  int64_t end = 0;
+  for (int64_t r = 0; r < nrounds; ++r) {
+    const int64_t off = h_rounds[2 * r], n = h_rounds[2 * r + 1];
+    YH_CHECK_ARG(off >= end && n >= 0 && off + n <= njobs, "round %lld", (long long)r);
+    YH_CHECK_ARG(n * slices <= 0x7fffffff, "round %lld: %lld workgroups", (long long)r, (long long)(n * slices));
+    end = off + n;
+    if (n == 0) continue;
+    const dim3 g((unsigned)(n * slices));
+    if (cplx)
+      hipLaunchKernelGGL(panel_round_kernel<true>, g, dim3(kInnerThreads), (size_t)lds, stream, d_blocks, d_jobs, off,
+                         (double2*)work, floors, b, slices, sweep, max_sweeps, d_state);
+    else
+      hipLaunchKernelGGL(panel_round_kernel<false>, g, dim3(kInnerThreads), (size_t)lds, stream, d_blocks, d_jobs, off,
+                         (double*)work, floors, b, slices, sweep, max_sweeps, d_state);
+    st = yh::hip_status(hipGetLastError(), "svd panel round launch");
+    if (st) return st;
+  }
+  return YH_OK;
+}
+
+extern "C" int yastn_hip_svd_panel_result(int32_t* d_state, int64_t state_numel, int launched, hipStream_t stream) {
+  YH_CHECK_ARG(launched >= 0 && state_numel >= kStateHead + launched, "%d sweeps, state of %lld words", launched,
+               (long long)state_numel);
+  hipLaunchKernelGGL(panel_result_kernel, dim3(1), dim3(64), 0, stream, d_state, launched);
+  return yh::hip_status(hipGetLastError(), "svd panel result launch");
 }
 
 extern "C" int yastn_hip_check_panel_jobs(const int64_t* h_blocks, int64_t nblocks, const int32_t* h_jobs,

@@ -5,7 +5,8 @@ Structure restated from the reference (yastn/tn/fpeps/envs/_env_ctm.py, _env_con
     tr (l, b), br (t, l), bl (r, t); edges t (l, D^2, r), l (b, D^2, t), b (r, D^2, l), r (t, D^2, b);
   * 2x2 enlarged corners (corner2x2_*, _env_contractions.py:429-475) on the double-layer site;
   * projectors from proj_corners (_env_ctm.py:1230-1258): rr = r0 r1^T, svd_with_truncation with
-    fix_signs=True, p0 = r1 (S^-1/2 V)^*, p1 = r0 (U S^-1/2)^*; use_qr=False (r_t = cor_tt, ...) as in
+    fix_signs=True, p0 = r1 (S^-1/2 V)^*, p1 = r0 (U S^-1/2)^*; r0, r1 the half-systems (use_qr=False,
+    r_t = cor_tt, ...) or their R factors (use_qr=True, the reference's default), as in
     update_extended_2x2_projectors_ (_env_ctm.py:1050-1183);
   * environment renormalisation of _update_env_ (_env_ctm.py:692-775) with norm(p='inf') rescaling;
   * an 'hv' step = horizontal move (projectors for the left and right bonds, then l, tl, bl, r, tr, br)
@@ -15,6 +16,7 @@ pairwise); the reference contracts ket and bra lazily.  The observable is <O> on
 Z(O)/Z(1) with Z(X) = Tr(cor_tl(X) cor_tr cor_br cor_bl).
 """
 import math
+import os
 import time
 
 import numpy as np
@@ -113,11 +115,23 @@ def _normalize(t):
     return t / t.config.backend.norm(t.data, 'inf')
 
 
+def _r_pairs(h1, h2, svd_opts):
+    """The (r0, r1) pairs of the two projector SVDs between the half-systems h1 and h2 (reference
+    _env_ctm.py:1095-1096, 1122-1123, 1153-1154, 1180-1181).  With ``use_qr`` (a key of svd_opts, default
+    False) r0 and r1 are the R factors of the half-systems, otherwise the half-systems themselves; both give
+    the same projectors up to the signs fix_signs picks.  Returns the pairs and svd_opts without the key."""
+    opts = dict(svd_opts or {})
+    if opts.pop('use_qr', False):
+        return [(bs.qr(h1, axes=(0, 1))[1], bs.qr(h2, axes=(1, 0))[1]),
+                (bs.qr(h1, axes=(1, 0))[1], bs.qr(h2, axes=(0, 1))[1])], opts
+    return [(h1, h2.transpose((1, 0))), (h1.transpose((1, 0)), h2)], opts
+
+
 def move_h(env, a, chi, svd_opts=None, batch=False):
     ctl, ctr, cbr, cbl = corner_tl(env, a), corner_tr(env, a), corner_br(env, a), corner_bl(env, a)
     cor_tt = ctl @ ctr
     cor_bb = cbr @ cbl
-    pairs = [(cor_tt, cor_bb.transpose((1, 0))), (cor_tt.transpose((1, 0)), cor_bb)]   # 'rh', 'lh'
+    pairs, svd_opts = _r_pairs(cor_tt, cor_bb, svd_opts)   # 'rh', 'lh'
     if batch:
         (hrb, hrt), (hlb, hlt) = proj_corners_many(pairs, chi, svd_opts)
     else:
@@ -145,7 +159,7 @@ def move_v(env, a, chi, svd_opts=None, batch=False):
     ctl, ctr, cbr, cbl = corner_tl(env, a), corner_tr(env, a), corner_br(env, a), corner_bl(env, a)
     cor_ll = cbl @ ctl
     cor_rr = ctr @ cbr
-    pairs = [(cor_ll, cor_rr.transpose((1, 0))), (cor_ll.transpose((1, 0)), cor_rr)]   # 'tv', 'bv'
+    pairs, svd_opts = _r_pairs(cor_ll, cor_rr, svd_opts)   # 'tv', 'bv'
     if batch:
         (vtr, vtl), (vbr, vbl) = proj_corners_many(pairs, chi, svd_opts)
     else:
@@ -616,44 +630,101 @@ def captured_fwd_bwd(A, env, sz, chi, warmup=2):
     return cs, (l2.detach(), d2.grad)
 
 
-def bench(steps=3, D=6, chi=64, plain=True, cpu=True):
-    """Seconds per CTMRG fwd+bwd step (D=6, chi=64, U(1), float64): batched HIP backend vs the plain
-    per-block GPU path vs the plain path on the host cores (torch CPU)."""
+def host_info():
+    """Host threads and BLAS of the CPU baselines (torch and numpy on the host cores)."""
+    import torch
+    info = {'torch_threads': torch.get_num_threads(), 'os_cpu_count': os.cpu_count(),
+            'OMP_NUM_THREADS': os.environ.get('OMP_NUM_THREADS'),
+            'OPENBLAS_NUM_THREADS': os.environ.get('OPENBLAS_NUM_THREADS')}
+    try:
+        from threadpoolctl import threadpool_info
+        info['blas'] = [{k: x.get(k) for k in ('internal_api', 'version', 'num_threads', 'prefix')}
+                        for x in threadpool_info() if x.get('user_api') == 'blas']
+    except Exception:  # threadpoolctl absent
+        pass
+    try:
+        info['torch_blas'] = torch.__config__.show().split('BLAS_INFO=')[1].split(',')[0]
+    except Exception:
+        pass
+    return info
+
+
+def bench(steps=3, D=6, chi=64, plain=True, cpu=True, host_runs=5):
+    """Seconds per CTMRG fwd+bwd step (D=6, chi=64, U(1), float64): batched HIP backend (eager, the
+    reference's data-dependent truncation) and the same step with static-structure truncation replayed as
+    one HIP graph, vs the plain per-block GPU path vs the plain path on the host cores (torch CPU, median of
+    host_runs runs)."""
     import torch
     from . import backend_hip, backend_plain
+    from . import _functions as F
     out = {'workload': f'CTMRG hv step fwd+bwd, U(1) iPEPS 1x1, D={D}, chi={chi}, float64; loss = 2x2 cluster <Sz>'}
 
-    def run(backend, device, n, batch=False):
+    def run(backend, device, n, batch=False, median=False):
         cfg = bs.make_config(backend, 'U1', device=device)
         A, env, sz = setup(cfg, D=D, chi=chi, seed=0)
         fwd_bwd(A, env, sz, chi, batch=batch)  # warm-up (plans)
         sync = torch.cuda.synchronize if device != 'cpu' else (lambda: None)
         sync()
-        t0 = time.perf_counter()
-        for _ in range(n):
-            loss, g = fwd_bwd(A, env, sz, chi, batch=batch)
-        sync()
-        return (time.perf_counter() - t0) / n, float(loss), g
+        ts = []
+        for _ in range(n if median else 1):
+            t0 = time.perf_counter()
+            for _ in range(1 if median else n):
+                loss, g = fwd_bwd(A, env, sz, chi, batch=batch)
+            sync()
+            ts.append((time.perf_counter() - t0) / (1 if median else n))
+        return float(np.median(ts)), float(loss), g
     tb, lb, gb = run(backend_hip, 'cuda:0', steps)
     out['s_per_step_batched'] = tb
     out['loss'] = lb
+    # the previous panel-path driver (a Gram launch and an inner-sweep launch per round)
+    F.FUSED_ROUNDS = False
+    try:
+        t_old, l_old, g_old = run(backend_hip, 'cuda:0', steps)
+    finally:
+        F.FUSED_ROUNDS = True
+    out['two_launch_rounds'] = {'s_per_step': t_old, 'loss_diff': abs(l_old - lb),
+                                'grad_rel_diff': float((g_old - gb).norm() / gb.norm())}
     # the two projector SVDs of each move in one batched call (the reference calls them one by one)
     tm, lm, gm = run(backend_hip, 'cuda:0', steps, batch=True)
     out['move_batched_svds'] = {'s_per_step': tm, 'loss_diff': abs(lm - lb),
                                 'grad_rel_diff': float((gm - gb).norm() / gb.norm())}
+    # static-structure truncation, the whole fwd+bwd step captured as one HIP graph
+    cfg = bs.make_config(backend_hip, 'U1', device='cuda:0')
+    A, env, sz = setup(cfg, D=D, chi=chi, seed=0)
+    cs, (le, ge) = captured_fwd_bwd(A, env, sz, chi)
+    cs.replay()
+    other = cs.validate()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cs.replay()
+    torch.cuda.synchronize()
+    tg = (time.perf_counter() - t0) / steps
+    cs.validate()
+    loss_g, grad_g = cs.replay()
+    cs.validate()
+    out['graph_static'] = {'s_per_step': tg, 'loss': float(loss_g),
+                           'bitwise_equal_eager': bool(torch.equal(loss_g, le) and torch.equal(grad_g, ge)),
+                           'truncation': 'static: per-sector kept counts = the environment split '
+                                         f'{ {k[0]: v for k, v in env_split(chi).items()} }',
+                           'projectors_where_global_rule_differs': other}
+    del cs
     if plain:
         tp, lp, gp = run(backend_plain, 'cuda:0', max(1, steps // 2))
         out['s_per_step_plain_gpu'] = tp
         out['speedup_vs_plain_gpu'] = tp / tb
         out['move_batched_svds']['speedup_vs_plain_gpu'] = tp / tm
+        out['graph_static']['speedup_vs_plain_gpu'] = tp / tg
         out['grad_rel_diff_vs_plain'] = float((gb - gp).norm() / gp.norm())
+        out['plain_svd'] = ('torch.linalg.svd per block, full_matrices=False, preferred linalg library '
+                            f'{torch.backends.cuda.preferred_linalg_library()}')
     if cpu:
-        import os
-        tc, lc, gc = run(backend_plain, 'cpu', 1)
+        tc, lc, gc = run(backend_plain, 'cpu', host_runs, median=True)
         out['s_per_step_host_torch'] = tc
-        out['host_threads'] = torch.get_num_threads()
+        out['host_runs'] = host_runs
+        out['host'] = host_info()
         out['speedup_vs_host'] = tc / tb
         out['move_batched_svds']['speedup_vs_host'] = tc / tm
+        out['graph_static']['speedup_vs_host'] = tc / tg
     return out
 
 
@@ -710,3 +781,37 @@ def bench_unrolled(D=8, chi=128, dtype='complex128', n_iter=20, segment=4, repea
                                               'energy_diff': abs(lm - lp),
                                               'grad_rel_diff_vs_plain': float((gm - gp).norm() / gp.norm())}}
     return out
+
+
+def _main():
+    """python -m yastn_amd.ctmrg [--steps N] [--graph] [--batch]: N fwd+bwd steps of the D=6, chi=64 CTMRG
+    workload on the HIP backend (eager, or replays of the captured static-structure step), for profiling."""
+    import argparse
+    import torch
+    from . import backend_hip
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--graph', action='store_true')
+    ap.add_argument('--batch', action='store_true', help='move-batched projector SVDs (eager only)')
+    ap.add_argument('--D', type=int, default=6)
+    ap.add_argument('--chi', type=int, default=64)
+    args = ap.parse_args()
+    cfg = bs.make_config(backend_hip, 'U1', device='cuda:0')
+    A, env, sz = setup(cfg, D=args.D, chi=args.chi, seed=0)
+    if args.graph:
+        cs, _ = captured_fwd_bwd(A, env, sz, args.chi)
+        run = lambda: cs.replay()
+    else:
+        fwd_bwd(A, env, sz, args.chi, batch=args.batch)
+        run = lambda: fwd_bwd(A, env, sz, args.chi, batch=args.batch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    torch.cuda.synchronize()
+    print(f'{args.steps} steps, {(time.perf_counter() - t0) / args.steps * 1e3:.2f} ms/step '
+          f'({"graph" if args.graph else "eager"})', flush=True)
+
+
+if __name__ == '__main__':
+    _main()
