@@ -211,6 +211,19 @@ int yastn_hip_check_splitk_plan(const int64_t* h_recs, int64_t nrec, const int64
 int yastn_hip_splitk_reduce(void* const* outs, const void* partials, int dtype, const int64_t* d_recs, int64_t nrec,
                             hipStream_t stream);
 
+/* No counterpart in the reference: yastn_hip_grouped_gemm followed by yastn_hip_splitk_reduce for a plan
+ * with split-K GEMMs, without any allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  The partial products go to a file-scope device scratch array of
+ * YH_NOALLOC_SCRATCH_BYTES (one per device, kept until the process ends; its address is taken with
+ * hipGetSymbolAddress on the first call on a device); part_numel elements (complex: 2 doubles each)
+ * beyond it are rejected.  outs[part_buf] must be NULL (the slot the plan's GEMMs write partials to).
+ * Every call on a device shares that scratch: calls that overlap on one device race. */
+#define YH_NOALLOC_SCRATCH_BYTES 33554432
+int yastn_hip_grouped_gemm_noalloc(void* const* outs, const void* const* ins, int dtype, const int64_t* d_gemms,
+                                   int64_t ngemms, const int64_t* d_terms, int64_t nterms, const int32_t* d_tiles,
+                                   int64_t ntiles, int tile_config, int part_buf, int64_t part_numel,
+                                   const int64_t* d_red, int64_t nred, hipStream_t stream);
+
 /* QR preconditioning of the panel path (SVD blocks only; no reference counterpart: the reference
  * calls LAPACK gesdd / rocSOLVER gesvd per block, backend_np.py:281-286, torch_svd_gesdd.py:15-21).
  * W0 = Q R by blocked Householder QR, one-sided Jacobi on X = R^H (q x q), W0 = (Q V_J) Sigma U_J^H.

@@ -34,6 +34,7 @@ YH_QROP_WORDS = 3
 YH_QRADJ_WORDS = 8
 YH_SVD_LDS_BYTES = 131072
 YH_DIAG_WORDS = 8
+YH_NOALLOC_SCRATCH_BYTES = 33554432
 
 _p = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -56,6 +57,7 @@ _SIGS = {
     'yastn_hip_grouped_gemm': [_p, _p, _i32, _p, _i64, _p, _i64, _p, _i64, _i32, _p],
     'yastn_hip_check_splitk_plan': [_p, _i64, _p, _i32, _i64],
     'yastn_hip_splitk_reduce': [_p, _p, _i32, _p, _i64, _p],
+    'yastn_hip_grouped_gemm_noalloc': [_p, _p, _i32, _p, _i64, _p, _i64, _p, _i64, _i32, _i32, _i64, _p, _i64, _p],
     'yastn_hip_dot': [_p, _p, _p, _i32, _p, _i64, _p, _i64, _p, _i64, _i32, _p],
     'yastn_hip_dot_adjoint': [_p, _p, _p, _p, _p, _i32, _p, _i64, _p, _i64, _p, _i64, _i32, _p],
     'yastn_hip_check_svd_plan': [_p, _i64, _p, _i64, _i64, _i32, _i32, _i64, _i64, _i64, _i64, _i64],

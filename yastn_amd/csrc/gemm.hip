@@ -11,6 +11,8 @@
 // k-major; each wave issues v_mfma_f64_16x16x4f64 on FM x FN 16x16 fragments.
 // Complex data is held as separate real / imaginary LDS planes and multiplied
 // with four real MFMAs per fragment.
+#include <mutex>
+
 #include "common.h"
 
 namespace {
@@ -533,4 +535,56 @@ extern "C" int yastn_hip_splitk_reduce(void* const* outs, const void* partials, 
     hipLaunchKernelGGL(splitk_reduce_kernel<true>, dim3((unsigned)nrec), dim3(kThreads), 0, stream, b,
                        (const double2*)partials, d_recs);
   return yh::hip_status(hipGetLastError(), "split-k reduce launch");
+}
+
+// ---- the split-K GEMM without allocation ----------------------------------------------------------
+// Scratch of yastn_hip_grouped_gemm_noalloc: the split-K partial products live in this file-scope
+// device array (one copy per device, held until the process ends).  Host code never passes its name
+// to a kernel: it takes the array's per-device address from hipGetSymbolAddress, once per device.
+namespace {
+constexpr int64_t kScratchDoubles = 4 * 1024 * 1024;  // 32 MiB
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+static int scratch_address(void** out) {
+  int dev = -1;
+  int st = yh::hip_status(hipGetDevice(&dev), "current device");
+  if (st) return st;
+  YH_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device %d", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    st = yh::hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address");
+    if (st) return st;
+    YH_CHECK_ARG(p != nullptr, "scratch address of device %d", dev);
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return YH_OK;
+}
+
+extern "C" int yastn_hip_grouped_gemm_noalloc(void* const* outs, const void* const* ins, int dtype,
+                                              const int64_t* d_gemms, int64_t ngemms, const int64_t* d_terms,
+                                              int64_t nterms, const int32_t* d_tiles, int64_t ntiles, int cfg,
+                                              int part_buf, int64_t part_numel, const int64_t* d_red, int64_t nred,
+                                              hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(part_buf >= 0 && part_buf < YH_MAX_BUFS, "partials buffer slot %d", part_buf);
+  YH_CHECK_ARG(outs[part_buf] == nullptr, "output slot %d of the partials is taken", part_buf);
+  YH_CHECK_ARG(part_numel >= 0, "partials size %lld", (long long)part_numel);
+  const int64_t doubles = part_numel * (dtype == YH_C128 ? 2 : 1);
+  YH_CHECK_ARG(doubles <= kScratchDoubles, "split-K partials of %lld bytes exceed the %lld-byte scratch",
+               (long long)(doubles * 8), (long long)(kScratchDoubles * 8));
+  void* part = nullptr;
+  int st = scratch_address(&part);
+  if (st) return st;
+  void* o[YH_MAX_BUFS];
+  for (int i = 0; i < YH_MAX_BUFS; ++i) o[i] = outs[i];
+  o[part_buf] = part;
+  st = yastn_hip_grouped_gemm(o, ins, dtype, d_gemms, ngemms, d_terms, nterms, d_tiles, ntiles, cfg, stream);
+  if (st || nred == 0) return st;
+  return yastn_hip_splitk_reduce(o, part, dtype, d_red, nred, stream);
 }
