@@ -91,9 +91,10 @@ def _pmc_traffic(kernel):
     return None
 
 
-def _dot_launch_stats(backend, workload_fn, reps=50):
+def _dot_launch_stats(backend, workload_fn, reps=50, top=None):
     """Per-launch duration (HIP events on the launch stream) of every grouped-GEMM launch of one
-    workload pass; returns [(flops, avg_ms, plan)] in launch order."""
+    workload pass (with ``top``: of the ``top`` launches with the most flops); returns
+    [(flops, avg_ms, plan, dtype, bytes, args)] in launch order."""
     import torch
     from yastn_amd import plans as P
     calls = []
@@ -110,6 +111,11 @@ def _dot_launch_stats(backend, workload_fn, reps=50):
         backend.dot = orig
     out = []
     stream = torch.cuda.current_stream()
+    if top is not None:
+        def _fl(c):
+            return sum(Da[0] * Da[1] * Db[1] for _, _, _, Da, _, Db in c[2])
+        keep = set(sorted(range(len(calls)), key=lambda i: -_fl(calls[i]))[:top])
+        calls = [c for i, c in enumerate(calls) if i in keep]
     for A, B, meta, Dsize in calls:
         cplx = A.is_complex() or B.is_complex()
         fl = sum((8 if cplx else 2) * Da[0] * Da[1] * Db[1] for _, _, _, Da, _, Db in meta)
@@ -117,13 +123,22 @@ def _dot_launch_stats(backend, workload_fn, reps=50):
         by = sum(isz * (Da[0] * Da[1] + Db[0] * Db[1] + Da[0] * Db[1]) for _, _, _, Da, _, Db in meta)
         for _ in range(3):
             backend.dot(A, B, meta, Dsize)
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-        for e0, e1 in evs:
-            e0.record(stream)
-            backend.dot(A, B, meta, Dsize)
-            e1.record(stream)
+        # `reps` launches back to back, captured as one HIP graph and replayed between two events on the
+        # launch stream: the average is the GPU's time per launch, free of the host's per-call cost
+        # (~13 us of Python and ctypes per eager call, more than a small launch takes on the GPU)
         torch.cuda.synchronize()
-        ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / reps
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(reps):
+                backend.dot(A, B, meta, Dsize)
+        g.replay()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        g.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del g
         dt = torch.promote_types(A.dtype, B.dtype)
         plan = P.dot_plan(meta, A.numel(), B.numel(), Dsize, A.device)
         out.append((fl, ms, plan, dt, by, (A, B, meta, Dsize)))
@@ -199,6 +214,18 @@ def bench_heff2(args, out):
             resp = res._replace(config=cfgp)
             tsp = _timeit(lambda: W.two_site_truncation(resp, 1024), 2, 1, sync)
             out['svd_truncation']['ms_plain'] = tsp * 1e3
+        if not args.no_cpu:  # the plain per-block path on the host cores (torch CPU; LAPACK per block)
+            cfgh = bs.make_config(backend_plain, 'U1', device='cpu')
+            resh = res._replace(config=cfgh, data=res.data.cpu())
+            W.two_site_truncation(resh, 1024)
+            th = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                W.two_site_truncation(resh, 1024)
+                th.append(time.perf_counter() - t0)
+            out['svd_truncation']['ms_host'] = float(np.median(th)) * 1e3
+            out['svd_truncation']['host'] = f'median of 5, torch CPU, {torch.get_num_threads()} threads'
+            out['svd_truncation']['speedup_vs_host'] = float(np.median(th)) / ts
     # roofline of the dominant kernel (the launch with the largest duration)
     stats = _dot_launch_stats(backend_hip, lambda: W.heff2(AA, FL, FR, Wm))
     fl, ms, plan, dt, by, _ = max(stats, key=lambda x: x[1])
@@ -219,6 +246,33 @@ def bench_heff2(args, out):
         out['roofline'] = {'bound': 'mfma', 'achieved': ach, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
                            'frac': ach / PEAK_FP64_TFLOPS, **common}
     return AA, FL, FR, Wm, flops
+
+
+def bench_ctmrg_gemm():
+    """The largest grouped-GEMM launches of the D=6, chi=64 CTMRG step's forward (the enlarged-corner
+    products, sectors up to 440 x 440 x 644): TFLOP/s and roofline of the one with the most flops."""
+    import torch
+    from yastn_amd import backend_hip
+    from yastn_amd import blocksparse as bs
+    from yastn_amd import ctmrg as C
+    cfg = bs.make_config(backend_hip, 'U1', device='cuda:0')
+    A, env, sz = C.setup(cfg, D=6, chi=64, seed=0)
+    with torch.no_grad():
+        C.loss_fn(A, env, sz, 64)  # plans and tile choices
+    stats = _dot_launch_stats(backend_hip, lambda: C.loss_fn(A, env, sz, 64), reps=20, top=4)
+    fl, ms, plan, dt, by, _ = max(stats, key=lambda x: x[0])
+    ach = fl / (ms * 1e-3) / 1e12
+    ridge = PEAK_FP64_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9)
+    bound = 'mfma' if fl / by >= ridge else 'hbm'
+    roof = ({'bound': 'mfma', 'achieved': ach, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
+             'frac': ach / PEAK_FP64_TFLOPS} if bound == 'mfma' else
+            {'bound': 'hbm', 'achieved': by / (ms * 1e-3) / 1e9, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+             'frac': by / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS})
+    return {'workload': 'CTMRG D=6 chi=64 forward: the grouped-GEMM launch with the most flops',
+            'kernel': _kernel_desc(plan), 'flops_per_launch': fl, 'avg_launch_ms': ms, 'tflops': ach,
+            'algorithmic_bytes_per_launch': by, 'arithmetic_intensity': fl / by, 'roofline': roof,
+            'top_launches': [{'gflop': x[0] / 1e9, 'ms': round(x[1], 5), 'tflops': x[0] / x[1] / 1e9}
+                             for x in stats]}
 
 
 def bench_u1xu1(args, out):
@@ -399,6 +453,7 @@ def main():
         from yastn_amd import ctmrg
         print('bench: ctmrg', flush=True)
         out['ctmrg'] = ctmrg.bench(steps=max(2, args.steps // 5))
+        out['ctmrg']['gemm'] = bench_ctmrg_gemm()
         print(json.dumps(out), flush=True)
         return
     print('bench: heff2', flush=True)
@@ -414,6 +469,7 @@ def main():
             from yastn_amd import ctmrg
             print('bench: ctmrg', flush=True)
             out['ctmrg'] = ctmrg.bench(steps=max(2, args.steps // 5))
+            out['ctmrg']['gemm'] = bench_ctmrg_gemm()
             if args.unrolled:
                 print('bench: unrolled ctmrg', flush=True)
                 out['ctmrg_unrolled'] = ctmrg.bench_unrolled(n_iter=args.unrolled_steps,

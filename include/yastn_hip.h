@@ -35,7 +35,7 @@ extern "C" {
 #define YH_F64 0
 #define YH_C128 1
 
-#define YH_ABI_VERSION 11
+#define YH_ABI_VERSION 12
 
 /* ---- box-gather plan (one job per copied or zero-filled box) -------------
  * int64 words per job (YH_JOB_WORDS):
@@ -164,9 +164,12 @@ int yastn_hip_eigh(void* S, void* U, const void* A, int dtype, const int64_t* d_
  * (block, panel i, panel j) int32 jobs of its rounds (h_rounds: nrounds (offset, count) pairs into
  * d_jobs); per round, two launches: partial Gram matrices of each job's 2b columns over chunks of 64
  * rows (part), then per job `slices` workgroups that each run the job's inner sweep of rotations on
- * its Gram matrix and update their share of the pair's rows of W and V in place (d_flags[job] = 1 if
- * it rotated, 0 if not, -1 if the LDS cannot hold the pair).  The host stops when a whole sweep
- * rotated nothing; finalize writes U, S, Vh (or S, U) like yastn_hip_svd.  Columns whose squared norm
+ * its Gram matrix and update their share of the pair's rows of W and V in place.  d_state (int32,
+ * state_numel >= 4 + max_sweeps, ZERO before sweep 0): word 4 + sweep is set when a pair rotated in
+ * that sweep, word 1 when the LDS cannot hold a pair; a round of sweep s > 0 whose previous sweep
+ * rotated nothing exits at once, so sweeps can be issued without a host read in between (HIP graph
+ * capture).  yastn_hip_svd_panel_result turns the words into the sweep count (word 3: -3 for an LDS
+ * plan too small); finalize writes U, S, Vh (or S, U) like yastn_hip_svd.  Columns whose squared norm
  * is below (eps ||A||_F)^2 (floors[b], set by init) are not rotated, here and in yastn_hip_svd: they are
  * at the rounding-noise level of the block.  maxch_p = ceil(max p / 64); part holds >= (max jobs per
  * round) maxch_p T (T + 1) / 2 256 (x2 complex) doubles, T = ceil(2b / 16).  b <= 32.
@@ -176,26 +179,15 @@ int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, double* floo
                              const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, hipStream_t stream);
 int yastn_hip_svd_panel_sweep(void* work, const double* floors, int dtype, const int64_t* d_blocks,
                               const int32_t* d_jobs, const int64_t* h_rounds, int64_t nrounds, int b, int maxch_p,
-                              int slices, void* part, int64_t part_numel, int32_t* d_flags, hipStream_t stream);
+                              int slices, void* part, int64_t part_numel, int sweep, int max_sweeps,
+                              int32_t* d_state, int64_t state_numel, hipStream_t stream);
 int yastn_hip_check_panel_jobs(const int64_t* h_blocks, int64_t nblocks, const int32_t* h_jobs, int64_t njobs, int b,
                                int mode, int maxch_p);
-/* One sweep of the panel path with one fused launch per round: per (job, slice) a workgroup forms the
- * job's 2b x 2b Gram matrix on the matrix cores, waits for the job's other slices (a counter per job),
- * runs the inner sweep and updates its slice of the stripes of W and V in place.  `sweep` indexes the
- * per-sweep rotation flags: a round of sweep s > 0 whose previous sweep rotated nothing exits at once,
- * so a fixed number of sweeps can be launched with no host read in between (HIP graph capture).
- * h_rounds: nrounds (offset, count) pairs into d_jobs (host); d_state: state_numel >= 4 + max_sweeps +
- * njobs int32 words, ZERO before sweep 0 (the caller clears them).  b <= 16.  The wait for sibling slices
- * polls a bounded number of times (it then abandons the iteration: result -2).
- * yastn_hip_svd_panel_result writes word 3 of the state: the sweeps used (the first of the `launched`
- * sweeps that rotated nothing, + 1), -1 if every launched sweep rotated, -2 if abandoned. */
-int yastn_hip_svd_panel_rounds(void* work, const double* floors, int dtype, const int64_t* d_blocks,
-                               const int32_t* d_jobs, const int64_t* h_rounds, int64_t nrounds, int b, int slices,
-                               int sweep, int max_sweeps, int32_t* d_state, int64_t state_numel, int64_t njobs,
-                               hipStream_t stream);
+/* Sweep count of the panel rounds: word 3 of the state <- the first of the `launched` sweeps that rotated
+ * nothing, + 1; -1 if every launched sweep rotated; -3 if a round found an LDS plan too small for a pair. */
 int yastn_hip_svd_panel_result(int32_t* d_state, int64_t state_numel, int launched, hipStream_t stream);
 /* d_sweeps (nullable): a device word holding the sweep count written to info (then `sweeps` is
- * ignored), e.g. word 3 of the panel rounds' state. */
+ * ignored), e.g. word 3 of the panel sweep state. */
 int yastn_hip_svd_panel_finalize(void* U, void* S, void* Vh, const void* work, const double* mu, int dtype, int mode,
                                  const int64_t* d_blocks, const int32_t* d_ids, int64_t nb, int32_t* d_info,
                                  int sweeps, const int32_t* d_sweeps, hipStream_t stream);

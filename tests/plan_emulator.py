@@ -156,8 +156,8 @@ def splitk_reduce(h_red, outs, part):
 
 
 class SlotOrder:
-    """Mirror of the kernel's slot order (jacobi.hip SlotOrder): slot k of sub-round r -> (i, j), j == nc
-    when the column has no partner."""
+    """Mirror of the inner kernel's slot order (panel_inner_kernel's parameter lanes, jacobi.hip): slot k of
+    sub-round r -> (i, j), j == nc when the column has no partner."""
 
     def __init__(self, intra, ci, cj, nc, k):
         self.intra, self.ci, self.cj, self.nc, self.k, self.r = intra, ci, cj, nc, k, 0
@@ -248,7 +248,7 @@ def inner_sweep(G, intra, ci, cj, tol, floor2, check_cover=True):
 
 
 def panel_sweeps(plan_jobs, plan_rounds, blocks_words, work, floors, b, max_sweeps, mode_tr):
-    """The fused rounds' iteration on W (p x q, column-major at work_off) and V (q x q after it):
+    """The panel rounds' iteration on W (p x q, column-major at work_off) and V (q x q after it):
     per sweep the rounds in order, per job the Gram matrix of its columns, the inner sweep and W <- W Q,
     V <- V Q.  Returns the sweeps used (or -1)."""
     for sweep in range(max_sweeps):

@@ -84,6 +84,52 @@ def test_dot_each_tile_config(B):
             plan.set_cfg(keep)
 
 
+def _random_dot_meta(shapes):
+    """meta_dot of one GEMM per block, blocks laid out one after the other in A, B and C."""
+    meta, oa, ob, oc = [], 0, 0, 0
+    for M, K, N in shapes:
+        meta.append(((oc, oc + M * N), (M, N), (oa, oa + M * K), (M, K), (ob, ob + K * N), (K, N)))
+        oa, ob, oc = oa + M * K, ob + K * N, oc + M * N
+    return tuple(meta), oa, ob, oc
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.complex128])
+def test_dot_and_adjoint_every_tile_config_ragged(B, dtype):
+    """Every tile shape on ragged sector GEMMs: M, N off every tile multiple, K of 1, below, at and past
+    the LDS slices (tails), the Heff2 sector sizes
+    (99 x 473 x 99, 106 x 106 x 318): the dot and both adjoint products (conjugated, transposed operand
+    strides) against numpy, <= 1e-12 relative."""
+    from yastn_amd import plans as P
+    shapes = [(1, 1, 1), (3, 17, 5), (16, 16, 16), (17, 33, 15), (33, 47, 65), (64, 64, 64), (65, 129, 63),
+              (99, 473, 99), (106, 106, 318), (2, 500, 3), (130, 5, 70)]
+    meta, na, nb, nc = _random_dot_meta(shapes)
+    rng = np.random.default_rng(11)
+    rnd = lambda n: rng.standard_normal(n) + (1j * rng.standard_normal(n) if dtype == np.complex128 else 0)
+    A, Bd, gC = rnd(na).astype(dtype), rnd(nb).astype(dtype), rnd(nc).astype(dtype)
+    want = O.dot(A, Bd, meta, nc)
+    wA, wB = O.dot_adjoint(gC, A, Bd, meta)
+    plan = P.dot_plan(meta, na, nb, nc, DEV)
+    adj = P.dot_adjoint_plan(meta, na, nb, nc, DEV)
+    keep = plan.cfg, adj.cfg
+    plan.tuned = adj.tuned = True
+    try:
+        for cfg in P.L.TILE_DIMS:
+            if not (plan.applicable(cfg) and adj.applicable(cfg)):
+                continue
+            print('tile config', cfg, np.dtype(dtype).name, flush=True)
+            plan.set_cfg(cfg)
+            adj.set_cfg(cfg)
+            tA, tB = _t(A).requires_grad_(True), _t(Bd).requires_grad_(True)
+            C = B.dot(tA, tB, meta, nc)
+            assert _rel(_n(C), want) <= 1e-12, cfg
+            C.backward(_t(gC))
+            assert _rel(_n(tA.grad), wA) <= 1e-12, cfg
+            assert _rel(_n(tB.grad), wB) <= 1e-12, cfg
+    finally:
+        plan.set_cfg(keep[0])
+        adj.set_cfg(keep[1])
+
+
 @pytest.mark.parametrize('scenario', ['mps_u1_fuse_contracted', 'u1xu1_c128', 'fuse_u1_fuse_contracted'])
 def test_dot_adjoint(B, scenario):
     rng = np.random.default_rng(7)

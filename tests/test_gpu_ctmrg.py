@@ -89,14 +89,22 @@ def test_hip_move_batched_projector_svds():
 @pytest.mark.parametrize('D,chi,dtype', [(3, 24, 'float64'), (2, 16, 'complex128')])
 def test_hip_qr_projectors_vs_plain(D, chi, dtype):
     """use_qr=True (R factors of the half-systems through the batched HIP QR and its adjoint) against the
-    plain path's per-block torch QR, autograd-differentiated: loss and gradient."""
+    plain path's per-block torch QR, autograd-differentiated: loss and gradient of one horizontal move +
+    <Sz>.  One move: after it the truncated environment makes the next move's half-systems numerically
+    rank-deficient (|R_jj| ~ 1e-21), where the QR adjoint, the reference's included, divides by ~0 and
+    the gradient is rounding noise or not finite."""
     BH, BP = _backends()
     print('ctmrg use_qr', D, chi, dtype, flush=True)
     res = []
     for B in (BH, BP):
         cfg = bs.make_config(B, 'U1', device='cuda:0')
         A, env, sz = C.setup(cfg, D=D, chi=chi, dtype=dtype, seed=3)
-        res.append(C.fwd_bwd(A, env, sz, chi, {'use_qr': True}))
+        x = A.data.detach().clone().requires_grad_(True)
+        A2 = A._replace(data=x)
+        val = C.cluster_expectation(C.move_h(env, C.double_layer(A2), chi, {'use_qr': True}), A2, sz)
+        loss = val.real if val.is_complex() else val
+        loss.backward()
+        res.append((loss.detach(), x.grad))
     (lh, gh), (lp, gp) = res
     assert abs(float(lh) - float(lp)) <= 1e-10 * max(1.0, abs(float(lp)))
     rel = float((gh - gp).norm() / gp.norm())

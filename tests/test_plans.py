@@ -309,11 +309,11 @@ def test_fast_apply_matches_public_apply(fast, monkeypatch):
     np.testing.assert_array_equal(gx.numpy(), np.full(5, 3.0))
 
 
-def test_fused_round_schedule_emulated():
-    """The schedule of the fused panel rounds (yastn_hip_svd_panel_rounds), run in numpy on a real
-    SvdPlan's job and round lists: on the QR-preconditioned blocks (X = R^H) every sub-round of every
-    pair writes every entry of its Gram matrix (double-buffered sweep), each column pair of a block is
-    rotated once per sweep, the iteration converges, and the column norms are the singular values."""
+def test_panel_round_schedule_emulated():
+    """The schedule of the panel rounds (yastn_hip_svd_panel_sweep's inner kernel), run in numpy on a
+    real SvdPlan's job and round lists: on the QR-preconditioned blocks (X = R^H) the items of every
+    sub-round of every pair cover every entry of its Gram matrix, each column pair of a block is rotated
+    once per sweep, the iteration converges, and the column norms are the singular values."""
     import plan_emulator as E
     rng = np.random.default_rng(5)
     shapes = [(200, 150), (130, 130), (300, 100), (90, 140), (17, 12)]

@@ -470,9 +470,6 @@ class DotDiag(_Fn):
 
 # ------------------------------------------------------------------ SVD / eigh
 
-# Fused panel rounds (Gram, inner sweep and update in one launch per round, per-sweep flags on the device);
-# off: the previous driver with two launches per round
-FUSED_ROUNDS = os.environ.get('YASTN_HIP_FUSED_ROUNDS', '1') != '0'
 # while a HIP graph is captured the sweeps cannot stop on a host read: the captured SVD launches the most
 # sweeps an eager call of the same plan used, plus this margin (the extra ones exit at once on the device;
 # a replay that needed more reports non-convergence through the deferred checks)
@@ -509,8 +506,8 @@ def _raise_on_info(info, what, nonfinite=None):
     if nf and nf[0]:
         # as the reference, whose scipy SVD / eigh check the input (backend_np.py:281-286)
         raise ValueError(f"{what}: the input contains infs or NaNs")
-    if lo == -2:
-        raise L.HipBackendError(f"{what}: the panel Jacobi was abandoned (a wait for sibling workgroups timed out)")
+    if lo == -3:
+        raise L.HipBackendError(f"{what}: panel Jacobi plan smaller than a panel pair (LDS)")
     if bad:
         raise L.HipBackendError(f"{what}: Jacobi did not converge in {MAX_SWEEPS} sweeps for {bad} block(s)")
 
@@ -561,7 +558,6 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
             L.call('yastn_hip_eigh', ptr(out_s), ptr(out_a), ptr(data), dt, plan.d_blocks.data_ptr(), plan.n_lds,
                    plan.d_order.data_ptr(), plan.n_lds, plan.lds_bytes, ptr(work), ptr(info), MAX_SWEEPS, lds_st)
     try:
-        sweeps = 0
         if plan.n_big:
             mu = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
             floors = torch.empty(plan.nblocks, dtype=torch.float64, device=data.device)
@@ -577,40 +573,28 @@ def _run_jacobi(mode, plan, data, out_a, out_s, out_b):
                             run_gemm(gp, (work,), (work,), dt, tune=False)
                 L.call('yastn_hip_svd_precond_init', ptr(work), dt, plan.d_qdesc.data_ptr(), plan.d_pre.data_ptr(),
                        plan.n_pre, st)
-            d_sweeps = None
-            if FUSED_ROUNDS and plan.panel_width <= 16:
-                state = torch.zeros(4 + MAX_SWEEPS + plan.njobs, dtype=torch.int32, device=data.device)
-                capturing = torch.cuda.is_current_stream_capturing()
-                budget = MAX_SWEEPS
-                if capturing and getattr(plan, 'sweeps_seen', None):
-                    budget = min(MAX_SWEEPS, plan.sweeps_seen + CAPTURE_SWEEP_MARGIN)
-                launched = 0
-                for sweep in range(budget):
-                    L.call('yastn_hip_svd_panel_rounds', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
-                           plan.d_jobs.data_ptr(), plan.h_rounds.ctypes.data, len(plan.rounds), plan.panel_width,
-                           plan.sweep_slices, sweep, MAX_SWEEPS, ptr(state), state.numel(), plan.njobs, st)
-                    launched += 1
-                    if not capturing and int(state[4 + sweep].item()) == 0:
-                        plan.sweeps_seen = max(getattr(plan, 'sweeps_seen', 0) or 0, sweep + 1)
-                        break
-                L.call('yastn_hip_svd_panel_result', ptr(state), state.numel(), launched, st)
-                d_sweeps = state.data_ptr() + 3 * state.element_size()
-            converged = d_sweeps is not None
-            if not converged:
-                part = torch.empty(max(plan.part_numel, 1), dtype=torch.float64, device=data.device)
-                flags = torch.zeros(max(plan.njobs, 1), dtype=torch.int32, device=data.device)
-            while not converged and sweeps < MAX_SWEEPS:
+            # two launches per round (Gram partials, then inner sweep + update); a round of a sweep after one
+            # that rotated nothing exits at once on the device, so under capture a fixed budget of sweeps is
+            # recorded with no host read in between
+            state = torch.zeros(4 + MAX_SWEEPS, dtype=torch.int32, device=data.device)
+            part = torch.empty(max(plan.part_numel, 1), dtype=torch.float64, device=data.device)
+            capturing = torch.cuda.is_current_stream_capturing()
+            budget = MAX_SWEEPS
+            if capturing and getattr(plan, 'sweeps_seen', None):
+                budget = min(MAX_SWEEPS, plan.sweeps_seen + CAPTURE_SWEEP_MARGIN)
+            launched = 0
+            for sweep in range(budget):
                 L.call('yastn_hip_svd_panel_sweep', ptr(work), ptr(floors), dt, plan.d_jblocks.data_ptr(),
                        plan.d_jobs.data_ptr(), plan.h_rounds.ctypes.data, len(plan.rounds), plan.panel_width,
-                       plan.maxch_p, plan.panel_slices, ptr(part), part.numel(), ptr(flags), st)
-                sweeps += 1
-                fmin, fmax = torch.stack([flags.min(), flags.max()]).tolist()
-                if fmin < 0:
-                    raise RuntimeError("yastn_amd: svd panel round: LDS plan smaller than a panel pair")
-                if fmax == 0:
-                    converged = True
+                       plan.maxch_p, plan.panel_slices, ptr(part), part.numel(), sweep, MAX_SWEEPS, ptr(state),
+                       state.numel(), st)
+                launched += 1
+                if not capturing and int(state[4 + sweep].item()) == 0:
+                    plan.sweeps_seen = max(getattr(plan, 'sweeps_seen', 0) or 0, sweep + 1)
                     break
-            done = sweeps if converged else -1
+            L.call('yastn_hip_svd_panel_result', ptr(state), state.numel(), launched, st)
+            d_sweeps = state.data_ptr() + 3 * state.element_size()
+            done = 0
             if plan.n_plain:
                 L.call('yastn_hip_svd_panel_finalize', ptr(out_a), ptr(out_s), ptr(out_b) if mode == 0 else None,
                        ptr(work), ptr(mu), dt, mode, plan.d_blocks.data_ptr(), plan.d_plain.data_ptr(), plan.n_plain,

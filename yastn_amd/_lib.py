@@ -12,7 +12,7 @@ LIB_PATH = os.environ.get('YASTN_HIP_LIB', os.path.join(HERE, 'libyastn_hip.so')
 # constants mirrored from include/yastn_hip.h
 YH_OK, YH_ERR_ARG, YH_ERR_HIP, YH_ERR_NUMERIC = 0, -1, -2, -3
 YH_F64, YH_C128 = 0, 1
-YH_ABI_VERSION = 11
+YH_ABI_VERSION = 12
 YH_MAXR = 8
 YH_JOB_WORDS = 32
 YH_GATHER_TILE = 2048
@@ -66,7 +66,8 @@ _SIGS = {
     'yastn_hip_fix_svd_signs': [_p, _p, _i32, _p, _i64, _p],
     'yastn_hip_device_info': [_p, _p],
     'yastn_hip_svd_panel_init': [_p, _p, _p, _p, _i32, _i32, _p, _p, _i64, _p],
-    'yastn_hip_svd_panel_sweep': [_p, _p, _i32, _p, _p, _p, _i64, _i32, _i32, _i32, _p, _i64, _p, _p],
+    'yastn_hip_svd_panel_sweep': [_p, _p, _i32, _p, _p, _p, _i64, _i32, _i32, _i32, _p, _i64, _i32, _i32, _p, _i64,
+                                  _p],
     'yastn_hip_check_panel_jobs': [_p, _i64, _p, _i64, _i32, _i32, _i32],
     'yastn_hip_check_qr_op_plan': [_p, _p, _i64, _i32, _i64, _i64, _i64, _i64, _i64],
     'yastn_hip_qr_init': [_p, _p, _i32, _p, _p, _i64, _p],
@@ -75,7 +76,6 @@ _SIGS = {
     'yastn_hip_qr_adjoint_core': [_p, _p, _p, _i32, _p, _i64, _p],
     'yastn_hip_qr_adjoint_trsm': [_p, _p, _p, _i32, _p, _i64, _i64, _i64, _p],
     'yastn_hip_svd_panel_finalize': [_p, _p, _p, _p, _p, _i32, _i32, _p, _p, _i64, _p, _i32, _p, _p],
-    'yastn_hip_svd_panel_rounds': [_p, _p, _i32, _p, _p, _p, _i64, _i32, _i32, _i32, _i32, _p, _i64, _i64, _p],
     'yastn_hip_svd_panel_result': [_p, _i64, _i32, _p],
     'yastn_hip_check_qr_plan': [_p, _p, _p, _i64, _i32, _i64, _i64],
     'yastn_hip_svd_qr_panel': [_p, _i32, _p, _p, _i64, _i32, _i64, _p],

@@ -1,3 +1,13 @@
+# Parts of this file restate the YASTN planner (yastn/tensor/_merging.py, _contractions.py, linalg.py) in
+# the reference's statement order, because the metas it produces must be bit-identical to the reference's:
+#   Copyright 2024 The YASTN Authors. All Rights Reserved.
+#   Licensed under the Apache License, Version 2.0 (the "License"); you may not use this file except in
+#   compliance with the License.  You may obtain a copy of the License at
+#   https://www.apache.org/licenses/LICENSE-2.0
+#   Unless required by applicable law or agreed to in writing, software distributed under the License is
+#   distributed on an "AS IS" BASIS, WITHOUT WARRANTIES OR CONDITIONS OF ANY KIND, either express or
+#   implied.  See the License for the specific language governing permissions and limitations under the
+#   License.
 """Restated block-sparse planner and the tensor layer the workloads need (abelian U1 / U1xU1).
 
 The backend never sees charges; it receives flat data buffers and integer meta tuples.  On a
@@ -14,6 +24,7 @@ Restated from (reference file: lines):
   tensordot             yastn/tensor/_contractions.py:57-346, 349-450 (no fusion), 453-528 (broadcast)
   masks                 yastn/tensor/_merging.py:554-583, _contractions.py:529-574
   svd / truncation      yastn/tensor/linalg.py:49-132, 135-380, 539-634 (multiplets), 637-735
+  qr                    yastn/tensor/linalg.py:738-830
   eigh                  yastn/tensor/linalg.py:831-1010
 Meta fusion (mfs), direct-sum fusion, leg-intersection masks and fermionic swap gates are not
 restated: the workloads here do not use them.

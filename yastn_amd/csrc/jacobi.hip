@@ -609,6 +609,16 @@ constexpr int kGramRows = 64;      // rows of W per workgroup of kernel 1
 constexpr int kChunkRows = 64;     // unit of the row-chunk count maxch_p passed by the host
 constexpr int kChunkThreads = 256;  // 4 waves
 constexpr int kInnerThreads = 1024;  // inner sweep: one thread per 2 x 2 block of G or (pair, row) of Q
+// Iteration state (int32 words, zero before sweep 0): 1 plan too small for a panel pair, 3 result (sweeps
+// used, -1 not converged, -3 plan too small), kStateHead + s: some pair rotated in sweep s.
+constexpr int kStateHead = 4;
+
+// The previous sweep rotated nothing (the iteration has converged): a round of this sweep has nothing
+// to do.  Launches of later sweeps can thus be issued without a host read in between.
+__device__ inline bool sweep_done(const int32_t* state, int sweep) {
+  return state != nullptr && sweep > 0 &&
+         __hip_atomic_load(&state[kStateHead + sweep - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+}
 
 struct PanelPair {
   int64_t p, q, ci, cj, c0i, c0j;
@@ -649,11 +659,13 @@ __device__ inline void tile_of(int tl, int T16, int& ti, int& tj) {
 template <bool CPLX>
 __global__ void __launch_bounds__(kChunkThreads)
     panel_gram_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
-                      const typename Num<CPLX>::T* __restrict__ work, double* __restrict__ part, int b, int maxch) {
+                      const typename Num<CPLX>::T* __restrict__ work, double* __restrict__ part, int b, int maxch,
+                      const int32_t* __restrict__ state, int sweep) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
   extern __shared__ double smem_raw[];
   constexpr int ld = kGramRows + 1;
+  if (sweep_done(state, sweep)) return;
   const int32_t* jb = jobs + 3 * (int64_t)blockIdx.y;
   const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
   const PanelPair pp = panel_pair(d, jb, b);
@@ -697,14 +709,16 @@ __global__ void __launch_bounds__(kChunkThreads)
 }
 
 // Inner sweep of one job on its Gram matrix (LDS: G and the rotation accumulator Q, nc x (2b + 1)
-// each), then the update of slice blockIdx.y of the pair's rows of W and V.  flags[job] = 1 if
-// anything rotated, 0 if not, -1 if lds_cap is too small for the pair.
+// each), then the update of slice blockIdx.y of the pair's rows of W and V.  Sets the sweep's rotation
+// word of the state if anything rotated, and word 1 if lds_cap is too small for the pair.  (A variant
+// with one barrier per sub-round, every thread computing its slots' rotations itself from a
+// double-buffered G, ran the D=6 chi=64 CTMRG step slower: 96 against 91 ms.)
 template <bool CPLX, int KS>
 __global__ void __launch_bounds__(kInnerThreads)
     panel_inner_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
                        typename Num<CPLX>::T* __restrict__ work, const double* __restrict__ floors,
-                       const double* __restrict__ part, int32_t* __restrict__ flags, int b, int maxch,
-                       int64_t lds_cap) {
+                       const double* __restrict__ part, int32_t* __restrict__ state, int sweep, int b,
+                       int maxch, int64_t lds_cap) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
   constexpr int kMaxPairs = 32;  // nc <= 64
@@ -713,6 +727,7 @@ __global__ void __launch_bounds__(kInnerThreads)
   __shared__ int4 s_ij[kMaxPairs];
   __shared__ double2 s_cs[kMaxPairs];
   __shared__ double s_si[kMaxPairs];
+  if (sweep_done(state, sweep)) return;
   const int32_t* jb = jobs + 3 * (int64_t)blockIdx.x;
   const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
   const PanelPair pp = panel_pair(d, jb, b);
@@ -725,7 +740,7 @@ __global__ void __launch_bounds__(kInnerThreads)
   T* Q = G + (int64_t)(2 * b) * nq;
   const int64_t need = (int64_t)2 * (2 * b) * nq * (int64_t)sizeof(T);
   if (need > lds_cap || nc > 2 * kMaxPairs || nc > 4 * KS) {
-    if (tid == 0 && blockIdx.y == 0) flags[blockIdx.x] = -1;  // uniform: plan too small for this panel pair
+    if (tid == 0 && blockIdx.y == 0) state[1] = 1;  // uniform: plan too small for this panel pair
     return;
   }
   if (tid == 0) s_rot = 0;
@@ -903,7 +918,7 @@ __global__ void __launch_bounds__(kInnerThreads)
       __syncthreads();
     }
   }
-  if (tid == 0 && blockIdx.y == 0) flags[blockIdx.x] = s_rot;
+  if (s_rot && tid == 0 && blockIdx.y == 0) state[kStateHead + sweep] = 1;
   if (!s_rot) return;  // uniform: nothing rotated, W and V of the pair are unchanged
   // ---- the pair's columns of this workgroup's stripes <- (.) Q in place (stripes are disjoint rows,
   // and a wave reads its whole stripe before writing it)
@@ -913,303 +928,8 @@ __global__ void __launch_bounds__(kInnerThreads)
   }
 }
 
-// ---- fused panel rounds ----------------------------------------------------------------------------
-// One launch per round of a sweep; a workgroup per (job, slice) of the round (slices consecutive):
-//   1. forms the Gram matrix G = Wp^H Wp of its pair's nc <= 32 columns over all p rows on the matrix
-//      cores (15 waves: 3 upper 16 x 16 tiles x 5 row chunks, summed in a fixed order),
-//   2. waits until the job's other slices have formed theirs (a counter per job: after this the pair's
-//      columns are written, so no slice may still be reading them),
-//   3. runs the inner sweep of rotations on G (below) accumulating Q, and
-//   4. updates its slice of the 16-row stripes of W and V by Q in place (stripes are disjoint).
-// Rounds are ordered by the kernel boundaries.  A round of sweep s > 0 whose previous sweep rotated
-// nothing exits at once, so a fixed number of sweeps can be launched without a host read (graph
-// capture); yastn_hip_svd_panel_result turns the per-sweep flags into the sweep count.
-//
-// Inner sweep: per sub-round every thread owns one item, a 2 x 2 block (slot pa rows, slot pb columns)
-// of G or a (row, slot pb) piece of Q.  It computes the rotations of its slots itself from the
-// sub-round's G (identical arithmetic in every thread, hence identical rotations), applies them and
-// writes the block to the other G buffer, so a sub-round needs one barrier.
-//
-// state (int32, zero before the first sweep): 2 abort (a sibling wait expired), 3 result (sweeps used,
-// -1 not converged, -2 abandoned), kStateHead + s: a pair rotated in sweep s, kStateHead + max_sweeps +
-// job: arrivals of the job's slices (its visit in sweep s completes at (s + 1) slices).  The sibling
-// wait polls a bounded number of times.
-constexpr int kStateHead = 4;
-constexpr int kSpinLimit = 1 << 22;
-constexpr int kSweepGramWaves = 15;
-
-// Thread 0: poll (relaxed loads, no cache invalidation per poll) until *p >= target.  False when a wait
-// expired somewhere (the abort flag), or after kSpinLimit polls (then the flag is set).
-__device__ bool wait_at_least(int32_t* p, int32_t target, int32_t* abort_flag) {
-  for (int n = 0; n < kSpinLimit; ++n) {
-    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
-    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-    __builtin_amdgcn_s_sleep(1);
-  }
-  __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return false;
-}
-
-// Slot k of the sub-round order of a pair: intra (one panel of nc columns, circle method over
-// ne = nc + (nc & 1) players) or cross (panel I of ci columns against panel J of cj columns, every
-// (i in I, j in J) once over max(ci, cj) sub-rounds).  j == nc marks a column without partner.
-struct SlotOrder {
-  int kind, k, nc, n1, ci, cj, ph1, ph2, r;
-  __device__ void init(bool intra, int ci_, int cj_, int nc_, int kk) {
-    nc = nc_;
-    ci = ci_;
-    cj = cj_;
-    k = kk;
-    r = 0;
-    if (intra) {
-      kind = 0;
-      const int ne = nc + (nc & 1);
-      n1 = ne - 1;
-      const int m = n1 > 0 ? n1 : 1;
-      ph1 = kk % m;
-      ph2 = (m - kk % m) % m;
-    } else if (ci <= cj) {
-      kind = 1;
-      ph1 = kk % cj;
-    } else {
-      kind = 2;
-      ph1 = kk % ci;
-    }
-  }
-  __device__ void get(int& i, int& j) const {
-    if (kind == 0) {
-      if (k == 0) {
-        i = r;
-        j = n1;
-      } else {
-        i = min(ph1, ph2);
-        j = max(ph1, ph2);
-      }
-    } else if (kind == 1) {
-      i = k < ci ? k : ci + ph1;
-      j = k < ci ? ci + ph1 : nc;
-    } else {
-      i = ph1;
-      j = k < cj ? ci + k : nc;
-    }
-  }
-  __device__ void advance() {
-    ++r;
-    if (kind == 0) {
-      ph1 = ph1 + 1 == n1 ? 0 : ph1 + 1;
-      ph2 = ph2 + 1 == n1 ? 0 : ph2 + 1;
-    } else if (kind == 1) {
-      ph1 = ph1 + 1 == cj ? 0 : ph1 + 1;
-    } else {
-      ph1 = ph1 + 1 == ci ? 0 : ph1 + 1;
-    }
-  }
-};
-
-// The rotation of slot (i, j) from G (column-major, leading dimension nq): go = rotated; (c, s e) and
-// s Im(e) as the inner kernel publishes them.
-template <bool CPLX>
-__device__ __attribute__((always_inline)) inline bool slot_rotation(const typename Num<CPLX>::T* G, int nq, int i,
-                                                                    int j, int nc, double tol, double floor2,
-                                                                    double& c, double& se, double& si) {
-  typedef Num<CPLX> N;
-  c = 1.0;
-  se = 0.0;
-  si = 0.0;
-  if (j >= nc) return false;
-  const typename N::T gij = G[j * nq + i];
-  double sn, er, ei;
-  if (!rotation(N::re(G[i * nq + i]), N::re(G[j * nq + j]), N::re(gij), N::im(gij), tol, floor2, c, sn, er, ei))
-    return false;
-  se = sn * er;
-  si = CPLX ? sn * ei : 0.0;
-  return true;
-}
-
-template <bool CPLX>
-__global__ void __launch_bounds__(kInnerThreads)
-    panel_round_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs, int64_t off,
-                       typename Num<CPLX>::T* __restrict__ work, const double* __restrict__ floors, int b,
-                       int slices, int sweep, int max_sweeps, int32_t* __restrict__ state) {
-  typedef Num<CPLX> N;
-  typedef typename N::T T;
-  constexpr int KS = 8;  // stripe registers: nc <= 32
-  extern __shared__ double smem_raw[];
-  __shared__ int s_ok;
-  int32_t* const rotated = state + kStateHead;
-  int32_t* const pair_cnt = state + kStateHead + max_sweeps;
-  // the previous sweep converged (or the iteration was abandoned): nothing to do
-  if (sweep > 0 && __hip_atomic_load(&rotated[sweep - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
-  if (__hip_atomic_load(&state[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nq = 2 * b + 1;
-  T* const Gb0 = reinterpret_cast<T*>(smem_raw);
-  T* const Gb1 = Gb0 + (int64_t)(2 * b) * nq;
-  T* const Q = Gb1 + (int64_t)(2 * b) * nq;
-  double* const part = reinterpret_cast<double*>(Gb1);  // Gram partials, before G1 and Q are used
-  const int64_t job = off + blockIdx.x / slices;
-  const int slice = (int)(blockIdx.x % slices);
-  const int32_t* jb = jobs + 3 * job;
-  const BlockDesc d = load_desc(blocks + (int64_t)jb[0] * YH_SVD_WORDS);
-  const PanelPair pp = panel_pair(d, jb, b);
-  const int64_t p = pp.p;
-  const int nc = pp.nc, ci = (int)pp.ci, cj = (int)pp.cj;
-  T* const Wd = work + d.work_off;
-  // ---- this slice's stripes of W (p rows) then V (q rows); each wave's first stripe is read now
-  const int sw = (int)((p + 15) / 16), sv = (int)((pp.q + 15) / 16);
-  const int per = (sw + sv + slices - 1) / slices;
-  const int s0 = slice * per, s1 = min(sw + sv, s0 + per);
-  auto stripe_col = [&](int st) {
-    const bool isv = st >= sw;
-    T* X = isv ? Wd + p * pp.q : Wd;
-    const int64_t ldx = isv ? pp.q : p;
-    return [=](int c) { return X + pp.col(c) * ldx; };
-  };
-  auto stripe_rows = [&](int st) { return st >= sw ? (int)pp.q : (int)p; };
-  auto stripe_idx = [&](int st) { return st >= sw ? st - sw : st; };
-  T a[KS];
-  const int st0 = s0 + wave;
-  if (st0 < s1) stripe_load<CPLX, KS>(a, stripe_col(st0), stripe_rows(st0), stripe_idx(st0), nc, lane);
-  // ---- 1. Gram matrix: wave w < 15 takes tile w / 5 (of the upper triangle) over row chunk w % 5
-  const int T16 = (nc + 15) / 16, ntile = T16 * (T16 + 1) / 2;
-  const int nchunk = kSweepGramWaves / ntile;  // 5 (three tiles) or 15 (one tile)
-  if (wave < ntile * nchunk) {
-    const int tl = wave / nchunk, chunk = wave % nchunk;
-    int ti, tj;
-    tile_of(tl, T16, ti, tj);
-    const int64_t rc = ((p + 4 * nchunk - 1) / (4 * nchunk)) * 4;
-    const int64_t r0 = chunk * rc, r1 = min(p, r0 + rc);
-    const int ca = ti * 16 + (lane & 15), cb = tj * 16 + (lane & 15);
-    const T* colA = Wd + pp.col(ca < nc ? ca : 0) * p;
-    const T* colB = Wd + pp.col(cb < nc ? cb : 0) * p;
-    d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t k0 = r0; k0 < r1; k0 += 4) {
-      const int64_t k = k0 + (lane >> 4);
-      const T av = (k < r1 && ca < nc) ? colA[k] : N::make(0.0, 0.0);
-      const T bv = (k < r1 && cb < nc) ? colB[k] : N::make(0.0, 0.0);
-      accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(av), N::re(bv), accR, 0, 0, 0);
-      if constexpr (CPLX) {  // conj(a) b
-        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(av), N::im(bv), accR, 0, 0, 0);
-        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(av), N::im(bv), accI, 0, 0, 0);
-        accI = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(av), N::re(bv), accI, 0, 0, 0);
-      }
-    }
-    double* dst = part + (int64_t)wave * 256 * (CPLX ? 2 : 1);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      dst[v * 64 + lane] = accR[v];
-      if constexpr (CPLX) dst[256 + v * 64 + lane] = accI[v];
-    }
-  }
-  __syncthreads();
-  for (int e = tid; e < ntile * 256; e += kInnerThreads) {
-    const int tl = e >> 8, idx = e & 255;
-    const int v = idx >> 6, ln = idx & 63;
-    int ti, tj;
-    tile_of(tl, T16, ti, tj);
-    const int i = ti * 16 + (ln >> 4) + 4 * v, j = tj * 16 + (ln & 15);  // D(l / 16 + 4 v, l % 16)
-    double gr = 0.0, gi = 0.0;
-    for (int ch = 0; ch < nchunk; ++ch) {
-      const double* src = part + (int64_t)(tl * nchunk + ch) * 256 * (CPLX ? 2 : 1);
-      gr += src[idx];
-      if constexpr (CPLX) gi += src[256 + idx];
-    }
-    if (i < nc && j < nc) {
-      Gb0[j * nq + i] = N::make(gr, gi);                // G(i, j)
-      if (ti != tj) Gb0[i * nq + j] = N::make(gr, -gi);  // G(j, i) of the mirrored tile
-    }
-  }
-  __syncthreads();  // the partials (aliasing G1 and Q) are consumed
-  for (int e = tid; e < nc * nq; e += kInnerThreads) {
-    const int l = e / nq, k = e - l * nq;
-    Q[e] = N::make(l == k ? 1.0 : 0.0, 0.0);
-  }
-  // ---- 2. the job's slices have all read the pair's columns before any of them writes
-  if (slices > 1) {
-    __syncthreads();
-    if (tid == 0) {
-      __hip_atomic_fetch_add(&pair_cnt[job], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_ok = wait_at_least(&pair_cnt[job], slices * (sweep + 1), &state[2]) ? 1 : 0;
-    }
-  }
-  __syncthreads();
-  if (slices > 1 && !s_ok) return;  // uniform: a sibling slice never arrived
-  // ---- 3. inner sweep
-  const double tol = DBL_EPSILON * sqrt((double)(p > 1 ? p : 1));
-  const double floor2 = floors[jb[0]];
-  const int ne = nc + (nc & 1);
-  const int npr = pp.intra ? ne / 2 : max(ci, cj);
-  const int nsub = pp.intra ? ne - 1 : npr;
-  const int ng = npr * npr;
-  // the item of this thread: G block (pa, pb), or Q piece (row, pb), or none
-  int pa = -1, pb = -1, row = -1;
-  if (tid < ng) {
-    pa = tid / npr;
-    pb = tid - pa * npr;
-  } else if (tid < ng + npr * nc) {
-    pb = (tid - ng) / nc;
-    row = (tid - ng) - pb * nc;
-  }
-  SlotOrder oa, ob;
-  oa.init(pp.intra, ci, cj, nc, pa < 0 ? 0 : pa);
-  ob.init(pp.intra, ci, cj, nc, pb < 0 ? 0 : pb);
-  int any = 0;
-  for (int r = 0; r < nsub; ++r) {
-    const T* Gc = (r & 1) ? Gb1 : Gb0;
-    T* Gn = (r & 1) ? Gb0 : Gb1;
-    if (pb >= 0) {
-      int i2, j2;
-      ob.get(i2, j2);
-      double cb_, seb, sib;
-      const bool gob = slot_rotation<CPLX>(Gc, nq, i2, j2, nc, tol, floor2, cb_, seb, sib);
-      if (pa >= 0) {
-        int i, j;
-        oa.get(i, j);
-        double ca_, sea, sia;
-        const bool goa = slot_rotation<CPLX>(Gc, nq, i, j, nc, tol, floor2, ca_, sea, sia);
-        const bool hj = j < nc, hj2 = j2 < nc;
-        T b00 = Gc[i2 * nq + i];
-        T b01 = hj2 ? Gc[j2 * nq + i] : N::make(0.0, 0.0);
-        T b10 = hj ? Gc[i2 * nq + j] : N::make(0.0, 0.0);
-        T b11 = (hj && hj2) ? Gc[j2 * nq + j] : N::make(0.0, 0.0);
-        if (goa) {  // rows: J^H
-          apply_rot<CPLX>(b00, b10, ca_, 1.0, sea, -sia);
-          apply_rot<CPLX>(b01, b11, ca_, 1.0, sea, -sia);
-        }
-        if (gob) {  // columns: J
-          apply_rot<CPLX>(b00, b01, cb_, 1.0, seb, sib);
-          apply_rot<CPLX>(b10, b11, cb_, 1.0, seb, sib);
-        }
-        Gn[i2 * nq + i] = b00;
-        if (hj2) Gn[j2 * nq + i] = b01;
-        if (hj) Gn[i2 * nq + j] = b10;
-        if (hj && hj2) Gn[j2 * nq + j] = b11;
-        any |= (goa || gob) ? 1 : 0;
-        oa.advance();
-      } else if (gob) {
-        T u = Q[i2 * nq + row], v = Q[j2 * nq + row];
-        apply_rot<CPLX>(u, v, cb_, 1.0, seb, sib);
-        Q[i2 * nq + row] = u;
-        Q[j2 * nq + row] = v;
-      }
-      ob.advance();
-    }
-    __syncthreads();
-  }
-  const int rot = __syncthreads_or(any);
-  if (rot && slice == 0 && tid == 0)
-    __hip_atomic_store(&rotated[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  // ---- 4. update of this slice's stripes (nothing to do when no rotation happened)
-  if (rot) {
-    for (int st = st0; st < s1; st += kInnerThreads / 64) {
-      if (st != st0) stripe_load<CPLX, KS>(a, stripe_col(st), stripe_rows(st), stripe_idx(st), nc, lane);
-      stripe_mul_store<CPLX, KS>(a, stripe_col(st), stripe_rows(st), stripe_idx(st), Q, nq, nc, lane);
-    }
-  }
-}
-
 // Sweep count from the per-sweep rotation flags: the first sweep that rotated nothing ends the
-// iteration (result = its number + 1); none among the `launched` sweeps: -1; abandoned: -2.
+// iteration (result = its number + 1); none among the `launched` sweeps: -1; plan too small: -3.
 __global__ void panel_result_kernel(int32_t* __restrict__ state, int launched) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int res = -1;
@@ -1218,7 +938,7 @@ __global__ void panel_result_kernel(int32_t* __restrict__ state, int launched) {
       res = s + 1;
       break;
     }
-  state[3] = state[2] ? -2 : res;
+  state[3] = state[1] ? -3 : res;
 }
 
 template <bool CPLX, int MODE>
@@ -1909,12 +1629,16 @@ extern "C" int yastn_hip_svd_panel_init(const void* A, void* work, double* mu, d
 
 extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int dtype, const int64_t* d_blocks,
                                          const int32_t* d_jobs, const int64_t* h_rounds, int64_t nrounds, int b,
-                                         int maxch_p, int slices, void* part, int64_t part_numel, int32_t* d_flags,
-                                         hipStream_t stream) {
+                                         int maxch_p, int slices, void* part, int64_t part_numel, int sweep,
+                                         int max_sweeps, int32_t* d_state, int64_t state_numel, hipStream_t stream) {
   YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
   YH_CHECK_ARG(b >= 1 && b <= 32, "panel width %d", b);
   YH_CHECK_ARG(maxch_p >= 1 && maxch_p <= 65535, "chunk count %d", maxch_p);
   YH_CHECK_ARG(slices >= 1 && slices <= 64, "update slices %d", slices);
+  YH_CHECK_ARG(max_sweeps >= 1 && max_sweeps <= 1000 && sweep >= 0 && sweep < max_sweeps, "sweep %d of %d", sweep,
+               max_sweeps);
+  YH_CHECK_ARG(d_state != nullptr && state_numel >= kStateHead + max_sweeps, "state buffer of %lld words",
+               (long long)state_numel);
   const bool cplx = dtype == YH_C128;
   const int T16m = (2 * b + 15) / 16, ntmax = T16m * (T16m + 1) / 2;
   const int64_t isz = cplx ? 16 : 8;
@@ -1944,66 +1668,30 @@ extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int d
     const int64_t off = h_rounds[2 * r], n = h_rounds[2 * r + 1];
     if (n == 0) continue;
     const int32_t* jobs = d_jobs + 3 * off;
-    int32_t* flags = d_flags + off;
     const dim3 gg((unsigned)gch, (unsigned)n), gi((unsigned)n, (unsigned)slices);
     if (cplx) {
       hipLaunchKernelGGL(panel_gram_kernel<true>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
-                         (const double2*)work, (double*)part, b, gch);
+                         (const double2*)work, (double*)part, b, gch, d_state, sweep);
       if (wide)
         hipLaunchKernelGGL((panel_inner_kernel<true, 16>), gi, dim3(kInnerThreads), (size_t)lds_inner, stream,
-                           d_blocks, jobs, (double2*)work, floors, (const double*)part, flags, b, gch, lds_inner);
+                           d_blocks, jobs, (double2*)work, floors, (const double*)part, d_state, sweep, b, gch,
+                           lds_inner);
       else
         hipLaunchKernelGGL((panel_inner_kernel<true, 8>), gi, dim3(kInnerThreads), (size_t)lds_inner, stream,
-                           d_blocks, jobs, (double2*)work, floors, (const double*)part, flags, b, gch, lds_inner);
+                           d_blocks, jobs, (double2*)work, floors, (const double*)part, d_state, sweep, b, gch,
+                           lds_inner);
     } else {
       hipLaunchKernelGGL(panel_gram_kernel<false>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
-                         (const double*)work, (double*)part, b, gch);
+                         (const double*)work, (double*)part, b, gch, d_state, sweep);
       if (wide)
         hipLaunchKernelGGL((panel_inner_kernel<false, 16>), gi, dim3(kInnerThreads), (size_t)lds_inner, stream,
-                           d_blocks, jobs, (double*)work, floors, (const double*)part, flags, b, gch, lds_inner);
+                           d_blocks, jobs, (double*)work, floors, (const double*)part, d_state, sweep, b, gch,
+                           lds_inner);
       else
         hipLaunchKernelGGL((panel_inner_kernel<false, 8>), gi, dim3(kInnerThreads), (size_t)lds_inner, stream,
-                           d_blocks, jobs, (double*)work, floors, (const double*)part, flags, b, gch, lds_inner);
+                           d_blocks, jobs, (double*)work, floors, (const double*)part, d_state, sweep, b, gch,
+                           lds_inner);
     }
-    st = yh::hip_status(hipGetLastError(), "svd panel round launch");
-    if (st) return st;
-  }
-  return YH_OK;
-}
-
-extern "C" int yastn_hip_svd_panel_rounds(void* work, const double* floors, int dtype, const int64_t* d_blocks,
-                                          const int32_t* d_jobs, const int64_t* h_rounds, int64_t nrounds, int b,
-                                          int slices, int sweep, int max_sweeps, int32_t* d_state,
-                                          int64_t state_numel, int64_t njobs, hipStream_t stream) {
-  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
-  YH_CHECK_ARG(b >= 1 && b <= 16, "panel width %d (the fused rounds take b <= 16)", b);
-  YH_CHECK_ARG(slices >= 1 && slices <= 64, "update slices %d", slices);
-  YH_CHECK_ARG(max_sweeps >= 1 && max_sweeps <= 1000 && sweep >= 0 && sweep < max_sweeps, "sweep %d of %d", sweep,
-               max_sweeps);
-  YH_CHECK_ARG(state_numel >= kStateHead + max_sweeps + njobs, "state buffer of %lld words", (long long)state_numel);
-  const bool cplx = dtype == YH_C128;
-  const int64_t isz = cplx ? 16 : 8;
-  const int64_t gsz = (int64_t)(2 * b) * (2 * b + 1) * isz;
-  const int64_t psz = (int64_t)kSweepGramWaves * 256 * isz;
-  const int64_t lds = gsz + std::max(2 * gsz, psz);
-  const void* fn = cplx ? (const void*)panel_round_kernel<true> : (const void*)panel_round_kernel<false>;
-  int st = yh::hip_status(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                          "svd panel round LDS size");
-  if (st) return st;
-  int64_t end = 0;
-  for (int64_t r = 0; r < nrounds; ++r) {
-    const int64_t off = h_rounds[2 * r], n = h_rounds[2 * r + 1];
-    YH_CHECK_ARG(off >= end && n >= 0 && off + n <= njobs, "round %lld", (long long)r);
-    YH_CHECK_ARG(n * slices <= 0x7fffffff, "round %lld: %lld workgroups", (long long)r, (long long)(n * slices));
-    end = off + n;
-    if (n == 0) continue;
-    const dim3 g((unsigned)(n * slices));
-    if (cplx)
-      hipLaunchKernelGGL(panel_round_kernel<true>, g, dim3(kInnerThreads), (size_t)lds, stream, d_blocks, d_jobs, off,
-                         (double2*)work, floors, b, slices, sweep, max_sweeps, d_state);
-    else
-      hipLaunchKernelGGL(panel_round_kernel<false>, g, dim3(kInnerThreads), (size_t)lds, stream, d_blocks, d_jobs, off,
-                         (double*)work, floors, b, slices, sweep, max_sweeps, d_state);
     st = yh::hip_status(hipGetLastError(), "svd panel round launch");
     if (st) return st;
   }

@@ -676,14 +676,6 @@ def bench(steps=3, D=6, chi=64, plain=True, cpu=True, host_runs=5):
     tb, lb, gb = run(backend_hip, 'cuda:0', steps)
     out['s_per_step_batched'] = tb
     out['loss'] = lb
-    # the previous panel-path driver (a Gram launch and an inner-sweep launch per round)
-    F.FUSED_ROUNDS = False
-    try:
-        t_old, l_old, g_old = run(backend_hip, 'cuda:0', steps)
-    finally:
-        F.FUSED_ROUNDS = True
-    out['two_launch_rounds'] = {'s_per_step': t_old, 'loss_diff': abs(l_old - lb),
-                                'grad_rel_diff': float((g_old - gb).norm() / gb.norm())}
     # the two projector SVDs of each move in one batched call (the reference calls them one by one)
     tm, lm, gm = run(backend_hip, 'cuda:0', steps, batch=True)
     out['move_batched_svds'] = {'s_per_step': tm, 'loss_diff': abs(lm - lb),

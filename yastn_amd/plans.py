@@ -780,8 +780,6 @@ class SvdPlan:
             self.h_rounds = np.ascontiguousarray(np.array(offs, dtype=np.int64).reshape(-1, 2))
             self.d_rounds = torch.from_numpy(self.h_rounds).to(device)
             self.max_round_jobs = maxn
-            # workgroups per job of the fused rounds (yastn_hip_svd_panel_rounds): as for the two-launch rounds
-            self.sweep_slices = self.panel_slices
             self.h_jobs = np.ascontiguousarray(np.array(jobs, dtype=np.int32).reshape(-1, 3))
             L.call('yastn_hip_check_panel_jobs', self.h_jblocks.ctypes.data, self.nblocks, self.h_jobs.ctypes.data,
                    len(jobs), width, mode, self.maxch_p)
