@@ -217,14 +217,15 @@ def bench_heff2(args, out):
         if not args.no_cpu:  # the plain per-block path on the host cores (torch CPU; LAPACK per block)
             cfgh = bs.make_config(backend_plain, 'U1', device='cpu')
             resh = res._replace(config=cfgh, data=res.data.cpu())
-            W.two_site_truncation(resh, 1024)
+            for _ in range(3):
+                W.two_site_truncation(resh, 1024)
             th = []
-            for _ in range(5):
+            for _ in range(10):
                 t0 = time.perf_counter()
                 W.two_site_truncation(resh, 1024)
                 th.append(time.perf_counter() - t0)
             out['svd_truncation']['ms_host'] = float(np.median(th)) * 1e3
-            out['svd_truncation']['host'] = f'median of 5, torch CPU, {torch.get_num_threads()} threads'
+            out['svd_truncation']['host'] = f'median of 10 after 3 warm-ups, torch CPU, {torch.get_num_threads()} threads'
             out['svd_truncation']['speedup_vs_host'] = float(np.median(th)) / ts
     # roofline of the dominant kernel (the launch with the largest duration)
     stats = _dot_launch_stats(backend_hip, lambda: W.heff2(AA, FL, FR, Wm))

@@ -649,20 +649,21 @@ def host_info():
     return info
 
 
-def bench(steps=3, D=6, chi=64, plain=True, cpu=True, host_runs=5):
+def bench(steps=3, D=6, chi=64, plain=True, cpu=True, host_runs=10, host_warmup=3):
     """Seconds per CTMRG fwd+bwd step (D=6, chi=64, U(1), float64): batched HIP backend (eager, the
     reference's data-dependent truncation) and the same step with static-structure truncation replayed as
     one HIP graph, vs the plain per-block GPU path vs the plain path on the host cores (torch CPU, median of
-    host_runs runs)."""
+    host_runs runs after host_warmup warm-up steps)."""
     import torch
     from . import backend_hip, backend_plain
     from . import _functions as F
     out = {'workload': f'CTMRG hv step fwd+bwd, U(1) iPEPS 1x1, D={D}, chi={chi}, float64; loss = 2x2 cluster <Sz>'}
 
-    def run(backend, device, n, batch=False, median=False):
+    def run(backend, device, n, batch=False, median=False, warmup=1):
         cfg = bs.make_config(backend, 'U1', device=device)
         A, env, sz = setup(cfg, D=D, chi=chi, seed=0)
-        fwd_bwd(A, env, sz, chi, batch=batch)  # warm-up (plans)
+        for _ in range(warmup):  # warm-up (plans)
+            fwd_bwd(A, env, sz, chi, batch=batch)
         sync = torch.cuda.synchronize if device != 'cpu' else (lambda: None)
         sync()
         ts = []
@@ -710,9 +711,10 @@ def bench(steps=3, D=6, chi=64, plain=True, cpu=True, host_runs=5):
         out['plain_svd'] = ('torch.linalg.svd per block, full_matrices=False, preferred linalg library '
                             f'{torch.backends.cuda.preferred_linalg_library()}')
     if cpu:
-        tc, lc, gc = run(backend_plain, 'cpu', host_runs, median=True)
+        tc, lc, gc = run(backend_plain, 'cpu', host_runs, median=True, warmup=host_warmup)
         out['s_per_step_host_torch'] = tc
         out['host_runs'] = host_runs
+        out['host_warmup'] = host_warmup
         out['host'] = host_info()
         out['speedup_vs_host'] = tc / tb
         out['move_batched_svds']['speedup_vs_host'] = tc / tm

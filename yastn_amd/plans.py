@@ -466,10 +466,28 @@ class GemmPlan:
     def histogram_key(self):
         return tuple(sorted(zip(self.h_gemms[:, 0].tolist(), self.h_gemms[:, 1].tolist(), self.k_of.tolist())))
 
-    def autotune(self, run, reps=3):
-        """Pick the fastest tile shape for this plan by timing ``run()`` (one launch of the plan) with
-        every shape; the choice is remembered per block-size histogram, so plans of other operands
-        with the same sector shapes reuse it."""
+    def operand_traffic(self, cfg):
+        """Operand elements the tiles of shape ``cfg`` request (A's row panel once per column tile, B's
+        column panel once per row tile, C once): the L2 / HBM reads a tile shape causes beyond the
+        algorithmic M K + K N + M N."""
+        bm, bn = L.TILE_DIMS[cfg]
+        M = self.h_gemms[:, 0].astype(np.float64)
+        N = self.h_gemms[:, 1].astype(np.float64)
+        K = self.k_of.astype(np.float64)
+        return float(np.sum(M * K * np.ceil(N / bn) + K * N * np.ceil(M / bm) + M * N))
+
+    def pick_tile(self, times, near=0.05):
+        """Of the timed shapes ({cfg: ms}) within ``near`` of the fastest, the one with the least
+        operand traffic."""
+        fastest = min(times.values())
+        return min((c for c in times if times[c] <= fastest * (1 + near)), key=self.operand_traffic)
+
+    def autotune(self, run, reps=3, near=0.05):
+        """Pick a tile shape for this plan by timing ``run()`` (one launch of the plan) with every shape:
+        of the shapes within ``near`` (relative) of the fastest, the one whose tiles request the fewest
+        operand elements (``operand_traffic``), so that near-ties go to the shape with fewer re-reads.
+        The choice is remembered per block-size histogram, so plans of other operands with the same
+        sector shapes reuse it."""
         if getattr(self, 'tuned', False):
             return
         key = self.histogram_key()
@@ -491,7 +509,7 @@ class GemmPlan:
             e1.record()
             e1.synchronize()
             times[cfg] = e0.elapsed_time(e1) / reps
-        best = min(times, key=times.get)
+        best = self.pick_tile(times, near)
         self.set_cfg(best)
         self.tune_times = times
         self.tuned = True
