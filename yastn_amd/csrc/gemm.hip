@@ -265,16 +265,19 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
 
 
 // Split-K reduction: for each record, C (=|+=) sum_s P_s in the order s = 0, 1, ... (deterministic).
+constexpr unsigned kSplitKBlocks = 32;
 // Record words (YH_SPLITK_WORDS): 0 M, 1 N, 2 c_off, 3 c_rs, 4 c_cs, 5 p_off (partial s at
 // p_off + s M N, row-major M x N), 6 S, 7 flags (bit 0 accumulate, bits 8-15 output buffer).
 template <bool CPLX>
 __global__ void __launch_bounds__(kThreads) splitk_reduce_kernel(Bufs bufs, const typename Elt<CPLX>::T* __restrict__ P,
                                                                  const int64_t* __restrict__ recs) {
   typedef typename Elt<CPLX>::T T;
-  const int64_t* w = recs + (int64_t)blockIdx.x * YH_SPLITK_WORDS;
+  const int64_t* w = recs + (int64_t)blockIdx.y * YH_SPLITK_WORDS;
   const int64_t M = w[0], N = w[1], c_off = w[2], c_rs = w[3], c_cs = w[4], p_off = w[5], S = w[6], fl = w[7];
   T* __restrict__ C = (T*)bufs.out[(fl >> 8) & 0xff];
-  for (int64_t e = threadIdx.x; e < M * N; e += kThreads) {
+  // kSplitKBlocks workgroups per record stride over its M N elements: a record's partials are read by
+  // many CUs at once (one workgroup per record was bound by a single CU's memory throughput)
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < M * N; e += (int64_t)gridDim.x * kThreads) {
     const int64_t i = e / N, j = e - i * N;
     T acc = P[p_off + e];
     for (int64_t s = 1; s < S; ++s) {
@@ -522,18 +525,20 @@ extern "C" int yastn_hip_check_splitk_plan(const int64_t* h_recs, int64_t nrec, 
 extern "C" int yastn_hip_splitk_reduce(void* const* outs, const void* partials, int dtype, const int64_t* d_recs,
                                        int64_t nrec, hipStream_t stream) {
   YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(nrec >= 0 && nrec <= 65535, "split-k records %lld", (long long)nrec);
   if (nrec == 0) return YH_OK;
+  const dim3 grid(kSplitKBlocks, (unsigned)nrec);
   Bufs b;
   for (int i = 0; i < YH_MAX_BUFS; ++i) {
     b.in[i] = nullptr;
     b.out[i] = outs[i];
   }
   if (dtype == YH_F64)
-    hipLaunchKernelGGL(splitk_reduce_kernel<false>, dim3((unsigned)nrec), dim3(kThreads), 0, stream, b,
-                       (const double*)partials, d_recs);
+    hipLaunchKernelGGL(splitk_reduce_kernel<false>, grid, dim3(kThreads), 0, stream, b, (const double*)partials,
+                       d_recs);
   else
-    hipLaunchKernelGGL(splitk_reduce_kernel<true>, dim3((unsigned)nrec), dim3(kThreads), 0, stream, b,
-                       (const double2*)partials, d_recs);
+    hipLaunchKernelGGL(splitk_reduce_kernel<true>, grid, dim3(kThreads), 0, stream, b, (const double2*)partials,
+                       d_recs);
   return yh::hip_status(hipGetLastError(), "split-k reduce launch");
 }
 
