@@ -152,6 +152,8 @@ def copy(x):
 def to_numpy(x):
     if isinstance(x, (int, float, complex)):
         return x
+    if F._is_wrapped(x):  # under a torch.func transform: read the plain tensor
+        return F.host_array(x).copy()
     return x.detach().cpu().resolve_conj().numpy().copy()
 
 

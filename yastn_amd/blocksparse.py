@@ -1302,13 +1302,18 @@ def _svd_many_same_dtype(items, fix_signs=False, **kwargs):
 
 def truncation_mask(S, tol=0, tol_block=0, D_block=float('inf'), D_total=float('inf'), truncate_multiplets=False,
                     mask_f=None, **kwargs):
-    """Boolean diagonal mask of the kept singular values (reference linalg.py:637-735)."""
+    """Boolean diagonal mask of the kept singular values (reference linalg.py:637-735).
+
+    The reference decides on the host from S, reading one count per block and the global one
+    (``.item()``, a device round trip each) and sorting on the device.  Here S is read to the host once
+    and the same rule runs in numpy (``np.argsort`` orders ties as the reference numpy backend does);
+    the kept indices ride along as the mask's ``host_index``, so applying it reads nothing back."""
     if mask_f is not None:
         return mask_f(S)
     be = S.config.backend
     nsym = S.config.sym.NSYM
-    Sdata = be.detach(S._data)
-    keep = Sdata > -float('inf')
+    Sh = np.asarray(be.to_numpy(be.detach(S._data))).real
+    keep = np.ones(Sh.shape, dtype=bool)
     if truncate_multiplets:
         tol_block, D_block = 0, float('inf')
     tol_null = 0. if isinstance(tol_block, dict) else tol_block
@@ -1317,32 +1322,34 @@ def truncation_mask(S, tol=0, tol_block=0, D_block=float('inf'), D_total=float('
         t = t[:nsym]
         lo, hi = sl.slcs[0]
         tol_rel = tol_block[t] if isinstance(tol_block, dict) and t in tol_block else tol_null
-        blk = Sdata[lo:hi]
-        D_tol = be.sum_elements(blk > tol_rel * be.max_abs(blk)).item()
+        blk = Sh[lo:hi]
+        D_tol = int(np.sum(blk > tol_rel * np.max(np.abs(blk)))) if hi > lo else 0
         D_bl = D_block[t] if isinstance(D_block, dict) and t in D_block else D_null
         D_bl = min(D_bl, D_tol)
         if 0 < D_bl < sl.Dp:
-            inds = be.argsort(blk)
+            inds = np.argsort(blk)
             keep[lo:hi][inds[:-D_bl]] = False
         elif D_bl == 0:
             keep[lo:hi] = False
-    tmp = Sdata * keep
-    D_tol = be.sum_elements(tmp > tol * be.max_abs(tmp)).item()
+    tmp = Sh * keep
+    D_tol = int(np.sum(tmp > tol * np.max(np.abs(tmp)))) if tmp.size else 0
     D_total = min(D_total, D_tol)
     if D_total == 0:
         keep[:] = False
-        return S._replace(data=keep)
-    inds = be.argsort(tmp)
-    if truncate_multiplets and D_total < len(inds):
-        gap = -1
-        for p in range(D_total, len(inds)):
-            gp = abs(Sdata[inds[-p]] - Sdata[inds[-p - 1]])
-            if gp > gap:
-                D_total, gap = p, gp
-            if gap > abs(Sdata[inds[-p]]):
-                break
-    keep[inds[:-D_total]] = False
-    return S._replace(data=keep)
+    else:
+        inds = np.argsort(tmp)
+        if truncate_multiplets and D_total < len(inds):
+            gap = -1
+            for p in range(D_total, len(inds)):
+                gp = abs(Sh[inds[-p]] - Sh[inds[-p - 1]])
+                if gp > gap:
+                    D_total, gap = p, gp
+                if gap > abs(Sh[inds[-p]]):
+                    break
+        keep[inds[:-D_total]] = False
+    m = S._replace(data=be.to_tensor(keep, dtype='bool', device=S.config.device))
+    m.host_index = {t[:nsym]: be.to_mask(keep[sl.slcs[0][0]:sl.slcs[0][1]]) for t, sl in zip(S.struct.t, S.slices)}
+    return m
 
 
 def truncation_mask_multiplets(S, tol=0, D_total=float('inf'), eps_multiplet=1e-13, hermitian=False, **kwargs):
