@@ -39,6 +39,9 @@ struct Elt<true> {
   typedef double2 T;
 };
 
+// One zero element (never written), read in place of operand elements outside a block.
+__device__ double2 kZeroElt[1];
+
 __device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
 // One workgroup per output tile.  Per term, the k loop is software-pipelined: the next BK slice of
@@ -59,6 +62,8 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
   constexpr int NSU = NS < KSUB ? NS : KSUB;  // partial accumulator sets actually used
   constexpr int A_PER = (BM * BK + kThreads - 1) / kThreads;
   constexpr int B_PER = (BN * BK + kThreads - 1) / kThreads;
+  constexpr bool A_FULL = (BM * BK) % kThreads == 0;  // every thread has all A_PER elements
+  constexpr bool B_FULL = (BN * BK) % kThreads == 0;
   static_assert(BK % 4 == 0, "BK");
   __shared__ double As[2][NPL][BK][BM + PAD];
   __shared__ double Bs[2][NPL][BK][BN + PAD];
@@ -110,7 +115,7 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
         a_i[p] = e % BM;
         a_k[p] = e / BM;
       }
-      a_in[p] = (e < BM * BK) && (m0 + a_i[p] < M);
+      a_in[p] = (A_FULL || e < BM * BK) && (m0 + a_i[p] < M);
     }
 #pragma unroll
     for (int p = 0; p < B_PER; ++p) {
@@ -122,35 +127,33 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
         b_k[p] = e % BK;
         b_j[p] = e / BK;
       }
-      b_in[p] = (e < BN * BK) && (n0 + b_j[p] < N);
+      b_in[p] = (B_FULL || e < BN * BK) && (n0 + b_j[p] < N);
     }
 
     // Two register sets: while slice s is consumed from LDS, slice s+1 waits in one set (loaded an
     // iteration earlier) and slice s+2 is being loaded into the other, so two slices of global loads
     // are in flight; slice s+1 goes to the other LDS buffer at the end of the iteration.
     T ra0[A_PER], rb0[B_PER], ra1[A_PER], rb1[B_PER];
+    // Every lane issues every load; an element outside the operand is read from kZeroElt.  (A select
+    // after the load would be hoisted by the scheduler and wait for loads still meant to be in flight.)
+    const T* zero = (const T*)kZeroElt;
+    const T* a_row[A_PER];
+    const T* b_col[B_PER];
+#pragma unroll
+    for (int p = 0; p < A_PER; ++p) a_row[p] = A + a_off + (m0 + a_i[p]) * a_rs + a_k[p] * a_cs;
+#pragma unroll
+    for (int p = 0; p < B_PER; ++p) b_col[p] = B + b_off + b_k[p] * b_rs + (n0 + b_j[p]) * b_cs;
     auto load = [&](int64_t k0, T* ra, T* rb) {
+      const int64_t da = k0 * a_cs, db = k0 * b_rs;
 #pragma unroll
-      for (int p = 0; p < A_PER; ++p) {
-        const int64_t gk = k0 + a_k[p];
-        if (a_in[p] && gk < K)
-          ra[p] = A[a_off + (m0 + a_i[p]) * a_rs + gk * a_cs];
-        else
-          ra[p] = T{};
-      }
+      for (int p = 0; p < A_PER; ++p) ra[p] = *((a_in[p] & (k0 + a_k[p] < K)) ? a_row[p] + da : zero);
 #pragma unroll
-      for (int p = 0; p < B_PER; ++p) {
-        const int64_t gk = k0 + b_k[p];
-        if (b_in[p] && gk < K)
-          rb[p] = B[b_off + gk * b_rs + (n0 + b_j[p]) * b_cs];
-        else
-          rb[p] = T{};
-      }
+      for (int p = 0; p < B_PER; ++p) rb[p] = *((b_in[p] & (k0 + b_k[p] < K)) ? b_col[p] + db : zero);
     };
     auto store = [&](int buf, const T* ra, const T* rb) {
 #pragma unroll
       for (int p = 0; p < A_PER; ++p) {
-        if (tid + p * kThreads < BM * BK) {
+        if (A_FULL || tid + p * kThreads < BM * BK) {
           if constexpr (CPLX) {
             As[buf][0][a_k[p]][a_i[p]] = ra[p].x;
             As[buf][1][a_k[p]][a_i[p]] = conjA ? -ra[p].y : ra[p].y;
@@ -161,7 +164,7 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
       }
 #pragma unroll
       for (int p = 0; p < B_PER; ++p) {
-        if (tid + p * kThreads < BN * BK) {
+        if (B_FULL || tid + p * kThreads < BN * BK) {
           if constexpr (CPLX) {
             Bs[buf][0][b_k[p]][b_j[p]] = rb[p].x;
             Bs[buf][1][b_k[p]][b_j[p]] = conjB ? -rb[p].y : rb[p].y;
@@ -171,59 +174,77 @@ __global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t
         }
       }
     };
-    // ---- MFMA over one BK slice of LDS buffer `buf`, 4 k per instruction
-    auto compute = [&](int buf) {
+    // ---- One BK slice in two steps: read_frag copies the wave's operand fragments of LDS buffer `buf`
+    // into registers, mma runs the MFMAs on them (4 k per instruction).  The fragments of slice s+1 are
+    // read while the MFMAs of slice s run, so the LDS round trip after each barrier is hidden.
+    typedef double FragA[KSUB][FM];
+    typedef double FragB[KSUB][FN];
+    FragA far0, fai0, far1, fai1;
+    FragB fbr0, fbi0, fbr1, fbi1;
+    auto read_frag = [&](int buf, FragA& ar, FragA& ai, FragB& br, FragB& bi) {
 #pragma unroll
       for (int ks = 0; ks < KSUB; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
-        constexpr int S = NSU;
-        const int s = ks % S;
-        double ar[FM], ai[FM], br[FN], bi[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
           const int col = wm * (BM / WM) + i * 16 + (lane & 15);
-          ar[i] = As[buf][0][kr][col];
-          if constexpr (CPLX) ai[i] = As[buf][1][kr][col];
+          ar[ks][i] = As[buf][0][kr][col];
+          if constexpr (CPLX) ai[ks][i] = As[buf][1][kr][col];
         }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
           const int col = wn * (BN / WN) + j * 16 + (lane & 15);
-          br[j] = Bs[buf][0][kr][col];
-          if constexpr (CPLX) bi[j] = Bs[buf][1][kr][col];
+          br[ks][j] = Bs[buf][0][kr][col];
+          if constexpr (CPLX) bi[ks][j] = Bs[buf][1][kr][col];
         }
+      }
+    };
+    auto mma = [&](const FragA& ar, const FragA& ai, const FragB& br, const FragB& bi) {
+#pragma unroll
+      for (int ks = 0; ks < KSUB; ++ks) {
+        const int s = ks % NSU;
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
-            accR[s][i][j] = mfma(ar[i], br[j], accR[s][i][j]);
+            accR[s][i][j] = mfma(ar[ks][i], br[ks][j], accR[s][i][j]);
             if constexpr (CPLX) {
-              accR[s][i][j] = mfma(-ai[i], bi[j], accR[s][i][j]);
-              accI[s][i][j] = mfma(ar[i], bi[j], accI[s][i][j]);
-              accI[s][i][j] = mfma(ai[i], br[j], accI[s][i][j]);
+              accR[s][i][j] = mfma(-ai[ks][i], bi[ks][j], accR[s][i][j]);
+              accI[s][i][j] = mfma(ar[ks][i], bi[ks][j], accI[s][i][j]);
+              accI[s][i][j] = mfma(ai[ks][i], br[ks][j], accI[s][i][j]);
             }
           }
       }
     };
 
     const int64_t ns = (K + BK - 1) / BK;  // slices
+    // The loads are issued unconditionally (slices past the end read only in-range fallbacks): a
+    // load under a branch leaves the compiler unsure how many are in flight, and it then waits for
+    // all of them before the next LDS store.  A buffer is overwritten only after a barrier that
+    // follows the completion of every wave's fragment reads of it (__syncthreads waits for them).
     load(0, ra0, rb0);
     store(0, ra0, rb0);
-    if (ns > 1) load(BK, ra0, rb0);
-    if (ns > 2) load(2 * BK, ra1, rb1);
+    load(BK, ra0, rb0);
+    load(2 * BK, ra1, rb1);
     __syncthreads();
-    for (int64_t s = 0; s < ns; s += 2) {
-      // even slice s in buffer 0; set 0 holds slice s+1, set 1 slice s+2
-      compute(0);
-      if (s + 1 < ns) store(1, ra0, rb0);
-      if (s + 3 < ns) load((s + 3) * BK, ra0, rb0);
+    read_frag(0, far0, fai0, fbr0, fbi0);
+    int64_t s = 0;
+    for (; s + 1 < ns; s += 2) {
+      // fragments 0 hold slice s; register set 0 slice s+1, set 1 slice s+2
+      store(1, ra0, rb0);
+      load((s + 3) * BK, ra0, rb0);
       __syncthreads();
-      if (s + 1 >= ns) break;
-      // odd slice s+1 in buffer 1; set 1 holds slice s+2, set 0 slice s+3
-      compute(1);
-      if (s + 2 < ns) store(0, ra1, rb1);
-      if (s + 4 < ns) load((s + 4) * BK, ra1, rb1);
+      read_frag(1, far1, fai1, fbr1, fbi1);
+      mma(far0, fai0, fbr0, fbi0);
+      // fragments 1 hold slice s+1; set 1 slice s+2 (all zeros past the end), set 0 slice s+3
+      store(0, ra1, rb1);
+      load((s + 4) * BK, ra1, rb1);
       __syncthreads();
+      read_frag(0, far0, fai0, fbr0, fbi0);
+      mma(far1, fai1, fbr1, fbi1);
     }
+    if (s < ns) mma(far0, fai0, fbr0, fbi0);  // odd slice count: fragments 0 hold the last slice
+    __syncthreads();  // the next term's first store reuses buffer 0
   }
 
   // ---- epilogue: v_mfma_f64_16x16x4f64 leaves D[lane/16 + 4 v][lane % 16] in item v of each lane
@@ -377,14 +398,16 @@ __global__ void __launch_bounds__(kThreads) gemm_smallm_kernel(Bufs bufs, const 
   }
 }
 
-// BK per configuration: the double-buffered LDS tiles stay within 64 KiB.
+// BK per configuration (measured on the Heff2 chi=1024 and U(1)xU(1) chi=2048 sector GEMMs): shorter
+// slices are faster now that two slices of global loads really are in flight; the double-buffered
+// LDS tiles stay within 64 KiB.
 template <bool CPLX>
 int launch(const Bufs& b, const int64_t* gemms, const int64_t* terms, const int32_t* tiles, int64_t ntiles,
            int cfg, hipStream_t s) {
   const dim3 grid((unsigned)ntiles), block(kThreads);
-  constexpr int BKW = CPLX ? 8 : 16;  // 64 / 128-wide tiles
+  constexpr int BKW = 8;              // 64 / 128-wide tiles
   constexpr int BKH = CPLX ? 4 : 8;   // 128 x 128
-  constexpr int BKS = CPLX ? 16 : 32; // 32-wide and skinny tiles: deeper slices, more loads in flight
+  constexpr int BKS = CPLX ? 8 : 16;  // 32-wide and skinny tiles
   switch (cfg) {
     case YH_TILE_32x32:
       hipLaunchKernelGGL((gemm_kernel<CPLX, 32, 32, 2, 2, BKS>), grid, block, 0, s, b, gemms, terms, tiles);
