@@ -56,8 +56,60 @@ __device__ inline double2 zero_of<double2>() { return make_double2(0.0, 0.0); }
 
 // The job words are read in place (wave-uniform, so they live in scalar registers), with the loops
 // over dimensions unrolled to YH_MAXR so that no per-thread array is indexed at run time; the
-// multi-index decoding uses 32-bit arithmetic (a box holds fewer than 2^31 elements, checked on the
-// host), the offsets 64-bit.
+// multi-index arithmetic is 32-bit (a box holds fewer than 2^31 elements, checked on the host), the
+// offsets 64-bit.  Every load of a thread is issued before the first wait: loads sit outside branches
+// (an index past the end of the box reads the box's first element, whose value is then not stored),
+// since a load under a per-lane branch gets its own wait.
+// Kind-0 tile of a rank-R box (R >= 2): the multi-index of the thread's first element e (divisions
+// once per thread), then each step of kThreads elements adds the uniform digits of kThreads in the box's
+// mixed radix, with one carry per dimension.
+template <typename T, int R>
+__device__ __forceinline__ void gather_box(T* __restrict__ dst, const T* __restrict__ src, const int64_t* __restrict__ w,
+                                           uint32_t e, uint32_t e1, int64_t src_off, int64_t dst_off) {
+  uint32_t n[R], idx[R], dig[R];
+  int64_t ss[R], ds[R];
+#pragma unroll
+  for (int d = 0; d < R; ++d) {
+    n[d] = (uint32_t)w[kShape + d];
+    ss[d] = w[kSrcStride + d];
+    ds[d] = w[kDstStride + d];
+  }
+  uint32_t rem = e, dr = kThreads;
+#pragma unroll
+  for (int d = R - 1; d >= 0; --d) {
+    const uint32_t q = rem / n[d], qd = dr / n[d];
+    idx[d] = rem - q * n[d];
+    dig[d] = dr - qd * n[d];
+    rem = q;
+    dr = qd;
+  }
+  T v[kRun];
+  int64_t dofs[kRun];
+#pragma unroll
+  for (int i = 0; i < kRun; ++i) {
+    if (i > 0) {
+      uint32_t carry = 0;
+#pragma unroll
+      for (int d = R - 1; d >= 0; --d) {
+        const uint32_t x = idx[d] + dig[d] + carry;
+        carry = x >= n[d];
+        idx[d] = carry ? x - n[d] : x;
+      }
+    }
+    int64_t so = 0, dof = 0;
+#pragma unroll
+    for (int d = 0; d < R; ++d) {
+      so += (int64_t)idx[d] * ss[d];
+      dof += (int64_t)idx[d] * ds[d];
+    }
+    if (src_off >= 0) v[i] = src[src_off + (e + i * kThreads < e1 ? so : 0)];
+    dofs[i] = dof;
+  }
+#pragma unroll
+  for (int i = 0; i < kRun; ++i)
+    if (e + i * kThreads < e1) dst[dst_off + dofs[i]] = (src_off >= 0) ? v[i] : zero_of<T>();
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads) gather_kernel(T* __restrict__ dst, const T* __restrict__ src,
                                                           const int64_t* __restrict__ jobs,
@@ -71,45 +123,40 @@ __global__ void __launch_bounds__(kThreads) gather_kernel(T* __restrict__ dst, c
   const int64_t local = t - w[5];
 
   if (kind == 0) {
+    // Lanes take consecutive destination elements (coalesced); thread element i is e + i kThreads.
     uint32_t numel = 1;
 #pragma unroll
     for (int d = 0; d < YH_MAXR; ++d)
       if (d < r) numel *= (uint32_t)w[kShape + d];
-    const uint32_t e0 = (uint32_t)local * YH_GATHER_TILE;
-    const uint32_t e1 = min(numel, e0 + (uint32_t)YH_GATHER_TILE);
-    const uint32_t L = (uint32_t)w[kShape + r - 1];
-    const int64_t ssl = w[kSrcStride + r - 1];
-    // lanes take consecutive destination elements (coalesced); each thread's kRun elements are
-    // decoded and loaded first, then stored, so that all its loads are in flight together
+    const uint32_t base = (uint32_t)local * YH_GATHER_TILE;
+    const uint32_t e1 = min(numel, base + (uint32_t)YH_GATHER_TILE);
+    const uint32_t e = base + threadIdx.x;
     T v[kRun];
-    int64_t dofs[kRun];
+    if (r == 1) {  // one contiguous destination run: no index arithmetic
+      const int64_t ss = w[kSrcStride];
+      if (src_off >= 0) {
 #pragma unroll
-    for (int i = 0; i < kRun; ++i) {
-      const uint32_t e = e0 + threadIdx.x + i * kThreads;
-      dofs[i] = -1;
-      if (e < e1) {
-        uint32_t rem = e / L;
-        const uint32_t c = e - rem * L;
-        int64_t so = (int64_t)c * ssl;
-        int64_t dof = c;  // dst stride of the last dim is 1
-#pragma unroll
-        for (int d = YH_MAXR - 2; d >= 0; --d) {
-          if (d <= r - 2) {
-            const uint32_t n = (uint32_t)w[kShape + d];
-            const uint32_t q = rem / n;
-            const uint32_t ii = rem - q * n;
-            so += (int64_t)ii * w[kSrcStride + d];
-            dof += (int64_t)ii * w[kDstStride + d];
-            rem = q;
-          }
+        for (int i = 0; i < kRun; ++i) {
+          const uint32_t ei = e + i * kThreads;
+          v[i] = src[src_off + (int64_t)(ei < e1 ? ei : 0) * ss];
         }
-        v[i] = (src_off >= 0) ? src[src_off + so] : zero_of<T>();
-        dofs[i] = dof;
       }
-    }
 #pragma unroll
-    for (int i = 0; i < kRun; ++i)
-      if (dofs[i] >= 0) dst[dst_off + dofs[i]] = v[i];
+      for (int i = 0; i < kRun; ++i) {
+        const uint32_t ei = e + i * kThreads;
+        if (ei < e1) dst[dst_off + ei] = (src_off >= 0) ? v[i] : zero_of<T>();
+      }
+      return;
+    }
+    switch (r) {
+      case 2: gather_box<T, 2>(dst, src, w, e, e1, src_off, dst_off); break;
+      case 3: gather_box<T, 3>(dst, src, w, e, e1, src_off, dst_off); break;
+      case 4: gather_box<T, 4>(dst, src, w, e, e1, src_off, dst_off); break;
+      case 5: gather_box<T, 5>(dst, src, w, e, e1, src_off, dst_off); break;
+      case 6: gather_box<T, 6>(dst, src, w, e, e1, src_off, dst_off); break;
+      case 7: gather_box<T, 7>(dst, src, w, e, e1, src_off, dst_off); break;
+      default: gather_box<T, YH_MAXR>(dst, src, w, e, e1, src_off, dst_off); break;
+    }
     return;
   }
 
@@ -137,16 +184,18 @@ __global__ void __launch_bounds__(kThreads) gather_kernel(T* __restrict__ dst, c
     }
   }
   const int64_t k0 = (int64_t)kt * kTE, j0 = (int64_t)jt * kTE;
-  const int64_t kk_lim = min((int64_t)kTE, shk - k0);
-  const int64_t jj_lim = min((int64_t)kTE, shj - j0);
+  const int kk_lim = (int)min((int64_t)kTE, shk - k0);
+  const int jj_lim = (int)min((int64_t)kTE, shj - j0);
   const int64_t ssj = w[kSrcStride + jd], dsk = w[kDstStride + k];
   const int lane32 = threadIdx.x & (kTE - 1);
   const int row8 = threadIdx.x >> 5;  // 0..7
-  // read: consecutive lanes walk the source-contiguous dim k
+  // read: consecutive lanes walk the source-contiguous dim k; cells outside the box are filled from
+  // the tile's last in-range element and never written out
+  const int kr = min(lane32, kk_lim - 1);
 #pragma unroll
   for (int p = 0; p < kTE / 8; ++p) {
-    const int jj = row8 + 8 * p;
-    if (lane32 < kk_lim && jj < jj_lim) tile[jj][lane32] = src[sbase + (k0 + lane32) /* ss[k]==1 */ + (j0 + jj) * ssj];
+    const int jj = min(row8 + 8 * p, jj_lim - 1);
+    tile[row8 + 8 * p][lane32] = src[sbase + (k0 + kr) /* ss[k]==1 */ + (j0 + jj) * ssj];
   }
   __syncthreads();
   // write: consecutive lanes walk the destination-contiguous dim
