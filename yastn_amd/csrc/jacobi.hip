@@ -24,6 +24,10 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// One zero element (never written), read in place of elements outside a matrix so that loads need no
+// per-lane branch: a load under such a branch gets a wait of its own, which serialises the loads.
+__device__ double2 kZeroElt[1];
+
 // 16 waves per block: the q/2 disjoint pairs of a round are spread over more waves
 constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
@@ -549,11 +553,12 @@ template <bool CPLX, int KS, typename COL>
 __device__ __attribute__((always_inline)) inline void stripe_load(typename Num<CPLX>::T (&a)[KS], COL col, int rows,
                                                                   int st, int nc, int lane) {
   typedef Num<CPLX> N;
-  const int nk4 = (nc + 3) / 4, ra = st * 16 + (lane & 15);
+  typedef typename N::T T;
+  const int ra = st * 16 + (lane & 15);
 #pragma unroll
   for (int sidx = 0; sidx < KS; ++sidx) {
     const int k = 4 * sidx + (lane >> 4);
-    a[sidx] = (sidx < nk4 && k < nc && ra < rows) ? col(k)[ra] : N::make(0.0, 0.0);
+    a[sidx] = *(((k < nc) & (ra < rows)) ? col(k) + ra : (const T*)kZeroElt);
   }
 }
 
@@ -656,7 +661,8 @@ __device__ inline void tile_of(int tl, int T16, int& ti, int& tj) {
 // stride, so the 16 columns of a tile fall in different banks); each wave forms upper-triangle tiles
 // on the matrix cores (lane l supplies A(l % 16, k) = conj(Wp[k, ti 16 + l % 16]) and B(k, l % 16) =
 // Wp[k, tj 16 + l % 16], k = k0 + l / 16) and stores them in the MFMA result order (item v at v 64 + l).
-template <bool CPLX>
+// NST = 2 b kGramRows / kChunkThreads staged elements per thread (8 for b <= 16, 16 for b <= 32).
+template <bool CPLX, int NST>
 __global__ void __launch_bounds__(kChunkThreads)
     panel_gram_kernel(const int64_t* __restrict__ blocks, const int32_t* __restrict__ jobs,
                       const typename Num<CPLX>::T* __restrict__ work, double* __restrict__ part, int b, int maxch,
@@ -675,9 +681,24 @@ __global__ void __launch_bounds__(kChunkThreads)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const T* W = work + d.work_off;
   T* S = reinterpret_cast<T*>(smem_raw);
-  for (int e = tid; e < pp.nc * kGramRows; e += kChunkThreads) {
-    const int l = e / kGramRows, r = e - l * kGramRows;
-    S[l * ld + r] = r < rows ? W[pp.col(l) * pp.p + r0 + r] : N::make(0.0, 0.0);
+  {
+    // all NST loads of a thread are issued before its first LDS store
+    // (an element outside the chunk reads W[0] and is stored as zero)
+    T v[NST];
+#pragma unroll
+    for (int i = 0; i < NST; ++i) {
+      const int e = tid + i * kChunkThreads;
+      const int l = e / kGramRows, r = e - l * kGramRows;
+      const bool ok = (l < pp.nc) & (r < rows);
+      v[i] = W[ok ? pp.col(l) * pp.p + r0 + r : 0];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the stores (no interleaving)
+#pragma unroll
+    for (int i = 0; i < NST; ++i) {
+      const int e = tid + i * kChunkThreads;
+      const int l = e / kGramRows, r = e - l * kGramRows;
+      if (l < pp.nc) S[l * ld + r] = r < rows ? v[i] : N::make(0.0, 0.0);
+    }
   }
   __syncthreads();
   const int T16 = (pp.nc + 15) / 16, ntile = T16 * (T16 + 1) / 2;
@@ -775,10 +796,23 @@ __global__ void __launch_bounds__(kInnerThreads)
       tile_of(tl, T16, ti, tj);
       const int i = ti * 16 + (ln >> 4) + 4 * v, j = tj * 16 + (ln & 15);  // D(l / 16 + 4 v, l % 16)
       double gr = 0.0, gi = 0.0;
-      for (int ch = 0; ch < nch; ++ch) {
-        const double* src = part + (((int64_t)blockIdx.x * maxch + ch) * ntmax + tl) * 256 * (CPLX ? 2 : 1);
-        gr += src[idx];
-        if constexpr (CPLX) gi += src[256 + idx];
+      // chunks in groups of kSumGroup with all loads of a group in flight; summed in chunk order
+      constexpr int kSumGroup = 8;
+      const double* zero = (const double*)kZeroElt;
+      for (int c0 = 0; c0 < nch; c0 += kSumGroup) {
+        double vr[kSumGroup], vi[kSumGroup];
+#pragma unroll
+        for (int g = 0; g < kSumGroup; ++g) {
+          const int ch = c0 + g;
+          const double* src = part + (((int64_t)blockIdx.x * maxch + ch) * ntmax + tl) * 256 * (CPLX ? 2 : 1);
+          vr[g] = *(ch < nch ? src + idx : zero);
+          if constexpr (CPLX) vi[g] = *(ch < nch ? src + 256 + idx : zero);
+        }
+#pragma unroll
+        for (int g = 0; g < kSumGroup; ++g) {
+          gr += vr[g];  // + 0.0 past the last chunk leaves the sum unchanged (it is never -0.0)
+          if constexpr (CPLX) gi += vi[g];
+        }
       }
       if (i < nc && j < nc) {
         G[j * nq + i] = N::make(gr, gi);  // G(i, j)
@@ -1659,9 +1693,11 @@ extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int d
                            : (wide ? (const void*)panel_inner_kernel<false, 16> : (const void*)panel_inner_kernel<false, 8>);
   int st = yh::hip_status(hipFuncSetAttribute(inner, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_inner),
                           "svd panel inner LDS size");
-  if (!st && cplx)
-    st = yh::hip_status(hipFuncSetAttribute((const void*)panel_gram_kernel<true>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_gram),
+  // the Gram kernel stages 2 b x kGramRows elements, NST per thread: 8 for b <= 16, else 16
+  const void* gram = cplx ? (wide ? (const void*)panel_gram_kernel<true, 16> : (const void*)panel_gram_kernel<true, 8>)
+                          : (wide ? (const void*)panel_gram_kernel<false, 16> : (const void*)panel_gram_kernel<false, 8>);
+  if (!st)
+    st = yh::hip_status(hipFuncSetAttribute(gram, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_gram),
                         "svd panel gram LDS size");
   if (st) return st;
   for (int64_t r = 0; r < nrounds; ++r) {
@@ -1670,8 +1706,12 @@ extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int d
     const int32_t* jobs = d_jobs + 3 * off;
     const dim3 gg((unsigned)gch, (unsigned)n), gi((unsigned)n, (unsigned)slices);
     if (cplx) {
-      hipLaunchKernelGGL(panel_gram_kernel<true>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
-                         (const double2*)work, (double*)part, b, gch, d_state, sweep);
+      if (wide)
+        hipLaunchKernelGGL((panel_gram_kernel<true, 16>), gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks,
+                           jobs, (const double2*)work, (double*)part, b, gch, d_state, sweep);
+      else
+        hipLaunchKernelGGL((panel_gram_kernel<true, 8>), gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks,
+                           jobs, (const double2*)work, (double*)part, b, gch, d_state, sweep);
       if (wide)
         hipLaunchKernelGGL((panel_inner_kernel<true, 16>), gi, dim3(kInnerThreads), (size_t)lds_inner, stream,
                            d_blocks, jobs, (double2*)work, floors, (const double*)part, d_state, sweep, b, gch,
@@ -1681,8 +1721,12 @@ extern "C" int yastn_hip_svd_panel_sweep(void* work, const double* floors, int d
                            d_blocks, jobs, (double2*)work, floors, (const double*)part, d_state, sweep, b, gch,
                            lds_inner);
     } else {
-      hipLaunchKernelGGL(panel_gram_kernel<false>, gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks, jobs,
-                         (const double*)work, (double*)part, b, gch, d_state, sweep);
+      if (wide)
+        hipLaunchKernelGGL((panel_gram_kernel<false, 16>), gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks,
+                           jobs, (const double*)work, (double*)part, b, gch, d_state, sweep);
+      else
+        hipLaunchKernelGGL((panel_gram_kernel<false, 8>), gg, dim3(kChunkThreads), (size_t)lds_gram, stream, d_blocks,
+                           jobs, (const double*)work, (double*)part, b, gch, d_state, sweep);
       if (wide)
         hipLaunchKernelGGL((panel_inner_kernel<false, 16>), gi, dim3(kInnerThreads), (size_t)lds_inner, stream,
                            d_blocks, jobs, (double*)work, floors, (const double*)part, d_state, sweep, b, gch,
