@@ -42,6 +42,10 @@ struct Elt<true> {
 // One zero element (never written), read in place of operand elements outside a block.
 __device__ double2 kZeroElt[1];
 
+// Workgroups per CU the register allocation must allow: 4 for 32x32 tiles (<= 128 VGPRs, no spills;
+// the short-K launches of many tiles are bound by how many tiles a CU holds at once), else no bound.
+#define GEMM_MINB(CPLX, BM, BN) ((BM) == 32 && (BN) == 32 ? 4 : 1)
+
 __device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
 // One workgroup per output tile.  Per term, the k loop is software-pipelined: the next BK slice of
@@ -50,7 +54,7 @@ __device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mf
 // fragment per wave keep NS partial accumulators (k sub-steps rotate over them, summed at the end)
 // so that consecutive MFMAs do not wait on each other.
 template <bool CPLX, int BM, int BN, int WM, int WN, int BK>
-__global__ void __launch_bounds__(kThreads) gemm_kernel(Bufs bufs, const int64_t* __restrict__ gemms,
+__global__ void __launch_bounds__(kThreads, GEMM_MINB(CPLX, BM, BN)) gemm_kernel(Bufs bufs, const int64_t* __restrict__ gemms,
                                                         const int64_t* __restrict__ terms,
                                                         const int32_t* __restrict__ tiles) {
   typedef typename Elt<CPLX>::T T;
