@@ -35,7 +35,7 @@ extern "C" {
 #define YH_F64 0
 #define YH_C128 1
 
-#define YH_ABI_VERSION 12
+#define YH_ABI_VERSION 13
 
 /* ---- box-gather plan (one job per copied or zero-filled box) -------------
  * int64 words per job (YH_JOB_WORDS):
@@ -48,11 +48,15 @@ extern "C" {
  *   5 tile_begin  index of the job's first tile
  *   6 ..  6+YH_MAXR-1    shape[YH_MAXR] (destination C order; the last dim has dst stride 1)
  *   6+YH_MAXR ..         src_stride[YH_MAXR]
- *   6+2*YH_MAXR ..       dst_stride[YH_MAXR]   (then padding to YH_JOB_WORDS)
+ *   6+2*YH_MAXR ..       dst_stride[YH_MAXR]
+ *   YH_JOB_BUFFER_WORD   buffer pair of the job: 0, or 1 for the second (destination, source) pair of
+ *                        the paired entry points (0 everywhere else)
+ *   (then padding to YH_JOB_WORDS)
  * tile_job[t] (int32) is the job owning tile t.
  */
 #define YH_MAXR 8
 #define YH_JOB_WORDS 32
+#define YH_JOB_BUFFER_WORD 30
 #define YH_GATHER_TILE 2048 /* elements per kind-0 tile */
 #define YH_TRANSPOSE_TILE 32 /* kind-1 tile edge */
 
@@ -60,7 +64,12 @@ extern "C" {
 int yastn_hip_check_gather_plan(const int64_t* h_jobs, int64_t njobs, const int32_t* h_tile_job,
                                 int64_t ntiles, int64_t src_numel, int64_t dst_numel);
 
-/* Generic engine behind the four entry points that follow. */
+/* The same check for a plan over two (source, destination) pairs (buffer word 0 or 1). */
+int yastn_hip_check_gather_plan2(const int64_t* h_jobs, int64_t njobs, const int32_t* h_tile_job,
+                                 int64_t ntiles, int64_t src_numel0, int64_t dst_numel0, int64_t src_numel1,
+                                 int64_t dst_numel1);
+
+/* Generic engine behind the single-pair entry points that follow. */
 int yastn_hip_block_gather(void* dst, int64_t dst_numel, const void* src, int64_t src_numel, int dtype,
                            const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job, int64_t ntiles,
                            hipStream_t stream);
@@ -78,6 +87,31 @@ int yastn_hip_transpose_and_merge(void* newdata, int64_t Dsize, const void* data
 int yastn_hip_transpose_and_merge_adjoint(void* grad_src, int64_t src_numel, const void* grad_new,
                                           int64_t Dsize, int dtype, const int64_t* d_jobs, int64_t njobs,
                                           const int32_t* d_tile_job, int64_t ntiles, hipStream_t stream);
+
+/* Generic engine of the paired entry points: one launch over the jobs of two gathers, each job writing
+ * dst0 from src0 or dst1 from src1 by its buffer word. */
+int yastn_hip_block_gather2(void* dst0, int64_t dst_numel0, const void* src0, int64_t src_numel0, void* dst1,
+                            int64_t dst_numel1, const void* src1, int64_t src_numel1, int dtype,
+                            const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job, int64_t ntiles,
+                            hipStream_t stream);
+
+/* A1 for both operands of one tensordot in one launch: replaces the pair of
+ *     _transpose_and_merge calls of _tensordot_fc (reference yastn/tensor/_contractions.py:169,174) or of
+ *     _merge_to_matrix in _tensordot_f2m (_contractions.py:145-146), when both operands need a merge and
+ *     share a dtype.  Buffer 0 is operand a, buffer 1 operand b; outputs equal two
+ *     yastn_hip_transpose_and_merge calls bit for bit. */
+int yastn_hip_transpose_and_merge_pair(void* newdata0, int64_t Dsize0, const void* data0, int64_t data_numel0,
+                                       void* newdata1, int64_t Dsize1, const void* data1, int64_t data_numel1,
+                                       int dtype, const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job,
+                                       int64_t ntiles, hipStream_t stream);
+
+/* A6 of the pair above (two kernel_transpose_and_merge.backward gathers,
+ *     _backend_torch_backwards.py:376-392, in one launch). */
+int yastn_hip_transpose_and_merge_adjoint_pair(void* grad_src0, int64_t src_numel0, const void* grad_new0,
+                                               int64_t Dsize0, void* grad_src1, int64_t src_numel1,
+                                               const void* grad_new1, int64_t Dsize1, int dtype,
+                                               const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job,
+                                               int64_t ntiles, hipStream_t stream);
 
 /* A2: replaces backend.unmerge(data, meta) (backend_np.py:664-669,
  *     _backend_torch_backwards.py:395-408). */

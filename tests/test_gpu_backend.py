@@ -57,6 +57,35 @@ def test_copies_bit_exact(B, scenario):
         np.testing.assert_array_equal(_n(got), outs[0])
 
 
+@pytest.mark.parametrize('scenario', ['mps_u1_fuse_contracted', 'u1xu1_c128', 'fuse_u1_fuse_to_matrix'])
+def test_merge_pair_equals_two_merges(B, scenario):
+    """transpose_and_merge_pair (both operands of a tensordot in one launch) against the golden merges
+    bit for bit, forward, and its adjoint against the single-merge adjoints bit for bit."""
+    merges = [args for fn, args, kw, outs in G.calls(scenario) if fn == 'transpose_and_merge']
+    refs = [outs[0] for fn, args, kw, outs in G.calls(scenario) if fn == 'transpose_and_merge']
+    pairs = [(i, j) for i in range(len(merges)) for j in range(i + 1, min(len(merges), i + 3))
+             if merges[i][0].dtype == merges[j][0].dtype]
+    assert pairs
+    for i, j in pairs:
+        print('pair', i, j, flush=True)
+        a, b = merges[i], merges[j]
+        da = _t(a[0]).requires_grad_(True)
+        db = _t(b[0]).requires_grad_(True)
+        oa, ob = B.transpose_and_merge_pair(da, *a[1:], db, *b[1:])
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(_n(oa.detach()), refs[i])
+        np.testing.assert_array_equal(_n(ob.detach()), refs[j])
+        ga = torch.randn(oa.shape, dtype=torch.float64, device=DEV).to(oa.dtype)
+        gb = torch.randn(ob.shape, dtype=torch.float64, device=DEV).to(ob.dtype)
+        gpa, gpb = torch.autograd.grad((oa, ob), (da, db), (ga, gb))
+        sa = _t(a[0]).requires_grad_(True)
+        sb = _t(b[0]).requires_grad_(True)
+        (gsa,) = torch.autograd.grad(B.transpose_and_merge(sa, *a[1:]), sa, ga)
+        (gsb,) = torch.autograd.grad(B.transpose_and_merge(sb, *b[1:]), sb, gb)
+        np.testing.assert_array_equal(_n(gpa), _n(gsa))
+        np.testing.assert_array_equal(_n(gpb), _n(gsb))
+
+
 @pytest.mark.parametrize('scenario', G.SCENARIOS)
 def test_dot(B, scenario):
     for fn, args, kw, outs in G.calls(scenario, 'dot'):

@@ -205,6 +205,48 @@ class TransposeAndMerge(_Fn):
         return run_gather('yastn_hip_transpose_and_merge_adjoint', plan, out, g), None, None, None, None
 
 
+class TransposeAndMergePair(_Fn):
+    """A1 / A6 of both operands of one tensordot in one launch each way: the two reference
+    kernel_transpose_and_merge calls (_backend_torch_backwards.py:338-392) of _tensordot_fc / f2m."""
+
+    @staticmethod
+    def forward(data_a, order_a, meta_new_a, meta_mrg_a, Dsize_a, data_b, order_b, meta_new_b, meta_mrg_b, Dsize_b):
+        data_a, data_b = raw(data_a), raw(data_b)
+        dev = data_a.device
+        pa = P.merge_plan(tuple(order_a), meta_new_a, meta_mrg_a, Dsize_a, data_a.numel(), dev)
+        pb = P.merge_plan(tuple(order_b), meta_new_b, meta_mrg_b, Dsize_b, data_b.numel(), dev)
+        plan = P.pair_plan(pa, pb, dev)
+        out_a = torch.empty(Dsize_a, dtype=data_a.dtype, device=dev)
+        out_b = torch.empty(Dsize_b, dtype=data_b.dtype, device=dev)
+        L.call('yastn_hip_transpose_and_merge_pair', ptr(out_a), Dsize_a, ptr(data_a), data_a.numel(), ptr(out_b),
+               Dsize_b, ptr(data_b), data_b.numel(), dtype_code(out_a), *plan.args(), stream_of(out_a))
+        return out_a, out_b
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        data_a, order_a, meta_new_a, meta_mrg_a, Dsize_a, data_b, order_b, meta_new_b, meta_mrg_b, Dsize_b = inputs
+        ctx.args_a = (tuple(order_a), meta_new_a, meta_mrg_a, Dsize_a, data_a.numel())
+        ctx.args_b = (tuple(order_b), meta_new_b, meta_mrg_b, Dsize_b, data_b.numel())
+
+    @staticmethod
+    @_plain_backward
+    def backward(ctx, ga, gb):
+        ga, gb = raw(ga), raw(gb)
+        dev = ga.device
+        pa = P.merge_adjoint_plan(*ctx.args_a, dev)
+        pb = P.merge_adjoint_plan(*ctx.args_b, dev)
+        out_a = torch.empty(ctx.args_a[4], dtype=ga.dtype, device=dev)
+        out_b = torch.empty(ctx.args_b[4], dtype=gb.dtype, device=dev)
+        if ga.dtype == gb.dtype:
+            plan = P.pair_plan(pa, pb, dev)
+            L.call('yastn_hip_transpose_and_merge_adjoint_pair', ptr(out_a), out_a.numel(), ptr(ga), ga.numel(),
+                   ptr(out_b), out_b.numel(), ptr(gb), gb.numel(), dtype_code(out_a), *plan.args(), stream_of(out_a))
+        else:
+            run_gather('yastn_hip_transpose_and_merge_adjoint', pa, out_a, ga)
+            run_gather('yastn_hip_transpose_and_merge_adjoint', pb, out_b, gb)
+        return out_a, None, None, None, None, out_b, None, None, None, None
+
+
 class Unmerge(_Fn):
     """A2 / A8 (reference _backend_torch_backwards.py:395-426)."""
 

@@ -12,9 +12,10 @@ LIB_PATH = os.environ.get('YASTN_HIP_LIB', os.path.join(HERE, 'libyastn_hip.so')
 # constants mirrored from include/yastn_hip.h
 YH_OK, YH_ERR_ARG, YH_ERR_HIP, YH_ERR_NUMERIC = 0, -1, -2, -3
 YH_F64, YH_C128 = 0, 1
-YH_ABI_VERSION = 12
+YH_ABI_VERSION = 13
 YH_MAXR = 8
 YH_JOB_WORDS = 32
+YH_JOB_BUFFER_WORD = 30
 YH_GATHER_TILE = 2048
 YH_TRANSPOSE_TILE = 32
 YH_GEMM_WORDS = 8
@@ -45,7 +46,12 @@ _dbl = ctypes.c_double
 _SIGS = {
     'yastn_hip_abi_version': [],
     'yastn_hip_check_gather_plan': [_p, _i64, _p, _i64, _i64, _i64],
+    'yastn_hip_check_gather_plan2': [_p, _i64, _p, _i64, _i64, _i64, _i64, _i64],
     'yastn_hip_block_gather': [_p, _i64, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
+    'yastn_hip_block_gather2': [_p, _i64, _p, _i64, _p, _i64, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
+    'yastn_hip_transpose_and_merge_pair': [_p, _i64, _p, _i64, _p, _i64, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
+    'yastn_hip_transpose_and_merge_adjoint_pair': [_p, _i64, _p, _i64, _p, _i64, _p, _i64, _i32, _p, _i64, _p,
+                                                   _i64, _p],
     'yastn_hip_transpose_and_merge': [_p, _i64, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
     'yastn_hip_transpose_and_merge_adjoint': [_p, _i64, _p, _i64, _i32, _p, _i64, _p, _i64, _p],
     'yastn_hip_unmerge': [_p, _p, _i64, _i32, _p, _i64, _p, _i64, _p],

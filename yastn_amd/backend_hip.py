@@ -707,6 +707,20 @@ def transpose_and_merge(data, order, meta_new, meta_mrg, Dsize):
     return F.TransposeAndMerge.apply(data, order, meta_new, meta_mrg, Dsize)
 
 
+def transpose_and_merge_pair(data_a, order_a, meta_new_a, meta_mrg_a, Dsize_a,
+                             data_b, order_b, meta_new_b, meta_mrg_b, Dsize_b):
+    """A1 of both operands of one tensordot (reference _contractions.py:169,174 and 145-146) in one gather
+    launch, adjoint A6 likewise; the same outputs as two transpose_and_merge calls.  Operands of
+    different dtypes take two launches."""
+    if data_a.dtype != data_b.dtype or data_a.device != data_b.device:
+        return (transpose_and_merge(data_a, order_a, meta_new_a, meta_mrg_a, Dsize_a),
+                transpose_and_merge(data_b, order_b, meta_new_b, meta_mrg_b, Dsize_b))
+    args = (data_a, order_a, meta_new_a, meta_mrg_a, Dsize_a, data_b, order_b, meta_new_b, meta_mrg_b, Dsize_b)
+    if not _needs_grad(data_a, data_b):
+        return F.TransposeAndMergePair.forward(*args)
+    return F.TransposeAndMergePair.apply(*args)
+
+
 def unmerge(data, meta):
     """A2: scatter fused blocks back to block layout (one grouped launch); adjoint A8."""
     if not _needs_grad(data):

@@ -578,6 +578,22 @@ def _transpose_and_merge(config, data, order, struct, slices, meta_mrg, inds=Non
     return config.backend.transpose_and_merge(data, order, meta_new, meta_mrg, struct.size)
 
 
+def _transpose_and_merge_pair(config, ops):
+    """_transpose_and_merge of two operands ((data, order, struct, slices, meta_mrg, inds) each); one backend
+    call for both when both need a merge and the backend fuses such pairs (transpose_and_merge_pair)."""
+    todo = []
+    for data, order, struct, slices, meta_mrg, inds in ops:
+        meta_new = tuple((x, y, z.slcs[0]) for x, y, z in zip(struct.t, struct.D, slices))
+        noop = inds is None and tuple(range(len(order))) == tuple(order) \
+            and struct.size == config.backend.get_size(data) \
+            and _no_change_in_transpose_and_merge(meta_mrg, meta_new, struct.size)
+        todo.append(None if noop else (data, order, meta_new, meta_mrg, struct.size))
+    pair = getattr(config.backend, 'transpose_and_merge_pair', None)
+    if pair is not None and todo[0] is not None and todo[1] is not None:
+        return pair(*todo[0], *todo[1])
+    return tuple(op[0] if t is None else config.backend.transpose_and_merge(*t) for op, t in zip(ops, todo))
+
+
 def _merge_to_matrix(a, axes, inds=None):
     order = axes[0] + axes[1]
     struct, slices, meta_mrg, ls_l, ls_r = _meta_merge_to_matrix(a.config, a.struct, a.slices, axes, inds)
@@ -994,8 +1010,11 @@ def tensordot(a, b, axes, conj=(0, 0)):
 def _tensordot_f2m(a, b, nout_a, nin_a, nin_b, nout_b, s_c):
     nsym = a.config.sym.NSYM
     ind_a, ind_b = _common_inds(a.struct.t, b.struct.t, nin_a, nin_b, a.ndim_n, b.ndim_n, nsym)
-    data_a, struct_a, slices_a, ls_l, ls_ac = _merge_to_matrix(a, (nout_a, nin_a), ind_a)
-    data_b, struct_b, slices_b, ls_bc, ls_r = _merge_to_matrix(b, (nin_b, nout_b), ind_b)
+    struct_a, slices_a, mrg_a, ls_l, ls_ac = _meta_merge_to_matrix(a.config, a.struct, a.slices, (nout_a, nin_a), ind_a)
+    struct_b, slices_b, mrg_b, ls_bc, ls_r = _meta_merge_to_matrix(b.config, b.struct, b.slices, (nin_b, nout_b), ind_b)
+    data_a, data_b = _transpose_and_merge_pair(
+        a.config, ((a._data, nout_a + nin_a, struct_a, slices_a, mrg_a, ind_a),
+                   (b._data, nin_b + nout_b, struct_b, slices_b, mrg_b, ind_b)))
     if ls_ac != ls_bc:
         raise BlockSparseError('Bond dimensions do not match.')
     meta_dot, struct_c, slices_c = _meta_tensordot_f2m(struct_a, slices_a, struct_b, slices_b)
@@ -1010,11 +1029,11 @@ def _tensordot_fc(a, b, nout_a, nin_a, nin_b, nout_b):
     axes_a = tuple((x,) for x in nout_a) + (nin_a,)
     order_a = nout_a + nin_a
     struct_a, slices_a, mrg_a, t_a, D_a = _meta_fuse_hard(a.config, a.struct, a.slices, axes_a, ind_a)
-    data_a = _transpose_and_merge(a.config, a._data, order_a, struct_a, slices_a, mrg_a)
     axes_b = (nin_b,) + tuple((x,) for x in nout_b)
     order_b = nin_b + nout_b
     struct_b, slices_b, mrg_b, t_b, D_b = _meta_fuse_hard(b.config, b.struct, b.slices, axes_b, ind_b)
-    data_b = _transpose_and_merge(b.config, b._data, order_b, struct_b, slices_b, mrg_b)
+    data_a, data_b = _transpose_and_merge_pair(a.config, ((a._data, order_a, struct_a, slices_a, mrg_a, None),
+                                                          (b._data, order_b, struct_b, slices_b, mrg_b, None)))
     if not all(D_a[i] == D_b[j] for i, j in zip(nin_a, nin_b)):
         raise BlockSparseError('Bond dimensions do not match.')
     meta_dot, struct_c, slices_c = _meta_tensordot_fc(struct_a, slices_a, struct_b, slices_b)

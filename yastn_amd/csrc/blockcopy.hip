@@ -45,6 +45,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kTE = YH_TRANSPOSE_TILE;
 constexpr int kShape = 6, kSrcStride = 6 + YH_MAXR, kDstStride = 6 + 2 * YH_MAXR;
+constexpr int kBuf = YH_JOB_BUFFER_WORD;  // 0: (dst0, src0), 1: (dst1, src1)
 constexpr int kRun = YH_GATHER_TILE / kThreads;  // elements per thread in a kind-0 tile
 
 template <typename T>
@@ -111,13 +112,17 @@ __device__ __forceinline__ void gather_box(T* __restrict__ dst, const T* __restr
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kThreads) gather_kernel(T* __restrict__ dst, const T* __restrict__ src,
+__global__ void __launch_bounds__(kThreads) gather_kernel(T* __restrict__ dst0, const T* __restrict__ src0,
+                                                          T* __restrict__ dst1, const T* __restrict__ src1,
                                                           const int64_t* __restrict__ jobs,
                                                           const int32_t* __restrict__ tile_job) {
   __shared__ T tile[kTE][kTE + 1];
   const int64_t t = blockIdx.x;
   const int jid = tile_job[t];
   const int64_t* __restrict__ w = jobs + (int64_t)jid * YH_JOB_WORDS;
+  const bool second = w[kBuf] != 0;  // wave-uniform
+  T* __restrict__ dst = second ? dst1 : dst0;
+  const T* __restrict__ src = second ? src1 : src0;
   const int64_t src_off = w[0], dst_off = w[1];
   const int r = (int)w[2], kind = (int)w[3], tdim = (int)w[4];
   const int64_t local = t - w[5];
@@ -208,8 +213,11 @@ __global__ void __launch_bounds__(kThreads) gather_kernel(T* __restrict__ dst, c
 
 }  // namespace
 
-extern "C" int yastn_hip_check_gather_plan(const int64_t* h_jobs, int64_t njobs, const int32_t* h_tile_job,
-                                           int64_t ntiles, int64_t src_numel, int64_t dst_numel) {
+namespace {
+
+// Bounds check of a plan over nbuf (source, destination) pairs; a job's buffer word selects the pair.
+int check_plan(const int64_t* h_jobs, int64_t njobs, const int32_t* h_tile_job, int64_t ntiles,
+               const int64_t* src_numels, const int64_t* dst_numels, int nbuf) {
   YH_CHECK_ARG(njobs >= 0 && ntiles >= 0, "negative plan length");
   YH_CHECK_ARG(njobs == 0 || (h_jobs && h_tile_job), "null plan");
   int64_t expect_tile = 0;
@@ -217,6 +225,9 @@ extern "C" int yastn_hip_check_gather_plan(const int64_t* h_jobs, int64_t njobs,
     const int64_t* w = h_jobs + q * YH_JOB_WORDS;
     const int64_t src_off = w[0], dst_off = w[1];
     const int rank = (int)w[2], kind = (int)w[3], tdim = (int)w[4];
+    YH_CHECK_ARG(w[kBuf] >= 0 && w[kBuf] < nbuf, "job %lld: buffer %lld of %d", (long long)q, (long long)w[kBuf],
+                 nbuf);
+    const int64_t src_numel = src_numels[w[kBuf]], dst_numel = dst_numels[w[kBuf]];
     YH_CHECK_ARG(rank >= 1 && rank <= YH_MAXR, "job %lld: rank %d", (long long)q, rank);
     YH_CHECK_ARG(kind == 0 || kind == 1, "job %lld: kind %d", (long long)q, kind);
     YH_CHECK_ARG(w[5] == expect_tile, "job %lld: tile_begin %lld != %lld", (long long)q, (long long)w[5],
@@ -255,23 +266,77 @@ extern "C" int yastn_hip_check_gather_plan(const int64_t* h_jobs, int64_t njobs,
   return YH_OK;
 }
 
+}  // namespace
+
+extern "C" int yastn_hip_check_gather_plan(const int64_t* h_jobs, int64_t njobs, const int32_t* h_tile_job,
+                                           int64_t ntiles, int64_t src_numel, int64_t dst_numel) {
+  return check_plan(h_jobs, njobs, h_tile_job, ntiles, &src_numel, &dst_numel, 1);
+}
+
+extern "C" int yastn_hip_check_gather_plan2(const int64_t* h_jobs, int64_t njobs, const int32_t* h_tile_job,
+                                            int64_t ntiles, int64_t src_numel0, int64_t dst_numel0,
+                                            int64_t src_numel1, int64_t dst_numel1) {
+  const int64_t sn[2] = {src_numel0, src_numel1}, dn[2] = {dst_numel0, dst_numel1};
+  return check_plan(h_jobs, njobs, h_tile_job, ntiles, sn, dn, 2);
+}
+
+namespace {
+
+int gather_launch(void* dst0, const void* src0, void* dst1, const void* src1, int dtype, const int64_t* d_jobs,
+                  int64_t njobs, const int32_t* d_tile_job, int64_t ntiles, hipStream_t stream) {
+  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
+  YH_CHECK_ARG(ntiles >= 0 && ntiles < (int64_t)INT32_MAX, "tile count %lld", (long long)ntiles);
+  if (ntiles == 0) return YH_OK;
+  YH_CHECK_ARG(dst0 && d_jobs && d_tile_job && njobs > 0, "null pointer");
+  const dim3 grid((unsigned)ntiles), block(kThreads);
+  if (dtype == YH_F64)
+    hipLaunchKernelGGL(gather_kernel<double>, grid, block, 0, stream, (double*)dst0, (const double*)src0,
+                       (double*)dst1, (const double*)src1, d_jobs, d_tile_job);
+  else
+    hipLaunchKernelGGL(gather_kernel<double2>, grid, block, 0, stream, (double2*)dst0, (const double2*)src0,
+                       (double2*)dst1, (const double2*)src1, d_jobs, d_tile_job);
+  return yh::hip_status(hipGetLastError(), "block_gather launch");
+}
+
+}  // namespace
+
 extern "C" int yastn_hip_block_gather(void* dst, int64_t dst_numel, const void* src, int64_t src_numel, int dtype,
                                       const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job,
                                       int64_t ntiles, hipStream_t stream) {
   (void)dst_numel;
   (void)src_numel;
-  YH_CHECK_ARG(dtype == YH_F64 || dtype == YH_C128, "dtype %d", dtype);
-  YH_CHECK_ARG(ntiles >= 0 && ntiles < (int64_t)INT32_MAX, "tile count %lld", (long long)ntiles);
-  if (ntiles == 0) return YH_OK;
-  YH_CHECK_ARG(dst && d_jobs && d_tile_job && njobs > 0, "null pointer");
-  const dim3 grid((unsigned)ntiles), block(kThreads);
-  if (dtype == YH_F64)
-    hipLaunchKernelGGL(gather_kernel<double>, grid, block, 0, stream, (double*)dst, (const double*)src, d_jobs,
-                       d_tile_job);
-  else
-    hipLaunchKernelGGL(gather_kernel<double2>, grid, block, 0, stream, (double2*)dst, (const double2*)src,
-                       d_jobs, d_tile_job);
-  return yh::hip_status(hipGetLastError(), "block_gather launch");
+  return gather_launch(dst, src, dst, src, dtype, d_jobs, njobs, d_tile_job, ntiles, stream);
+}
+
+extern "C" int yastn_hip_block_gather2(void* dst0, int64_t dst_numel0, const void* src0, int64_t src_numel0,
+                                       void* dst1, int64_t dst_numel1, const void* src1, int64_t src_numel1,
+                                       int dtype, const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job,
+                                       int64_t ntiles, hipStream_t stream) {
+  (void)dst_numel0;
+  (void)src_numel0;
+  (void)dst_numel1;
+  (void)src_numel1;
+  YH_CHECK_ARG(ntiles == 0 || dst1, "null pointer");
+  return gather_launch(dst0, src0, dst1, src1, dtype, d_jobs, njobs, d_tile_job, ntiles, stream);
+}
+
+extern "C" int yastn_hip_transpose_and_merge_pair(void* newdata0, int64_t Dsize0, const void* data0,
+                                                  int64_t data_numel0, void* newdata1, int64_t Dsize1,
+                                                  const void* data1, int64_t data_numel1, int dtype,
+                                                  const int64_t* d_jobs, int64_t njobs, const int32_t* d_tile_job,
+                                                  int64_t ntiles, hipStream_t stream) {
+  return yastn_hip_block_gather2(newdata0, Dsize0, data0, data_numel0, newdata1, Dsize1, data1, data_numel1, dtype,
+                                 d_jobs, njobs, d_tile_job, ntiles, stream);
+}
+
+extern "C" int yastn_hip_transpose_and_merge_adjoint_pair(void* grad_src0, int64_t src_numel0, const void* grad_new0,
+                                                          int64_t Dsize0, void* grad_src1, int64_t src_numel1,
+                                                          const void* grad_new1, int64_t Dsize1, int dtype,
+                                                          const int64_t* d_jobs, int64_t njobs,
+                                                          const int32_t* d_tile_job, int64_t ntiles,
+                                                          hipStream_t stream) {
+  return yastn_hip_block_gather2(grad_src0, src_numel0, grad_new0, Dsize0, grad_src1, src_numel1, grad_new1, Dsize1,
+                                 dtype, d_jobs, njobs, d_tile_job, ntiles, stream);
 }
 
 extern "C" int yastn_hip_transpose_and_merge(void* newdata, int64_t Dsize, const void* data, int64_t data_numel,

@@ -155,6 +155,31 @@ class GatherPlan:
         return (self.d_jobs.data_ptr(), self.njobs, self.d_tile_job.data_ptr(), self.ntiles)
 
 
+class GatherPairPlan:
+    """Two gather plans in one launch: the jobs of ``pb`` follow those of ``pa`` with the buffer word set
+    to 1 (see include/yastn_hip.h), tile indices shifted by ``pa``'s tile count."""
+
+    def __init__(self, pa, pb, device):
+        jb = pb.h_jobs.copy()
+        jb[:, 5] += pa.ntiles
+        jb[:, L.YH_JOB_BUFFER_WORD] = 1
+        self.h_jobs = np.ascontiguousarray(np.concatenate([pa.h_jobs, jb]))
+        self.h_tile_job = np.ascontiguousarray(np.concatenate([pa.h_tile_job, pb.h_tile_job + pa.njobs]).astype(np.int32))
+        self.njobs, self.ntiles = pa.njobs + pb.njobs, pa.ntiles + pb.ntiles
+        L.call('yastn_hip_check_gather_plan2', self.h_jobs.ctypes.data, self.njobs, self.h_tile_job.ctypes.data,
+               self.ntiles, pa.src_numel, pa.dst_numel, pb.src_numel, pb.dst_numel)
+        self.d_jobs = torch.from_numpy(self.h_jobs).to(device)
+        self.d_tile_job = torch.from_numpy(self.h_tile_job).to(device)
+
+    def args(self):
+        return (self.d_jobs.data_ptr(), self.njobs, self.d_tile_job.data_ptr(), self.ntiles)
+
+
+def pair_plan(pa, pb, device):
+    """The launch of two gather plans together (cached on the pair)."""
+    return _CACHE.get(('pair', id(pa), id(pb), _device_key(device)), (pa, pb), lambda: GatherPairPlan(pa, pb, device))
+
+
 def _uncovered_boxes(Dn, boxes):
     """Boxes of the C-ordered block of shape Dn not covered by any of ``boxes`` ((start, stop) per dim)."""
     r = len(Dn)
