@@ -145,6 +145,7 @@ int yastn_hip_unmerge_adjoint(void* grad_src, const void* grad_new, int64_t nume
 #define YH_TILE_64x16 4   /* skinny N */
 #define YH_TILE_128x128 5 /* large sectors */
 #define YH_TILE_SMALLM 6  /* M <= 16: streaming kernel, one thread per column (16 x 256 tiles) */
+#define YH_TILE_RD32 7    /* 32 x 32, operands read straight into registers, the 4 waves split K (long K) */
 
 int yastn_hip_check_gemm_plan(const int64_t* h_gemms, int64_t ngemms, const int64_t* h_terms, int64_t nterms,
                               const int32_t* h_tiles, int64_t ntiles, int tile_config,

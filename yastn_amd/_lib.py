@@ -23,9 +23,10 @@ YH_TERM_WORDS = 8
 YH_MAX_BUFS = 4
 YH_TILE_32x32, YH_TILE_64x64, YH_TILE_128x64, YH_TILE_16x64, YH_TILE_64x16, YH_TILE_128x128 = 0, 1, 2, 3, 4, 5
 YH_TILE_SMALLM = 6
+YH_TILE_RD32 = 7
 TILE_DIMS = {YH_TILE_32x32: (32, 32), YH_TILE_64x64: (64, 64), YH_TILE_128x64: (128, 64),
              YH_TILE_16x64: (16, 64), YH_TILE_64x16: (64, 16), YH_TILE_128x128: (128, 128),
-             YH_TILE_SMALLM: (16, 256)}
+             YH_TILE_SMALLM: (16, 256), YH_TILE_RD32: (32, 32)}
 YH_SVD_WORDS = 8
 YH_SVD_MAX_Q = 2048
 YH_QR_WORDS = 8

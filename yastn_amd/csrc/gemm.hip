@@ -289,6 +289,146 @@ __global__ void __launch_bounds__(kThreads, GEMM_MINB(CPLX, BM, BN)) gemm_kernel
 }
 
 
+// Register-direct 32x32 tile (YH_TILE_RD32): no LDS in the k loop and no barrier.  The four waves split
+// each term's k-steps (4 k each) into quarters; every wave computes the whole tile (2 x 2 fragments of
+// 16 x 16) over its quarter, lane l supplying A(row l % 16, k l / 16) and B(k l / 16, col l % 16) of each
+// step straight from memory, kRdDepth steps ahead of the MFMAs (an element outside the operand, or a
+// step past the wave's quarter, is read from kZeroElt and adds zero).  The quarters are summed through
+// LDS in wave order, one fragment per wave, so the result does not depend on timing.
+constexpr int kRdDepth = 4;
+template <bool CPLX>
+__global__ void __launch_bounds__(kThreads, 4) gemm_rd_kernel(Bufs bufs, const int64_t* __restrict__ gemms,
+                                                             const int64_t* __restrict__ terms,
+                                                             const int32_t* __restrict__ tiles) {
+  typedef typename Elt<CPLX>::T T;
+  constexpr int NPL = CPLX ? 2 : 1;
+  __shared__ double red[4][NPL][4][4][64];  // [wave][plane][fragment][item][lane]
+  // the wave index as a scalar: the k loop bounds depend on it, and a loop the compiler thinks divergent
+  // ends each trip waiting for every load in flight
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r16 = lane & 15, kq = lane >> 4;
+  const int32_t* tl = tiles + 3 * (int64_t)blockIdx.x;
+  const int64_t g = tl[0];
+  const int64_t m0 = tl[1], n0 = tl[2];
+  const int64_t* gw = gemms + g * YH_GEMM_WORDS;
+  const int64_t M = gw[0], N = gw[1], c_off = gw[2], c_rs = gw[3], c_cs = gw[4];
+  const int64_t tb = gw[5], tc = gw[6], gflags = gw[7];
+  const T* zero = (const T*)kZeroElt;
+  d4 accR[2][2], accI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = d4{0, 0, 0, 0};
+      accI[i][j] = d4{0, 0, 0, 0};
+    }
+  const bool oka0 = m0 + r16 < M, oka1 = m0 + 16 + r16 < M;
+  const bool okb0 = n0 + r16 < N, okb1 = n0 + 16 + r16 < N;
+  for (int64_t q = tb; q < tb + tc; ++q) {
+    const int64_t* tw = terms + q * YH_TERM_WORDS;
+    const int64_t a_off = tw[0], a_rs = tw[1], a_cs = tw[2];
+    const int64_t b_off = tw[3], b_rs = tw[4], b_cs = tw[5], K = tw[6], tf = tw[7];
+    if (K <= 0) continue;
+    const bool conjA = tf & 1, conjB = (tf >> 1) & 1;
+    const T* __restrict__ A = (const T*)bufs.in[(tf >> 8) & 0xff];
+    const T* __restrict__ B = (const T*)bufs.in[(tf >> 16) & 0xff];
+    const T* pa0 = A + a_off + (m0 + r16) * a_rs;
+    const T* pa1 = pa0 + 16 * a_rs;
+    const T* pb0 = B + b_off + (n0 + r16) * b_cs;
+    const T* pb1 = pb0 + 16 * b_cs;
+    const int64_t nst = (K + 3) / 4, per = (nst + 3) / 4;
+    const int64_t s0 = wave * per, s1 = min(nst, s0 + per);
+    // step s of this lane: k = 4 s + kq; values a0, a1, b0, b1
+    auto load = [&](int64_t s, T* v) {
+      const int64_t k = 4 * s + kq;
+      const bool ok = (s < s1) & (k < K);
+      v[0] = *((ok & oka0) ? pa0 + k * a_cs : zero);
+      v[1] = *((ok & oka1) ? pa1 + k * a_cs : zero);
+      v[2] = *((ok & okb0) ? pb0 + k * b_rs : zero);
+      v[3] = *((ok & okb1) ? pb1 + k * b_rs : zero);
+    };
+    T buf[kRdDepth][4];
+#pragma unroll
+    for (int d = 0; d < kRdDepth; ++d) load(s0 + d, buf[d]);
+    for (int64_t s = s0; s < s1; s += kRdDepth) {
+#pragma unroll
+      for (int d = 0; d < kRdDepth; ++d) {
+        double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          if constexpr (CPLX) {
+            ar[i] = buf[d][i].x;
+            ai[i] = conjA ? -buf[d][i].y : buf[d][i].y;
+            br[i] = buf[d][2 + i].x;
+            bi[i] = conjB ? -buf[d][2 + i].y : buf[d][2 + i].y;
+          } else {
+            ar[i] = buf[d][i];
+            br[i] = buf[d][2 + i];
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            accR[i][j] = mfma(ar[i], br[j], accR[i][j]);
+            if constexpr (CPLX) {
+              accR[i][j] = mfma(-ai[i], bi[j], accR[i][j]);
+              accI[i][j] = mfma(ar[i], bi[j], accI[i][j]);
+              accI[i][j] = mfma(ai[i], br[j], accI[i][j]);
+            }
+          }
+        // refill the slot after its last use, so that the slot keeps its registers from trip to trip (an
+        // overlap of old and new values would need copies, and a copy waits for its load); the program
+        // order is kept: the scheduler would otherwise hoist consumers next to their loads
+        __builtin_amdgcn_sched_barrier(0);
+        load(s + d + kRdDepth, buf[d]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  // quarters -> LDS; wave w sums fragment w over the waves in order and writes it
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        red[wave][0][i * 2 + j][v][lane] = accR[i][j][v];
+        if constexpr (CPLX) red[wave][NPL - 1][i * 2 + j][v][lane] = accI[i][j][v];
+      }
+  __syncthreads();
+  T* __restrict__ C = (T*)bufs.out[(gflags >> 8) & 0xff];
+  const bool accumulate = gflags & 1;
+  const int fi = wave >> 1, fj = wave & 1;
+  const int64_t col = n0 + fj * 16 + r16;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    double sr = red[0][0][wave][v][lane], si = CPLX ? red[0][NPL - 1][wave][v][lane] : 0.0;
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      sr += red[w][0][wave][v][lane];
+      if constexpr (CPLX) si += red[w][NPL - 1][wave][v][lane];
+    }
+    const int64_t row = m0 + fi * 16 + kq + 4 * v;
+    if (row < M && col < N) {
+      T* p = C + c_off + row * c_rs + col * c_cs;
+      if constexpr (CPLX) {
+        double2 r = make_double2(sr, si);
+        if (accumulate) {
+          const double2 o = *p;
+          r.x += o.x;
+          r.y += o.y;
+        }
+        *p = r;
+      } else {
+        double r = sr;
+        if (accumulate) r += *p;
+        *p = r;
+      }
+    }
+  }
+}
+
 // Split-K reduction: for each record, C (=|+=) sum_s P_s in the order s = 0, 1, ... (deterministic).
 constexpr unsigned kSplitKBlocks = 32;
 // Record words (YH_SPLITK_WORDS): 0 M, 1 N, 2 c_off, 3 c_rs, 4 c_cs, 5 p_off (partial s at
@@ -431,6 +571,9 @@ int launch(const Bufs& b, const int64_t* gemms, const int64_t* terms, const int3
     case YH_TILE_128x128:
       hipLaunchKernelGGL((gemm_kernel<CPLX, 128, 128, 2, 2, BKH>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
+    case YH_TILE_RD32:
+      hipLaunchKernelGGL(gemm_rd_kernel<CPLX>, grid, block, 0, s, b, gemms, terms, tiles);
+      break;
     case YH_TILE_SMALLM:
       hipLaunchKernelGGL((gemm_smallm_kernel<CPLX>), grid, block, 0, s, b, gemms, terms, tiles);
       break;
@@ -449,6 +592,7 @@ void tile_dims(int cfg, int* bm, int* bn) {
     case YH_TILE_16x64: *bm = 16; *bn = 64; break;
     case YH_TILE_128x128: *bm = 128; *bn = 128; break;
     case YH_TILE_SMALLM: *bm = kSmallM; *bn = kSmallN; break;
+    case YH_TILE_RD32: *bm = 32; *bn = 32; break;
     default: *bm = 64; *bn = 16; break;
   }
 }
@@ -458,7 +602,7 @@ void tile_dims(int cfg, int* bm, int* bn) {
 extern "C" int yastn_hip_check_gemm_plan(const int64_t* h_gemms, int64_t ngemms, const int64_t* h_terms,
                                          int64_t nterms, const int32_t* h_tiles, int64_t ntiles, int cfg,
                                          const int64_t* in_numel, int nin, const int64_t* out_numel, int nout) {
-  YH_CHECK_ARG(cfg >= YH_TILE_32x32 && cfg <= YH_TILE_SMALLM, "tile config %d", cfg);
+  YH_CHECK_ARG(cfg >= YH_TILE_32x32 && cfg <= YH_TILE_RD32, "tile config %d", cfg);
   YH_CHECK_ARG(nin <= YH_MAX_BUFS && nout <= YH_MAX_BUFS, "too many buffers");
   int bm, bn;
   tile_dims(cfg, &bm, &bn);

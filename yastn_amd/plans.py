@@ -623,7 +623,7 @@ def choose_tile(plan):
            L.YH_TILE_64x16: 0.3, L.YH_TILE_128x128: 1.1, L.YH_TILE_SMALLM: 0.3}
     best, best_cost = None, None
     for cfg, (bm, bn) in L.TILE_DIMS.items():
-        if not plan.applicable(cfg):
+        if not plan.applicable(cfg) or cfg == L.YH_TILE_RD32:  # the register-direct tile: by timing only
             continue
         tiles = np.ceil(M / bm) * np.ceil(N / bn)
         work = bm * bn * K
