@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(kThreads, GEMM_MINB(CPLX, BM, BN)) gemm_kernel
 // LDS in wave order, one fragment per wave, so the result does not depend on timing.
 constexpr int kRdDepth = 4;
 template <bool CPLX>
-__global__ void __launch_bounds__(kThreads, 4) gemm_rd_kernel(Bufs bufs, const int64_t* __restrict__ gemms,
+__global__ void __launch_bounds__(kThreads, CPLX ? 2 : 4) gemm_rd_kernel(Bufs bufs, const int64_t* __restrict__ gemms,
                                                              const int64_t* __restrict__ terms,
                                                              const int32_t* __restrict__ tiles) {
   typedef typename Elt<CPLX>::T T;
