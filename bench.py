@@ -145,7 +145,8 @@ def _dot_launch_stats(backend, workload_fn, reps=50, top=None):
     return out
 
 
-KNAME = {0: '32x32', 1: '64x64', 2: '128x64', 3: '16x64', 4: '64x16', 5: '128x128', 6: 'small-M streaming'}
+KNAME = {0: '32x32', 1: '64x64', 2: '128x64', 3: '16x64', 4: '64x16', 5: '128x128', 6: 'small-M streaming',
+         7: '32x32 register-direct'}
 
 
 def _kernel_desc(plan):
