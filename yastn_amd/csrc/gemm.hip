@@ -39,8 +39,11 @@ struct Elt<true> {
   typedef double2 T;
 };
 
-// One zero element (never written), read in place of operand elements outside a block.
-__device__ double2 kZeroElt[1];
+// Zero elements (never written), read in place of operand elements outside a block; thread t reads
+// element t mod kZeroSpan, so that the fallback loads of a wave spread over cache lines like ordinary
+// coalesced loads instead of all hitting one line.
+constexpr int kZeroSpan = 256;
+__device__ double2 kZeroElt[kZeroSpan];
 
 // Workgroups per CU the register allocation must allow: 4 for 32x32 tiles (<= 128 VGPRs, no spills;
 // the short-K launches of many tiles are bound by how many tiles a CU holds at once), else no bound.
@@ -140,7 +143,7 @@ __global__ void __launch_bounds__(kThreads, GEMM_MINB(CPLX, BM, BN)) gemm_kernel
     T ra0[A_PER], rb0[B_PER], ra1[A_PER], rb1[B_PER];
     // Every lane issues every load; an element outside the operand is read from kZeroElt.  (A select
     // after the load would be hoisted by the scheduler and wait for loads still meant to be in flight.)
-    const T* zero = (const T*)kZeroElt;
+    const T* zero = (const T*)kZeroElt + (tid & (kZeroSpan - 1));
     const T* a_row[A_PER];
     const T* b_col[B_PER];
 #pragma unroll
@@ -313,7 +316,7 @@ __global__ void __launch_bounds__(kThreads, CPLX ? 2 : 4) gemm_rd_kernel(Bufs bu
   const int64_t* gw = gemms + g * YH_GEMM_WORDS;
   const int64_t M = gw[0], N = gw[1], c_off = gw[2], c_rs = gw[3], c_cs = gw[4];
   const int64_t tb = gw[5], tc = gw[6], gflags = gw[7];
-  const T* zero = (const T*)kZeroElt;
+  const T* zero = (const T*)kZeroElt + (threadIdx.x & (kZeroSpan - 1));
   d4 accR[2][2], accI[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)

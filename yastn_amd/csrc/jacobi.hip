@@ -24,9 +24,11 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-// One zero element (never written), read in place of elements outside a matrix so that loads need no
-// per-lane branch: a load under such a branch gets a wait of its own, which serialises the loads.
-__device__ double2 kZeroElt[1];
+// Zero elements (never written), read in place of elements outside a matrix so that loads need no
+// per-lane branch (a load under such a branch gets a wait of its own, which serialises the loads); a
+// thread reads element (its index mod kZeroSpan), so a wave's fallback loads spread over cache lines.
+constexpr int kZeroSpan = 256;
+__device__ double2 kZeroElt[kZeroSpan];
 
 // 16 waves per block: the q/2 disjoint pairs of a round are spread over more waves
 constexpr int kThreads = 1024;
@@ -558,7 +560,7 @@ __device__ __attribute__((always_inline)) inline void stripe_load(typename Num<C
 #pragma unroll
   for (int sidx = 0; sidx < KS; ++sidx) {
     const int k = 4 * sidx + (lane >> 4);
-    a[sidx] = *(((k < nc) & (ra < rows)) ? col(k) + ra : (const T*)kZeroElt);
+    a[sidx] = *(((k < nc) & (ra < rows)) ? col(k) + ra : (const T*)kZeroElt + lane);
   }
 }
 
@@ -798,7 +800,7 @@ __global__ void __launch_bounds__(kInnerThreads)
       double gr = 0.0, gi = 0.0;
       // chunks in groups of kSumGroup with all loads of a group in flight; summed in chunk order
       constexpr int kSumGroup = 8;
-      const double* zero = (const double*)kZeroElt;
+      const double* zero = (const double*)kZeroElt + (idx & (kZeroSpan - 1));
       for (int c0 = 0; c0 < nch; c0 += kSumGroup) {
         double vr[kSumGroup], vi[kSumGroup];
 #pragma unroll

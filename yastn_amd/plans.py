@@ -507,12 +507,14 @@ class GemmPlan:
         fastest = min(times.values())
         return min((c for c in times if times[c] <= fastest * (1 + near)), key=self.operand_traffic)
 
-    def autotune(self, run, reps=3, near=0.05):
+    def autotune(self, run, reps=5, batches=3, near=0.05):
         """Pick a tile shape for this plan by timing ``run()`` (one launch of the plan) with every shape:
-        of the shapes within ``near`` (relative) of the fastest, the one whose tiles request the fewest
-        operand elements (``operand_traffic``), so that near-ties go to the shape with fewer re-reads.
-        The choice is remembered per block-size histogram, so plans of other operands with the same
-        sector shapes reuse it."""
+        each shape's time is the fastest of ``batches`` event-timed batches of ``reps`` launches (a single
+        batch let a stray delay decide between shapes a few percent apart); of the shapes within ``near``
+        (relative) of the fastest, the one whose tiles request the fewest operand elements
+        (``operand_traffic``), so that near-ties go to the shape with fewer re-reads.  The choice is
+        remembered per block-size histogram, so plans of other operands with the same sector shapes reuse
+        it."""
         if getattr(self, 'tuned', False):
             return
         key = self.histogram_key()
@@ -526,14 +528,18 @@ class GemmPlan:
                 continue
             self.set_cfg(cfg)
             run()
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(reps):
-                run()
-            e1.record()
-            e1.synchronize()
-            times[cfg] = e0.elapsed_time(e1) / reps
+            best_t = None
+            for _ in range(batches):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    run()
+                e1.record()
+                e1.synchronize()
+                t = e0.elapsed_time(e1) / reps
+                best_t = t if best_t is None else min(best_t, t)
+            times[cfg] = best_t
         best = self.pick_tile(times, near)
         self.set_cfg(best)
         self.tune_times = times
