@@ -507,7 +507,7 @@ class GemmPlan:
         fastest = min(times.values())
         return min((c for c in times if times[c] <= fastest * (1 + near)), key=self.operand_traffic)
 
-    def autotune(self, run, reps=5, batches=3, near=0.05):
+    def autotune(self, run, reps=3, batches=3, near=0.05):
         """Pick a tile shape for this plan by timing ``run()`` (one launch of the plan) with every shape:
         each shape's time is the fastest of ``batches`` event-timed batches of ``reps`` launches (a single
         batch let a stray delay decide between shapes a few percent apart); of the shapes within ``near``
