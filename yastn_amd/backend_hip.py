@@ -711,8 +711,8 @@ def transpose_and_merge_pair(data_a, order_a, meta_new_a, meta_mrg_a, Dsize_a,
                              data_b, order_b, meta_new_b, meta_mrg_b, Dsize_b):
     """A1 of both operands of one tensordot (reference _contractions.py:169,174 and 145-146) in one gather
     launch, adjoint A6 likewise; the same outputs as two transpose_and_merge calls.  Operands of
-    different dtypes take two launches."""
-    if data_a.dtype != data_b.dtype or data_a.device != data_b.device:
+    different dtypes, and empty outputs, take two launches."""
+    if data_a.dtype != data_b.dtype or data_a.device != data_b.device or Dsize_a == 0 or Dsize_b == 0:
         return (transpose_and_merge(data_a, order_a, meta_new_a, meta_mrg_a, Dsize_a),
                 transpose_and_merge(data_b, order_b, meta_new_b, meta_mrg_b, Dsize_b))
     args = (data_a, order_a, meta_new_a, meta_mrg_a, Dsize_a, data_b, order_b, meta_new_b, meta_mrg_b, Dsize_b)
