@@ -55,6 +55,29 @@ def test_transpose_dot_sum(ca, cb):
         assert np.linalg.norm(g - gr) <= 1e-12 * np.linalg.norm(gr)
 
 
+@pytest.mark.parametrize('ca,cb', [(False, False), (True, True)])
+def test_transpose_dot_sum_every_tile_config(ca, cb):
+    """The multi-term grouped GEMM of transpose_dot_sum (outputs summing several products) on every tile
+    shape, the register-direct one included, against the oracle (<= 1e-12)."""
+    from yastn_amd import backend_hip as B, plans as P
+    rng = np.random.default_rng(43)
+    A, Bd, meta_dot, Ar, Br, Ao, Bo, Dsize = tds_case.make(rng, ca, cb)
+    ref = O.transpose_dot_sum(A, Bd, meta_dot, Ar, Br, Ao, Bo, Dsize)
+    tA, tB = torch.as_tensor(A, device=DEV), torch.as_tensor(Bd, device=DEV)
+    B.transpose_dot_sum(tA, tB, meta_dot, Ar, Br, Ao, Bo, Dsize)  # builds and tunes the plan
+    plan = P.tds_plan(meta_dot, Ar, Br, Ao, Bo, A.size, Bd.size, Dsize, DEV).dot
+    assert max(int(r[6]) for r in plan.h_gemms) > 1  # some outputs sum several products
+    keep = plan.cfg
+    for cfg in P.L.TILE_DIMS:
+        if not plan.applicable(cfg):
+            continue
+        print('tile config', cfg, flush=True)
+        plan.set_cfg(cfg)
+        got = B.transpose_dot_sum(tA, tB, meta_dot, Ar, Br, Ao, Bo, Dsize).cpu().numpy()
+        assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref), cfg
+    plan.set_cfg(keep)
+
+
 def test_transpose_dot_sum_reference_metas():
     """The reference planner's own no-fusion calls (tests/golden nofusion_u1: list-typed meta_dot with
     [a block, b block] product lists, real, mixed real x complex and complex): output vs the reference numpy
