@@ -57,6 +57,68 @@ def test_copies_bit_exact(B, scenario):
         np.testing.assert_array_equal(_n(got), outs[0])
 
 
+def _random_box_jobs(rng, ranks, max_dim):
+    """Permuted copies of C-ordered source blocks of the given ranks (sides 2 ... max_dim up to rank 4, 2 ... 5
+    above) into consecutive C-ordered destination boxes, plus zero-fill boxes between them:
+    (jobs, src_numel, dst_numel)."""
+    jobs, so, do = [], 0, 0
+    for r in ranks:
+        shape = tuple(int(x) for x in rng.integers(2, (max_dim if r <= 4 else 5) + 1, size=r))
+        perm = rng.permutation(r)
+        cs = np.cumprod((1,) + shape[::-1])[:-1][::-1]  # C strides of the source block
+        dshape = tuple(shape[k] for k in perm)
+        dcs = np.cumprod((1,) + dshape[::-1])[:-1][::-1]
+        jobs.append((so, do, dshape, tuple(int(cs[k]) for k in perm), tuple(int(x) for x in dcs)))
+        n = int(np.prod(shape))
+        so, do = so + n, do + n
+        gap = int(rng.integers(0, 3000))
+        if gap:
+            jobs.append((-1, do, (gap,), (0,), (1,)))
+            do += gap
+    return jobs, so, do
+
+
+def _np_gather(jobs, src, dst_numel):
+    out = np.full(dst_numel, np.nan, dtype=src.dtype)
+    for src_off, dst_off, shape, ss, ds in jobs:
+        idx = np.indices(shape).reshape(len(shape), -1)
+        d = dst_off + (np.array(ds)[:, None] * idx).sum(0)
+        out[d] = 0 if src_off < 0 else src[src_off + (np.array(ss)[:, None] * idx).sum(0)]
+    return out
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.complex128])
+def test_box_gather_every_rank(dtype):
+    """The box-gather engine on permuted boxes of rank 1 ... 8 (each rank its own specialised path; boxes
+    span several 2048-element tiles, so the per-thread index stepping crosses tile and dimension
+    boundaries), the 32 x 32 transpose tiles, zero fills, and the paired launch: bit-exact against numpy."""
+    from yastn_amd import plans as P, _functions as F
+    rng = np.random.default_rng(7)
+    for trial in range(3):
+        jobs, sn, dn = _random_box_jobs(rng, [1, 2, 3, 4, 5, 6, 7, 8, 3, 2], 5 if trial < 2 else 9)
+        src = rng.standard_normal(sn)
+        if dtype == np.complex128:
+            src = src + 1j * rng.standard_normal(sn)
+        plan = P.GatherPlan(jobs, sn, dn, DEV)
+        print('trial', trial, 'jobs', plan.njobs, 'tiles', plan.ntiles, 'kinds', sorted(set(plan.h_jobs[:, 3])), flush=True)
+        out = torch.empty(dn, dtype=_t(src).dtype, device=DEV)
+        F.run_gather('yastn_hip_transpose_and_merge', plan, out, _t(src))
+        np.testing.assert_array_equal(_n(out), _np_gather(jobs, src, dn))
+        # the same two plans in one paired launch
+        jobs2, sn2, dn2 = _random_box_jobs(rng, [2, 5, 3], 6)
+        src2 = rng.standard_normal(sn2).astype(src.dtype)
+        plan2 = P.GatherPlan(jobs2, sn2, dn2, DEV)
+        pp = P.pair_plan(plan, plan2, DEV)
+        o1 = torch.empty(dn, dtype=out.dtype, device=DEV)
+        o2 = torch.empty(dn2, dtype=out.dtype, device=DEV)
+        from yastn_amd import _lib as L
+        s1, s2 = _t(src), _t(src2)  # held: a raw pointer does not keep its tensor alive
+        L.call('yastn_hip_transpose_and_merge_pair', o1.data_ptr(), dn, s1.data_ptr(), sn, o2.data_ptr(), dn2,
+               s2.data_ptr(), sn2, F.dtype_code(o1), *pp.args(), F.stream_of(o1))
+        np.testing.assert_array_equal(_n(o1), _np_gather(jobs, src, dn))
+        np.testing.assert_array_equal(_n(o2), _np_gather(jobs2, src2, dn2))
+
+
 @pytest.mark.parametrize('scenario', ['mps_u1_fuse_contracted', 'u1xu1_c128', 'fuse_u1_fuse_to_matrix'])
 def test_merge_pair_equals_two_merges(B, scenario):
     """transpose_and_merge_pair (both operands of a tensordot in one launch) against the golden merges
