@@ -296,6 +296,20 @@ def test_fix_svd_signs(B):
             U3, V3 = B.fix_svd_signs(tU, _t(Vd), meta)
             np.testing.assert_allclose(_n(U3), outs[0], atol=1e-14)
             np.testing.assert_allclose(_n(V3), outs[1], atol=1e-14)
+            # its adjoint against torch autograd of the per-block torch formulation (same pivots)
+            from yastn_amd import backend_plain as BPl
+            gu = torch.randn(U3.shape, dtype=torch.float64, device=DEV).to(U3.dtype)
+            gv = torch.randn(V3.shape, dtype=torch.float64, device=DEV).to(V3.dtype)
+            tU, tV = _t(Ud).requires_grad_(True), _t(Vd).requires_grad_(True)
+            hU, hV = torch.autograd.grad(B.fix_svd_signs(tU, tV, meta), (tU, tV), (gu, gv))
+            pU, pV = _t(Ud).requires_grad_(True), _t(Vd).requires_grad_(True)
+            rU, rV = torch.autograd.grad(BPl.fix_svd_signs(pU, pV, meta), (pU, pV), (gu, gv))
+            assert _rel(_n(hU), _n(rU)) <= 1e-13 and _rel(_n(hV), _n(rV)) <= 1e-13
+            # deterministic: the same adjoint twice, bit for bit
+            tU2, tV2 = _t(Ud).requires_grad_(True), _t(Vd).requires_grad_(True)
+            hU2, hV2 = torch.autograd.grad(B.fix_svd_signs(tU2, tV2, meta), (tU2, tV2), (gu, gv))
+            np.testing.assert_array_equal(_n(hU), _n(hU2))
+            np.testing.assert_array_equal(_n(hV), _n(hV2))
 
 
 def test_eigh(B):

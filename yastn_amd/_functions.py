@@ -857,29 +857,27 @@ class Eigh(_Fn):
 _SIGN_MAPS = P._Cache()
 
 
-def _sign_maps(meta, u_numel, v_numel, s_numel, device):
+def _sign_maps(meta, device):
+    """Per-block words of the sign kernels, and the dot_diag metas (reference layout (sln, slb, Db, sla)) that
+    scale the columns of U and the rows of Vh by one phase per singular value."""
     def build():
-        col = np.zeros(u_numel, dtype=np.int64)
-        row = np.zeros(v_numel, dtype=np.int64)
-        blocks = []
+        blocks, mu, mv = [], [], []
         for sl, D, slU, DU, slS, slV, DV in meta:
-            m, k = DU
-            k2, n = DV
-            col[slU[0]:slU[1]] = (slS[0] + np.tile(np.arange(k), m))
-            row[slV[0]:slV[1]] = (slS[0] + np.repeat(np.arange(k2), n))
             blocks.append((sl[0], D[0], D[1], slU[0], slS[0], slV[0], 0, 0))
+            mu.append((slU, slU, DU, slS))
+            mv.append((slV, slV, DV, slS))
         h = np.array(blocks, dtype=np.int64).reshape(-1, L.YH_SVD_WORDS)
-        return (torch.from_numpy(col).to(device), torch.from_numpy(row).to(device), torch.from_numpy(h).to(device),
-                len(blocks))
-    return _SIGN_MAPS.get(('signs', id(meta), u_numel, v_numel, s_numel, device.index), (meta,), build)
+        return torch.from_numpy(h).to(device), len(blocks), tuple(mu), tuple(mv)
+    return _SIGN_MAPS.get(('signs', id(meta), device.index), (meta,), build)
 
 
 def fix_svd_signs(Udata, Vdata, meta):
     """A11: pivots from the HIP kernel; phases applied with differentiable tensor ops when a gradient
-    is needed (as the reference torch backend does), else in place by the HIP kernel."""
-    s_numel = sum(slS[1] - slS[0] for _, _, _, _, slS, _, _ in meta)
+    is needed (as the reference torch backend does), else in place by the HIP kernel.  On the gradient
+    route the phases scale U's columns and Vh's rows through dot_diag, whose adjoint sums each phase's
+    contributions in a fixed order (deterministic, no sort)."""
     s_numel = max([slS[1] for _, _, _, _, slS, _, _ in meta], default=0)
-    col, row, blocks, nb = _sign_maps(meta, Udata.numel(), Vdata.numel(), s_numel, Udata.device)
+    blocks, nb, mu, mv = _sign_maps(meta, Udata.device)
     dt = dtype_code(Udata)
     if not (torch.is_grad_enabled() and (Udata.requires_grad or Vdata.requires_grad)):
         U2 = raw(Udata).clone()
@@ -893,9 +891,12 @@ def fix_svd_signs(Udata, Vdata, meta):
         return piv
     U0 = Udata.detach()
     piv = _Opaque.apply(pivots, U0) if _is_wrapped(U0) else pivots(U0)
-    ph = Udata[piv[:s_numel]]
+    # one pivot per column, all at distinct positions of U: the adjoint of the selection adds into each
+    # position once
+    ph = torch.index_select(Udata, 0, piv[:s_numel])
     ph = ph / ph.abs()
-    return Udata * ph.conj()[col], Vdata * ph[row]
+    return (DotDiag.apply(ph.conj(), Udata, mu, Udata.numel(), 1, 2),
+            DotDiag.apply(ph, Vdata, mv, Vdata.numel(), 0, 2))
 
 def qr_forward(data, meta, sizes, plan=None):
     """Batched Householder QR of every block (plan: plans.QrPlan)."""
