@@ -1295,8 +1295,7 @@ __global__ void __launch_bounds__(kThreads)
   extern __shared__ double smem_raw[];
   __shared__ double s_red[kWaves];
   __shared__ double s_tr[NB], s_ti[NB];          // tau_j
-  __shared__ double s_Tr[NB][NB], s_Ti[NB][NB];  // T (upper triangular), [row][col]
-  __shared__ double s_zr[NB], s_zi[NB];
+  __shared__ double s_Zr[NB][NB], s_Zi[NB][NB];  // Z = V^H V (upper triangle), [row][col]
   const QrDesc d = load_qr(qdesc + (int64_t)ids[blockIdx.x] * YH_QR_WORDS);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t k0 = (int64_t)step * NB;
@@ -1399,47 +1398,91 @@ __global__ void __launch_bounds__(kThreads)
     if (tid == 0) xp[nb - 1] = N::make(pbeta, 0.0);
   }
   __syncthreads();
-  // T by columns: T(j, j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) z, z_i = V(:, i)^H v_j
-  // (computing every z first and running the recurrence on one wave measured slower: 88 against 53 us
-  // per 32-column panel of a 440-row block)
-  for (int j = 0; j < nb; ++j) {
-    const T* vj = P + (int64_t)j * ld;
-    for (int i = wave; i < j; i += kWaves) {
-      const T* vi = P + (int64_t)i * ld;
-      double zr = 0.0, zi = 0.0;
-      for (int64_t r = j + 1 + lane; r < L; r += 64) {
-        const T a = vi[r], b = vj[r];
-        zr += N::re(a) * N::re(b) + N::im(a) * N::im(b);
-        zi += N::re(a) * N::im(b) - N::im(a) * N::re(b);
-      }
-      zr = wave_sum(zr);
-      if (CPLX) zi = wave_sum(zi);
-      if (lane == 0) {  // row j of v_i times the unit entry of v_j
-        s_zr[i] = zr + N::re(vi[j]);
-        s_zi[i] = zi - N::im(vi[j]);
+  // T from T^-1 = diag(1 / tau) + striu(V^H V) (Joffrain, Low, Quintana-Orti, van de Geijn, Van Zee,
+  // "Accumulating Householder transformations, revisited", ACM TOMS 32 (2006)): column j of T solves a
+  // triangular system, T(j, j) = tau_j and T(i, j) = -tau_i sum_{i < l <= j} Z(i, l) T(l, j), Z = V^H V
+  // (tau_i = 0 gives a zero row and column, as the larft recurrence does).
+  // Z's upper 16 x 16 tiles (00, 01, 11) are formed on the matrix cores by waves 0-11 (tile wave / 4,
+  // the rows cut into four parts whose products are summed in part order); then each wave solves two
+  // columns of T with no workgroup barrier, lanes 0-31 column wave and lanes 32-63 column wave + 16,
+  // lane l holding T(l, column).
+  {
+    const int tile = wave >> 2, part = wave & 3;
+    const int ti = tile == 2 ? 1 : 0, tj = tile == 0 ? 0 : 1;
+    const bool active = tile < 3 && 16 * tj < nb;  // wave-uniform
+    d4 accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
+    if (active) {
+      const int64_t rbeg = 16 * ti;  // V(r, c) = 0 above the diagonal: rows below 16 ti give nothing
+      const int64_t ns = L > rbeg ? (L - rbeg + 3) / 4 : 0;
+      const int64_t q4 = (ns + 3) / 4;
+      const int64_t sb = part * q4, se = ns < sb + q4 ? ns : sb + q4;
+      const int ca = 16 * ti + (lane & 15), cb = 16 * tj + (lane & 15);
+      const int64_t cac = ca < nb ? ca : nb - 1, cbc = cb < nb ? cb : nb - 1;
+      const T one = N::make(1.0, 0.0), zero = N::make(0.0, 0.0);
+      for (int64_t st = sb; st < se; ++st) {
+        const int64_t r = rbeg + 4 * st + (lane >> 4);
+        const int64_t rc = r < L ? r : L - 1;
+        const T va = P[rc + cac * ld], vb = P[rc + cbc * ld];
+        // V(r, c): 0 above the diagonal, 1 on it, the stored vector below
+        const T a = ((ca < nb) & (r < L) & (r >= ca)) ? (r == ca ? one : va) : zero;
+        const T bb = ((cb < nb) & (r < L) & (r >= cb)) ? (r == cb ? one : vb) : zero;
+        accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::re(bb), accR, 0, 0, 0);
+        if constexpr (CPLX) {  // conj(a) b
+          accR = __builtin_amdgcn_mfma_f64_16x16x4f64(N::im(a), N::im(bb), accR, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f64_16x16x4f64(N::re(a), N::im(bb), accI, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f64_16x16x4f64(-N::im(a), N::re(bb), accI, 0, 0, 0);
+        }
       }
     }
-    __syncthreads();
-    if (tid < j) {
-      double ar = 0.0, ai = 0.0;
-      for (int l = tid; l < j; ++l) {
-        ar += s_Tr[tid][l] * s_zr[l] - s_Ti[tid][l] * s_zi[l];
-        ai += s_Tr[tid][l] * s_zi[l] + s_Ti[tid][l] * s_zr[l];
+    // item v of lane l is Z(16 ti + l / 16 + 4 v, 16 tj + l % 16)
+    const int zr0 = 16 * ti + (lane >> 4), zc = 16 * tj + (lane & 15);
+    for (int pp = 0; pp < 4; ++pp) {
+      if (active && part == pp) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int zrow = zr0 + 4 * v;
+          s_Zr[zrow][zc] = pp == 0 ? accR[v] : s_Zr[zrow][zc] + accR[v];
+          if constexpr (CPLX) s_Zi[zrow][zc] = pp == 0 ? accI[v] : s_Zi[zrow][zc] + accI[v];
+        }
       }
-      const double tr_ = s_tr[j], ti_ = s_ti[j];
-      s_Tr[tid][j] = -(tr_ * ar - ti_ * ai);
-      s_Ti[tid][j] = -(tr_ * ai + ti_ * ar);
+      __syncthreads();
     }
-    if (tid == 0) {
-      s_Tr[j][j] = s_tr[j];
-      s_Ti[j][j] = s_ti[j];
-    }
-    __syncthreads();
   }
   T* Tm = work + d.tm_off + (int64_t)step * NB * NB;
-  for (int e = tid; e < NB * NB; e += kThreads) {
-    const int c = e / NB, r = e - c * NB;
-    Tm[e] = (r < nb && c < nb && r <= c) ? N::make(-s_Tr[r][c], -s_Ti[r][c]) : N::make(0.0, 0.0);
+  {
+    const int h = lane >> 5, l = lane & 31, jc = wave + 16 * h;
+    const bool col_ok = jc < nb;
+    double xr = 0.0, xi = 0.0;
+    if (col_ok && l == jc) {
+      xr = s_tr[jc];
+      xi = s_ti[jc];
+    }
+    for (int i = wave + 15; i >= 0; --i) {  // wave-uniform trip count; half 0 needs only i < wave
+      const double zr = s_Zr[i][l], zi = CPLX ? s_Zi[i][l] : 0.0;
+      // Z(i, l) T(l, jc) over i < l <= jc (T(l, jc) is zero for l > jc; Z below the diagonal is unset)
+      const bool use = (l > i) & (l <= jc) & col_ok;
+      double cr = use ? zr * xr - zi * xi : 0.0;
+      double cim = use ? zr * xi + zi * xr : 0.0;
+      cr += dpp_move<0xB1>(cr);
+      cr += dpp_move<0x4E>(cr);
+      cr += dpp_move<0x141>(cr);
+      cr += dpp_move<0x140>(cr);
+      const double sr = h ? read_lane(cr, 32) + read_lane(cr, 48) : read_lane(cr, 0) + read_lane(cr, 16);
+      double si = 0.0;
+      if constexpr (CPLX) {
+        cim += dpp_move<0xB1>(cim);
+        cim += dpp_move<0x4E>(cim);
+        cim += dpp_move<0x141>(cim);
+        cim += dpp_move<0x140>(cim);
+        si = h ? read_lane(cim, 32) + read_lane(cim, 48) : read_lane(cim, 0) + read_lane(cim, 16);
+      }
+      if ((l == i) & (i < jc) & col_ok) {
+        const double tr_ = s_tr[i], ti_ = s_ti[i];
+        xr = -(tr_ * sr - ti_ * si);
+        xi = -(tr_ * si + ti_ * sr);
+      }
+    }
+    Tm[l + jc * NB] = (col_ok && l <= jc) ? N::make(-xr, -xi) : N::make(0.0, 0.0);
   }
   for (int64_t e = tid; e < L * nb; e += kThreads) {
     const int64_t c = e / L, r = e - c * L;
