@@ -1285,73 +1285,64 @@ __device__ inline QrDesc load_qr(const int64_t* w) {
 // The panel is staged in LDS (INPLACE false) or, when it does not fit, factored in place in W0
 // (leading dimension p; the workgroup's barriers order its global reads and writes, and the
 // panel of one block, a few hundred KB, stays in L2).
-template <bool CPLX, bool INPLACE>
-__global__ void __launch_bounds__(kThreads)
-    qr_panel_kernel(const int64_t* __restrict__ qdesc, const int32_t* __restrict__ ids,
-                    typename Num<CPLX>::T* __restrict__ work, int step) {
+// Householder reflector of a column (LAPACK larfg): alpha = x[j], xn2 = |x[j+1:]|^2; H = I - tau v v^H
+// with H^H x = beta e_1 (beta real), v = [1; s x[j+1:]], s = scr + i sci = 1 / (alpha - beta).
+template <bool CPLX>
+__device__ __attribute__((always_inline)) inline void house_coeffs(typename Num<CPLX>::T alpha, double xn2,
+                                                                   double& beta, double& taur, double& taui,
+                                                                   double& scr, double& sci) {
+  typedef Num<CPLX> N;
+  taur = taui = scr = sci = 0.0;
+  if (xn2 == 0.0 && N::im(alpha) == 0.0) {
+    beta = N::re(alpha);  // H = I
+    return;
+  }
+  const double nrm = sqrt(N::abs2(alpha) + xn2);
+  beta = N::re(alpha) >= 0.0 ? -nrm : nrm;
+  const double rb = 1.0 / beta;
+  taur = (beta - N::re(alpha)) * rb;
+  taui = -N::im(alpha) * rb;
+  const double dr = N::re(alpha) - beta, di = N::im(alpha);
+  const double rden = 1.0 / (dr * dr + di * di);
+  scr = dr * rden;
+  sci = -di * rden;
+}
+
+// Column j - 1 of the panel becomes [R(0:j-1, j-1); beta; s x[j:]] (its reflector scaled in place).
+template <bool CPLX>
+__device__ __attribute__((always_inline)) inline void scale_column(typename Num<CPLX>::T* xp, int64_t j, int64_t L,
+                                                                   double scr, double sci, double beta, int tid) {
   typedef Num<CPLX> N;
   typedef typename N::T T;
-  constexpr int NB = YH_QR_NB;
-  extern __shared__ double smem_raw[];
-  __shared__ double s_red[kWaves];
-  __shared__ double s_tr[NB], s_ti[NB];          // tau_j
-  __shared__ double s_Zr[NB][NB], s_Zi[NB][NB];  // Z = V^H V (upper triangle), [row][col]
-  const QrDesc d = load_qr(qdesc + (int64_t)ids[blockIdx.x] * YH_QR_WORDS);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t k0 = (int64_t)step * NB;
-  const int nb = (int)((d.k - k0) < NB ? (d.k - k0) : NB);
-  const int64_t L = d.p - k0;
-  const int64_t p = d.p;
-  T* W0 = work + d.w_off;
-  T* Vall = work + d.vall_off;
-  // the L x nb panel, column-major with leading dimension ld
-  const int64_t ld = INPLACE ? p : L;
-  T* P = INPLACE ? W0 + k0 + k0 * p : reinterpret_cast<T*>(smem_raw);
-  if (!INPLACE) {
-    for (int64_t e = tid; e < L * nb; e += kThreads) {
-      const int64_t c = e / L, r = e - c * L;
-      P[e] = W0[(k0 + r) + (k0 + c) * p];
-    }
-    __syncthreads();
+  for (int64_t r = j + tid; r < L; r += kThreads) {
+    const T v = xp[r];
+    xp[r] = N::make(N::re(v) * scr - N::im(v) * sci, N::re(v) * sci + N::im(v) * scr);
   }
-  // One workgroup barrier per column.  Column j's reflector is applied to the later columns with
-  // v = [1; s x[j+1:]] formed on the fly from the unscaled column (s = 1 / (alpha - beta), the same
-  // products as scaling first); the column itself is scaled during the next column's update, which
-  // does not read it.  The wave that updates column j + 1 also sums the squares of its new tail, the
-  // next reflector's norm, so that no block-wide reduction sits in the loop.
-  double xn2 = 0.0;
-  {
-    double loc = 0.0;
-    for (int64_t r = 1 + tid; r < L; r += kThreads) loc += N::abs2(P[r]);
-    xn2 = block_sum(loc, s_red, lane, wave);
-  }
-  __shared__ double s_nrm[2];  // by column parity: column j's norm is read while j + 1's is written
+  if (tid == 0) xp[j - 1] = N::make(beta, 0.0);
+}
+
+// The column loop of the panel factorisation, one workgroup barrier per column.  Column j's reflector
+// is applied to the later columns with v = [1; s x[j+1:]] formed on the fly from the unscaled column
+// (the same products as scaling first); the column itself is scaled during the next column's update,
+// which does not read it.  The wave that updates column j + 1 also sums the squares of its new tail,
+// the next reflector's norm, so that no block-wide reduction sits in the loop.
+// qr_columns_mem: every column update reads and writes the panel (LDS or, in place, global memory),
+// one wave per later column.
+template <bool CPLX>
+__device__ __attribute__((always_inline)) inline void qr_columns_mem(typename Num<CPLX>::T* P, int64_t ld,
+                                                                     int64_t L, int nb, double xn2, double* s_tr,
+                                                                     double* s_ti, double* s_nrm, int tid, int lane,
+                                                                     int wave) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
   double pscr = 0.0, psci = 0.0, pbeta = 0.0;  // the previous column's scale factor and beta
   for (int j = 0; j < nb; ++j) {
     T* x = P + (int64_t)j * ld;
     if (j > 0) xn2 = s_nrm[j & 1];
     const T alpha = x[j];
-    double beta, taur = 0.0, taui = 0.0, scr = 0.0, sci = 0.0;
-    if (xn2 == 0.0 && N::im(alpha) == 0.0) {
-      beta = N::re(alpha);  // H_j = I
-    } else {
-      const double nrm = sqrt(N::abs2(alpha) + xn2);
-      beta = N::re(alpha) >= 0.0 ? -nrm : nrm;
-      taur = (beta - N::re(alpha)) / beta;
-      taui = -N::im(alpha) / beta;
-      const double dr = N::re(alpha) - beta, di = N::im(alpha);  // v = x / (alpha - beta)
-      const double den = dr * dr + di * di;
-      scr = dr / den;
-      sci = -di / den;
-    }
-    if (j > 0) {  // scale the previous column (no wave reads it any more)
-      T* xp = P + (int64_t)(j - 1) * ld;
-      for (int64_t r = j + tid; r < L; r += kThreads) {
-        const T v = xp[r];
-        xp[r] = N::make(N::re(v) * pscr - N::im(v) * psci, N::re(v) * psci + N::im(v) * pscr);
-      }
-      if (tid == 0) xp[j - 1] = N::make(pbeta, 0.0);
-    }
+    double beta, taur, taui, scr, sci;
+    house_coeffs<CPLX>(alpha, xn2, beta, taur, taui, scr, sci);
+    if (j > 0) scale_column<CPLX>(P + (int64_t)(j - 1) * ld, j, L, pscr, psci, pbeta, tid);
     if (tid == 0) {
       s_tr[j] = taur;
       s_ti[j] = taui;
@@ -1389,14 +1380,153 @@ __global__ void __launch_bounds__(kThreads)
     pbeta = beta;
     __syncthreads();  // updated columns and s_nrm visible; every thread has read alpha and s_nrm
   }
-  {  // scale the last column
-    T* xp = P + (int64_t)(nb - 1) * ld;
-    for (int64_t r = nb + tid; r < L; r += kThreads) {
-      const T v = xp[r];
-      xp[r] = N::make(N::re(v) * pscr - N::im(v) * psci, N::re(v) * psci + N::im(v) * pscr);
+  scale_column<CPLX>(P + (int64_t)(nb - 1) * ld, nb, L, pscr, psci, pbeta, tid);
+}
+
+// Rows per lane of the register instance for the largest LDS-resident panels.
+template <bool CPLX>
+constexpr int kRegRows = CPLX ? 5 : 9;
+
+// qr_columns_reg (panel in LDS, L <= 64 RPL): column c stays in the registers of wave c % kWaves
+// (slot c / kWaves; lane l holds rows l + 64 u) until it becomes the reflector; its owner writes it
+// back to the LDS panel one iteration earlier, so every column is in the LDS when it is read as a
+// reflector and the LDS traffic of a column step is one read of the reflector per wave.
+template <bool CPLX, int RPL>
+__device__ __attribute__((always_inline)) inline void qr_columns_reg(typename Num<CPLX>::T* P, int64_t ld,
+                                                                     int64_t L, int nb, double xn2, double* s_tr,
+                                                                     double* s_ti, double* s_nrm, int tid, int lane,
+                                                                     int wave) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  constexpr int NS = YH_QR_NB / kWaves;  // column slots per wave
+  const T one = N::make(1.0, 0.0), zero = N::make(0.0, 0.0);
+  T Y[NS][RPL];
+#pragma unroll
+  for (int sl = 0; sl < NS; ++sl) {
+    const int c = wave + kWaves * sl;
+    const int64_t cc = c < nb ? c : nb - 1;
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) {
+      const int64_t r = lane + 64 * u;
+      const T v = P[(r < L ? r : L - 1) + cc * ld];
+      Y[sl][u] = ((c < nb) & (r < L)) ? v : zero;
     }
-    if (tid == 0) xp[nb - 1] = N::make(pbeta, 0.0);
   }
+  double pscr = 0.0, psci = 0.0, pbeta = 0.0;
+  for (int j = 0; j < nb; ++j) {
+    const T* x = P + (int64_t)j * ld;
+    if (j > 0) xn2 = s_nrm[j & 1];
+    const T alpha = x[j];
+    double beta, taur, taui, scr, sci;
+    house_coeffs<CPLX>(alpha, xn2, beta, taur, taui, scr, sci);
+    if (j > 0) scale_column<CPLX>(P + (int64_t)(j - 1) * ld, j, L, pscr, psci, pbeta, tid);
+    if (tid == 0) {
+      s_tr[j] = taur;
+      s_ti[j] = taui;
+    }
+    if (wave + kWaves * (NS - 1) > j) {  // wave-uniform: this wave owns a later column
+      T v[RPL];  // v = [0 above j; 1; s x[j+1:]]
+#pragma unroll
+      for (int u = 0; u < RPL; ++u) {
+        const int64_t r = lane + 64 * u;
+        const T xr = x[r < L ? r : L - 1];
+        const T sx = N::make(N::re(xr) * scr - N::im(xr) * sci, N::re(xr) * sci + N::im(xr) * scr);
+        v[u] = ((r > j) & (r < L)) ? sx : (r == j ? one : zero);
+      }
+#pragma unroll
+      for (int sl = 0; sl < NS; ++sl) {
+        const int c = wave + kWaves * sl;
+        if (c > j && c < nb) {  // wave-uniform
+          double wr = 0.0, wi = 0.0;  // w = v^H y
+#pragma unroll
+          for (int u = 0; u < RPL; ++u) {
+            wr += N::re(v[u]) * N::re(Y[sl][u]) + N::im(v[u]) * N::im(Y[sl][u]);
+            if constexpr (CPLX) wi += N::re(v[u]) * N::im(Y[sl][u]) - N::im(v[u]) * N::re(Y[sl][u]);
+          }
+          wr = wave_sum(wr);
+          if constexpr (CPLX) wi = wave_sum(wi);
+          const double cr = taur * wr + taui * wi, ci = taur * wi - taui * wr;  // conj(tau) w
+          double loc = 0.0;
+#pragma unroll
+          for (int u = 0; u < RPL; ++u) {
+            const int64_t r = lane + 64 * u;
+            Y[sl][u] = N::make(N::re(Y[sl][u]) - (N::re(v[u]) * cr - N::im(v[u]) * ci),
+                               N::im(Y[sl][u]) - (N::re(v[u]) * ci + N::im(v[u]) * cr));
+            loc += r > j + 1 ? N::abs2(Y[sl][u]) : 0.0;  // rows past L hold zeros
+          }
+          if (c == j + 1) {  // the next reflector: its tail norm, and the column itself into the LDS
+            loc = wave_sum(loc);
+            if (lane == 0) s_nrm[(j + 1) & 1] = loc;
+            T* y = P + (int64_t)c * ld;
+#pragma unroll
+            for (int u = 0; u < RPL; ++u) {
+              const int64_t r = lane + 64 * u;
+              if (r < L) y[r] = Y[sl][u];
+            }
+          }
+        }
+      }
+    }
+    pscr = scr;
+    psci = sci;
+    pbeta = beta;
+    __syncthreads();  // the next reflector and s_nrm visible; every thread has read alpha and s_nrm
+  }
+  scale_column<CPLX>(P + (int64_t)(nb - 1) * ld, nb, L, pscr, psci, pbeta, tid);
+}
+constexpr int64_t kQrMaxRows = (int64_t)1 << 27;  // rows x YH_QR_NB panel elements < 2^32 (32-bit indices)
+
+template <bool CPLX, bool INPLACE>
+__global__ void __launch_bounds__(kThreads)
+    qr_panel_kernel(const int64_t* __restrict__ qdesc, const int32_t* __restrict__ ids,
+                    typename Num<CPLX>::T* __restrict__ work, int step) {
+  typedef Num<CPLX> N;
+  typedef typename N::T T;
+  constexpr int NB = YH_QR_NB;
+  extern __shared__ double smem_raw[];
+  __shared__ double s_red[kWaves];
+  __shared__ double s_tr[NB], s_ti[NB];          // tau_j
+  __shared__ double s_Zr[NB][NB], s_Zi[NB][NB];  // Z = V^H V (upper triangle), [row][col]
+  const QrDesc d = load_qr(qdesc + (int64_t)ids[blockIdx.x] * YH_QR_WORDS);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t k0 = (int64_t)step * NB;
+  const int nb = (int)((d.k - k0) < NB ? (d.k - k0) : NB);
+  const int64_t L = d.p - k0;
+  const int64_t p = d.p;
+  T* W0 = work + d.w_off;
+  T* Vall = work + d.vall_off;
+  // the L x nb panel, column-major with leading dimension ld
+  const int64_t ld = INPLACE ? p : L;
+  T* P = INPLACE ? W0 + k0 + k0 * p : reinterpret_cast<T*>(smem_raw);
+  if (!INPLACE) {  // (the panel fits the LDS, so 32-bit element indices)
+    const uint32_t Lu = (uint32_t)L, n = (uint32_t)(L * nb);
+    for (uint32_t e = tid; e < n; e += kThreads) {
+      const uint32_t c = e / Lu, r = e - c * Lu;
+      P[e] = W0[(k0 + r) + (k0 + c) * p];
+    }
+    __syncthreads();
+  }
+  double xn2 = 0.0;
+  {
+    double loc = 0.0;
+    for (int64_t r = 1 + tid; r < L; r += kThreads) loc += N::abs2(P[r]);
+    xn2 = block_sum(loc, s_red, lane, wave);
+  }
+  __shared__ double s_nrm[2];  // by column parity: column j's norm is read while j + 1's is written
+  bool done = false;
+  if constexpr (!INPLACE) {  // the later columns in registers (rows per lane: the smallest instance)
+    if (L <= 64 * 2) {
+      qr_columns_reg<CPLX, 2>(P, ld, L, nb, xn2, s_tr, s_ti, s_nrm, tid, lane, wave);
+      done = true;
+    } else if (!CPLX && L <= 64 * 4) {
+      qr_columns_reg<CPLX, 4>(P, ld, L, nb, xn2, s_tr, s_ti, s_nrm, tid, lane, wave);
+      done = true;
+    } else if (L <= 64 * kRegRows<CPLX>) {
+      qr_columns_reg<CPLX, kRegRows<CPLX>>(P, ld, L, nb, xn2, s_tr, s_ti, s_nrm, tid, lane, wave);
+      done = true;
+    }
+  }
+  if (!done) qr_columns_mem<CPLX>(P, ld, L, nb, xn2, s_tr, s_ti, s_nrm, tid, lane, wave);
   __syncthreads();
   // T from T^-1 = diag(1 / tau) + striu(V^H V) (Joffrain, Low, Quintana-Orti, van de Geijn, Van Zee,
   // "Accumulating Householder transformations, revisited", ACM TOMS 32 (2006)): column j of T solves a
@@ -1485,7 +1615,7 @@ __global__ void __launch_bounds__(kThreads)
     Tm[l + jc * NB] = (col_ok && l <= jc) ? N::make(-xr, -xi) : N::make(0.0, 0.0);
   }
   for (int64_t e = tid; e < L * nb; e += kThreads) {
-    const int64_t c = e / L, r = e - c * L;
+    const int64_t c = (int64_t)((uint32_t)e / (uint32_t)L), r = e - c * L;  // L nb < 2^32 (q <= YH_SVD_MAX_Q)
     const T v = P[r + c * ld];  // in place: the same element as the W0 store below
     W0[(k0 + r) + (k0 + c) * p] = r <= c ? v : N::make(0.0, 0.0);
     Vall[(k0 + r) + (k0 + c) * p] = r < c ? N::make(0.0, 0.0) : (r == c ? N::make(1.0, 0.0) : v);
@@ -1890,6 +2020,7 @@ extern "C" int yastn_hip_check_qr_plan(const int64_t* h_qdesc, const int64_t* h_
     YH_CHECK_ARG(b >= 0 && q >= 1 && p >= q && q <= YH_SVD_MAX_Q, "qr block %d: %lld x %lld", b, (long long)p,
                  (long long)q);
     YH_CHECK_ARG((sm >= sn ? sm : sn) == p && (sm < sn ? sm : sn) == q, "qr block %d: shape mismatch", b);
+    YH_CHECK_ARG(p <= kQrMaxRows, "qr block %d: %lld rows (at most %lld)", b, (long long)p, (long long)kQrMaxRows);
     YH_CHECK_ARG(lds_bytes == 0 || p * YH_QR_NB * isz <= lds_bytes, "qr block %d: panel of %lld rows exceeds LDS",
                  b, (long long)p);
     const bool ok = w[2] >= 0 && w[2] + p * q <= work_numel && w[3] >= 0 && w[3] + p * q <= work_numel &&
@@ -1975,6 +2106,8 @@ extern "C" int yastn_hip_check_qr_op_plan(const int64_t* h_qdesc, const int64_t*
                  (long long)m, (long long)n, (long long)k);
     YH_CHECK_ARG(lds_bytes == 0 || m * YH_QR_NB * isz <= lds_bytes, "qr block %lld: panel of %lld rows exceeds LDS",
                  (long long)b, (long long)m);
+    YH_CHECK_ARG(m <= kQrMaxRows, "qr block %lld: %lld rows (at most %lld)", (long long)b, (long long)m,
+                 (long long)kQrMaxRows);
     const int64_t np = (k + YH_QR_NB - 1) / YH_QR_NB;
     const bool ok = w[2] >= 0 && w[2] + m * n <= work_numel && w[3] >= 0 && w[3] + m * k <= work_numel &&
                     w[4] >= 0 && w[4] + np * YH_QR_NB * YH_QR_NB <= work_numel && w[5] >= 0 &&
