@@ -1277,14 +1277,6 @@ __device__ inline QrDesc load_qr(const int64_t* w) {
   return d;
 }
 
-// One panel of the blocked Householder QR (LAPACK geqrf / larfg / larft conventions): the
-// (p - k0) x nb panel at column k0 = step * YH_QR_NB is factored column by column,
-// H_j = I - tau_j v_j v_j^H with H_j^H x = beta e_1 (beta real), applying H_j^H to the panel's
-// later columns.  The vectors go to Vall, the R part back to W0, and -T, with H_1 ... H_nb =
-// I - V T V^H, to Tm.  The columns right of the panel are updated by grouped GEMMs on the host's plan.
-// The panel is staged in LDS (INPLACE false) or, when it does not fit, factored in place in W0
-// (leading dimension p; the workgroup's barriers order its global reads and writes, and the
-// panel of one block, a few hundred KB, stays in L2).
 // Householder reflector of a column (LAPACK larfg): alpha = x[j], xn2 = |x[j+1:]|^2; H = I - tau v v^H
 // with H^H x = beta e_1 (beta real), v = [1; s x[j+1:]], s = scr + i sci = 1 / (alpha - beta).
 template <bool CPLX>
@@ -1474,8 +1466,17 @@ __device__ __attribute__((always_inline)) inline void qr_columns_reg(typename Nu
   }
   scale_column<CPLX>(P + (int64_t)(nb - 1) * ld, nb, L, pscr, psci, pbeta, tid);
 }
+
 constexpr int64_t kQrMaxRows = (int64_t)1 << 27;  // rows x YH_QR_NB panel elements < 2^32 (32-bit indices)
 
+// One panel of the blocked Householder QR (LAPACK geqrf / larfg / larft conventions): the
+// (p - k0) x nb panel at column k0 = step * YH_QR_NB is factored column by column,
+// H_j = I - tau_j v_j v_j^H with H_j^H x = beta e_1 (beta real), applying H_j^H to the panel's
+// later columns.  The vectors go to Vall, the R part back to W0, and -T, with H_1 ... H_nb =
+// I - V T V^H, to Tm.  The columns right of the panel are updated by grouped GEMMs on the host's plan.
+// The panel is staged in LDS (INPLACE false) or, when it does not fit, factored in place in W0
+// (leading dimension p; the workgroup's barriers order its global reads and writes, and the
+// panel of one block, a few hundred KB, stays in L2).
 template <bool CPLX, bool INPLACE>
 __global__ void __launch_bounds__(kThreads)
     qr_panel_kernel(const int64_t* __restrict__ qdesc, const int32_t* __restrict__ ids,
