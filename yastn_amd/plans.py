@@ -705,6 +705,9 @@ def _circle_pair(ne, r, k):
 
 
 PRECOND = True  # QR-precondition the panel path of SVD blocks (off: plain one-sided Jacobi on W0)
+# SVD blocks with more columns than this take the QR-preconditioned panel path even where they fit the
+# LDS kernel, whose unpreconditioned sweeps on one workgroup outlast the panel rounds past this size.
+SVD_LDS_MAX_Q = int(os.environ.get('YASTN_HIP_SVD_LDS_MAX_Q', '16'))
 # Panel width b of the panel Jacobi (a workgroup rotates 2b columns per round): smaller panels give more
 # workgroups per round but more rounds per sweep.  Tuning knob; the LDS limit may lower it further.
 PANEL_WIDTH = int(os.environ.get('YASTN_HIP_PANEL_WIDTH', '16'))
@@ -741,7 +744,7 @@ class SvdPlan:
             p, q = (n, m) if tr else (m, n)
             need = p * q + q * q
             jw = None
-            if need * itemsize <= lds_limit:
+            if need * itemsize <= lds_limit and not (mode == 0 and PRECOND and q > SVD_LDS_MAX_Q):
                 small.append(b)
                 lds_bytes = max(lds_bytes, need * itemsize)
                 w_off = 0
