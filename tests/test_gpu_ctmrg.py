@@ -150,15 +150,17 @@ def test_rvb_kagome_reference_ctmrg_on_gpu(k):
         assert np.abs(g - g_ref).max() <= 1e-9 * np.abs(g_ref).max()
 
 
-def test_captured_step_replays_equal_eager():
+@pytest.mark.parametrize('batch', [False, True])
+def test_captured_step_replays_equal_eager(batch):
     """D=6, chi=64 CTMRG step + <Sz>, forward and backward, with static-structure projectors, captured as
-    one HIP graph: every replay (with eager work and synchronisations in between) gives the eager loss and
-    gradient bitwise; the Jacobi convergence flags and the structure check are read after each replay."""
+    one HIP graph (also with the two projector SVDs of each move in one call): every replay (with eager work
+    and synchronisations in between) gives the eager loss and gradient bitwise; the Jacobi convergence
+    flags and the structure check are read after each replay."""
     from yastn_amd import backend_hip
     cfg = bs.make_config(backend_hip, 'U1', device='cuda:0')
     A, env, sz = C.setup(cfg, D=6, chi=64, seed=0)
-    print('capturing D=6 chi=64 fwd+bwd', flush=True)
-    cs, (l_eager, g_eager) = C.captured_fwd_bwd(A, env, sz, 64)
+    print('capturing D=6 chi=64 fwd+bwd, move-batched', batch, flush=True)
+    cs, (l_eager, g_eager) = C.captured_fwd_bwd(A, env, sz, 64, batch=batch)
     for k in range(3):
         loss, grad = cs.replay()
         other = cs.validate()

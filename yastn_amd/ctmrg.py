@@ -612,15 +612,16 @@ class CapturedStep:
         return other_set
 
 
-def captured_fwd_bwd(A, env, sz, chi, warmup=2):
+def captured_fwd_bwd(A, env, sz, chi, warmup=2, batch=False):
     """One CTMRG step + <Sz> with static-structure projectors (fwd + bwd) as a CapturedStep; also returns the
-    eager (loss, grad) of the same step for comparison."""
+    eager (loss, grad) of the same step for comparison.  ``batch``: the two projector SVDs of each move in one
+    batched backend call."""
     import torch
     check = torch.zeros(2, dtype=torch.int64, device=A.data.device)
     opts = static_svd_opts(chi, check)
 
     def run(data):
-        loss = loss_fn(A._replace(data=data), env, sz, chi, opts)
+        loss = loss_fn(A._replace(data=data), env, sz, chi, opts, batch)
         loss.backward()
         return loss
     data = A.data.detach().clone().requires_grad_(True)
@@ -700,6 +701,24 @@ def bench(steps=3, D=6, chi=64, plain=True, cpu=True, host_runs=10, host_warmup=
                            'truncation': 'static: per-sector kept counts = the environment split '
                                          f'{ {k[0]: v for k, v in env_split(chi).items()} }',
                            'projectors_where_global_rule_differs': other}
+    loss_g, grad_g = loss_g.clone(), grad_g.clone()  # (the graph's outputs go with the graph)
+    del cs
+    # the same, with the two projector SVDs of each move in one batched call
+    cs, (lm_e, gm_e) = captured_fwd_bwd(A, env, sz, chi, batch=True)
+    cs.replay()
+    cs.validate()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cs.replay()
+    torch.cuda.synchronize()
+    tgm = (time.perf_counter() - t0) / steps
+    cs.validate()
+    loss_gm, grad_gm = cs.replay()
+    cs.validate()
+    out['graph_static_move_batched'] = {
+        's_per_step': tgm, 'bitwise_equal_eager': bool(torch.equal(loss_gm, lm_e) and torch.equal(grad_gm, gm_e)),
+        'loss_diff_vs_graph_static': abs(float(loss_gm) - float(loss_g)),
+        'grad_rel_diff_vs_graph_static': float((grad_gm - grad_g).norm() / grad_g.norm())}
     del cs
     if plain:
         tp, lp, gp = run(backend_plain, 'cuda:0', max(1, steps // 2))
@@ -707,6 +726,7 @@ def bench(steps=3, D=6, chi=64, plain=True, cpu=True, host_runs=10, host_warmup=
         out['speedup_vs_plain_gpu'] = tp / tb
         out['move_batched_svds']['speedup_vs_plain_gpu'] = tp / tm
         out['graph_static']['speedup_vs_plain_gpu'] = tp / tg
+        out['graph_static_move_batched']['speedup_vs_plain_gpu'] = tp / tgm
         out['grad_rel_diff_vs_plain'] = float((gb - gp).norm() / gp.norm())
         out['plain_svd'] = ('torch.linalg.svd per block, full_matrices=False, preferred linalg library '
                             f'{torch.backends.cuda.preferred_linalg_library()}')
@@ -719,6 +739,7 @@ def bench(steps=3, D=6, chi=64, plain=True, cpu=True, host_runs=10, host_warmup=
         out['speedup_vs_host'] = tc / tb
         out['move_batched_svds']['speedup_vs_host'] = tc / tm
         out['graph_static']['speedup_vs_host'] = tc / tg
+        out['graph_static_move_batched']['speedup_vs_host'] = tc / tgm
     return out
 
 
