@@ -308,18 +308,31 @@ def _inverse_transpose_meta(meta):
 def _mask_host(mask):
     """Host copy of a mask dict (charge -> kept indices).  A value may also be a ``slice`` (the
     reference's flip_charges embeds with {0: slice(None)}, tensor/_single.py:222-225); it is resolved
-    against the masked dimension of each block when the plan is built."""
-    out = {}
+    against the masked dimension of each block when the plan is built.  The tensor values of one dtype
+    and device are concatenated and copied in one transfer (each copy to the host waits for the device)."""
+    def as_index(arr):
+        if arr.dtype == np.bool_:
+            arr = np.nonzero(arr.ravel())[0]
+        return arr.astype(np.int64).ravel()
+    vals, groups = {}, {}
     for k, v in mask.items():
         key = tuple(k) if isinstance(k, (list, tuple)) else k
         if isinstance(v, slice):
-            out[key] = v
-            continue
-        arr = host_array(v) if torch.is_tensor(v) else np.asarray(v)
-        if arr.dtype == np.bool_:
-            arr = np.nonzero(arr.ravel())[0]
-        out[key] = arr.astype(np.int64).ravel()
-    return out
+            vals[key] = v
+        elif torch.is_tensor(v):
+            vals[key] = None
+            groups.setdefault((v.dtype, v.device), []).append((key, v))
+        else:
+            vals[key] = as_index(np.asarray(v))
+    for items in groups.values():
+        if len(items) == 1:
+            arrs = [host_array(items[0][1])]
+        else:
+            flat = host_array(torch.cat([v.reshape(-1) for _, v in items]))
+            arrs = np.split(flat, np.cumsum([v.numel() for _, v in items])[:-1])
+        for (key, _), arr in zip(items, arrs):
+            vals[key] = as_index(arr)
+    return vals
 
 
 def _mask_key(mh):
